@@ -1,0 +1,143 @@
+/*
+ * tmce.h -- C ABI of libtmce.so, the MI355X (gfx950) batched Classic McEliece KEM engine.
+ *
+ * Two layers:
+ *  1. The libmceliece-20250507 crypto_kem interface for the five parameter sets, same names,
+ *     argument meaning, sizes and return conventions, so a program linked against libmceliece
+ *     can link against this library instead:
+ *       mceliece_kem_<set>_keypair / _enc / _dec / _implementation / _compiler
+ *     (LM:doc/api.md:1-115; symbol names from LM:hdef; reference callers
+ *     mceliece-gpu-params/main.cu:17-21,159-167 and mceliece-gpu-bmma/main.cu:60,139).
+ *     Randomness comes from randombytes() exactly as in libmceliece: keypair draws 32 bytes;
+ *     enc draws one 32-byte seed per call and expands it with the seeded chain below.  Install a
+ *     source with tmce_set_randombytes() (default: getrandom()).
+ *  2. Batched extensions (no counterpart in libmceliece; the reference harness has no exported
+ *     API -- its per-stage kernels are launched from mceliece-gpu-params/main.cu:213-262
+ *     (encapsulation) and :326-370 (decapsulation)):
+ *       device-resident key handles, enc/dec over n operations from host or device buffers.
+ *     Encapsulation is seeded per operation: op i uses the chain
+ *       state = 0x40 || seed_i;  each randombytes(buf,len) returns SHAKE256(state, len) and
+ *       then sets state[1..32] = buf[0..31]
+ *     (the reference's GPU gen_e, mceliece-gpu-params/ref/348864.cuh:3434-3486), so its output
+ *     is byte-identical to libmceliece enc() run with that randombytes source.
+ *
+ * Errors: functions return 0 on success; negative TMCE_E* codes for API misuse or device
+ * failure (tmce_last_error() describes the last one).  Per-operation KEM results go to the
+ * status array with libmceliece's meaning (0, or -1 for the 6960119 padding rules).
+ * Thread safety: calls may come from several host threads; they are serialised per device.
+ */
+#ifndef TMCE_H
+#define TMCE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- sizes (LM:api) */
+#define mceliece_kem_348864_PUBLICKEYBYTES 261120
+#define mceliece_kem_348864_SECRETKEYBYTES 6492
+#define mceliece_kem_348864_CIPHERTEXTBYTES 96
+#define mceliece_kem_348864_BYTES 32
+#define mceliece_kem_460896_PUBLICKEYBYTES 524160
+#define mceliece_kem_460896_SECRETKEYBYTES 13608
+#define mceliece_kem_460896_CIPHERTEXTBYTES 156
+#define mceliece_kem_460896_BYTES 32
+#define mceliece_kem_6688128_PUBLICKEYBYTES 1044992
+#define mceliece_kem_6688128_SECRETKEYBYTES 13932
+#define mceliece_kem_6688128_CIPHERTEXTBYTES 208
+#define mceliece_kem_6688128_BYTES 32
+#define mceliece_kem_6960119_PUBLICKEYBYTES 1047319
+#define mceliece_kem_6960119_SECRETKEYBYTES 13948
+#define mceliece_kem_6960119_CIPHERTEXTBYTES 194
+#define mceliece_kem_6960119_BYTES 32
+#define mceliece_kem_8192128_PUBLICKEYBYTES 1357824
+#define mceliece_kem_8192128_SECRETKEYBYTES 14120
+#define mceliece_kem_8192128_CIPHERTEXTBYTES 208
+#define mceliece_kem_8192128_BYTES 32
+
+/* ---------------------------------------------------------------- layer 1: libmceliece API */
+#define TMCE_DECLARE_SET(S)                                                                     \
+  void mceliece_kem_##S##_keypair(unsigned char *pk, unsigned char *sk);                       \
+  int mceliece_kem_##S##_enc(unsigned char *c, unsigned char *k, const unsigned char *pk);     \
+  int mceliece_kem_##S##_dec(unsigned char *k, const unsigned char *c, const unsigned char *sk); \
+  const char *mceliece_kem_##S##_implementation(void);                                          \
+  const char *mceliece_kem_##S##_compiler(void);
+
+/* LM:doc/api.md:39-116, one block per parameter set */
+TMCE_DECLARE_SET(348864)
+TMCE_DECLARE_SET(460896)
+TMCE_DECLARE_SET(6688128)
+TMCE_DECLARE_SET(6960119)
+TMCE_DECLARE_SET(8192128)
+
+/* randomness source of keypair/enc (replaces librandombytes' randombytes(),
+   librandombytes include/randombytes.h:10); NULL restores getrandom() */
+typedef void (*tmce_randombytes_fn)(void *out, long long len);
+void tmce_set_randombytes(tmce_randombytes_fn fn);
+
+/* ---------------------------------------------------------------- layer 2: batched engine */
+enum {
+  TMCE_SET_348864 = 0,
+  TMCE_SET_460896 = 1,
+  TMCE_SET_6688128 = 2,
+  TMCE_SET_6960119 = 3,
+  TMCE_SET_8192128 = 4
+};
+
+enum {
+  TMCE_OK = 0,
+  TMCE_EINVAL = -22,   /* bad argument (set id, null pointer, key of the wrong kind) */
+  TMCE_ENODEV = -19,   /* no usable gfx950 device */
+  TMCE_ENOMEM = -12,   /* device allocation failed */
+  TMCE_EDEVICE = -5    /* a HIP call or kernel launch failed */
+};
+
+typedef struct tmce_key tmce_key;
+
+/* initialise the engine on a device (implicit on first use with device 0) */
+int tmce_init(int device);
+const char *tmce_last_error(void);
+
+/* device-resident key material: from a public key (encapsulation) or a secret key
+   (decapsulation); pk/sk byte layouts as libmceliece */
+int tmce_key_from_pk(int set, const unsigned char *pk, tmce_key **out);
+int tmce_key_from_sk(int set, const unsigned char *sk, tmce_key **out);
+void tmce_key_free(tmce_key *key);
+/* bytes of device memory held by a key handle (for admission control) */
+size_t tmce_key_device_bytes(const tmce_key *key);
+
+/* seeded batched keypair: keypair i uses randombytes = the seeded chain above on seeds[32*i] */
+int tmce_keypair_seeded(int set, unsigned char *pk, unsigned char *sk, const unsigned char *seed);
+
+/* batched KEM over host buffers: ct[n*CIPHERTEXTBYTES], k[n*32], status[n] (may be NULL),
+   seeds[n*32] (encapsulation seeds, see above); synchronous */
+int tmce_enc_batch(tmce_key *pk, unsigned char *ct, unsigned char *k, int *status,
+                   const unsigned char *seeds, size_t n);
+int tmce_dec_batch(tmce_key *sk, unsigned char *k, int *status, const unsigned char *ct, size_t n);
+
+/* the same over device buffers, asynchronous on `stream` (a hipStream_t; NULL = the engine's
+   stream); inputs resident in HBM */
+int tmce_enc_batch_device(tmce_key *pk, unsigned char *d_ct, unsigned char *d_k, int *d_status,
+                          const unsigned char *d_seeds, size_t n, void *stream);
+int tmce_dec_batch_device(tmce_key *sk, unsigned char *d_k, int *d_status, const unsigned char *d_ct,
+                          size_t n, void *stream);
+
+/* device-memory helpers for callers without a HIP runtime binding */
+void *tmce_dev_alloc(size_t bytes);
+void tmce_dev_free(void *p);
+int tmce_memcpy_h2d(void *d, const void *h, size_t bytes);
+int tmce_memcpy_d2h(void *h, const void *d, size_t bytes);
+int tmce_sync(void);
+
+/* kernel timing: while enabled, every kernel launch of the engine is bracketed by HIP events on
+   its stream; tmce_timing_read() returns per-kernel launch counts and total milliseconds */
+int tmce_timing_enable(int on);
+int tmce_timing_read(int max, char *names /* max*48 bytes */, long *counts, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TMCE_H */

@@ -1,0 +1,256 @@
+"""tensormceliece_amd -- Python binding of libtmce.so, the gfx950 batched Classic McEliece KEM.
+
+The shared library is the product; this module is a thin ctypes layer over its C ABI
+(include/tmce.h) mirroring libmceliece's crypto_kem interface (keypair / enc / dec per parameter
+set, LM:doc/api.md) plus the batched, device-resident entry points.  There is no CPU fallback:
+importing works anywhere, but every KEM call needs the built library and a gfx950 GPU and raises
+otherwise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtmce.so")
+
+SETS = {"348864": 0, "460896": 1, "6688128": 2, "6960119": 3, "8192128": 4}
+# (m, n, t) and byte sizes; sizes as libmceliece's api (LM:api)
+PARAMS = {
+    "348864": dict(m=12, n=3488, t=64, pk=261120, sk=6492, ct=96),
+    "460896": dict(m=13, n=4608, t=96, pk=524160, sk=13608, ct=156),
+    "6688128": dict(m=13, n=6688, t=128, pk=1044992, sk=13932, ct=208),
+    "6960119": dict(m=13, n=6960, t=119, pk=1047319, sk=13948, ct=194),
+    "8192128": dict(m=13, n=8192, t=128, pk=1357824, sk=14120, ct=208),
+}
+KEY_BYTES = 32
+
+
+class TmceError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Loads libtmce.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TmceError(f"{LIB_PATH} is missing: run tensormceliece_amd/build.py (or __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    P, S, I, V = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, None
+    sig = {
+        "tmce_init": (I, [I]),
+        "tmce_last_error": (ctypes.c_char_p, []),
+        "tmce_key_from_pk": (I, [I, P, ctypes.POINTER(P)]),
+        "tmce_key_from_sk": (I, [I, P, ctypes.POINTER(P)]),
+        "tmce_key_free": (V, [P]),
+        "tmce_key_device_bytes": (S, [P]),
+        "tmce_keypair_seeded": (I, [I, P, P, P]),
+        "tmce_enc_batch": (I, [P, P, P, P, P, S]),
+        "tmce_dec_batch": (I, [P, P, P, P, S]),
+        "tmce_enc_batch_device": (I, [P, P, P, P, P, S, P]),
+        "tmce_dec_batch_device": (I, [P, P, P, P, S, P]),
+        "tmce_dev_alloc": (P, [S]),
+        "tmce_dev_free": (V, [P]),
+        "tmce_memcpy_h2d": (I, [P, P, S]),
+        "tmce_memcpy_d2h": (I, [P, P, S]),
+        "tmce_sync": (I, []),
+        "tmce_timing_enable": (I, [I]),
+        "tmce_timing_read": (I, [I, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = load().tmce_last_error()
+        raise TmceError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def init(device=0):
+    _check(load().tmce_init(device), "tmce_init")
+
+
+def keypair_seeded(set_name, seed):
+    """libmceliece keypair() with randombytes = the seeded chain on `seed` (32 bytes)."""
+    p = PARAMS[set_name]
+    pk = np.zeros(p["pk"], np.uint8)
+    sk = np.zeros(p["sk"], np.uint8)
+    sd = np.frombuffer(bytes(seed), np.uint8).copy()
+    assert sd.size == 32
+    _check(load().tmce_keypair_seeded(SETS[set_name], _ptr(pk), _ptr(sk), _ptr(sd)), "keypair")
+    return pk, sk
+
+
+class _Key:
+    def __init__(self, set_name, raw, kind):
+        self.set_name = set_name
+        self.params = PARAMS[set_name]
+        self._h = ctypes.c_void_p()
+        arr = np.ascontiguousarray(np.frombuffer(bytes(raw), np.uint8) if not isinstance(raw, np.ndarray) else raw,
+                                   dtype=np.uint8)
+        fn = load().tmce_key_from_pk if kind == "pk" else load().tmce_key_from_sk
+        _check(fn(SETS[set_name], _ptr(arr), ctypes.byref(self._h)), f"tmce_key_from_{kind}")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_bytes(self):
+        return load().tmce_key_device_bytes(self._h)
+
+    def close(self):
+        if self._h:
+            load().tmce_key_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PublicKey(_Key):
+    """Device-resident public key; enc(seeds) runs batched encapsulation."""
+
+    def __init__(self, set_name, pk):
+        super().__init__(set_name, pk, "pk")
+
+    def enc(self, seeds):
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint8).reshape(-1, 32)
+        n = seeds.shape[0]
+        ct = np.zeros((n, self.params["ct"]), np.uint8)
+        k = np.zeros((n, KEY_BYTES), np.uint8)
+        st = np.zeros(n, np.int32)
+        _check(load().tmce_enc_batch(self._h, _ptr(ct), _ptr(k), _ptr(st), _ptr(seeds), n), "tmce_enc_batch")
+        return ct, k, st
+
+
+class SecretKey(_Key):
+    """Device-resident secret key; dec(ct) runs batched decapsulation."""
+
+    def __init__(self, set_name, sk):
+        super().__init__(set_name, sk, "sk")
+
+    def dec(self, ct):
+        ct = np.ascontiguousarray(ct, dtype=np.uint8).reshape(-1, self.params["ct"])
+        n = ct.shape[0]
+        k = np.zeros((n, KEY_BYTES), np.uint8)
+        st = np.zeros(n, np.int32)
+        _check(load().tmce_dec_batch(self._h, _ptr(k), _ptr(st), _ptr(ct), n), "tmce_dec_batch")
+        return k, st
+
+
+class DeviceBuffer:
+    """Device memory owned through the library (for benchmarks with inputs resident in HBM)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = load().tmce_dev_alloc(self.nbytes)
+        if not self.ptr:
+            raise TmceError("tmce_dev_alloc failed")
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(load().tmce_memcpy_h2d(self.ptr, _ptr(arr), arr.nbytes), "h2d")
+
+    def download(self, shape, dtype=np.uint8):
+        out = np.zeros(shape, dtype)
+        _check(load().tmce_memcpy_d2h(_ptr(out), self.ptr, out.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            load().tmce_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def enc_device(pk, d_ct, d_k, d_status, d_seeds, n, stream=None):
+    _check(load().tmce_enc_batch_device(pk.handle, d_ct.ptr, d_k.ptr, d_status.ptr if d_status else None,
+                                        d_seeds.ptr, n, stream), "tmce_enc_batch_device")
+
+
+def dec_device(sk, d_k, d_status, d_ct, n, stream=None):
+    _check(load().tmce_dec_batch_device(sk.handle, d_k.ptr, d_status.ptr if d_status else None, d_ct.ptr, n,
+                                        stream), "tmce_dec_batch_device")
+
+
+def sync():
+    _check(load().tmce_sync(), "tmce_sync")
+
+
+def timing_enable(on=True):
+    _check(load().tmce_timing_enable(1 if on else 0), "timing_enable")
+
+
+def timing_read():
+    """{kernel name: (launch count, total ms)} of the engine's kernels since timing_enable(True)."""
+    mx = 64
+    names = ctypes.create_string_buffer(48 * mx)
+    counts = np.zeros(mx, np.int64)
+    tot = np.zeros(mx, np.float64)
+    n = load().tmce_timing_read(mx, names, _ptr(counts), _ptr(tot))
+    if n < 0:
+        raise TmceError("timing_read failed")
+    raw = names.raw
+    out = {}
+    for i in range(n):
+        nm = raw[48 * i:48 * (i + 1)].split(b"\0", 1)[0].decode()
+        out[nm] = (int(counts[i]), float(tot[i]))
+    return out
+
+
+# ---- libmceliece-style single-operation entry points (mceliece_kem_<set>_{keypair,enc,dec})
+def kem_keypair(set_name):
+    p = PARAMS[set_name]
+    pk = np.zeros(p["pk"], np.uint8)
+    sk = np.zeros(p["sk"], np.uint8)
+    f = getattr(load(), f"mceliece_kem_{set_name}_keypair")
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    f.restype = None
+    f(_ptr(pk), _ptr(sk))
+    return pk, sk
+
+
+def kem_enc(set_name, pk):
+    p = PARAMS[set_name]
+    ct = np.zeros(p["ct"], np.uint8)
+    k = np.zeros(KEY_BYTES, np.uint8)
+    pk = np.ascontiguousarray(pk, np.uint8)
+    f = getattr(load(), f"mceliece_kem_{set_name}_enc")
+    f.argtypes = [ctypes.c_void_p] * 3
+    f.restype = ctypes.c_int
+    r = f(_ptr(ct), _ptr(k), _ptr(pk))
+    return r, ct, k
+
+
+def kem_dec(set_name, ct, sk):
+    k = np.zeros(KEY_BYTES, np.uint8)
+    ct = np.ascontiguousarray(ct, np.uint8)
+    sk = np.ascontiguousarray(sk, np.uint8)
+    f = getattr(load(), f"mceliece_kem_{set_name}_dec")
+    f.argtypes = [ctypes.c_void_p] * 3
+    f.restype = ctypes.c_int
+    r = f(_ptr(k), _ptr(ct), _ptr(sk))
+    return r, k

@@ -1,0 +1,513 @@
+// api.cpp -- the C ABI of libtmce.so (declared in include/tmce.h): engine context, device-resident
+// key handles, batched encapsulation/decapsulation and the libmceliece-compatible entry points.
+#include <hip/hip_runtime.h>
+#include <sys/random.h>
+
+#include <cerrno>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/tmce.h"
+#include "host_math.h"
+#include "kem_kernels.h"
+
+namespace tmce {
+KernelTimer *g_ktimer = nullptr;
+}
+
+using namespace tmce;
+
+struct tmce_key {
+  int set = -1;
+  int kind = 0;  // 1 = public (encapsulation), 2 = secret (decapsulation)
+  size_t bytes = 0;
+  bool pad_bad = false;
+  uint32_t *cols = nullptr;
+  uint32_t *F = nullptr;
+  uint16_t *alpha = nullptr, *la = nullptr, *lu = nullptr;
+  uint8_t *rej = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(TMCE_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, bytes) != hipSuccess) return fail(TMCE_ENOMEM, "hipMalloc scratch");
+    cap = bytes;
+    return 0;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct EventTimer : KernelTimer {
+  struct Rec { std::string name; hipEvent_t a, b; };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<long, double>> acc;
+  hipEvent_t get() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  void begin(const char *name, hipStream_t s) override {
+    Rec r{name, get(), get()};
+    (void)hipEventRecord(r.a, s);
+    pending.push_back(r);
+  }
+  void end(hipStream_t s) override { (void)hipEventRecord(pending.back().b, s); }
+  void drain() {
+    for (auto &r : pending) {
+      (void)hipEventSynchronize(r.b);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, r.a, r.b);
+      auto &x = acc[r.name];
+      x.first += 1;
+      x.second += ms;
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    pending.clear();
+  }
+};
+
+struct SetTables {
+  uint16_t *twl = nullptr, *plg = nullptr, *powt = nullptr;
+};
+
+struct Engine {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  uint16_t *lg[2] = {nullptr, nullptr}, *ex[2] = {nullptr, nullptr};
+  SetTables st[NUM_SETS];
+  DevBuf seeds, ct, key, status, ind, msg, S, attempts, queue;
+  EventTimer timer;
+};
+
+Engine *g_engine = nullptr;
+std::mutex g_engine_mu;
+
+template <class T>
+int upload(T **dst, const std::vector<T> &src) {
+  if (hipMalloc((void **)dst, src.size() * sizeof(T)) != hipSuccess) return fail(TMCE_ENOMEM, "hipMalloc table");
+  if (hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(TMCE_EDEVICE, "hipMemcpy table");
+  return 0;
+}
+
+int engine_init(int device) {
+  std::lock_guard<std::mutex> lk(g_engine_mu);
+  if (g_engine) return g_engine->device == device ? 0 : fail(TMCE_EINVAL, "engine already bound to another device");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return fail(TMCE_ENODEV, "no HIP device");
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(TMCE_ENODEV, std::string("libtmce is built for gfx950, device is ") + prop.gcnArchName);
+  HIPCHK(hipSetDevice(device));
+  auto e = std::make_unique<Engine>();
+  e->device = device;
+  HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  for (int w = 0; w < 2; ++w) {
+    const HostField &F = host_field(12 + w);
+    std::vector<uint16_t> ex(F.ex.begin(), F.ex.begin() + F.q);
+    if (int r = upload(&e->lg[w], F.lg)) return r;
+    if (int r = upload(&e->ex[w], ex)) return r;
+  }
+  for (int s = 0; s < NUM_SETS; ++s) {
+    const SetInfo &si = set_info(s);
+    const int D = si.t == 64 ? 64 : 128;
+    FftTables t = build_fft_tables(si.m, D);
+    if (int r = upload(&e->st[s].twl, t.twl)) return r;
+    if (int r = upload(&e->st[s].plg, t.plg)) return r;
+    if (si.t == D) {
+      const HostField &F = host_field(si.m);
+      std::vector<uint16_t> pw(size_t(1) << si.m);
+      for (size_t a = 0; a < pw.size(); ++a) pw[a] = F.pw((uint16_t)a, (uint32_t)si.t);
+      if (int r = upload(&e->st[s].powt, pw)) return r;
+    }
+  }
+  g_engine = e.release();
+  return 0;
+}
+
+Engine *engine() {
+  if (!g_engine && engine_init(0) != 0) return nullptr;
+  return g_engine;
+}
+
+bool valid_set(int set) { return set >= 0 && set < NUM_SETS; }
+
+// ---------------------------------------------------------------- randomness
+tmce_randombytes_fn g_randombytes = nullptr;
+
+void sys_randombytes(void *out, long long len) {
+  uint8_t *p = static_cast<uint8_t *>(out);
+  while (len > 0) {
+    ssize_t r = getrandom(p, (size_t)(len > 256 ? 256 : len), 0);
+    if (r < 0) {
+      if (errno == EINTR) continue;
+      std::abort();
+    }
+    p += r;
+    len -= r;
+  }
+}
+
+void randombytes(void *out, long long len) {
+  if (g_randombytes) g_randombytes(out, len);
+  else sys_randombytes(out, len);
+}
+
+void rnd_global(uint8_t *out, long long len, void *) { randombytes(out, len); }
+
+// the seeded chain: state = 0x40||seed; out = SHAKE256(state,len); state[1..32] = out[0..31]
+struct SeedChain { uint8_t state[33]; };
+void rnd_chain(uint8_t *out, long long len, void *ctx) {
+  SeedChain *c = static_cast<SeedChain *>(ctx);
+  shake256(out, len, c->state, 33);
+  std::memcpy(c->state + 1, out, (size_t)(len < 32 ? len : 32));
+}
+
+// ---------------------------------------------------------------- batched operations
+int enc_device(tmce_key *k, uint8_t *d_ct, uint8_t *d_k, int *d_status, const uint8_t *d_seeds, size_t n,
+               hipStream_t s) {
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  if (!k || k->kind != 1) return fail(TMCE_EINVAL, "encapsulation needs a public-key handle");
+  if (n == 0) return 0;
+  if (n > (size_t)INT32_MAX / 2) return fail(TMCE_EINVAL, "batch too large");
+  const SetInfo &si = set_info(k->set);
+  if (!s) s = e->stream;
+  if (k->pad_bad) {  // LM:crypto_kem/6960119/vec/kem_enc.c: outputs cleared, return -1
+    HIPCHK(launch_fill_u8(d_ct, 0, n * si.synd_bytes, s));
+    HIPCHK(launch_fill_u8(d_k, 0, n * 32, s));
+    if (d_status) HIPCHK(launch_fill_i32(d_status, -1, n, s));
+    return 0;
+  }
+  const int MSGW = (1 + si.n / 8 + si.synd_bytes) / 136 * 34 + 34;
+  if (int r = e->ind.ensure(n * ((si.t + 1) & ~1) * sizeof(uint16_t))) return r;
+  if (int r = e->msg.ensure(n * MSGW * sizeof(uint32_t))) return r;
+  if (int r = e->queue.ensure(64)) return r;
+  EncArgs a{};
+  a.seeds = d_seeds;
+  a.cols = k->cols;
+  a.ct = d_ct;
+  a.key = d_k;
+  a.status = d_status;
+  a.ind = e->ind.as<uint16_t>();
+  a.msg = e->msg.as<uint32_t>();
+  a.attempts = nullptr;
+  a.queue = e->queue.as<unsigned>();
+  a.nops = (int)n;
+  a.gen_blocks = 0;
+  HIPCHK(launch_enc_set(k->set, a, s));
+  return 0;
+}
+
+int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, size_t n, hipStream_t s) {
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  if (!k || k->kind != 2) return fail(TMCE_EINVAL, "decapsulation needs a secret-key handle");
+  if (n == 0) return 0;
+  if (n > (size_t)INT32_MAX / 2) return fail(TMCE_EINVAL, "batch too large");
+  const SetInfo &si = set_info(k->set);
+  if (!s) s = e->stream;
+  const int TT = ((2 * si.t + 31) / 32) * 32;
+  const int RW = si.m * TT / 32;
+  const int MSGW = (1 + si.n / 8 + si.synd_bytes) / 136 * 34 + 34;
+  if (int r = e->S.ensure(n * RW * sizeof(uint32_t))) return r;
+  if (int r = e->msg.ensure(n * MSGW * sizeof(uint32_t))) return r;
+  DecArgs a{};
+  a.ct = d_ct;
+  a.F = k->F;
+  a.alpha = k->alpha;
+  a.la = k->la;
+  a.lu = k->lu;
+  a.rej = k->rej;
+  const int w = si.m - 12;
+  a.ft = FieldTables{e->lg[w], e->ex[w], e->st[k->set].twl, e->st[k->set].plg, e->st[k->set].powt};
+  a.key = d_k;
+  a.status = d_status;
+  a.S = e->S.as<uint32_t>();
+  a.msg = e->msg.as<uint32_t>();
+  a.nops = (int)n;
+  a.decode_blocks = 0;
+  HIPCHK(launch_dec_set(k->set, a, s));
+  return 0;
+}
+
+// key cache for the single-operation libmceliece entry points
+struct CachedKey {
+  std::vector<uint8_t> bytes;
+  tmce_key *key = nullptr;
+};
+std::mutex g_cache_mu;
+std::map<std::pair<int, int>, CachedKey> g_cache;  // (set, kind) -> last key
+
+tmce_key *cached_key(int set, int kind, const uint8_t *bytes, size_t len) {
+  auto &c = g_cache[{set, kind}];
+  if (c.key && c.bytes.size() == len && std::memcmp(c.bytes.data(), bytes, len) == 0) return c.key;
+  if (c.key) tmce_key_free(c.key);
+  c.key = nullptr;
+  tmce_key *k = nullptr;
+  int r = kind == 1 ? tmce_key_from_pk(set, bytes, &k) : tmce_key_from_sk(set, bytes, &k);
+  if (r) return nullptr;
+  c.key = k;
+  c.bytes.assign(bytes, bytes + len);
+  return k;
+}
+
+int single_enc(int set, unsigned char *c, unsigned char *k, const unsigned char *pk) {
+  uint8_t seed[32];
+  randombytes(seed, 32);
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  tmce_key *key = cached_key(set, 1, pk, (size_t)set_info(set).pk_bytes);
+  if (!key) std::abort();
+  int st = 0;
+  if (tmce_enc_batch(key, c, k, &st, seed, 1) != 0) std::abort();
+  return st;
+}
+
+int single_dec(int set, unsigned char *k, const unsigned char *c, const unsigned char *sk) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  const SetInfo &si = set_info(set);
+  uint8_t ctc[256];
+  std::memcpy(ctc, c, si.synd_bytes);  // k and c may alias
+  tmce_key *key = cached_key(set, 2, sk, (size_t)si.sk_bytes);
+  if (!key) std::abort();
+  int st = 0;
+  if (tmce_dec_batch(key, k, &st, ctc, 1) != 0) std::abort();
+  return st;
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+int tmce_init(int device) { return engine_init(device); }
+const char *tmce_last_error(void) { return g_err.c_str(); }
+
+void tmce_set_randombytes(tmce_randombytes_fn fn) { g_randombytes = fn; }
+
+int tmce_key_from_pk(int set, const unsigned char *pk, tmce_key **out) {
+  if (!valid_set(set) || !pk || !out) return fail(TMCE_EINVAL, "tmce_key_from_pk: bad argument");
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  std::lock_guard<std::mutex> lk(e->mu);
+  auto k = std::make_unique<tmce_key>();
+  k->set = set;
+  k->kind = 1;
+  k->pad_bad = pk_padding_bad(set, pk);
+  std::vector<uint32_t> cols;
+  build_enc_cols(set, pk, cols);
+  if (int r = upload(&k->cols, cols)) return r;
+  k->bytes = cols.size() * 4;
+  *out = k.release();
+  return 0;
+}
+
+int tmce_key_from_sk(int set, const unsigned char *sk, tmce_key **out) {
+  if (!valid_set(set) || !sk || !out) return fail(TMCE_EINVAL, "tmce_key_from_sk: bad argument");
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  std::lock_guard<std::mutex> lk(e->mu);
+  auto k = std::make_unique<tmce_key>();
+  k->set = set;
+  k->kind = 2;
+  DecKeyHost h;
+  build_dec_key(set, sk, h);
+  if (int r = upload(&k->F, h.F)) return r;
+  if (int r = upload(&k->alpha, h.alpha)) return r;
+  if (int r = upload(&k->la, h.la)) return r;
+  if (int r = upload(&k->lu, h.lu)) return r;
+  if (int r = upload(&k->rej, h.rej)) return r;
+  k->bytes = h.F.size() * 4 + (h.alpha.size() + h.la.size() + h.lu.size()) * 2 + h.rej.size();
+  *out = k.release();
+  return 0;
+}
+
+void tmce_key_free(tmce_key *k) {
+  if (!k) return;
+  if (Engine *e = g_engine) {
+    std::lock_guard<std::mutex> lk(e->mu);
+    (void)hipStreamSynchronize(e->stream);
+  }
+  (void)hipFree(k->cols);
+  (void)hipFree(k->F);
+  (void)hipFree(k->alpha);
+  (void)hipFree(k->la);
+  (void)hipFree(k->lu);
+  (void)hipFree(k->rej);
+  delete k;
+}
+
+size_t tmce_key_device_bytes(const tmce_key *k) { return k ? k->bytes : 0; }
+
+int tmce_keypair_seeded(int set, unsigned char *pk, unsigned char *sk, const unsigned char *seed) {
+  if (!valid_set(set) || !pk || !sk || !seed) return fail(TMCE_EINVAL, "tmce_keypair_seeded: bad argument");
+  SeedChain c;
+  c.state[0] = 0x40;
+  std::memcpy(c.state + 1, seed, 32);
+  host_keypair(set, pk, sk, rnd_chain, &c);
+  return 0;
+}
+
+int tmce_enc_batch_device(tmce_key *pk, unsigned char *d_ct, unsigned char *d_k, int *d_status,
+                          const unsigned char *d_seeds, size_t n, void *stream) {
+  if (!pk || (n && (!d_ct || !d_k || !d_seeds))) return fail(TMCE_EINVAL, "tmce_enc_batch_device: bad argument");
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  std::lock_guard<std::mutex> lk(e->mu);
+  return enc_device(pk, d_ct, d_k, d_status, d_seeds, n, static_cast<hipStream_t>(stream));
+}
+
+int tmce_dec_batch_device(tmce_key *sk, unsigned char *d_k, int *d_status, const unsigned char *d_ct, size_t n,
+                          void *stream) {
+  if (!sk || (n && (!d_ct || !d_k))) return fail(TMCE_EINVAL, "tmce_dec_batch_device: bad argument");
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  std::lock_guard<std::mutex> lk(e->mu);
+  return dec_device(sk, d_k, d_status, d_ct, n, static_cast<hipStream_t>(stream));
+}
+
+int tmce_enc_batch(tmce_key *pk, unsigned char *ct, unsigned char *k, int *status, const unsigned char *seeds,
+                   size_t n) {
+  if (!pk || (n && (!ct || !k || !seeds))) return fail(TMCE_EINVAL, "tmce_enc_batch: bad argument");
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  if (n == 0) return 0;
+  std::lock_guard<std::mutex> lk(e->mu);
+  const SetInfo &si = set_info(pk->set);
+  if (int r = e->seeds.ensure(n * 32)) return r;
+  if (int r = e->ct.ensure(n * si.synd_bytes)) return r;
+  if (int r = e->key.ensure(n * 32)) return r;
+  if (int r = e->status.ensure(n * sizeof(int))) return r;
+  HIPCHK(hipMemcpyAsync(e->seeds.p, seeds, n * 32, hipMemcpyHostToDevice, e->stream));
+  if (int r = enc_device(pk, e->ct.as<uint8_t>(), e->key.as<uint8_t>(), e->status.as<int>(), e->seeds.as<uint8_t>(), n,
+                         e->stream))
+    return r;
+  HIPCHK(hipMemcpyAsync(ct, e->ct.p, n * si.synd_bytes, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipMemcpyAsync(k, e->key.p, n * 32, hipMemcpyDeviceToHost, e->stream));
+  if (status) HIPCHK(hipMemcpyAsync(status, e->status.p, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+int tmce_dec_batch(tmce_key *sk, unsigned char *k, int *status, const unsigned char *ct, size_t n) {
+  if (!sk || (n && (!ct || !k))) return fail(TMCE_EINVAL, "tmce_dec_batch: bad argument");
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  if (n == 0) return 0;
+  std::lock_guard<std::mutex> lk(e->mu);
+  const SetInfo &si = set_info(sk->set);
+  if (int r = e->ct.ensure(n * si.synd_bytes)) return r;
+  if (int r = e->key.ensure(n * 32)) return r;
+  if (int r = e->status.ensure(n * sizeof(int))) return r;
+  HIPCHK(hipMemcpyAsync(e->ct.p, ct, n * si.synd_bytes, hipMemcpyHostToDevice, e->stream));
+  if (int r = dec_device(sk, e->key.as<uint8_t>(), e->status.as<int>(), e->ct.as<uint8_t>(), n, e->stream)) return r;
+  HIPCHK(hipMemcpyAsync(k, e->key.p, n * 32, hipMemcpyDeviceToHost, e->stream));
+  if (status) HIPCHK(hipMemcpyAsync(status, e->status.p, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+void *tmce_dev_alloc(size_t bytes) {
+  if (!engine()) return nullptr;
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+  return p;
+}
+void tmce_dev_free(void *p) { (void)hipFree(p); }
+int tmce_memcpy_h2d(void *d, const void *h, size_t bytes) {
+  HIPCHK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+int tmce_memcpy_d2h(void *h, const void *d, size_t bytes) {
+  HIPCHK(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+int tmce_sync(void) {
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipDeviceSynchronize());
+  return 0;
+}
+
+int tmce_timing_enable(int on) {
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  std::lock_guard<std::mutex> lk(e->mu);
+  e->timer.drain();
+  if (on) {
+    e->timer.acc.clear();
+    g_ktimer = &e->timer;
+  } else {
+    g_ktimer = nullptr;
+  }
+  return 0;
+}
+
+int tmce_timing_read(int max, char *names, long *counts, double *total_ms) {
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  std::lock_guard<std::mutex> lk(e->mu);
+  e->timer.drain();
+  int i = 0;
+  for (auto &kv : e->timer.acc) {
+    if (i >= max) break;
+    std::snprintf(names + 48 * i, 48, "%s", kv.first.c_str());
+    counts[i] = kv.second.first;
+    total_ms[i] = kv.second.second;
+    ++i;
+  }
+  return i;
+}
+
+#define TMCE_DEFINE_SET(S, ID)                                                                    \
+  void mceliece_kem_##S##_keypair(unsigned char *pk, unsigned char *sk) {                         \
+    host_keypair(ID, pk, sk, rnd_global, nullptr);                                                \
+  }                                                                                               \
+  int mceliece_kem_##S##_enc(unsigned char *c, unsigned char *k, const unsigned char *pk) {       \
+    return single_enc(ID, c, k, pk);                                                              \
+  }                                                                                               \
+  int mceliece_kem_##S##_dec(unsigned char *k, const unsigned char *c, const unsigned char *sk) { \
+    return single_dec(ID, k, c, sk);                                                              \
+  }                                                                                               \
+  const char *mceliece_kem_##S##_implementation(void) { return "tmce-gfx950"; }                   \
+  const char *mceliece_kem_##S##_compiler(void) { return "hipcc " __VERSION__; }
+
+TMCE_DEFINE_SET(348864, 0)
+TMCE_DEFINE_SET(460896, 1)
+TMCE_DEFINE_SET(6688128, 2)
+TMCE_DEFINE_SET(6960119, 3)
+TMCE_DEFINE_SET(8192128, 4)
+
+}  // extern "C"

@@ -1,0 +1,471 @@
+// host_math.cpp -- host-side field arithmetic, key precomputation and key generation.
+#include "host_math.h"
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+
+#include "params.h"
+
+namespace tmce {
+
+// ------------------------------------------------------------------ GF(2^m)
+// m=12: z^12+z^3+1, m=13: z^13+z^4+z^3+z+1 (LM:crypto_kem/<set>/vec/gf.c)
+static uint16_t mul_poly(int m, uint16_t a, uint16_t b) {
+  const uint32_t poly = m == 12 ? 0x1009u : 0x201Bu;
+  uint32_t r = 0;
+  for (int i = 0; i < m; ++i)
+    if ((b >> i) & 1) r ^= (uint32_t)a << i;
+  for (int i = 2 * m - 2; i >= m; --i)
+    if ((r >> i) & 1) r ^= poly << (i - m);
+  return (uint16_t)r;
+}
+
+HostField::HostField(int m_) : m(m_), q((1u << m_) - 1) {
+  lg.assign(1u << m, 0);
+  ex.assign(2 * q, 0);
+  for (uint16_t g = 2;; ++g) {  // smallest generator of the multiplicative group
+    uint16_t x = 1;
+    uint32_t ord = 0;
+    do { x = mul_poly(m, x, g); ++ord; } while (x != 1 && ord <= q);
+    if (ord != q) continue;
+    x = 1;
+    for (uint32_t i = 0; i < 2 * q; ++i) {
+      ex[i] = x;
+      if (i < q) lg[x] = (uint16_t)i;
+      x = mul_poly(m, x, g);
+    }
+    break;
+  }
+}
+
+uint16_t HostField::pw(uint16_t a, uint32_t e) const {
+  if (e == 0) return 1;
+  if (!a) return 0;
+  return ex[(uint64_t)lg[a] * e % q];
+}
+
+const HostField &host_field(int m) {
+  static HostField f12(12), f13(13);
+  if (m == 12) return f12;
+  if (m == 13) return f13;
+  throw std::invalid_argument("unsupported field");
+}
+
+uint16_t bitrev(uint16_t a, int m) {
+  uint16_t r = 0;
+  for (int i = 0; i < m; ++i) r |= (uint16_t)(((a >> i) & 1u) << (m - 1 - i));
+  return r;
+}
+
+// ------------------------------------------------------------------ Benes control bits
+void perm_from_controlbits(int m, const uint8_t *cb, std::vector<uint16_t> &pi) {
+  const size_t n = size_t(1) << m;
+  pi.resize(n);
+  for (size_t i = 0; i < n; ++i) pi[i] = (uint16_t)i;
+  size_t bit = 0;
+  for (int layer = 0; layer < 2 * m - 1; ++layer) {
+    const int s = layer < m ? layer : 2 * m - 2 - layer;
+    const size_t stride = size_t(1) << s;
+    for (size_t blk = 0; blk < n; blk += 2 * stride)
+      for (size_t j = 0; j < stride; ++j, ++bit)
+        if ((cb[bit >> 3] >> (bit & 7)) & 1) std::swap(pi[blk + j], pi[blk + j + stride]);
+  }
+}
+
+// The recursion of LM:crypto_kem/<set>/vec/controlbits.c (cbrecursion).  Every sort below is on
+// distinct non-negative 32-bit keys, so std::sort reproduces the library's order exactly.
+static void cbrec(uint8_t *out, long long pos, long long step, const uint16_t *pi, int w, size_t n) {
+  auto flip = [&](long long p, uint32_t b) { out[p >> 3] ^= (uint8_t)((b & 1u) << (p & 7)); };
+  if (w == 1) { flip(pos, pi[0]); return; }
+  std::vector<uint32_t> A(n), B(n);
+  for (size_t x = 0; x < n; ++x) A[x] = ((uint32_t)(pi[x] ^ 1u) << 16) | pi[x ^ 1];
+  std::sort(A.begin(), A.end());
+  for (size_t x = 0; x < n; ++x) {
+    const uint32_t px = A[x] & 0xffffu;
+    B[x] = (px << 16) | std::min<uint32_t>(px, (uint32_t)x);
+  }
+  for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 16) | (uint32_t)x;
+  std::sort(A.begin(), A.end());
+  for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 16) + (B[x] >> 16);
+  std::sort(A.begin(), A.end());
+  if (w <= 10) {
+    for (size_t x = 0; x < n; ++x) B[x] = ((A[x] & 0xffffu) << 10) | (B[x] & 0x3ffu);
+    for (int i = 1; i < w - 1; ++i) {
+      for (size_t x = 0; x < n; ++x) A[x] = ((B[x] & ~0x3ffu) << 6) | (uint32_t)x;
+      std::sort(A.begin(), A.end());
+      for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 20) | B[x];
+      std::sort(A.begin(), A.end());
+      for (size_t x = 0; x < n; ++x)
+        B[x] = std::min<uint32_t>((A[x] & 0xffc00u) | (B[x] & 0x3ffu), A[x] & 0xfffffu);
+    }
+    for (size_t x = 0; x < n; ++x) B[x] &= 0x3ffu;
+  } else {
+    for (size_t x = 0; x < n; ++x) B[x] = (A[x] << 16) | (B[x] & 0xffffu);
+    for (int i = 1; i < w - 1; ++i) {
+      for (size_t x = 0; x < n; ++x) A[x] = (B[x] & ~0xffffu) | (uint32_t)x;
+      std::sort(A.begin(), A.end());
+      for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 16) | (B[x] & 0xffffu);
+      if (i < w - 2) {
+        for (size_t x = 0; x < n; ++x) B[x] = (A[x] & ~0xffffu) | (B[x] >> 16);
+        std::sort(B.begin(), B.end());
+        for (size_t x = 0; x < n; ++x) B[x] = (B[x] << 16) | (A[x] & 0xffffu);
+      }
+      std::sort(A.begin(), A.end());
+      for (size_t x = 0; x < n; ++x) B[x] = std::min<uint32_t>(B[x], (B[x] & ~0xffffu) | (A[x] & 0xffffu));
+    }
+    for (size_t x = 0; x < n; ++x) B[x] &= 0xffffu;
+  }
+  for (size_t x = 0; x < n; ++x) A[x] = ((uint32_t)pi[x] << 16) + (uint32_t)x;
+  std::sort(A.begin(), A.end());
+  for (size_t j = 0; j < n / 2; ++j) {
+    const size_t z = 2 * j;
+    const uint32_t fj = B[z] & 1u, Fz = (uint32_t)z + fj;
+    flip(pos, fj);
+    pos += step;
+    B[z] = (A[z] << 16) | Fz;
+    B[z + 1] = (A[z + 1] << 16) | (Fz ^ 1u);
+  }
+  std::sort(B.begin(), B.end());
+  pos += (2 * w - 3) * step * (long long)(n / 2);
+  for (size_t k = 0; k < n / 2; ++k) {
+    const size_t y = 2 * k;
+    const uint32_t lk = B[y] & 1u, Ly = (uint32_t)y + lk;
+    flip(pos, lk);
+    pos += step;
+    A[y] = (Ly << 16) | (B[y] & 0xffffu);
+    A[y + 1] = ((Ly ^ 1u) << 16) | (B[y + 1] & 0xffffu);
+  }
+  std::sort(A.begin(), A.end());
+  pos -= (2 * w - 2) * step * (long long)(n / 2);
+  std::vector<uint16_t> q(n);
+  for (size_t j = 0; j < n / 2; ++j) {
+    q[j] = (uint16_t)((A[2 * j] & 0xffffu) >> 1);
+    q[j + n / 2] = (uint16_t)((A[2 * j + 1] & 0xffffu) >> 1);
+  }
+  cbrec(out, pos, step * 2, q.data(), w - 1, n / 2);
+  cbrec(out, pos + step, step * 2, q.data() + n / 2, w - 1, n / 2);
+}
+
+void controlbits_from_perm(int m, const std::vector<uint16_t> &pi, uint8_t *cb) {
+  const size_t n = size_t(1) << m;
+  std::memset(cb, 0, ((2 * m - 1) * n / 2 + 7) / 8);
+  cbrec(cb, 0, 1, pi.data(), m, n);
+}
+
+// ------------------------------------------------------------------ additive FFT tables
+// Gao-Mateer recursion on the standard basis (1, z, ..., z^(m-1)) so that output index c is the
+// field element c.  Level d: twist by the last basis element b (coefficient i times b^i), Taylor
+// expansion at x^2+x, recurse on span(gamma_i^2 + gamma_i) with gamma_i = basis_i / b; the
+// bottom-up butterfly at level d multiplies by p(c) = sum_i c_i gamma_i.
+FftTables build_fft_tables(int m, int D) {
+  const HostField &F = host_field(m);
+  int KD = 0;
+  while ((1 << KD) < D) ++KD;
+  FftTables t;
+  t.twl.assign(2 * D, 0);
+  t.plg.assign(size_t(1) << m, 0xFFFF);
+  std::vector<uint16_t> basis(m);
+  for (int i = 0; i < m; ++i) basis[i] = (uint16_t)(1u << i);
+  for (int d = 0; d < KD; ++d) {
+    const int Ld = D >> d;
+    const uint16_t b = basis.back();
+    for (int i = 0; i < Ld; ++i) t.twl[2 * D - 2 * Ld + i] = F.lg[F.pw(b, (uint32_t)i)];
+    const uint16_t binv = F.inv(b);
+    std::vector<uint16_t> gam(basis.size() - 1);
+    for (size_t i = 0; i + 1 < basis.size(); ++i) gam[i] = F.mul(basis[i], binv);
+    const size_t half = size_t(1) << gam.size();
+    const size_t off = (size_t(1) << m) - (size_t(1) << (m - d));
+    for (size_t c = 0; c < half; ++c) {
+      uint16_t p = 0;
+      for (size_t i = 0; i < gam.size(); ++i)
+        if ((c >> i) & 1) p ^= gam[i];
+      t.plg[off + c] = F.logv(p);
+    }
+    for (size_t i = 0; i < gam.size(); ++i) gam[i] = (uint16_t)(F.mul(gam[i], gam[i]) ^ gam[i]);
+    basis = gam;
+  }
+  return t;
+}
+
+// ------------------------------------------------------------------ per-key precomputation
+template <int SET>
+static void build_dec_key_t(const uint8_t *sk, DecKeyHost &o) {
+  using P = Par<SET>;
+  const HostField &F = host_field(P::M);
+  std::vector<uint16_t> pi;
+  perm_from_controlbits(P::M, sk + 40 + P::IRR_BYTES, pi);
+  std::vector<uint16_t> g(P::T + 1);
+  for (int i = 0; i < P::T; ++i) g[i] = (uint16_t)((sk[40 + 2 * i] | (sk[41 + 2 * i] << 8)) & ((1 << P::M) - 1));
+  g[P::T] = 1;
+  o.alpha.resize(P::N);
+  o.la.resize(P::N);
+  o.lu.resize(P::N);
+  std::vector<uint16_t> u(P::N);
+  for (int i = 0; i < P::N; ++i) {
+    const uint16_t a = bitrev(pi[i], P::M);
+    uint16_t v = g[P::T];
+    for (int k = P::T - 1; k >= 0; --k) v = (uint16_t)(F.mul(v, a) ^ g[k]);
+    u[i] = F.inv(F.mul(v, v));
+    o.alpha[i] = a;
+    o.la[i] = F.logv(a);
+    o.lu[i] = F.logv(u[i]);
+  }
+  // front-end matrix: row b*TT + j, column i (i < KBITS): bit b of alpha_i^j u_i
+  o.F.assign((size_t)P::R * P::KWP, 0);
+  for (int i = 0; i < P::KBITS; ++i) {
+    uint16_t v = u[i];
+    for (int j = 0; j < 2 * P::T; ++j) {
+      for (int b = 0; b < P::M; ++b)
+        if ((v >> b) & 1) o.F[(size_t)(b * P::TT + j) * P::KWP + i / 32] |= 1u << (i % 32);
+      v = F.mul(v, o.alpha[i]);
+    }
+  }
+  o.rej.assign(sk + 40 + P::IRR_BYTES + P::COND_BYTES, sk + 40 + P::IRR_BYTES + P::COND_BYTES + P::N / 8);
+}
+
+void build_dec_key(int set, const uint8_t *sk, DecKeyHost &out) {
+  switch (set) {
+    case S348864: return build_dec_key_t<S348864>(sk, out);
+    case S460896: return build_dec_key_t<S460896>(sk, out);
+    case S6688128: return build_dec_key_t<S6688128>(sk, out);
+    case S6960119: return build_dec_key_t<S6960119>(sk, out);
+    case S8192128: return build_dec_key_t<S8192128>(sk, out);
+  }
+  throw std::invalid_argument("set");
+}
+
+template <int SET>
+static void build_enc_cols_t(const uint8_t *pk, std::vector<uint32_t> &cols) {
+  using P = Par<SET>;
+  cols.assign((size_t)P::N * P::CWP, 0);
+  for (int p = 0; p < P::MT; ++p) cols[(size_t)p * P::CWP + p / 32] = 1u << (p % 32);
+  for (int r = 0; r < P::MT; ++r) {
+    const uint8_t *row = pk + (size_t)r * P::PK_ROW_BYTES;
+    for (int j = 0; j < P::K; ++j)
+      if ((row[j >> 3] >> (j & 7)) & 1) cols[(size_t)(P::MT + j) * P::CWP + r / 32] |= 1u << (r % 32);
+  }
+}
+
+void build_enc_cols(int set, const uint8_t *pk, std::vector<uint32_t> &cols) {
+  switch (set) {
+    case S348864: return build_enc_cols_t<S348864>(pk, cols);
+    case S460896: return build_enc_cols_t<S460896>(pk, cols);
+    case S6688128: return build_enc_cols_t<S6688128>(pk, cols);
+    case S6960119: return build_enc_cols_t<S6960119>(pk, cols);
+    case S8192128: return build_enc_cols_t<S8192128>(pk, cols);
+  }
+  throw std::invalid_argument("set");
+}
+
+// LM:crypto_kem/6960119/vec/kem_enc.c check_pk_padding
+bool pk_padding_bad(int set, const uint8_t *pk) {
+  if (set != S6960119) return false;
+  using P = Par<S6960119>;
+  uint8_t b = 0;
+  for (int i = 0; i < P::MT; ++i) b |= pk[(size_t)i * P::PK_ROW_BYTES + P::PK_ROW_BYTES - 1];
+  return (b >> (P::K % 8)) != 0;
+}
+
+// ------------------------------------------------------------------ SHAKE256 (host)
+static inline uint64_t rol(uint64_t x, int r) { return r ? (x << r) | (x >> (64 - r)) : x; }
+static void keccakf(uint64_t s[25]) {
+  static const uint64_t RC[24] = {
+      0x0000000000000001ULL, 0x0000000000008082ULL, 0x800000000000808aULL, 0x8000000080008000ULL,
+      0x000000000000808bULL, 0x0000000080000001ULL, 0x8000000080008081ULL, 0x8000000000008009ULL,
+      0x000000000000008aULL, 0x0000000000000088ULL, 0x0000000080008009ULL, 0x000000008000000aULL,
+      0x000000008000808bULL, 0x800000000000008bULL, 0x8000000000008089ULL, 0x8000000000008003ULL,
+      0x8000000000008002ULL, 0x8000000000000080ULL, 0x000000000000800aULL, 0x800000008000000aULL,
+      0x8000000080008081ULL, 0x8000000000008080ULL, 0x0000000080000001ULL, 0x8000000080008008ULL};
+  static const int rot[25] = {0, 1, 62, 28, 27, 36, 44, 6, 55, 20, 3, 10, 43, 25, 39, 41, 45, 15, 21, 8, 18, 2, 61, 56, 14};
+  for (int r = 0; r < 24; ++r) {
+    uint64_t c[5], b[25];
+    for (int x = 0; x < 5; ++x) c[x] = s[x] ^ s[x + 5] ^ s[x + 10] ^ s[x + 15] ^ s[x + 20];
+    for (int x = 0; x < 5; ++x) {
+      const uint64_t dx = c[(x + 4) % 5] ^ rol(c[(x + 1) % 5], 1);
+      for (int y = 0; y < 5; ++y) s[x + 5 * y] ^= dx;
+    }
+    for (int x = 0; x < 5; ++x)
+      for (int y = 0; y < 5; ++y) b[y + 5 * ((2 * x + 3 * y) % 5)] = rol(s[x + 5 * y], rot[x + 5 * y]);
+    for (int y = 0; y < 5; ++y)
+      for (int x = 0; x < 5; ++x) s[x + 5 * y] = b[x + 5 * y] ^ (~b[(x + 1) % 5 + 5 * y] & b[(x + 2) % 5 + 5 * y]);
+    s[0] ^= RC[r];
+  }
+}
+
+void shake256(uint8_t *out, long long outlen, const uint8_t *in, long long inlen) {
+  uint64_t s[25] = {0};
+  uint8_t blk[136];
+  auto absorb = [&](const uint8_t *p) {
+    for (int i = 0; i < 17; ++i) {
+      uint64_t w = 0;
+      for (int b = 0; b < 8; ++b) w |= (uint64_t)p[8 * i + b] << (8 * b);
+      s[i] ^= w;
+    }
+    keccakf(s);
+  };
+  while (inlen >= 136) { absorb(in); in += 136; inlen -= 136; }
+  std::memset(blk, 0, sizeof blk);
+  std::memcpy(blk, in, (size_t)inlen);
+  blk[inlen] ^= 0x1F;
+  blk[135] ^= 0x80;
+  for (int i = 0; i < 17; ++i) {
+    uint64_t w = 0;
+    for (int b = 0; b < 8; ++b) w |= (uint64_t)blk[8 * i + b] << (8 * b);
+    s[i] ^= w;
+  }
+  while (outlen > 0) {
+    keccakf(s);
+    const long long take = outlen < 136 ? outlen : 136;
+    for (long long i = 0; i < take; ++i) out[i] = (uint8_t)(s[i / 8] >> (8 * (i % 8)));
+    out += take;
+    outlen -= take;
+  }
+}
+
+// ------------------------------------------------------------------ key generation
+// GF((2^m)^t) multiplication modulo the set's irreducible F(y) (LM:crypto_kem/<set>/vec/gf_2mt_mul.c)
+template <int SET>
+static void ext_mul(const HostField &F, const uint16_t *a, const uint16_t *b, uint16_t *out) {
+  using P = Par<SET>;
+  constexpr int T = P::T;
+  uint16_t prod[2 * T - 1] = {0};
+  for (int i = 0; i < T; ++i)
+    if (a[i])
+      for (int j = 0; j < T; ++j) prod[i + j] ^= F.mul(a[i], b[j]);
+  for (int i = 2 * T - 2; i >= T; --i) {
+    const uint16_t c = prod[i];
+    if constexpr (SET == S348864) { prod[i - T + 3] ^= c; prod[i - T + 1] ^= c; prod[i - T] ^= F.mul(c, 2); }
+    else if constexpr (SET == S460896) { prod[i - T + 10] ^= c; prod[i - T + 9] ^= c; prod[i - T + 6] ^= c; prod[i - T] ^= c; }
+    else if constexpr (SET == S6960119) { prod[i - T + 8] ^= c; prod[i - T] ^= c; }
+    else { prod[i - T + 7] ^= c; prod[i - T + 2] ^= c; prod[i - T + 1] ^= c; prod[i - T] ^= c; }
+  }
+  std::memcpy(out, prod, sizeof(uint16_t) * T);
+}
+
+// Goppa polynomial = minimal polynomial of f over GF(2^m) (LM:.../sk_gen.c genpoly_gen);
+// false when 1, f, ..., f^(t-1) are linearly dependent.
+template <int SET>
+static bool min_poly(const HostField &F, const uint16_t *f, uint16_t *g) {
+  using P = Par<SET>;
+  constexpr int T = P::T;
+  std::vector<uint16_t> cols((size_t)(T + 1) * T, 0);  // column j = coefficients of f^j
+  cols[0] = 1;
+  std::memcpy(&cols[T], f, sizeof(uint16_t) * T);
+  for (int j = 2; j <= T; ++j) ext_mul<SET>(F, &cols[(size_t)(j - 1) * T], f, &cols[(size_t)j * T]);
+  // Gauss-Jordan on rows = coefficient index, columns 0..T-1 = powers, column T = rhs
+  auto at = [&](int row, int col) -> uint16_t & { return cols[(size_t)col * T + row]; };
+  for (int c = 0; c < T; ++c) {
+    int piv = c;
+    while (piv < T && !at(piv, c)) ++piv;
+    if (piv == T) return false;
+    if (piv != c)
+      for (int j = 0; j <= T; ++j) std::swap(at(piv, j), at(c, j));
+    const uint16_t inv = F.inv(at(c, c));
+    for (int j = c; j <= T; ++j) at(c, j) = F.mul(at(c, j), inv);
+    for (int r = 0; r < T; ++r) {
+      if (r == c || !at(r, c)) continue;
+      const uint16_t fac = at(r, c);
+      for (int j = c; j <= T; ++j) at(r, j) ^= F.mul(fac, at(c, j));
+    }
+  }
+  for (int i = 0; i < T; ++i) g[i] = at(i, T);
+  return true;
+}
+
+// Systematic form of the parity-check matrix; false when the left mt x mt block is singular.
+template <int SET>
+static bool systematic_pk(const HostField &F, const uint16_t *g, const uint16_t *alpha, uint8_t *pk) {
+  using P = Par<SET>;
+  constexpr int W = ceil_div(P::N, 64);
+  std::vector<uint64_t> H((size_t)P::MT * W, 0);
+  for (int j = 0; j < P::N; ++j) {
+    uint16_t v = g[P::T];
+    for (int k = P::T - 1; k >= 0; --k) v = (uint16_t)(F.mul(v, alpha[j]) ^ g[k]);
+    v = F.inv(v);
+    for (int i = 0; i < P::T; ++i) {
+      for (int b = 0; b < P::M; ++b)
+        if ((v >> b) & 1) H[(size_t)(i * P::M + b) * W + j / 64] |= 1ull << (j % 64);
+      v = F.mul(v, alpha[j]);
+    }
+  }
+  for (int c = 0; c < P::MT; ++c) {
+    const int cw = c / 64;
+    const uint64_t cm = 1ull << (c % 64);
+    int piv = c;
+    while (piv < P::MT && !(H[(size_t)piv * W + cw] & cm)) ++piv;
+    if (piv == P::MT) return false;
+    uint64_t *rc = &H[(size_t)c * W];
+    if (piv != c) std::swap_ranges(rc + cw, rc + W, &H[(size_t)piv * W + cw]);
+    for (int r = 0; r < P::MT; ++r) {
+      uint64_t *rr = &H[(size_t)r * W];
+      if (r != c && (rr[cw] & cm))
+        for (int w = cw; w < W; ++w) rr[w] ^= rc[w];
+    }
+  }
+  std::memset(pk, 0, (size_t)P::PK_BYTES);
+  for (int r = 0; r < P::MT; ++r)
+    for (int j = 0; j < P::K; ++j) {
+      const int col = P::MT + j;
+      if ((H[(size_t)r * W + col / 64] >> (col % 64)) & 1) pk[(size_t)r * P::PK_ROW_BYTES + j / 8] |= (uint8_t)(1u << (j % 8));
+    }
+  return true;
+}
+
+// LM:crypto_kem/<set>/vec/kem_keypair.c: seed expansion by SHAKE256, Goppa polynomial, random
+// support by sorting 32-bit keys, systematic public key, Benes control bits, rejection string.
+template <int SET>
+static void keypair_t(uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx) {
+  using P = Par<SET>;
+  const HostField &F = host_field(P::M);
+  constexpr int NQ = 1 << P::M;
+  uint8_t seed[33];
+  seed[0] = 64;
+  rnd(seed + 1, 32, ctx);
+  const size_t rlen = P::N / 8 + 4 * (size_t)NQ + 2 * P::T + 32;
+  std::vector<uint8_t> r(rlen);
+  std::vector<uint16_t> f(P::T), g(P::T + 1), alpha(P::N), pi(NQ);
+  std::vector<std::pair<uint32_t, uint16_t>> keys(NQ);
+  for (;;) {
+    shake256(r.data(), (long long)rlen, seed, 33);
+    std::memcpy(sk, seed + 1, 32);
+    std::memcpy(seed + 1, &r[rlen - 32], 32);
+    const uint8_t *fp = &r[rlen - 32 - 2 * P::T];
+    for (int i = 0; i < P::T; ++i) f[i] = (uint16_t)((fp[2 * i] | (fp[2 * i + 1] << 8)) & ((1 << P::M) - 1));
+    if (!min_poly<SET>(F, f.data(), g.data())) continue;
+    g[P::T] = 1;
+    const uint8_t *pp = &r[P::N / 8];
+    for (int i = 0; i < NQ; ++i)
+      keys[i] = {(uint32_t)pp[4 * i] | ((uint32_t)pp[4 * i + 1] << 8) | ((uint32_t)pp[4 * i + 2] << 16) |
+                     ((uint32_t)pp[4 * i + 3] << 24),
+                 (uint16_t)i};
+    std::sort(keys.begin(), keys.end());
+    bool dup = false;
+    for (int i = 1; i < NQ && !dup; ++i) dup = keys[i - 1].first == keys[i].first;
+    if (dup) continue;
+    for (int i = 0; i < NQ; ++i) pi[i] = keys[i].second;
+    for (int i = 0; i < P::N; ++i) alpha[i] = bitrev(pi[i], P::M);
+    if (!systematic_pk<SET>(F, g.data(), alpha.data(), pk)) continue;
+    uint8_t *skp = sk + 40;
+    for (int i = 0; i < P::T; ++i) { skp[2 * i] = (uint8_t)g[i]; skp[2 * i + 1] = (uint8_t)(g[i] >> 8); }
+    controlbits_from_perm(P::M, pi, skp + P::IRR_BYTES);
+    std::memcpy(skp + P::IRR_BYTES + P::COND_BYTES, r.data(), P::N / 8);
+    const uint8_t piv[8] = {0xFF, 0xFF, 0xFF, 0xFF, 0, 0, 0, 0};
+    std::memcpy(sk + 32, piv, 8);
+    return;
+  }
+}
+
+void host_keypair(int set, uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx) {
+  switch (set) {
+    case S348864: return keypair_t<S348864>(pk, sk, rnd, ctx);
+    case S460896: return keypair_t<S460896>(pk, sk, rnd, ctx);
+    case S6688128: return keypair_t<S6688128>(pk, sk, rnd, ctx);
+    case S6960119: return keypair_t<S6960119>(pk, sk, rnd, ctx);
+    case S8192128: return keypair_t<S8192128>(pk, sk, rnd, ctx);
+  }
+  throw std::invalid_argument("set");
+}
+
+}  // namespace tmce

@@ -1,0 +1,54 @@
+// host_math.h -- host-side GF(2^m) arithmetic and per-key / per-field precomputation for the
+// device engine, plus the host key generator behind the crypto_kem keypair entry points.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace tmce {
+
+struct HostField {
+  int m = 0;
+  uint32_t q = 0;
+  std::vector<uint16_t> lg, ex;  // ex has 2q entries (no reduction needed for a sum of two logs)
+  explicit HostField(int m_);
+  uint16_t mul(uint16_t a, uint16_t b) const { return (a && b) ? ex[lg[a] + lg[b]] : 0; }
+  uint16_t inv(uint16_t a) const { return a ? ex[(q - lg[a]) % q] : 0; }
+  uint16_t pw(uint16_t a, uint32_t e) const;
+  uint16_t logv(uint16_t a) const { return a ? lg[a] : 0xFFFF; }
+};
+const HostField &host_field(int m);
+
+uint16_t bitrev(uint16_t a, int m);
+
+// Permutation of 2^m elements encoded by Benes control bits laid out as in libmceliece's
+// secret key (2m-1 layers of 2^(m-1) bits; strides 1,2,..,2^(m-1),..,2,1).
+void perm_from_controlbits(int m, const uint8_t *cb, std::vector<uint16_t> &pi);
+// Control bits of a permutation (Nassimi-Sahni recursion, libmceliece controlbits.c).
+void controlbits_from_perm(int m, const std::vector<uint16_t> &pi, uint8_t *cb);
+
+// Additive-FFT tables for evaluating a D-coefficient polynomial at all 2^m points.
+struct FftTables {
+  std::vector<uint16_t> twl;  // 2D entries
+  std::vector<uint16_t> plg;  // 2^m entries
+};
+FftTables build_fft_tables(int m, int D);
+
+// Per-key data of decapsulation, derived from sk.
+struct DecKeyHost {
+  std::vector<uint16_t> alpha, la, lu;
+  std::vector<uint32_t> F;   // R x KWP words
+  std::vector<uint8_t> rej;  // N/8 bytes
+};
+void build_dec_key(int set, const uint8_t *sk, DecKeyHost &out);
+
+// Per-key data of encapsulation, derived from pk: column-major [I | T], N x CWP words.
+void build_enc_cols(int set, const uint8_t *pk, std::vector<uint32_t> &cols);
+bool pk_padding_bad(int set, const uint8_t *pk);
+
+// Host key generation with libmceliece's keypair semantics; rand(out,len) supplies randomness.
+typedef void (*rand_fn)(uint8_t *out, long long len, void *ctx);
+void host_keypair(int set, uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx);
+
+void shake256(uint8_t *out, long long outlen, const uint8_t *in, long long inlen);
+
+}  // namespace tmce

@@ -1,0 +1,244 @@
+// kem_enc.hip -- batched Classic McEliece encapsulation on gfx950.
+//
+// Engine form (two launches):
+//   k_gen_e    one op per lane; seeded fixed-weight sampling (SHAKE256 chain, range filter,
+//              collision rejection) with a device-wide work queue so lanes whose attempt is
+//              rejected retry while finished lanes take new ops (no wave-wide straggling).
+//              The bits of e are set with atomicOr straight into the op's session-key message
+//              row; the returned old word detects a repeated index (libmceliece sorts the t
+//              indices and compares neighbours -- the same predicate).  The support is kept in
+//              draw order: only its set matters downstream.
+//   k_enc_finish one op per lane; c = XOR of t columns of [I | T] (sparse column gather from the
+//              device-resident, column-major public key; L2 resident), then K = SHAKE256(1||e||c).
+//
+// Semantics: LM:crypto_kem/<set>/vec/encrypt.c (gen_e :41-92, syndrome :94-132) and kem_enc.c,
+// with randombytes() replaced by the per-op seeded chain of the reference harness
+// (mceliece-gpu-params/ref/348864.cuh:3434-3486): state = 0x40||seed, draws = SHAKE256(state),
+// next state = first 32 bytes of the draws.
+#include "kem_kernels.h"
+
+namespace tmce {
+
+constexpr int GEN_WG = 256;
+
+// e occupies bits 8..8+N-1 of the message row (byte 0 is the 0x01 prefix of 1||e||c).
+template <int SET>
+__global__ __launch_bounds__(GEN_WG) void k_gen_e(const uint8_t *__restrict__ seeds, uint16_t *__restrict__ ind_out,
+                                                  uint32_t *__restrict__ msg, uint32_t *__restrict__ attempts,
+                                                  int nops, unsigned *__restrict__ queue) {
+  using P = Par<SET>;
+  constexpr uint32_t GFMASK = (1u << P::M) - 1u;
+  constexpr int EW2 = ceil_div(P::N + 8, 64);  // 8-byte words holding the prefix byte and e
+  const int lane = lane_id();
+
+  int op = -1;
+  bool done = false;
+  uint32_t st[8];  // the 32-byte chained rng state
+  uint32_t tries = 0;
+
+  for (;;) {
+    // ---- refill idle lanes from the device-wide queue
+    const bool need = (op < 0) && !done;
+    const uint64_t needmask = __ballot(need);
+    if (needmask) {
+      const int leader = __ffsll((long long)needmask) - 1;
+      unsigned base = 0;
+      if (lane == leader) base = atomicAdd(queue, (unsigned)__popcll(needmask));
+      base = (unsigned)__shfl((int)base, leader);
+      if (need) {
+        const unsigned r = base + mbcnt64(needmask);
+        if (r < (unsigned)nops) {
+          op = (int)r;
+          tries = 0;
+          const uint32_t *sp = reinterpret_cast<const uint32_t *>(seeds + 32 * (size_t)op);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) st[i] = sp[i];
+        } else {
+          done = true;
+        }
+      }
+    }
+    if (!__any(op >= 0)) break;
+    if (op < 0) continue;
+
+    // ---- one attempt: draws = SHAKE256(0x40 || st, DRAW_BYTES)
+    ++tries;
+    uint32_t *mrow = msg + (size_t)op * P::MSG_WORDS;
+    uint64_t *mrow2 = reinterpret_cast<uint64_t *>(mrow);
+    uint16_t *io = ind_out + (size_t)op * P::IND_STRIDE;
+    for (int i = 0; i < EW2; ++i) mrow2[i] = 0;
+    __threadfence_block();  // the clears complete before this lane's atomics on the same words
+    uint64_t A[25];
+#pragma unroll
+    for (int i = 0; i < 25; ++i) A[i] = 0;
+    {
+      const uint64_t s0 = st[0] | ((uint64_t)st[1] << 32), s1 = st[2] | ((uint64_t)st[3] << 32);
+      const uint64_t s2 = st[4] | ((uint64_t)st[5] << 32), s3 = st[6] | ((uint64_t)st[7] << 32);
+      A[0] = 0x40ull | (s0 << 8);
+      A[1] = (s0 >> 56) | (s1 << 8);
+      A[2] = (s1 >> 56) | (s2 << 8);
+      A[3] = (s2 >> 56) | (s3 << 8);
+      A[4] = (s3 >> 56) | (0x1Full << 8);
+      A[16] = 0x80ull << 56;
+    }
+    uint32_t nst[8];
+    int count = 0;
+    uint32_t coll = 0;
+#pragma unroll
+    for (int blk = 0; blk < P::DRAW_BLOCKS; ++blk) {
+      keccak_f1600(A);
+      if (blk == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) { nst[2 * i] = (uint32_t)A[i]; nst[2 * i + 1] = (uint32_t)(A[i] >> 32); }
+      }
+#pragma unroll
+      for (int w = 0; w < 17; ++w)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int idx = blk * 68 + w * 4 + q;
+          if (idx < P::NDRAW) {
+            const uint32_t v = (uint32_t)(A[w] >> (16 * q)) & GFMASK;
+            if (v < (uint32_t)P::N && count < P::T) {
+              io[count] = (uint16_t)v;
+              const uint32_t pos = v + 8u, bit = 1u << (pos & 31u);
+              coll |= atomicOr(mrow + (pos >> 5), bit) & bit;
+              ++count;
+            }
+          }
+        }
+    }
+    if (count == P::T && coll == 0) {
+      atomicOr(mrow, 0x01u);
+      if (attempts) attempts[op] = tries;
+      op = -1;
+    } else if (tries >= (uint32_t)P::MAX_ATTEMPTS) {
+      io[0] = 0xFFFFu;  // abandoned (never expected): k_enc_finish reports status -2
+      if (attempts) attempts[op] = tries;
+      op = -1;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) st[i] = nst[i];
+    }
+  }
+}
+
+template <int SET>
+__global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__ ind, const uint4 *__restrict__ cols,
+                                                    uint32_t *__restrict__ msg, uint8_t *__restrict__ ct,
+                                                    uint8_t *__restrict__ key, int *__restrict__ status, int nops) {
+  using P = Par<SET>;
+  constexpr int C4 = P::CWP / 4;
+  const int op = blockIdx.x * blockDim.x + threadIdx.x;
+  if (op >= nops) return;
+
+  // ---- c = sum of columns over the support of e
+  uint4 c[C4];
+#pragma unroll
+  for (int q = 0; q < C4; ++q) c[q] = make_uint4(0, 0, 0, 0);
+  const uint32_t *ip = reinterpret_cast<const uint32_t *>(ind + (size_t)op * P::IND_STRIDE);
+  if ((ip[0] & 0xFFFFu) == 0xFFFFu) {  // sampler gave up on this op
+    for (int h = 0; h < P::SYND_BYTES; ++h) ct[(size_t)op * P::SYND_BYTES + h] = 0;
+    for (int h = 0; h < 32; ++h) key[32 * (size_t)op + h] = 0;
+    if (status) status[op] = -2;
+    return;
+  }
+#pragma unroll 4
+  for (int i = 0; i < P::T; i += 2) {
+    const uint32_t pr = ip[i / 2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (i + h < P::T) {
+        const uint32_t p = h ? (pr >> 16) : (pr & 0xFFFFu);
+        const uint4 *col = cols + (size_t)p * C4;
+#pragma unroll
+        for (int q = 0; q < C4; ++q) {
+          const uint4 x = col[q];
+          c[q].x ^= x.x; c[q].y ^= x.y; c[q].z ^= x.z; c[q].w ^= x.w;
+        }
+      }
+    }
+  }
+  uint32_t cw[P::CWP];
+#pragma unroll
+  for (int q = 0; q < C4; ++q) { cw[4 * q] = c[q].x; cw[4 * q + 1] = c[q].y; cw[4 * q + 2] = c[q].z; cw[4 * q + 3] = c[q].w; }
+
+  // ---- ciphertext bytes (SYND_BYTES is even for every set)
+  uint16_t *ctp = reinterpret_cast<uint16_t *>(ct + (size_t)op * P::SYND_BYTES);
+#pragma unroll
+  for (int h = 0; h < P::SYND_BYTES / 2; ++h) ctp[h] = (uint16_t)(cw[h / 2] >> (16 * (h & 1)));
+
+  // ---- finish the message: c at byte 1+N/8, then SHAKE padding
+  uint32_t *mrow = msg + (size_t)op * P::MSG_WORDS;
+  constexpr int OFF = 1 + P::N / 8;
+  constexpr int KB = OFF / 4, RS = 8 * (OFF % 4);
+  constexpr int PADW = P::MSG_LEN / 4, PADS = 8 * (P::MSG_LEN % 4);
+  const uint32_t first = mrow[KB];
+#pragma unroll
+  for (int k = KB; k < P::MSG_WORDS; ++k) {
+    const int j = k - KB;
+    uint32_t v = 0;
+    if (j < P::CW) v |= cw[j] << RS;
+    if (RS != 0 && j >= 1 && j - 1 < P::CW) v |= cw[j - 1] >> ((32 - RS) & 31);
+    if (k == KB) v |= first;
+    if (k == PADW) v |= 0x1Fu << PADS;
+    if (k == P::MSG_WORDS - 1) v |= 0x80u << 24;
+    mrow[k] = v;
+  }
+  uint64_t out[4];
+  shake256_blocks_32(reinterpret_cast<const uint64_t *>(mrow), P::MSG_BLOCKS, out);
+  uint64_t *kp = reinterpret_cast<uint64_t *>(key + 32 * (size_t)op);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) kp[i] = out[i];
+  if (status) status[op] = 0;
+}
+
+__global__ void k_fill_u32(unsigned *p, unsigned v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u8(uint8_t *p, uint8_t v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_i32(int *p, int v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s) {
+  if (n) k_fill_u8<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+hipError_t launch_fill_i32(int *p, int v, size_t n, hipStream_t s) {
+  if (n) k_fill_i32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+
+template <int SET>
+hipError_t launch_enc(const EncArgs &a, hipStream_t s) {
+  using P = Par<SET>;
+  if (a.nops <= 0) return hipSuccess;
+  k_fill_u32<<<1, 64, 0, s>>>(a.queue, 0u, 1);
+  // persistent grid: enough resident waves to fill the chip, lanes pull ops from the queue
+  int grid = a.gen_blocks > 0 ? a.gen_blocks : 1024;
+  const int maxgrid = ceil_div(a.nops, GEN_WG);
+  if (grid > maxgrid) grid = maxgrid;
+  TMCE_TIMED("gen_e", s, k_gen_e<SET><<<grid, GEN_WG, 0, s>>>(a.seeds, a.ind, a.msg, a.attempts, a.nops, a.queue));
+  TMCE_TIMED("enc_finish", s,
+             k_enc_finish<SET><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.ind, reinterpret_cast<const uint4 *>(a.cols),
+                                                                     a.msg, a.ct, a.key, a.status, a.nops));
+  return hipGetLastError();
+}
+
+hipError_t launch_enc_set(int set, const EncArgs &a, hipStream_t s) {
+  switch (set) {
+    case S348864: return launch_enc<S348864>(a, s);
+    case S460896: return launch_enc<S460896>(a, s);
+    case S6688128: return launch_enc<S6688128>(a, s);
+    case S6960119: return launch_enc<S6960119>(a, s);
+    case S8192128: return launch_enc<S8192128>(a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace tmce

@@ -1,0 +1,70 @@
+// kem_kernels.h -- launch interface between the host engine (api.cpp) and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "params.h"
+#include "device_common.h"
+
+namespace tmce {
+
+// Device buffers of one encapsulation batch (all pointers are device pointers).
+struct EncArgs {
+  const uint8_t *seeds;  // [n][32]
+  const uint32_t *cols;  // key: [N][CWP] column-major [I | T]
+  uint8_t *ct;           // [n][SYND_BYTES]
+  uint8_t *key;          // [n][32]
+  int *status;           // [n] 0, or -2 if the sampler gave up (optional)
+  uint16_t *ind;         // scratch [n][IND_STRIDE] support of e
+  uint32_t *msg;         // scratch [n][MSG_WORDS] session-key message
+  uint32_t *attempts;    // optional [n] gen_e attempts
+  unsigned *queue;       // scratch work-queue counter
+  int nops;
+  int gen_blocks;        // persistent grid size of the sampler (0 = default)
+};
+
+// Device-resident tables of one GF(2^m) field and of the additive FFT used by decapsulation.
+struct FieldTables {
+  const uint16_t *lg;    // [2^m]
+  const uint16_t *ex;    // [2^m - 1]
+  const uint16_t *twl;   // FFT twist logs, level d at offset 2D - 2(D >> d)
+  const uint16_t *plg;   // FFT combine-multiplier logs (0xFFFF = zero), level d at 2^m - 2^(m-d)
+  const uint16_t *powt;  // a^t for every field element a (monic x^t term), or null
+};
+
+// Device buffers of one decapsulation batch.
+struct DecArgs {
+  const uint8_t *ct;     // [n][SYND_BYTES]
+  const uint32_t *F;     // key: [R][KWP] front-end matrix (plane-major rows)
+  const uint16_t *alpha; // key: [N] support
+  const uint16_t *la;    // key: [N] log alpha (0xFFFF for alpha = 0)
+  const uint16_t *lu;    // key: [N] log 1/g(alpha)^2 (0xFFFF for 0)
+  const uint8_t *rej;    // key: [N/8] implicit-rejection string s
+  FieldTables ft;
+  uint8_t *key;          // [n][32]
+  int *status;           // [n] 0, or -1 for a ciphertext with nonzero padding bits (6960119)
+  uint32_t *S;           // scratch [n][RW] syndrome bit planes
+  uint32_t *msg;         // scratch [n][MSG_WORDS]
+  int nops;
+  int decode_blocks;     // grid of the decoder (0 = default)
+};
+
+hipError_t launch_enc_set(int set, const EncArgs &a, hipStream_t s);
+hipError_t launch_dec_set(int set, const DecArgs &a, hipStream_t s);
+hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
+hipError_t launch_fill_i32(int *p, int v, size_t n, hipStream_t s);
+
+// Optional per-kernel timing: when set, each engine launch is bracketed by events on its stream.
+struct KernelTimer {
+  virtual void begin(const char *name, hipStream_t s) = 0;
+  virtual void end(hipStream_t s) = 0;
+  virtual ~KernelTimer() = default;
+};
+extern KernelTimer *g_ktimer;
+#define TMCE_TIMED(name, stream, ...)                   \
+  do {                                                  \
+    if (::tmce::g_ktimer) ::tmce::g_ktimer->begin(name, stream); \
+    __VA_ARGS__;                                        \
+    if (::tmce::g_ktimer) ::tmce::g_ktimer->end(stream);  \
+  } while (0)
+
+}  // namespace tmce

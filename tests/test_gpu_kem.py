@@ -1,0 +1,130 @@
+"""GPU parity of the batched KEM engine (libtmce.so) against the CPU oracle, all five sets.
+
+Bar: bit-exact ciphertexts and session keys.  Keys and seeds follow the reference harness's
+xorshift64 stream (mceliece-gpu-params/main.cu:75-76,184-188).
+"""
+import numpy as np
+import pytest
+
+from conftest import SETS
+
+pytestmark = pytest.mark.gpu
+
+_KEYS = {}
+
+
+@pytest.fixture(scope="module")
+def tm():
+    import tensormceliece_amd as tm
+    tm.init(0)
+    return tm
+
+
+def keys(orc, s):
+    if s not in _KEYS:
+        kseed, st = orc.xorshift_seeds(1)
+        pk, sk = orc.keypair_seeded(s, kseed[0].tobytes())
+        _KEYS[s] = (pk, sk, st)
+    return _KEYS[s]
+
+
+def seeds_for(orc, s, n):
+    _, _, st = keys(orc, s)
+    return orc.xorshift_seeds(n, st)[0]
+
+
+@pytest.mark.parametrize("s", SETS)
+def test_enc_parity(tm, orc, s):
+    pk, sk, _ = keys(orc, s)
+    seeds = seeds_for(orc, s, 256)
+    print(f"enc parity {s} n=256", flush=True)
+    ct, k, st = tm.PublicKey(s, pk).enc(seeds)
+    ct_ref, k_ref, ret = orc.enc_batch(s, pk, seeds)
+    assert not st.any() and not ret.any()
+    assert np.array_equal(ct, ct_ref)
+    assert np.array_equal(k, k_ref)
+
+
+@pytest.mark.parametrize("s", SETS)
+def test_dec_parity_valid(tm, orc, s):
+    pk, sk, _ = keys(orc, s)
+    seeds = seeds_for(orc, s, 256)
+    ct_ref, k_ref, _ = orc.enc_batch(s, pk, seeds)
+    print(f"dec parity {s} n=256", flush=True)
+    k, st = tm.SecretKey(s, sk).dec(ct_ref)
+    assert not st.any()
+    assert np.array_equal(k, k_ref)
+
+
+@pytest.mark.parametrize("s", SETS)
+def test_dec_parity_invalid(tm, orc, s):
+    """Implicit rejection: random ciphertexts and valid ones with one byte changed."""
+    pk, sk, _ = keys(orc, s)
+    rng = np.random.default_rng(1234)
+    nbytes = orc.params(s).synd_bytes
+    rnd = rng.integers(0, 256, size=(128, nbytes), dtype=np.uint8)
+    ct_ref, _, _ = orc.enc_batch(s, pk, seeds_for(orc, s, 128))
+    tam = ct_ref.copy()
+    idx = rng.integers(0, nbytes, size=128)
+    tam[np.arange(128), idx] += rng.integers(1, 256, size=128).astype(np.uint8)
+    cts = np.concatenate([rnd, tam])
+    print(f"dec invalid {s} n={len(cts)}", flush=True)
+    k, st = tm.SecretKey(s, sk).dec(cts)
+    k_ref, ret = orc.dec_batch(s, sk, cts)
+    assert np.array_equal(st, ret)
+    assert np.array_equal(k, k_ref)
+
+
+def test_6960119_padding(tm, orc):
+    s = "6960119"
+    pk, sk, _ = keys(orc, s)
+    p = orc.params(s)
+    # public key with a nonzero padding bit: enc returns -1 and clears ct and k
+    bad = pk.copy()
+    bad[p.pk_row_bytes - 1] |= 0x80
+    seeds = seeds_for(orc, s, 8)
+    ct, k, st = tm.PublicKey(s, bad).enc(seeds)
+    ct_ref, k_ref, ret = orc.enc_batch(s, bad, seeds)
+    assert (st == -1).all() and (ret == -1).all()
+    assert np.array_equal(ct, ct_ref) and np.array_equal(k, k_ref) and not ct.any() and not k.any()
+    # ciphertext with a nonzero padding bit: dec returns -1 and k = 0xFF..FF
+    ct_ok, _, _ = orc.enc_batch(s, pk, seeds)
+    ct_bad = ct_ok.copy()
+    ct_bad[:, -1] |= 0x10
+    k, st = tm.SecretKey(s, sk).dec(ct_bad)
+    k_ref, ret = orc.dec_batch(s, sk, ct_bad)
+    assert (st == -1).all() and (ret == -1).all()
+    assert np.array_equal(k, k_ref) and (k == 0xFF).all()
+
+
+@pytest.mark.parametrize("s,n", [("348864", 65536), ("460896", 16384), ("6688128", 16384), ("6960119", 16384),
+                                 ("8192128", 65536)])
+def test_roundtrip_large(tm, orc, s, n):
+    """At full batch sizes: dec(enc(seed)) recovers the session key for every op, every e has
+    weight t (checked through a sample against the oracle), and results do not depend on N."""
+    pk, sk, _ = keys(orc, s)
+    rng = np.random.default_rng(7)
+    seeds = rng.integers(0, 256, size=(n, 32), dtype=np.uint8)
+    print(f"roundtrip {s} n={n}", flush=True)
+    ct, k, st = tm.PublicKey(s, pk).enc(seeds)
+    k2, st2 = tm.SecretKey(s, sk).dec(ct)
+    assert not st.any() and not st2.any()
+    assert np.array_equal(k, k2)
+    sample = rng.choice(n, 64, replace=False)
+    ct_ref, k_ref, _ = orc.enc_batch(s, pk, seeds[sample])
+    assert np.array_equal(ct[sample], ct_ref) and np.array_equal(k[sample], k_ref)
+    # batch-size independence: the sample alone gives the same bytes
+    ct_s, k_s, _ = tm.PublicKey(s, pk).enc(seeds[sample])
+    assert np.array_equal(ct_s, ct_ref) and np.array_equal(k_s, k_ref)
+
+
+@pytest.mark.parametrize("s", SETS)
+def test_single_op_api(tm, orc, s):
+    """mceliece_kem_<set>_enc/_dec entry points (libmceliece interface) on one op."""
+    pk, sk, _ = keys(orc, s)
+    r, ct, k = tm.kem_enc(s, pk)
+    assert r == 0
+    r2, k2 = tm.kem_dec(s, ct, sk)
+    assert r2 == 0 and np.array_equal(k, k2)
+    k_ref, _ = orc.dec_batch(s, sk, ct[None, :])
+    assert np.array_equal(k2, k_ref[0])
