@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched Classic McEliece KEM engine on one MI355X.
+
+One step = one batch of N encapsulations (seeded, inputs resident in HBM) followed by one batch of
+N decapsulations of the resulting ciphertexts, for one parameter set (default mceliece348864,
+N = 65536, the reference's largest batch).  Prints ONE JSON line (see the driver contract):
+value = KEM operations per second over the timed steps (N encaps + N decaps per step), with
+encaps/s and decaps/s, per-kernel HIP-event timings, a roofline object for the dominant kernel and
+an all-core CPU baseline (the C oracle, bounded sample) beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def xorshift_seeds(n, state=0x9E3779B97F4A7C15):
+    """Seed bytes as the reference harness draws them (mceliece-gpu-params/main.cu:75-76,184-188)."""
+    out = np.empty(n * 32, np.uint8)
+    x = state
+    M = (1 << 64) - 1
+    for i in range(n * 32):
+        x ^= (x << 13) & M
+        x ^= x >> 7
+        x ^= (x << 17) & M
+        out[i] = x & 0xFF
+    return out.reshape(n, 32), x
+
+
+def algorithmic_bytes(kernel, p, n):
+    """HBM bytes one launch must move at minimum (inputs once, outputs once), per kernel."""
+    ct, e8 = p["ct"], p["n"] // 8
+    msg = ((1 + e8 + ct) // 136 + 1) * 136
+    per_op = {
+        "gen_e": 32 + 2 * p["t"] + msg,           # seed in; support list + message row out
+        "enc_finish": 2 * p["t"] + msg + ct + 32,  # support + message in; ct + key out
+        "dec_front": ct + 4 * p["m"] * ((2 * p["t"] + 31) // 32),
+        "dec_decode": 4 * p["m"] * ((2 * p["t"] + 31) // 32) + ct + msg,
+        "dec_hash": msg + ct + 32 + 4,
+    }[kernel]
+    return per_op * n
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--set", default="348864")
+    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-verify", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(every op is independent: N GPUs would each run this step on N/G ops)", file=sys.stderr)
+        sys.exit(2)
+
+    import tensormceliece_amd as tm
+    tm.init(0)
+    s, n = args.set, args.n
+    p = tm.PARAMS[s]
+
+    kseed, st = xorshift_seeds(1)
+    pk, sk = tm.keypair_seeded(s, kseed[0].tobytes())
+    seeds, _ = xorshift_seeds(n, st)
+    PK, SK = tm.PublicKey(s, pk), tm.SecretKey(s, sk)
+
+    d_seeds = tm.DeviceBuffer(n * 32)
+    d_ct = tm.DeviceBuffer(n * p["ct"])
+    d_k = tm.DeviceBuffer(n * 32)
+    d_st = tm.DeviceBuffer(n * 4)
+    d_k2 = tm.DeviceBuffer(n * 32)
+    d_st2 = tm.DeviceBuffer(n * 4)
+    d_seeds.upload(seeds)
+
+    def enc():
+        tm.enc_device(PK, d_ct, d_k, d_st, d_seeds, n)
+
+    def dec():
+        tm.dec_device(SK, d_k2, d_st2, d_ct, n)
+
+    for _ in range(args.warmup):
+        enc()
+        dec()
+    tm.sync()
+
+    # ---- timed region: K steps of (N encaps + N decaps), device-resident inputs
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        enc()
+        dec()
+    tm.sync()
+    t_step = (time.perf_counter() - t0) / args.steps
+
+    # ---- per-phase and per-kernel timing (HIP events on the engine stream)
+    reps = max(3, args.steps // 2)
+    tm.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        enc()
+    tm.sync()
+    t_enc = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dec()
+    tm.sync()
+    t_dec = (time.perf_counter() - t0) / reps
+    tm.timing_enable(True)
+    for _ in range(reps):
+        enc()
+        dec()
+    tm.sync()
+    kt = tm.timing_read()
+    tm.timing_enable(False)
+    kernels = {k: {"launches": c, "avg_ms": tot / c} for k, (c, tot) in kt.items() if c}
+
+    # ---- correctness of the measured batch
+    verified = None
+    if not args.no_verify:
+        ct = d_ct.download((n, p["ct"]))
+        k1 = d_k.download((n, 32))
+        k2 = d_k2.download((n, 32))
+        st1 = d_st.download(n, np.int32)
+        st2 = d_st2.download(n, np.int32)
+        ok = bool(np.array_equal(k1, k2) and not st1.any() and not st2.any())
+        from oracle import oracle as orc
+        idx = np.random.default_rng(0).choice(n, min(n, 256), replace=False)
+        ct_ref, k_ref, _ = orc.enc_batch(s, pk, seeds[idx])
+        ok = ok and bool(np.array_equal(ct[idx], ct_ref) and np.array_equal(k1[idx], k_ref))
+        verified = ok
+
+    # ---- roofline of the dominant kernel (HBM; algorithmic bytes per launch / launch time)
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
+    abytes = algorithmic_bytes(dom, p, n)
+    achieved = abytes / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "algorithmic_bytes_per_launch": abytes}
+
+    # ---- CPU baseline: the oracle on all host cores, bounded sample of the same workload
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        cores = len(os.sched_getaffinity(0))
+        ne = max(cores, 64)
+        t0 = time.perf_counter()
+        ct_c, k_c, _ = orc.enc_batch(s, pk, seeds[:ne], cores)
+        te = time.perf_counter() - t0
+        # scale the sample to ~cpu_seconds/2 for each of enc and dec
+        ne2 = int(min(n, max(ne, ne * (args.cpu_seconds / 2) / max(te, 1e-3))))
+        t0 = time.perf_counter()
+        ct_c, k_c, _ = orc.enc_batch(s, pk, seeds[:ne2], cores)
+        te = time.perf_counter() - t0
+        nd = max(cores, 64)
+        t0 = time.perf_counter()
+        orc.dec_batch(s, sk, ct_c[:nd], cores)
+        td = time.perf_counter() - t0
+        nd2 = int(min(ne2, max(nd, nd * (args.cpu_seconds / 2) / max(td, 1e-3))))
+        t0 = time.perf_counter()
+        k_d, _ = orc.dec_batch(s, sk, ct_c[:nd2], cores)
+        td = time.perf_counter() - t0
+        assert np.array_equal(k_d, k_c[:nd2])
+        enc_s, dec_s = ne2 / te, nd2 / td
+        cpu = {"value": round(2.0 / (1.0 / enc_s + 1.0 / dec_s), 1), "unit": "KEM ops/s", "cores": cores,
+               "kind": "port", "encaps_per_s": round(enc_s, 1), "decaps_per_s": round(dec_s, 1),
+               "sample": f"oracle C restatement, {ne2} encaps + {nd2} decaps of the same seeded {s} inputs, "
+                         f"{cores} threads"}
+
+    kem_ops_per_s = 2 * n / t_step
+    line = {
+        "metric": "encaps/s & decaps/s at N=65536 vs plain GPU form (bit-exact); serving p50/p99",
+        "value": round(kem_ops_per_s, 1),
+        "unit": "KEM ops/s (encaps+decaps)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_step * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8/u16 GF(2^m)", "data": "synthetic (seeded xorshift64 keypair and encapsulation seeds)",
+        "config": {"workload": f"mceliece{s} KEM: {n} seeded encapsulations + {n} decapsulations per step",
+                   "set": s, "batch": n},
+        "encaps_per_s": round(n / t_enc, 1), "decaps_per_s": round(n / t_dec, 1),
+        "bit_exact": verified,
+        "kernels": kernels,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

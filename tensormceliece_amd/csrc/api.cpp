@@ -96,7 +96,7 @@ struct EventTimer : KernelTimer {
 };
 
 struct SetTables {
-  uint16_t *twl = nullptr, *plg = nullptr, *powt = nullptr;
+  uint16_t *twl = nullptr, *plo = nullptr, *phi = nullptr, *ptlo = nullptr, *pthi = nullptr;
 };
 
 struct Engine {
@@ -144,12 +144,15 @@ int engine_init(int device) {
     const int D = si.t == 64 ? 64 : 128;
     FftTables t = build_fft_tables(si.m, D);
     if (int r = upload(&e->st[s].twl, t.twl)) return r;
-    if (int r = upload(&e->st[s].plg, t.plg)) return r;
-    if (si.t == D) {
+    if (int r = upload(&e->st[s].plo, t.plo)) return r;
+    if (int r = upload(&e->st[s].phi, t.phi)) return r;
+    if (si.t == D) {  // a -> a^t is additive for t = 2^k: a^t = (lane)^t + (64 r)^t
       const HostField &F = host_field(si.m);
-      std::vector<uint16_t> pw(size_t(1) << si.m);
-      for (size_t a = 0; a < pw.size(); ++a) pw[a] = F.pw((uint16_t)a, (uint32_t)si.t);
-      if (int r = upload(&e->st[s].powt, pw)) return r;
+      std::vector<uint16_t> lo(64), hi(size_t(1) << (si.m - 6));
+      for (size_t l = 0; l < 64; ++l) lo[l] = F.pw((uint16_t)l, (uint32_t)si.t);
+      for (size_t r = 0; r < hi.size(); ++r) hi[r] = F.pw((uint16_t)(64 * r), (uint32_t)si.t);
+      if (int r = upload(&e->st[s].ptlo, lo)) return r;
+      if (int r = upload(&e->st[s].pthi, hi)) return r;
     }
   }
   g_engine = e.release();
@@ -251,7 +254,8 @@ int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, si
   a.lu = k->lu;
   a.rej = k->rej;
   const int w = si.m - 12;
-  a.ft = FieldTables{e->lg[w], e->ex[w], e->st[k->set].twl, e->st[k->set].plg, e->st[k->set].powt};
+  const SetTables &tb = e->st[k->set];
+  a.ft = FieldTables{e->lg[w], e->ex[w], tb.twl, tb.plo, tb.phi, tb.ptlo, tb.pthi};
   a.key = d_k;
   a.status = d_status;
   a.S = e->S.as<uint32_t>();

@@ -165,7 +165,11 @@ FftTables build_fft_tables(int m, int D) {
   while ((1 << KD) < D) ++KD;
   FftTables t;
   t.twl.assign(2 * D, 0);
-  t.plg.assign(size_t(1) << m, 0xFFFF);
+  t.plo.assign((size_t)KD * 64, 0);
+  size_t phi_total = 0;
+  for (int d = 0; d < KD; ++d) phi_total += size_t(1) << (m - 7 - d < 0 ? 0 : m - 7 - d);
+  t.phi.assign(phi_total, 0);
+  size_t phi_off = 0;
   std::vector<uint16_t> basis(m);
   for (int i = 0; i < m; ++i) basis[i] = (uint16_t)(1u << i);
   for (int d = 0; d < KD; ++d) {
@@ -175,14 +179,16 @@ FftTables build_fft_tables(int m, int D) {
     const uint16_t binv = F.inv(b);
     std::vector<uint16_t> gam(basis.size() - 1);
     for (size_t i = 0; i + 1 < basis.size(); ++i) gam[i] = F.mul(basis[i], binv);
-    const size_t half = size_t(1) << gam.size();
-    const size_t off = (size_t(1) << m) - (size_t(1) << (m - d));
-    for (size_t c = 0; c < half; ++c) {
+    auto pval = [&](size_t c) {
       uint16_t p = 0;
       for (size_t i = 0; i < gam.size(); ++i)
         if ((c >> i) & 1) p ^= gam[i];
-      t.plg[off + c] = F.logv(p);
-    }
+      return p;
+    };
+    for (size_t l = 0; l < 64; ++l) t.plo[(size_t)d * 64 + l] = pval(l);
+    const size_t nhi = size_t(1) << (m - 7 - d);
+    for (size_t j = 0; j < nhi; ++j) t.phi[phi_off + j] = pval(j << 6);
+    phi_off += nhi;
     for (size_t i = 0; i < gam.size(); ++i) gam[i] = (uint16_t)(F.mul(gam[i], gam[i]) ^ gam[i]);
     basis = gam;
   }

@@ -27,9 +27,12 @@ void perm_from_controlbits(int m, const uint8_t *cb, std::vector<uint16_t> &pi);
 void controlbits_from_perm(int m, const std::vector<uint16_t> &pi, uint8_t *cb);
 
 // Additive-FFT tables for evaluating a D-coefficient polynomial at all 2^m points.
+// The bottom-up butterfly multiplier of level d at point index c is p_d(c) = sum_i c_i gamma_i,
+// linear in the bits of c; with c = 64*j + lane it splits as plo[d][lane] ^ phi[d][j].
 struct FftTables {
-  std::vector<uint16_t> twl;  // 2D entries
-  std::vector<uint16_t> plg;  // 2^m entries
+  std::vector<uint16_t> twl;   // 2D entries: twist logs, level d at 2D - 2(D >> d)
+  std::vector<uint16_t> plo;   // KD x 64 raw multipliers (lane part)
+  std::vector<uint16_t> phi;   // register part, level d at offset PHI_OFF(d) = sum_{d'<d} 2^(m-7-d')
 };
 FftTables build_fft_tables(int m, int D);
 

@@ -4,10 +4,11 @@
 //   k_dec_front   one op per lane; the 2t syndromes s_j = sum_i c_i alpha_i^j / g(alpha_i)^2 as a
 //                 GF(2) matrix-vector product (AND + parity over 32-bit words) with the key's
 //                 front-end matrix staged through LDS (all lanes read the same row: broadcast).
-//   k_dec_decode  one op per wavefront: Berlekamp-Massey with the coefficients spread over the
-//                 64 lanes, Gao-Mateer additive FFT of the locator over all 2^m field elements in
-//                 LDS, root gather at the support, weight check, re-encryption check (syndrome of
-//                 the found error recomputed from the key's log tables), session-key message.
+//   k_dec_decode  four ops per wavefront: Berlekamp-Massey on 16-lane rows (DPP reductions and
+//                 shifts, log-domain tables in LDS), Gao-Mateer additive FFT of the locator over
+//                 all 2^m field elements with the values in registers, root gather at the support,
+//                 weight check, re-encryption check (syndrome of the found error recomputed from the
+//                 key's log tables), session-key message.
 //   k_hash_msg    one op per lane; K = SHAKE256(b || (b ? e : s) || c); 6960119 padding rule.
 //
 // Semantics: LM:crypto_kem/<set>/vec/decrypt.c (decrypt :135-190, weight_check :101-116) and
@@ -20,7 +21,7 @@
 namespace tmce {
 
 constexpr int FRONT_WG = 256;
-constexpr int DEC_WG = 256;
+constexpr int DEC_WG = 512;
 constexpr int DEC_WAVES = DEC_WG / 64;
 
 // ------------------------------------------------------------------ front end
@@ -60,17 +61,34 @@ __global__ __launch_bounds__(FRONT_WG) void k_dec_front(const uint8_t *__restric
   }
 }
 
-// ------------------------------------------------------------------ decoder (one op per wave)
+// ------------------------------------------------------------------ decoder
+// LDS field tables: lg[a] = log a with lg[0] = ZL = 2q (sentinel); ex[i] = g^(i mod q) for
+// i < 2q and 0 for 2q <= i <= 4q, so a*b = ex[lg[a] + lg[b]] with no zero test and no reduction.
 template <int SET>
-struct DecLds {
+struct DecWave {
   using P = Par<SET>;
-  uint16_t s[P::TT];
-  uint16_t f[P::D];
+  uint16_t s[4][P::TT];           // syndromes of the wave's four ops
+  uint16_t f[4][P::D];            // locator coefficients (FFT input), per op
   uint16_t tmp[P::D];
-  uint16_t pos[P::T];
+  uint16_t pos[P::IND_STRIDE];    // error positions of the op being finished
+  uint32_t Z[(1 << P::M) / 32];   // zero flags of sigma over all field elements
   uint32_t ebits[ceil_div(P::N, 64) * 2];
-  uint16_t V[1 << P::M];
 };
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+// row_ror:n -- lane l of a 16-lane row reads lane (l - n) mod 16 of the same row
+constexpr int DPP_ROR1 = 0x121, DPP_ROR2 = 0x122, DPP_ROR4 = 0x124, DPP_ROR8 = 0x128;
+
+__device__ __forceinline__ uint32_t row_xor16(uint32_t v) {
+  v ^= dpp<DPP_ROR8>(v);
+  v ^= dpp<DPP_ROR4>(v);
+  v ^= dpp<DPP_ROR2>(v);
+  v ^= dpp<DPP_ROR1>(v);
+  return v;
+}
 
 template <int SET>
 __device__ __forceinline__ uint32_t msg_byte(int b, bool ok, const uint8_t *ebytes, const uint8_t *rej,
@@ -86,200 +104,268 @@ __device__ __forceinline__ uint32_t msg_byte(int b, bool ok, const uint8_t *ebyt
   return v;
 }
 
+// One wavefront decodes four ops at a time: Berlekamp-Massey runs on the four 16-lane rows in
+// parallel (coefficient i of op `row` lives in lane 16*row + i%16, slot i/16; row reductions and
+// the x-shifts of B and of the syndrome window are DPP row operations), then the wave finishes the
+// ops one by one: additive FFT with the 2^m values in registers (point p = 64*r + lane, so every
+// butterfly pairs two registers of one lane), root flags, weight and re-encryption checks, message.
 template <int SET>
 __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
   using P = Par<SET>;
-  constexpr int M = P::M, T = P::T, Dn = P::D, KD = P::KD;
+  constexpr int M = P::M, T = P::T, Dn = P::D, KD = P::KD, Q = P::Q;
   constexpr int NPTS = 1 << M;
+  constexpr int NR = NPTS / 64;  // registers per lane holding field values (== D)
+  constexpr uint32_t ZL = 2u * Q;
+  constexpr int NSB = ceil_div(T + 1, 16);  // BM coefficient slots per lane
+  static_assert(M - KD == 6 && NR == Dn, "FFT layout assumes 2^(m-KD) = 64 points per constant");
+  constexpr int NPHI = (1 << (M - 6)) - (1 << (M - 6 - KD));  // sum over levels of 2^(m-7-d)
   __shared__ uint16_t lg_s[NPTS];
-  __shared__ uint16_t ex_s[NPTS];
-  __shared__ DecLds<SET> W[DEC_WAVES];
+  __shared__ uint16_t ex_s[4 * Q + 2];
+  __shared__ uint16_t phi_s[NPHI];
+  __shared__ uint16_t pthi_s[NR];
+  __shared__ DecWave<SET> Wv[DEC_WAVES];
 
   const int tid = threadIdx.x;
-  for (int i = tid; i < NPTS; i += DEC_WG) {
-    lg_s[i] = a.ft.lg[i];
-    ex_s[i] = i < P::Q ? a.ft.ex[i] : 0;
-  }
+  for (int i = tid; i < NPTS; i += DEC_WG) lg_s[i] = i ? a.ft.lg[i] : (uint16_t)ZL;
+  for (int i = tid; i < 4 * Q + 2; i += DEC_WG) ex_s[i] = i < 2 * Q ? a.ft.ex[i < Q ? i : i - Q] : 0;
+  for (int i = tid; i < NPHI; i += DEC_WG) phi_s[i] = a.ft.phi[i];
+  for (int i = tid; i < NR; i += DEC_WG) pthi_s[i] = P::TOP_TERM ? a.ft.pthi[i] : 0;
   __syncthreads();
-  const GfLds gf{lg_s, ex_s, (uint32_t)P::Q};
-  const int wave = tid >> 6, l = tid & 63;
-  DecLds<SET> &w = W[wave];
+  const int wave = tid >> 6, lane = tid & 63, row = lane >> 4, l16 = lane & 15;
+  uint32_t plo[KD];
+#pragma unroll
+  for (int d = 0; d < KD; ++d) plo[d] = a.ft.plo[64 * d + lane];
+  const uint32_t ptlo = P::TOP_TERM ? a.ft.ptlo[lane] : 0u;
+  DecWave<SET> &w = Wv[wave];
 
-  for (int op = blockIdx.x * DEC_WAVES + wave; op < a.nops; op += gridDim.x * DEC_WAVES) {
-    // ---- 1. syndromes from the bit planes
-    const uint32_t *Srow = a.S + (size_t)op * P::RW;
-    for (int j = l; j < P::TT; j += 64) {
-      uint32_t v = 0;
-#pragma unroll
-      for (int b = 0; b < M; ++b) v |= ((Srow[b * (P::TT / 32) + (j >> 5)] >> (j & 31)) & 1u) << b;
-      w.s[j] = (uint16_t)v;
-    }
-    wave_lds_sync();
-
-    // ---- 2. Berlekamp-Massey (coefficient i = slot*64 + lane)
-    constexpr int NS = P::BM_SLOTS;
-    uint32_t C[NS], B[NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int i = s * 64 + l;
-      C[s] = (i == 0) ? 1u : 0u;
-      B[s] = (i == 1) ? 1u : 0u;
-    }
-    int L = 0;
-    uint32_t bb = 1;
-    for (int N = 0; N < 2 * T; ++N) {
-      uint32_t dp = 0;
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const int i = s * 64 + l;
-        if (i <= T && i <= N) dp ^= gf.mul(C[s], w.s[N - i]);
-      }
-      const uint32_t d = wave_xor(dp);
-      if (d != 0) {
-        const uint32_t lf = gf.addmod(lg_s[d], P::Q - lg_s[bb]);
-        const bool upd = (2 * L <= N);
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-          const uint32_t tc = C[s];
-          C[s] ^= gf.mul_log(B[s], lf);
-          if (upd) B[s] = tc;
-        }
-        if (upd) { L = N + 1 - L; bb = d; }
-      }
-      // B <- x * B (coefficient t+1 and above dropped)
-      uint32_t nb[NS];
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const uint32_t up = (uint32_t)__shfl_up((int)B[s], 1);
-        const uint32_t carry = (s > 0) ? (uint32_t)__shfl((int)B[s > 0 ? s - 1 : 0], 63) : 0u;
-        nb[s] = (l == 0) ? carry : up;
-      }
-#pragma unroll
-      for (int s = 0; s < NS; ++s) B[s] = (s * 64 + l <= T) ? nb[s] : 0u;
-    }
-
-    // ---- 3. locator coefficients sigma_j = C_{t-j}, j < D (sigma_t = C_0 = 1 via the power table)
-    for (int j = l; j < Dn; j += 64) w.f[j] = 0;
-    wave_lds_sync();
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int i = s * 64 + l;
-      const int j = T - i;
-      if (i <= T && j < Dn) w.f[j] = (uint16_t)C[s];
-    }
-    wave_lds_sync();
-
-    // ---- 4. additive FFT, top-down: twist, Taylor expansion at x^2+x, split even/odd
+  for (int base = (blockIdx.x * DEC_WAVES + wave) * 4; base < a.nops; base += gridDim.x * DEC_WAVES * 4) {
+    // ---- 1. syndromes of the four ops from their bit planes
 #pragma unroll 1
-    for (int d = 0; d < KD; ++d) {
-      const int Ld = Dn >> d;
-      const uint16_t *tw = a.ft.twl + (2 * Dn - 2 * Ld);
-      for (int c = l; c < Dn; c += 64) w.f[c] = (uint16_t)gf.mul_log(w.f[c], tw[c & (Ld - 1)]);
-      wave_lds_sync();
-      for (int len = Ld; len >= 4; len >>= 1) {
-        const int q4 = len >> 2;
-        for (int u = l; u < Dn / 4; u += 64) {
-          const int base = (u / q4) * len, i = u % q4;
-          w.f[base + 2 * q4 + i] ^= w.f[base + 3 * q4 + i];
+    for (int q = 0; q < 4; ++q) {
+      const int op = base + q;
+      for (int j = lane; j < P::TT; j += 64) {
+        uint32_t v = 0;
+        if (op < a.nops) {
+          const uint32_t *Srow = a.S + (size_t)op * P::RW;
+#pragma unroll
+          for (int b = 0; b < M; ++b) v |= ((Srow[b * (P::TT / 32) + (j >> 5)] >> (j & 31)) & 1u) << b;
         }
-        wave_lds_sync();
-        for (int u = l; u < Dn / 4; u += 64) {
-          const int base = (u / q4) * len, i = u % q4;
-          w.f[base + q4 + i] ^= w.f[base + 2 * q4 + i];
-        }
-        wave_lds_sync();
+        w.s[q][j] = (uint16_t)v;
       }
-      for (int c = l; c < Dn; c += 64) {
-        const int base = c & ~(Ld - 1), i = c & (Ld - 1);
-        w.tmp[base + (i >> 1) + (i & 1) * (Ld >> 1)] = w.f[c];
-      }
-      wave_lds_sync();
-      for (int c = l; c < Dn; c += 64) w.f[c] = w.tmp[c];
-      wave_lds_sync();
     }
-    // ---- 5. bottom-up butterflies over all 2^m points
-    for (int c = l; c < NPTS; c += 64) w.V[c] = w.f[c >> (M - KD)];
     wave_lds_sync();
+
+    // ---- 2. Berlekamp-Massey, one op per row; C in normal form, B and the syndrome window
+    //         W_i = s_{N-i} in log form
+    {
+      const uint16_t *sq = w.s[row];
+      uint32_t C[NSB], Bl[NSB], Wl[NSB];
+#pragma unroll
+      for (int s = 0; s < NSB; ++s) {
+        const int i = 16 * s + l16;
+        C[s] = (i == 0) ? 1u : 0u;
+        Bl[s] = (i == 1) ? 0u : ZL;
+        Wl[s] = ZL;
+      }
+      int L = 0;
+      uint32_t bb = 1;
 #pragma unroll 1
-    for (int d = KD - 1; d >= 0; --d) {
-      const int hb = M - d - 1;  // log2(half)
-      const uint16_t *pl = a.ft.plg + (NPTS - (NPTS >> d));
-      for (int u = l; u < NPTS / 2; u += 64) {
-        const int c = u & ((1 << hb) - 1);
-        const int base = (u >> hb) << (hb + 1);
-        uint32_t x = w.V[base + c];
-        const uint32_t y = w.V[base + (1 << hb) + c];
-        x ^= gf.mul_log(y, pl[c]);
-        w.V[base + c] = (uint16_t)x;
-        w.V[base + (1 << hb) + c] = (uint16_t)(x ^ y);
-      }
-      wave_lds_sync();
-    }
-
-    // ---- 6. roots at the support: e_i = [sigma(alpha_i) == 0]
-    int wt = 0;
-    for (int it = 0; it < ceil_div(P::N, 64); ++it) {
-      const int i = it * 64 + l;
-      const bool valid = i < P::N;
-      const uint32_t av = valid ? a.alpha[i] : 0u;
-      uint32_t val = w.V[av];
-      if (P::TOP_TERM) val ^= a.ft.powt[av];
-      const bool ei = valid && (val == 0);
-      const uint64_t mask = __ballot(ei);
-      if (ei) {
-        const int idx = wt + (int)mbcnt64(mask);
-        if (idx < T) w.pos[idx] = (uint16_t)i;
-      }
-      if (l == 0) { w.ebits[2 * it] = (uint32_t)mask; w.ebits[2 * it + 1] = (uint32_t)(mask >> 32); }
-      wt += __popcll(mask);
-    }
-    wave_lds_sync();
-
-    // ---- 7. re-encryption check: syndrome of e equals the ciphertext syndrome
-    bool ok = (wt == T);
-    if (ok) {
-      uint32_t acc[ceil_div(2 * T, 64)];
+      for (int N = 0; N < 2 * T; ++N) {
+        uint32_t R[NSB];
 #pragma unroll
-      for (int k = 0; k < ceil_div(2 * T, 64); ++k) acc[k] = 0;
-      for (int k = 0; k < T; ++k) {
-        const int p = w.pos[k];
-        const uint32_t la = a.la[p], lu = a.lu[p];
+        for (int s = 0; s < NSB; ++s) R[s] = dpp<DPP_ROR1>(Wl[s]);
+        const uint32_t lsN = lg_s[sq[N]];
 #pragma unroll
-        for (int kk = 0; kk < ceil_div(2 * T, 64); ++kk) {
-          const uint32_t j = (uint32_t)(kk * 64 + l);
-          uint32_t v;
-          if (lu == 0xFFFFu) v = 0;
-          else if (la == 0xFFFFu) v = (j == 0) ? ex_s[lu] : 0u;
-          else {
-            uint32_t e = j * la + lu;
-            e = (e & P::Q) + (e >> M);
-            e = (e & P::Q) + (e >> M);
-            e = e >= (uint32_t)P::Q ? e - P::Q : e;
-            v = ex_s[e];
+        for (int s = 0; s < NSB; ++s) Wl[s] = (l16 == 0) ? (s ? R[s > 0 ? s - 1 : 0] : lsN) : R[s];
+        uint32_t dp = 0, lc[NSB];
+#pragma unroll
+        for (int s = 0; s < NSB; ++s) {
+          lc[s] = lg_s[C[s]];
+          dp ^= ex_s[lc[s] + Wl[s]];
+        }
+        const uint32_t d = row_xor16(dp);
+        if (d != 0) {
+          uint32_t lf = (uint32_t)lg_s[d] + Q - lg_s[bb];
+          lf = lf >= (uint32_t)Q ? lf - Q : lf;
+          const bool upd = (2 * L <= N);
+#pragma unroll
+          for (int s = 0; s < NSB; ++s) {
+            const uint32_t nc = C[s] ^ ex_s[lf + Bl[s]];
+            if (upd) Bl[s] = lc[s];
+            C[s] = nc;
           }
-          acc[kk] ^= v;
+          if (upd) { L = N + 1 - L; bb = d; }
+        }
+#pragma unroll
+        for (int s = 0; s < NSB; ++s) R[s] = dpp<DPP_ROR1>(Bl[s]);
+#pragma unroll
+        for (int s = 0; s < NSB; ++s) {
+          uint32_t v = (l16 == 0) ? (s ? R[s > 0 ? s - 1 : 0] : ZL) : R[s];
+          Bl[s] = (16 * s + l16 <= T) ? v : ZL;
         }
       }
-      bool mism = false;
+      // locator sigma_j = C_{t-j} (sigma_t = C_0 = 1 is added from the power table when t = D)
+      for (int j = l16; j < Dn; j += 16) w.f[row][j] = 0;
+      wave_lds_sync();
 #pragma unroll
-      for (int kk = 0; kk < ceil_div(2 * T, 64); ++kk) {
-        const int j = kk * 64 + l;
-        if (j < 2 * T && acc[kk] != w.s[j]) mism = true;
+      for (int s = 0; s < NSB; ++s) {
+        const int i = 16 * s + l16;
+        if (i <= T && T - i < Dn) w.f[row][T - i] = (uint16_t)C[s];
       }
-      ok = !__any(mism);
+      wave_lds_sync();
     }
 
-    // ---- 8. session-key message [b][b ? e : s][c][padding]
-    const uint8_t *ebytes = reinterpret_cast<const uint8_t *>(w.ebits);
-    const uint8_t *ctrow = a.ct + (size_t)op * P::SYND_BYTES;
-    uint32_t *mrow = a.msg + (size_t)op * P::MSG_WORDS;
-    for (int k = l; k < P::MSG_WORDS; k += 64) {
-      uint32_t v = 0;
+    // ---- 3. finish the four ops one after the other
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q) {
+      const int op = base + q;
+      if (op >= a.nops) break;
+      uint16_t *fq = w.f[q];
+      // additive FFT, top-down on the D coefficients: twist, Taylor at x^2+x, even/odd split
+#pragma unroll 1
+      for (int d = 0; d < KD; ++d) {
+        const int Ld = Dn >> d;
+        const uint16_t *tw = a.ft.twl + (2 * Dn - 2 * Ld);
+        for (int c = lane; c < Dn; c += 64) fq[c] = ex_s[lg_s[fq[c]] + tw[c & (Ld - 1)]];
+        wave_lds_sync();
+        for (int len = Ld; len >= 4; len >>= 1) {
+          const int q4 = len >> 2;
+          for (int u = lane; u < Dn / 4; u += 64) {
+            const int bs = (u / q4) * len, i = u % q4;
+            fq[bs + 2 * q4 + i] ^= fq[bs + 3 * q4 + i];
+          }
+          wave_lds_sync();
+          for (int u = lane; u < Dn / 4; u += 64) {
+            const int bs = (u / q4) * len, i = u % q4;
+            fq[bs + q4 + i] ^= fq[bs + 2 * q4 + i];
+          }
+          wave_lds_sync();
+        }
+        for (int c = lane; c < Dn; c += 64) {
+          const int bs = c & ~(Ld - 1), i = c & (Ld - 1);
+          w.tmp[bs + (i >> 1) + (i & 1) * (Ld >> 1)] = fq[c];
+        }
+        wave_lds_sync();
+        for (int c = lane; c < Dn; c += 64) fq[c] = w.tmp[c];
+        wave_lds_sync();
+      }
+      // bottom-up butterflies in registers: V2[r] packs point 64r+lane (low half) and point
+      // 64(r+NR/2)+lane (high half); the two butterflies of a packed pair share their multiplier
+      constexpr int NH = NR / 2;
+      uint32_t V2[NH];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) v |= msg_byte<SET>(4 * k + q, ok, ebytes, a.rej, ctrow) << (8 * q);
-      mrow[k] = v;
+      for (int r = 0; r < NH; ++r) V2[r] = (uint32_t)fq[r] | ((uint32_t)fq[r + NH] << 16);
+#pragma unroll
+      for (int d = KD - 1; d >= 1; --d) {  // deepest level first
+        constexpr int dummy = 0;
+        (void)dummy;
+        const int step = 1 << (M - 7 - d);
+        const int phoff = (1 << (M - 6)) - (1 << (M - 6 - d));
+#pragma unroll
+        for (int j = 0; j < step; ++j) {
+          const uint32_t lp = lg_s[plo[d] ^ phi_s[phoff + j]];
+#pragma unroll
+          for (int r = j; r < NH; r += 2 * step) {
+            const uint32_t y2 = V2[r + step];
+            const uint32_t plo_ = ex_s[lg_s[y2 & 0xFFFFu] + lp];
+            const uint32_t phi_ = ex_s[lg_s[y2 >> 16] + lp];
+            const uint32_t x2 = V2[r] ^ (plo_ | (phi_ << 16));
+            V2[r] = x2;
+            V2[r + step] = x2 ^ y2;
+          }
+        }
+      }
+      {  // level 0 (half = 2^(m-1)): the pair is (low, high) of one register
+#pragma unroll
+        for (int r = 0; r < NH; ++r) {
+          const uint32_t lp = lg_s[plo[0] ^ phi_s[r]];
+          const uint32_t x = V2[r] & 0xFFFFu, y = V2[r] >> 16;
+          const uint32_t xn = x ^ ex_s[lg_s[y] + lp];
+          V2[r] = xn | ((xn ^ y) << 16);
+        }
+      }
+      // zero flags of sigma (adding the monic x^t term where it lies outside the FFT)
+#pragma unroll
+      for (int r = 0; r < NH; ++r) {
+        uint32_t lo = V2[r] & 0xFFFFu, hi = V2[r] >> 16;
+        if (P::TOP_TERM) {
+          lo ^= ptlo ^ pthi_s[r];
+          hi ^= ptlo ^ pthi_s[r + NH];
+        }
+        const uint64_t zl = __ballot(lo == 0);
+        const uint64_t zh = __ballot(hi == 0);
+        if (lane == 0) {
+          w.Z[2 * r] = (uint32_t)zl; w.Z[2 * r + 1] = (uint32_t)(zl >> 32);
+          w.Z[2 * (r + NH)] = (uint32_t)zh; w.Z[2 * (r + NH) + 1] = (uint32_t)(zh >> 32);
+        }
+      }
+      wave_lds_sync();
+      // roots at the support
+      int wt = 0;
+      for (int it = 0; it < ceil_div(P::N, 64); ++it) {
+        const int i = it * 64 + lane;
+        const bool valid = i < P::N;
+        const uint32_t av = valid ? a.alpha[i] : 0u;
+        const bool ei = valid && ((w.Z[av >> 5] >> (av & 31)) & 1u);
+        const uint64_t mask = __ballot(ei);
+        if (ei) {
+          const int idx = wt + (int)mbcnt64(mask);
+          if (idx < T) w.pos[idx] = (uint16_t)i;
+        }
+        if (lane == 0) { w.ebits[2 * it] = (uint32_t)mask; w.ebits[2 * it + 1] = (uint32_t)(mask >> 32); }
+        wt += __popcll(mask);
+      }
+      wave_lds_sync();
+      // re-encryption: syndromes of e (sum over its t positions of alpha^j / g(alpha)^2) must
+      // equal the ciphertext's
+      bool ok = (wt == T);
+      if (ok) {
+        constexpr int JS = ceil_div(2 * T, 64);
+        uint32_t acc[JS];
+#pragma unroll
+        for (int k = 0; k < JS; ++k) acc[k] = 0;
+        for (int k = 0; k < T; ++k) {
+          const int p = w.pos[k];
+          const uint32_t la = a.la[p], lu = a.lu[p];
+          if (lu == 0xFFFFu) continue;
+          if (la == 0xFFFFu) {
+            if (lane == 0) acc[0] ^= ex_s[lu];
+            continue;
+          }
+          uint32_t e = (uint32_t)lane * la + lu;
+          e = (e & Q) + (e >> M);
+          e = (e & Q) + (e >> M);
+          e = e >= (uint32_t)Q ? e - Q : e;
+          uint32_t inc = 64u * la;
+          inc = (inc & Q) + (inc >> M);
+          inc = inc >= (uint32_t)Q ? inc - Q : inc;
+#pragma unroll
+          for (int k2 = 0; k2 < JS; ++k2) {
+            acc[k2] ^= ex_s[e];
+            e += inc;
+            e = e >= (uint32_t)Q ? e - Q : e;
+          }
+        }
+        bool mism = false;
+#pragma unroll
+        for (int k2 = 0; k2 < JS; ++k2) {
+          const int j = k2 * 64 + lane;
+          if (j < 2 * T && acc[k2] != w.s[q][j]) mism = true;
+        }
+        ok = !__any(mism);
+      }
+      // session-key message [b][b ? e : s][c][padding]
+      const uint8_t *ebytes = reinterpret_cast<const uint8_t *>(w.ebits);
+      const uint8_t *ctrow = a.ct + (size_t)op * P::SYND_BYTES;
+      uint32_t *mrow = a.msg + (size_t)op * P::MSG_WORDS;
+      for (int k = lane; k < P::MSG_WORDS; k += 64) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) v |= msg_byte<SET>(4 * k + qq, ok, ebytes, a.rej, ctrow) << (8 * qq);
+        mrow[k] = v;
+      }
+      wave_lds_sync();
     }
-    wave_lds_sync();
   }
 }
 
@@ -313,8 +399,8 @@ hipError_t launch_dec(const DecArgs &a, hipStream_t s) {
   TMCE_TIMED("dec_front", s,
              k_dec_front<SET><<<ceil_div(a.nops, FRONT_WG), FRONT_WG, 0, s>>>(
                  a.ct, reinterpret_cast<const uint4 *>(a.F), a.S, a.nops));
-  int grid = a.decode_blocks > 0 ? a.decode_blocks : 1024;
-  const int maxgrid = ceil_div(a.nops, DEC_WAVES);
+  int grid = a.decode_blocks > 0 ? a.decode_blocks : 256;
+  const int maxgrid = ceil_div(a.nops, DEC_WAVES * 4);
   if (grid > maxgrid) grid = maxgrid;
   TMCE_TIMED("dec_decode", s, k_dec_decode<SET><<<grid, DEC_WG, 0, s>>>(a));
   TMCE_TIMED("dec_hash", s,

@@ -27,8 +27,10 @@ struct FieldTables {
   const uint16_t *lg;    // [2^m]
   const uint16_t *ex;    // [2^m - 1]
   const uint16_t *twl;   // FFT twist logs, level d at offset 2D - 2(D >> d)
-  const uint16_t *plg;   // FFT combine-multiplier logs (0xFFFF = zero), level d at 2^m - 2^(m-d)
-  const uint16_t *powt;  // a^t for every field element a (monic x^t term), or null
+  const uint16_t *plo;   // FFT butterfly multipliers, lane part [KD][64]
+  const uint16_t *phi;   // FFT butterfly multipliers, register part (levels concatenated)
+  const uint16_t *ptlo;  // (lane)^t, [64]       (monic x^t term when t = D), or null
+  const uint16_t *pthi;  // (64 r)^t, [2^m / 64]
 };
 
 // Device buffers of one decapsulation batch.
