@@ -17,14 +17,45 @@ __constant__ static const uint64_t kKeccakRC[24] = {
     0x8000000000008002ULL, 0x8000000000000080ULL, 0x000000000000800aULL, 0x800000008000000aULL,
     0x8000000080008081ULL, 0x8000000000008080ULL, 0x0000000080000001ULL, 0x8000000080008008ULL};
 
+// 64-bit rotate as two 32-bit funnel shifts (v_alignbit_b32): for 0 < r < 32,
+// hi' = (hi:lo) >> (32-r), lo' = (lo:hi) >> (32-r); r >= 32 swaps the halves first.
 template <int R>
 __device__ __forceinline__ uint64_t rotl64(uint64_t x) {
-  if constexpr (R == 0) return x;
-  else return (x << R) | (x >> (64 - R));
+  if constexpr (R == 0) {
+    return x;
+  } else {
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    if constexpr (R >= 32) { const uint32_t t = lo; lo = hi; hi = t; }
+    constexpr int r = R & 31;
+    if constexpr (r == 0) {
+      return ((uint64_t)hi << 32) | lo;
+    } else {
+      const uint32_t nhi = __builtin_amdgcn_alignbit(hi, lo, 32 - r);
+      const uint32_t nlo = __builtin_amdgcn_alignbit(lo, hi, 32 - r);
+      return ((uint64_t)nhi << 32) | nlo;
+    }
+  }
 }
 
+// Round constants held in two VGPRs: lane r (r < 24) has RC[r]; read back with v_readlane, so the
+// rolled round loop has no scalar-memory wait per round.
+struct KeccakRC {
+  uint32_t lo, hi;
+  __device__ __forceinline__ KeccakRC() {
+    const int l = __lane_id();
+    const uint64_t rc = kKeccakRC[l < 24 ? l : 0];
+    lo = (uint32_t)rc;
+    hi = (uint32_t)(rc >> 32);
+  }
+  __device__ __forceinline__ uint64_t get(int r) const {
+    return (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)lo, r) |
+           ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)hi, r) << 32);
+  }
+};
+
 // One lane computes the whole permutation; the state stays in 50 VGPRs.  Lane (x,y) is A[x+5y].
-__device__ __forceinline__ void keccak_f1600(uint64_t (&A)[25]) {
+// All lanes of the wave must be active when `rc` is read (it is broadcast with v_readlane).
+__device__ __forceinline__ void keccak_f1600(uint64_t (&A)[25], const KeccakRC &rc) {
 #pragma unroll 1
   for (int round = 0; round < 24; ++round) {
     uint64_t C0 = A[0] ^ A[5] ^ A[10] ^ A[15] ^ A[20];
@@ -71,21 +102,21 @@ __device__ __forceinline__ void keccak_f1600(uint64_t (&A)[25]) {
       A[5 * y + 3] = b3 ^ (~b4 & b0);
       A[5 * y + 4] = b4 ^ (~b0 & b1);
     }
-    A[0] ^= kKeccakRC[round];
+    A[0] ^= rc.get(round);
   }
 }
 
 // SHAKE256 over a message already laid out in memory as whole 136-byte blocks with the
 // padding applied (0x1F after the message, 0x80 in the last byte); writes 32 bytes.
 __device__ __forceinline__ void shake256_blocks_32(const uint64_t *__restrict__ msg, int nblocks,
-                                                   uint64_t out[4]) {
+                                                   uint64_t out[4], const KeccakRC &rc) {
   uint64_t A[25];
 #pragma unroll
   for (int i = 0; i < 25; ++i) A[i] = 0;
   for (int b = 0; b < nblocks; ++b) {
 #pragma unroll
     for (int w = 0; w < 17; ++w) A[w] ^= msg[17 * b + w];
-    keccak_f1600(A);
+    keccak_f1600(A, rc);
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) out[i] = A[i];

@@ -373,10 +373,13 @@ template <int SET>
 __global__ __launch_bounds__(256) void k_hash_msg(const uint32_t *__restrict__ msg, const uint8_t *__restrict__ ct,
                                                   uint8_t *__restrict__ key, int *__restrict__ status, int nops) {
   using P = Par<SET>;
-  const int op = blockIdx.x * blockDim.x + threadIdx.x;
-  if (op >= nops) return;
+  // every lane stays active through the hash (round constants via v_readlane)
+  const KeccakRC rc;
+  const int gop = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = gop < nops;
+  const int op = live ? gop : nops - 1;
   uint64_t out[4];
-  shake256_blocks_32(reinterpret_cast<const uint64_t *>(msg + (size_t)op * P::MSG_WORDS), P::MSG_BLOCKS, out);
+  shake256_blocks_32(reinterpret_cast<const uint64_t *>(msg + (size_t)op * P::MSG_WORDS), P::MSG_BLOCKS, out, rc);
   int st = 0;
   if (SET == S6960119) {
     const uint8_t last = ct[(size_t)op * P::SYND_BYTES + P::SYND_BYTES - 1];
@@ -386,6 +389,7 @@ __global__ __launch_bounds__(256) void k_hash_msg(const uint32_t *__restrict__ m
       for (int i = 0; i < 4; ++i) out[i] = ~0ull;
     }
   }
+  if (!live) return;
   uint64_t *kp = reinterpret_cast<uint64_t *>(key + 32 * (size_t)op);
 #pragma unroll
   for (int i = 0; i < 4; ++i) kp[i] = out[i];

@@ -3,11 +3,9 @@
 // Engine form (two launches):
 //   k_gen_e    one op per lane; seeded fixed-weight sampling (SHAKE256 chain, range filter,
 //              collision rejection) with a device-wide work queue so lanes whose attempt is
-//              rejected retry while finished lanes take new ops (no wave-wide straggling).
-//              The bits of e are set with atomicOr straight into the op's session-key message
-//              row; the returned old word detects a repeated index (libmceliece sorts the t
-//              indices and compares neighbours -- the same predicate).  The support is kept in
-//              draw order: only its set matters downstream.
+//              rejected retry while finished lanes take new ops.  Collision checks run
+//              wave-cooperatively on an LDS bitmap; e and the support (in draw order -- only its
+//              set matters downstream) leave through coalesced stores.
 //   k_enc_finish one op per lane; c = XOR of t columns of [I | T] (sparse column gather from the
 //              device-resident, column-major public key; L2 resident), then K = SHAKE256(1||e||c).
 //
@@ -20,16 +18,31 @@
 namespace tmce {
 
 constexpr int GEN_WG = 256;
+constexpr int GEN_WAVES = GEN_WG / 64;
 
-// e occupies bits 8..8+N-1 of the message row (byte 0 is the 0x01 prefix of 1||e||c).
+// Sampler.  Phase A (per lane): one SHAKE256 attempt of the lane's op; the first t draws below N
+// go to a lane-private LDS list.  Phase B (per wave): the candidate ops of the wave are checked
+// one after another by all 64 lanes -- each lane sets the bits of two of the op's indices in a
+// per-wave LDS bitmap with ds_or and a returned bit already set means a repeated index (the
+// predicate libmceliece evaluates by sorting and comparing neighbours).  An accepted op's bitmap
+// is e: the wave writes it (shifted behind the 0x01 prefix byte of 1||e||c) and the support list
+// to HBM with coalesced stores.  Rejected ops keep the lane and retry with the chained state.
 template <int SET>
 __global__ __launch_bounds__(GEN_WG) void k_gen_e(const uint8_t *__restrict__ seeds, uint16_t *__restrict__ ind_out,
                                                   uint32_t *__restrict__ msg, uint32_t *__restrict__ attempts,
                                                   int nops, unsigned *__restrict__ queue) {
   using P = Par<SET>;
   constexpr uint32_t GFMASK = (1u << P::M) - 1u;
-  constexpr int EW2 = ceil_div(P::N + 8, 64);  // 8-byte words holding the prefix byte and e
+  constexpr int NEW = ceil_div(P::N, 32);  // 32-bit words of e
+  __shared__ uint16_t lst[P::T * GEN_WG];  // [index][thread]
+  __shared__ uint32_t bmp[GEN_WAVES][4][NEW + 1];
+  const int tid = threadIdx.x;
   const int lane = lane_id();
+  const int wave = tid >> 6;
+  const int grp = lane >> 4, g16 = lane & 15;  // phase B: four 16-lane groups
+  uint32_t *bm = bmp[wave][grp];
+  for (int i = lane; i < 4 * (NEW + 1); i += 64) bmp[wave][i / (NEW + 1)][i % (NEW + 1)] = 0;
+  const KeccakRC rc;
 
   int op = -1;
   bool done = false;
@@ -59,19 +72,15 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e(const uint8_t *__restrict__ se
       }
     }
     if (!__any(op >= 0)) break;
-    if (op < 0) continue;
 
-    // ---- one attempt: draws = SHAKE256(0x40 || st, DRAW_BYTES)
-    ++tries;
-    uint32_t *mrow = msg + (size_t)op * P::MSG_WORDS;
-    uint64_t *mrow2 = reinterpret_cast<uint64_t *>(mrow);
-    uint16_t *io = ind_out + (size_t)op * P::IND_STRIDE;
-    for (int i = 0; i < EW2; ++i) mrow2[i] = 0;
-    __threadfence_block();  // the clears complete before this lane's atomics on the same words
-    uint64_t A[25];
-#pragma unroll
-    for (int i = 0; i < 25; ++i) A[i] = 0;
+    // ---- phase A: one attempt, draws = SHAKE256(0x40 || st, DRAW_BYTES)
+    int count = 0;
+    uint32_t nst[8];
     {
+      tries += (op >= 0) ? 1u : 0u;
+      uint64_t A[25];
+#pragma unroll
+      for (int i = 0; i < 25; ++i) A[i] = 0;
       const uint64_t s0 = st[0] | ((uint64_t)st[1] << 32), s1 = st[2] | ((uint64_t)st[3] << 32);
       const uint64_t s2 = st[4] | ((uint64_t)st[5] << 32), s3 = st[6] | ((uint64_t)st[7] << 32);
       A[0] = 0x40ull | (s0 << 8);
@@ -80,44 +89,97 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e(const uint8_t *__restrict__ se
       A[3] = (s2 >> 56) | (s3 << 8);
       A[4] = (s3 >> 56) | (0x1Full << 8);
       A[16] = 0x80ull << 56;
-    }
-    uint32_t nst[8];
-    int count = 0;
-    uint32_t coll = 0;
 #pragma unroll
-    for (int blk = 0; blk < P::DRAW_BLOCKS; ++blk) {
-      keccak_f1600(A);
-      if (blk == 0) {
+      for (int blk = 0; blk < P::DRAW_BLOCKS; ++blk) {
+        keccak_f1600(A, rc);  // idle lanes compute too: the round constants need the full wave
+        if (blk == 0) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) { nst[2 * i] = (uint32_t)A[i]; nst[2 * i + 1] = (uint32_t)(A[i] >> 32); }
-      }
+          for (int i = 0; i < 4; ++i) { nst[2 * i] = (uint32_t)A[i]; nst[2 * i + 1] = (uint32_t)(A[i] >> 32); }
+        }
 #pragma unroll
-      for (int w = 0; w < 17; ++w)
+        for (int w = 0; w < 17; ++w)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int idx = blk * 68 + w * 4 + q;
-          if (idx < P::NDRAW) {
-            const uint32_t v = (uint32_t)(A[w] >> (16 * q)) & GFMASK;
-            if (v < (uint32_t)P::N && count < P::T) {
-              io[count] = (uint16_t)v;
-              const uint32_t pos = v + 8u, bit = 1u << (pos & 31u);
-              coll |= atomicOr(mrow + (pos >> 5), bit) & bit;
-              ++count;
+          for (int q = 0; q < 4; ++q) {
+            const int idx = blk * 68 + w * 4 + q;
+            if (idx < P::NDRAW) {
+              const uint32_t v = (uint32_t)(A[w] >> (16 * q)) & GFMASK;
+              if (op >= 0 && v < (uint32_t)P::N && count < P::T) {
+                lst[count * GEN_WG + tid] = (uint16_t)v;
+                ++count;
+              }
             }
           }
-        }
+      }
     }
-    if (count == P::T && coll == 0) {
-      atomicOr(mrow, 0x01u);
-      if (attempts) attempts[op] = tries;
-      op = -1;
-    } else if (tries >= (uint32_t)P::MAX_ATTEMPTS) {
-      io[0] = 0xFFFFu;  // abandoned (never expected): k_enc_finish reports status -2
-      if (attempts) attempts[op] = tries;
-      op = -1;
-    } else {
+    wave_lds_sync();
+
+    // ---- phase B: collision check of the wave's candidates, four at a time (one per 16-lane group)
+    uint64_t cand = __ballot(op >= 0 && count == P::T);
+    uint64_t okmask = 0;
+    while (cand) {
+      // candidate of this group: the grp-th lowest set bit (none if fewer remain)
+      uint64_t c = cand;
+      for (int k = 0; k < grp && c; ++k) c &= c - 1;
+      const bool have = c != 0;
+      const int j = have ? __ffsll((long long)c) - 1 : 0;
+      for (int k = 0; k < 4 && cand; ++k) cand &= cand - 1;
+      const int opj = __shfl(op, j);
+      const uint16_t *lj = lst + (wave * 64 + j);
+      bool coll = false;
+      if (have) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) st[i] = nst[i];
+        for (int k = 0; k < ceil_div(P::T, 16); ++k) {
+          const int idx = k * 16 + g16;
+          if (idx < P::T) {
+            const uint32_t v = lj[idx * GEN_WG];
+            const uint32_t bit = 1u << (v & 31u);
+            coll |= (atomicOr(bm + (v >> 5), bit) & bit) != 0;
+          }
+        }
+      }
+      const uint64_t cm = __ballot(coll);
+      const bool ok = have && ((cm >> (16 * grp)) & 0xFFFFull) == 0;
+      wave_lds_sync();
+      if (ok) {
+        uint32_t *mrow = msg + (size_t)opj * P::MSG_WORDS;
+        for (int k = g16; k <= NEW; k += 16) {
+          const uint32_t cur = k < NEW ? bm[k] : 0u;
+          const uint32_t prev = k ? bm[k - 1] : 0x01000000u;
+          mrow[k] = (cur << 8) | (prev >> 24);
+        }
+        uint16_t *io = ind_out + (size_t)opj * P::IND_STRIDE;
+        for (int idx = g16; idx < P::T; idx += 16) io[idx] = lj[idx * GEN_WG];
+      }
+      {
+        // lanes g16 == 0 of accepting groups publish their candidate index
+        const uint64_t acc = __ballot(ok && g16 == 0);
+        for (int g = 0; g < 4; ++g)
+          if ((acc >> (16 * g)) & 1ull) okmask |= 1ull << __shfl(j, 16 * g);
+      }
+      wave_lds_sync();
+      if (have) {
+#pragma unroll
+        for (int k = 0; k < ceil_div(P::T, 16); ++k) {
+          const int idx = k * 16 + g16;
+          if (idx < P::T) bm[lj[idx * GEN_WG] >> 5] = 0;
+        }
+      }
+      wave_lds_sync();
+    }
+
+    // ---- per-lane bookkeeping
+    if (op >= 0) {
+      if ((okmask >> lane) & 1ull) {
+        if (attempts) attempts[op] = tries;
+        op = -1;
+      } else if (tries >= (uint32_t)P::MAX_ATTEMPTS) {
+        ind_out[(size_t)op * P::IND_STRIDE] = 0xFFFFu;  // abandoned (never expected): status -2
+        if (attempts) attempts[op] = tries;
+        op = -1;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) st[i] = nst[i];
+      }
     }
   }
 }
@@ -128,32 +190,33 @@ __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__
                                                     uint8_t *__restrict__ key, int *__restrict__ status, int nops) {
   using P = Par<SET>;
   constexpr int C4 = P::CWP / 4;
-  const int op = blockIdx.x * blockDim.x + threadIdx.x;
-  if (op >= nops) return;
+  // Every lane stays active through the hash (the round constants are read with v_readlane):
+  // lanes past the batch redo the last op and store nothing.
+  const KeccakRC rc;
+  const int gop = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = gop < nops;
+  const int op = live ? gop : nops - 1;
 
   // ---- c = sum of columns over the support of e
   uint4 c[C4];
 #pragma unroll
   for (int q = 0; q < C4; ++q) c[q] = make_uint4(0, 0, 0, 0);
   const uint32_t *ip = reinterpret_cast<const uint32_t *>(ind + (size_t)op * P::IND_STRIDE);
-  if ((ip[0] & 0xFFFFu) == 0xFFFFu) {  // sampler gave up on this op
-    for (int h = 0; h < P::SYND_BYTES; ++h) ct[(size_t)op * P::SYND_BYTES + h] = 0;
-    for (int h = 0; h < 32; ++h) key[32 * (size_t)op + h] = 0;
-    if (status) status[op] = -2;
-    return;
-  }
+  const bool abandoned = (ip[0] & 0xFFFFu) == 0xFFFFu;  // sampler gave up on this op (never expected)
+  if (!abandoned) {
 #pragma unroll 4
-  for (int i = 0; i < P::T; i += 2) {
-    const uint32_t pr = ip[i / 2];
+    for (int i = 0; i < P::T; i += 2) {
+      const uint32_t pr = ip[i / 2];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      if (i + h < P::T) {
-        const uint32_t p = h ? (pr >> 16) : (pr & 0xFFFFu);
-        const uint4 *col = cols + (size_t)p * C4;
+      for (int h = 0; h < 2; ++h) {
+        if (i + h < P::T) {
+          const uint32_t p = h ? (pr >> 16) : (pr & 0xFFFFu);
+          const uint4 *col = cols + (size_t)p * C4;
 #pragma unroll
-        for (int q = 0; q < C4; ++q) {
-          const uint4 x = col[q];
-          c[q].x ^= x.x; c[q].y ^= x.y; c[q].z ^= x.z; c[q].w ^= x.w;
+          for (int q = 0; q < C4; ++q) {
+            const uint4 x = col[q];
+            c[q].x ^= x.x; c[q].y ^= x.y; c[q].z ^= x.z; c[q].w ^= x.w;
+          }
         }
       }
     }
@@ -164,8 +227,10 @@ __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__
 
   // ---- ciphertext bytes (SYND_BYTES is even for every set)
   uint16_t *ctp = reinterpret_cast<uint16_t *>(ct + (size_t)op * P::SYND_BYTES);
+  if (live) {
 #pragma unroll
-  for (int h = 0; h < P::SYND_BYTES / 2; ++h) ctp[h] = (uint16_t)(cw[h / 2] >> (16 * (h & 1)));
+    for (int h = 0; h < P::SYND_BYTES / 2; ++h) ctp[h] = (uint16_t)(cw[h / 2] >> (16 * (h & 1)));
+  }
 
   // ---- finish the message: c at byte 1+N/8, then SHAKE padding
   uint32_t *mrow = msg + (size_t)op * P::MSG_WORDS;
@@ -182,14 +247,18 @@ __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__
     if (k == KB) v |= first;
     if (k == PADW) v |= 0x1Fu << PADS;
     if (k == P::MSG_WORDS - 1) v |= 0x80u << 24;
-    mrow[k] = v;
+    if (live) mrow[k] = v;
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // this lane's message stores complete
   uint64_t out[4];
-  shake256_blocks_32(reinterpret_cast<const uint64_t *>(mrow), P::MSG_BLOCKS, out);
+  shake256_blocks_32(reinterpret_cast<const uint64_t *>(mrow), P::MSG_BLOCKS, out, rc);
   uint64_t *kp = reinterpret_cast<uint64_t *>(key + 32 * (size_t)op);
+  if (live) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i) kp[i] = out[i];
-  if (status) status[op] = 0;
+    for (int i = 0; i < 4; ++i) kp[i] = abandoned ? 0ull : out[i];
+    if (abandoned) for (int h = 0; h < P::SYND_BYTES / 2; ++h) ctp[h] = 0;
+    if (status) status[op] = abandoned ? -2 : 0;
+  }
 }
 
 __global__ void k_fill_u32(unsigned *p, unsigned v, int n) {
