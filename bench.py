@@ -59,6 +59,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-stage GPU form")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -125,6 +126,45 @@ def main():
     tm.timing_enable(False)
     kernels = {k: {"launches": c, "avg_ms": tot / c} for k, (c, tot) in kt.items() if c}
 
+    # ---- the plain one-launch-per-stage form on the same device-resident inputs
+    plain = None
+    if not args.no_plain:
+        tm.set_form("plain")
+        enc()
+        dec()
+        tm.sync()
+        preps = max(2, reps // 2)
+        t0 = time.perf_counter()
+        for _ in range(preps):
+            enc()
+        tm.sync()
+        pt_enc = (time.perf_counter() - t0) / preps
+        t0 = time.perf_counter()
+        for _ in range(preps):
+            dec()
+        tm.sync()
+        pt_dec = (time.perf_counter() - t0) / preps
+        tm.timing_enable(True)
+        enc()
+        dec()
+        tm.sync()
+        pkt = tm.timing_read()
+        tm.timing_enable(False)
+        plain_ct = d_ct.download((n, p["ct"]))
+        plain_k = d_k.download((n, 32))
+        plain_k2 = d_k2.download((n, 32))
+        tm.set_form("engine")
+        enc()
+        dec()
+        tm.sync()
+        same = bool(np.array_equal(plain_ct, d_ct.download((n, p["ct"]))) and
+                    np.array_equal(plain_k, d_k.download((n, 32))) and
+                    np.array_equal(plain_k2, d_k2.download((n, 32))))
+        plain = {"encaps_per_s": round(n / pt_enc, 1), "decaps_per_s": round(n / pt_dec, 1),
+                 "kem_ops_per_s": round(2 * n / (pt_enc + pt_dec), 1),
+                 "identical_to_engine": same,
+                 "kernels": {k: {"launches": c, "avg_ms": tot / c} for k, (c, tot) in pkt.items() if c}}
+
     # ---- correctness of the measured batch
     verified = None
     if not args.no_verify:
@@ -190,6 +230,10 @@ def main():
                    "set": s, "batch": n},
         "encaps_per_s": round(n / t_enc, 1), "decaps_per_s": round(n / t_dec, 1),
         "bit_exact": verified,
+        "plain_gpu_form": plain,
+        "speedup_vs_plain": None if plain is None else {
+            "encaps": round((n / t_enc) / plain["encaps_per_s"], 2),
+            "decaps": round((n / t_dec) / plain["decaps_per_s"], 2)},
         "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,

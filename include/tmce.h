@@ -125,6 +125,12 @@ int tmce_enc_batch_device(tmce_key *pk, unsigned char *d_ct, unsigned char *d_k,
 int tmce_dec_batch_device(tmce_key *sk, unsigned char *d_k, int *d_status, const unsigned char *d_ct,
                           size_t n, void *stream);
 
+/* select the pipeline used by subsequent batched calls: 0 = engine form (default: fused,
+   persistent sampler, row-parallel decoder), 1 = plain one-launch-per-stage form (the baseline
+   mirroring the reference pipelines, mceliece-gpu-params/main.cu:213-262,326-370).  Both give
+   byte-identical results. */
+int tmce_set_form(int form);
+
 /* device-memory helpers for callers without a HIP runtime binding */
 void *tmce_dev_alloc(size_t bytes);
 void tmce_dev_free(void *p);

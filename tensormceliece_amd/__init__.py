@@ -60,6 +60,7 @@ def load():
         "tmce_memcpy_d2h": (I, [P, P, S]),
         "tmce_sync": (I, []),
         "tmce_timing_enable": (I, [I]),
+        "tmce_set_form": (I, [I]),
         "tmce_timing_read": (I, [I, P, P, P]),
     }
     for name, (res, args) in sig.items():
@@ -194,6 +195,11 @@ def enc_device(pk, d_ct, d_k, d_status, d_seeds, n, stream=None):
 def dec_device(sk, d_k, d_status, d_ct, n, stream=None):
     _check(load().tmce_dec_batch_device(sk.handle, d_k.ptr, d_status.ptr if d_status else None, d_ct.ptr, n,
                                         stream), "tmce_dec_batch_device")
+
+
+def set_form(form):
+    """0 = engine form (default), 1 = plain one-launch-per-stage form (the baseline)."""
+    _check(load().tmce_set_form({"engine": 0, "plain": 1}.get(form, form)), "tmce_set_form")
 
 
 def sync():

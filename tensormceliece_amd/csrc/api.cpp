@@ -28,6 +28,7 @@ struct tmce_key {
   size_t bytes = 0;
   bool pad_bad = false;
   uint32_t *cols = nullptr;
+  uint32_t *Trows = nullptr;  // plain form: T as MT rows of ceil(K/32) words
   uint32_t *F = nullptr;
   uint16_t *alpha = nullptr, *la = nullptr, *lu = nullptr;
   uint8_t *rej = nullptr;
@@ -106,6 +107,8 @@ struct Engine {
   uint16_t *lg[2] = {nullptr, nullptr}, *ex[2] = {nullptr, nullptr};
   SetTables st[NUM_SETS];
   DevBuf seeds, ct, key, status, ind, msg, S, attempts, queue;
+  DevBuf p_erow, p_ens, p_s, p_loc, p_weight;  // plain-form scratch
+  int form = 0;                                // 0 = engine form, 1 = plain per-stage form
   EventTimer timer;
 };
 
@@ -214,6 +217,17 @@ int enc_device(tmce_key *k, uint8_t *d_ct, uint8_t *d_k, int *d_status, const ui
     return 0;
   }
   const int MSGW = (1 + si.n / 8 + si.synd_bytes) / 136 * 34 + 34;
+  if (e->form == 1) {
+    const int NEW = (si.n + 31) / 32, KWE = (si.k + 31) / 32, CW = (si.mt + 31) / 32;
+    if (int r = e->p_erow.ensure(n * NEW * 4)) return r;
+    if (int r = e->p_ens.ensure(n * KWE * 4)) return r;
+    if (int r = e->p_s.ensure(n * CW * 4)) return r;
+    if (int r = e->msg.ensure(n * MSGW * sizeof(uint32_t))) return r;
+    PlainEncArgs pa{d_seeds, k->Trows, d_ct, d_k, d_status, e->p_erow.as<uint32_t>(), e->p_ens.as<uint32_t>(),
+                    e->p_s.as<uint32_t>(), e->msg.as<uint32_t>(), (int)n};
+    HIPCHK(launch_plain_enc_set(k->set, pa, s));
+    return 0;
+  }
   if (int r = e->ind.ensure(n * ((si.t + 1) & ~1) * sizeof(uint16_t))) return r;
   if (int r = e->msg.ensure(n * MSGW * sizeof(uint32_t))) return r;
   if (int r = e->queue.ensure(64)) return r;
@@ -262,6 +276,15 @@ int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, si
   a.msg = e->msg.as<uint32_t>();
   a.nops = (int)n;
   a.decode_blocks = 0;
+  if (e->form == 1) {
+    const int NEW2 = 2 * ((si.n + 63) / 64);
+    if (int r = e->p_loc.ensure(n * (si.t + 1) * 2)) return r;
+    if (int r = e->p_erow.ensure(n * NEW2 * 4)) return r;
+    if (int r = e->p_weight.ensure(n * 4)) return r;
+    PlainDecScratch x{e->p_loc.as<uint16_t>(), e->p_erow.as<uint32_t>(), e->p_weight.as<int>()};
+    HIPCHK(launch_plain_dec_set(k->set, a, x, s));
+    return 0;
+  }
   HIPCHK(launch_dec_set(k->set, a, s));
   return 0;
 }
@@ -329,10 +352,12 @@ int tmce_key_from_pk(int set, const unsigned char *pk, tmce_key **out) {
   k->set = set;
   k->kind = 1;
   k->pad_bad = pk_padding_bad(set, pk);
-  std::vector<uint32_t> cols;
+  std::vector<uint32_t> cols, trows;
   build_enc_cols(set, pk, cols);
+  build_enc_rows(set, pk, trows);
   if (int r = upload(&k->cols, cols)) return r;
-  k->bytes = cols.size() * 4;
+  if (int r = upload(&k->Trows, trows)) return r;
+  k->bytes = (cols.size() + trows.size()) * 4;
   *out = k.release();
   return 0;
 }
@@ -364,6 +389,7 @@ void tmce_key_free(tmce_key *k) {
     (void)hipStreamSynchronize(e->stream);
   }
   (void)hipFree(k->cols);
+  (void)hipFree(k->Trows);
   (void)hipFree(k->F);
   (void)hipFree(k->alpha);
   (void)hipFree(k->la);
@@ -462,6 +488,15 @@ int tmce_sync(void) {
   if (!e) return TMCE_ENODEV;
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(hipDeviceSynchronize());
+  return 0;
+}
+
+int tmce_set_form(int form) {
+  if (form != 0 && form != 1) return fail(TMCE_EINVAL, "form must be 0 (engine) or 1 (plain)");
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  std::lock_guard<std::mutex> lk(e->mu);
+  e->form = form;
   return 0;
 }
 

@@ -106,6 +106,57 @@ __device__ __forceinline__ void keccak_f1600(uint64_t (&A)[25], const KeccakRC &
   }
 }
 
+// Same permutation with the round constant read from constant memory: safe in divergent code
+// (the plain per-thread kernels), at the cost of a scalar load per round.
+__device__ __forceinline__ void keccak_f1600_c(uint64_t (&A)[25]) {
+  struct ConstRC {
+    __device__ __forceinline__ uint64_t get(int r) const { return kKeccakRC[r]; }
+  };
+#pragma unroll 1
+  for (int round = 0; round < 24; ++round) {
+    uint64_t C0 = A[0] ^ A[5] ^ A[10] ^ A[15] ^ A[20];
+    uint64_t C1 = A[1] ^ A[6] ^ A[11] ^ A[16] ^ A[21];
+    uint64_t C2 = A[2] ^ A[7] ^ A[12] ^ A[17] ^ A[22];
+    uint64_t C3 = A[3] ^ A[8] ^ A[13] ^ A[18] ^ A[23];
+    uint64_t C4 = A[4] ^ A[9] ^ A[14] ^ A[19] ^ A[24];
+    uint64_t D0 = C4 ^ rotl64<1>(C1), D1 = C0 ^ rotl64<1>(C2), D2 = C1 ^ rotl64<1>(C3);
+    uint64_t D3 = C2 ^ rotl64<1>(C4), D4 = C3 ^ rotl64<1>(C0);
+    uint64_t B[25];
+    B[0] = rotl64<0>(A[0] ^ D0);   B[10] = rotl64<1>(A[1] ^ D1);  B[20] = rotl64<62>(A[2] ^ D2);
+    B[5] = rotl64<28>(A[3] ^ D3);  B[15] = rotl64<27>(A[4] ^ D4); B[16] = rotl64<36>(A[5] ^ D0);
+    B[1] = rotl64<44>(A[6] ^ D1);  B[11] = rotl64<6>(A[7] ^ D2);  B[21] = rotl64<55>(A[8] ^ D3);
+    B[6] = rotl64<20>(A[9] ^ D4);  B[7] = rotl64<3>(A[10] ^ D0);  B[17] = rotl64<10>(A[11] ^ D1);
+    B[2] = rotl64<43>(A[12] ^ D2); B[12] = rotl64<25>(A[13] ^ D3); B[22] = rotl64<39>(A[14] ^ D4);
+    B[23] = rotl64<41>(A[15] ^ D0); B[8] = rotl64<45>(A[16] ^ D1); B[18] = rotl64<15>(A[17] ^ D2);
+    B[3] = rotl64<21>(A[18] ^ D3); B[13] = rotl64<8>(A[19] ^ D4); B[14] = rotl64<18>(A[20] ^ D0);
+    B[24] = rotl64<2>(A[21] ^ D1); B[9] = rotl64<61>(A[22] ^ D2); B[19] = rotl64<56>(A[23] ^ D3);
+    B[4] = rotl64<14>(A[24] ^ D4);
+#pragma unroll
+    for (int y = 0; y < 5; ++y) {
+      uint64_t b0 = B[5 * y], b1 = B[5 * y + 1], b2 = B[5 * y + 2], b3 = B[5 * y + 3], b4 = B[5 * y + 4];
+      A[5 * y + 0] = b0 ^ (~b1 & b2);
+      A[5 * y + 1] = b1 ^ (~b2 & b3);
+      A[5 * y + 2] = b2 ^ (~b3 & b4);
+      A[5 * y + 3] = b3 ^ (~b4 & b0);
+      A[5 * y + 4] = b4 ^ (~b0 & b1);
+    }
+    A[0] ^= ConstRC().get(round);
+  }
+}
+
+__device__ __forceinline__ void shake256_blocks_32_c(const uint64_t *__restrict__ msg, int nblocks, uint64_t out[4]) {
+  uint64_t A[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) A[i] = 0;
+  for (int b = 0; b < nblocks; ++b) {
+#pragma unroll
+    for (int w = 0; w < 17; ++w) A[w] ^= msg[17 * b + w];
+    keccak_f1600_c(A);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) out[i] = A[i];
+}
+
 // SHAKE256 over a message already laid out in memory as whole 136-byte blocks with the
 // padding applied (0x1F after the message, 0x80 in the last byte); writes 32 bytes.
 __device__ __forceinline__ void shake256_blocks_32(const uint64_t *__restrict__ msg, int nblocks,

@@ -265,6 +265,15 @@ void build_enc_cols(int set, const uint8_t *pk, std::vector<uint32_t> &cols) {
   throw std::invalid_argument("set");
 }
 
+void build_enc_rows(int set, const uint8_t *pk, std::vector<uint32_t> &rows) {
+  const SetInfo &si = set_info(set);
+  const int kw = (si.k + 31) / 32;
+  rows.assign((size_t)si.mt * kw, 0);
+  for (int r = 0; r < si.mt; ++r)
+    for (int j = 0; j < si.k; ++j)
+      if ((pk[(size_t)r * si.pk_row_bytes + j / 8] >> (j % 8)) & 1) rows[(size_t)r * kw + j / 32] |= 1u << (j % 32);
+}
+
 // LM:crypto_kem/6960119/vec/kem_enc.c check_pk_padding
 bool pk_padding_bad(int set, const uint8_t *pk) {
   if (set != S6960119) return false;

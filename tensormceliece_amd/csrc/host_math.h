@@ -47,6 +47,8 @@ void build_dec_key(int set, const uint8_t *sk, DecKeyHost &out);
 // Per-key data of encapsulation, derived from pk: column-major [I | T], N x CWP words.
 void build_enc_cols(int set, const uint8_t *pk, std::vector<uint32_t> &cols);
 bool pk_padding_bad(int set, const uint8_t *pk);
+// The public key T as MT rows of ceil(K/32) little-endian words (plain form's GEMM operand).
+void build_enc_rows(int set, const uint8_t *pk, std::vector<uint32_t> &rows);
 
 // Host key generation with libmceliece's keypair semantics; rand(out,len) supplies randomness.
 typedef void (*rand_fn)(uint8_t *out, long long len, void *ctx);

@@ -412,6 +412,38 @@ hipError_t launch_dec(const DecArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int SET>
+hipError_t front_only(const DecArgs &a, hipStream_t s) {
+  k_dec_front<SET><<<ceil_div(a.nops, FRONT_WG), FRONT_WG, 0, s>>>(a.ct, reinterpret_cast<const uint4 *>(a.F), a.S,
+                                                                    a.nops);
+  return hipGetLastError();
+}
+template <int SET>
+hipError_t hash_only(const DecArgs &a, hipStream_t s) {
+  k_hash_msg<SET><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, a.ct, a.key, a.status, a.nops);
+  return hipGetLastError();
+}
+hipError_t launch_dec_front_only(int set, const DecArgs &a, hipStream_t s) {
+  switch (set) {
+    case S348864: return front_only<S348864>(a, s);
+    case S460896: return front_only<S460896>(a, s);
+    case S6688128: return front_only<S6688128>(a, s);
+    case S6960119: return front_only<S6960119>(a, s);
+    case S8192128: return front_only<S8192128>(a, s);
+  }
+  return hipErrorInvalidValue;
+}
+hipError_t launch_dec_hash_only(int set, const DecArgs &a, hipStream_t s) {
+  switch (set) {
+    case S348864: return hash_only<S348864>(a, s);
+    case S460896: return hash_only<S460896>(a, s);
+    case S6688128: return hash_only<S6688128>(a, s);
+    case S6960119: return hash_only<S6960119>(a, s);
+    case S8192128: return hash_only<S8192128>(a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_dec_set(int set, const DecArgs &a, hipStream_t s) {
   switch (set) {
     case S348864: return launch_dec<S348864>(a, s);

@@ -50,7 +50,30 @@ struct DecArgs {
   int decode_blocks;     // grid of the decoder (0 = default)
 };
 
+// Plain one-launch-per-stage form (baseline).
+struct PlainEncArgs {
+  const uint8_t *seeds;
+  const uint32_t *Trows;  // key: public key T as MT rows of ceil(K/32) words
+  uint8_t *ct;
+  uint8_t *key;
+  int *status;
+  uint32_t *erow;  // scratch [n][ceil(N/32)]
+  uint32_t *Ens;   // scratch [ceil(K/32)][n]
+  uint32_t *S;     // scratch [CW][n]
+  uint32_t *msg;   // scratch [n][MSG_WORDS]
+  int nops;
+};
+struct PlainDecScratch {
+  uint16_t *loc;   // [n][T+1]
+  uint32_t *erow;  // [n][2*ceil(N/64)]
+  int *weight;     // [n]
+};
+
 hipError_t launch_enc_set(int set, const EncArgs &a, hipStream_t s);
+hipError_t launch_plain_enc_set(int set, const PlainEncArgs &a, hipStream_t s);
+hipError_t launch_plain_dec_set(int set, const DecArgs &a, const PlainDecScratch &x, hipStream_t s);
+hipError_t launch_dec_front_only(int set, const DecArgs &a, hipStream_t s);
+hipError_t launch_dec_hash_only(int set, const DecArgs &a, hipStream_t s);
 hipError_t launch_dec_set(int set, const DecArgs &a, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_i32(int *p, int v, size_t n, hipStream_t s);

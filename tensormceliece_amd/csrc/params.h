@@ -42,6 +42,7 @@ template <int SET> struct Par : SetBase<SET> {
   static constexpr int NDRAW = (SET == S8192128) ? T : 2 * T;
   static constexpr int DRAW_BYTES = 2 * NDRAW;
   static constexpr int DRAW_BLOCKS = ceil_div(DRAW_BYTES, 136);
+  static constexpr int TP = next_pow2(T);            // sort width of the plain per-thread sampler
   static constexpr int IND_STRIDE = round_up(T, 2);  // u16 entries per op of the support list
   static constexpr int MAX_ATTEMPTS = 256;           // P(256 rejections) < 2^-128 for every set
   // syndrome columns: column p (0 <= p < N) is the unit vector e_p for p < MT and column p-MT of

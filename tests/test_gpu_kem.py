@@ -128,3 +128,24 @@ def test_single_op_api(tm, orc, s):
     assert r2 == 0 and np.array_equal(k, k2)
     k_ref, _ = orc.dec_batch(s, sk, ct[None, :])
     assert np.array_equal(k2, k_ref[0])
+
+
+@pytest.mark.parametrize("s", SETS)
+def test_plain_form_parity(tm, orc, s):
+    """The plain one-launch-per-stage form gives the same bytes as the oracle (and the engine)."""
+    pk, sk, _ = keys(orc, s)
+    seeds = seeds_for(orc, s, 192)
+    ct_ref, k_ref, _ = orc.enc_batch(s, pk, seeds)
+    rng = np.random.default_rng(5)
+    rnd = rng.integers(0, 256, size=(64, orc.params(s).synd_bytes), dtype=np.uint8)
+    print(f"plain form {s}", flush=True)
+    tm.set_form("plain")
+    try:
+        ct, k, st = tm.PublicKey(s, pk).enc(seeds)
+        k2, st2 = tm.SecretKey(s, sk).dec(np.concatenate([ct_ref, rnd]))
+    finally:
+        tm.set_form("engine")
+    assert not st.any()
+    assert np.array_equal(ct, ct_ref) and np.array_equal(k, k_ref)
+    k2_ref, ret = orc.dec_batch(s, sk, np.concatenate([ct_ref, rnd]))
+    assert np.array_equal(k2, k2_ref) and np.array_equal(st2, ret)
