@@ -136,6 +136,7 @@ void *tmce_dev_alloc(size_t bytes);
 void tmce_dev_free(void *p);
 int tmce_memcpy_h2d(void *d, const void *h, size_t bytes);
 int tmce_memcpy_d2h(void *h, const void *d, size_t bytes);
+int tmce_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int tmce_sync(void);
 
 /* kernel timing: while enabled, every kernel launch of the engine is bracketed by HIP events on

@@ -58,6 +58,7 @@ def load():
         "tmce_dev_free": (V, [P]),
         "tmce_memcpy_h2d": (I, [P, P, S]),
         "tmce_memcpy_d2h": (I, [P, P, S]),
+        "tmce_memcpy_d2d": (I, [P, P, S]),
         "tmce_sync": (I, []),
         "tmce_timing_enable": (I, [I]),
         "tmce_set_form": (I, [I]),
@@ -174,6 +175,9 @@ class DeviceBuffer:
         out = np.zeros(shape, dtype)
         _check(load().tmce_memcpy_d2h(_ptr(out), self.ptr, out.nbytes), "d2h")
         return out
+
+    def copy_from(self, other, nbytes):
+        _check(load().tmce_memcpy_d2d(self.ptr, other.ptr, nbytes), "d2d")
 
     def free(self):
         if self.ptr:

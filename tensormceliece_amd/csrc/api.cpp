@@ -483,6 +483,13 @@ int tmce_memcpy_d2h(void *h, const void *d, size_t bytes) {
   HIPCHK(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
   return 0;
 }
+int tmce_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return 0;
+}
 int tmce_sync(void) {
   Engine *e = engine();
   if (!e) return TMCE_ENODEV;
