@@ -125,6 +125,37 @@ int tmce_enc_batch_device(tmce_key *pk, unsigned char *d_ct, unsigned char *d_k,
 int tmce_dec_batch_device(tmce_key *sk, unsigned char *d_k, int *d_status, const unsigned char *d_ct,
                           size_t n, void *stream);
 
+/* batched syndrome (PKE encryption of given error vectors): ct_i = H e_i = e_i[0:mt) + T e_i[mt:n)
+   over GF(2), bits >= mt of the last byte zero.  Replaces libmceliece's internal syndrome()
+   (LM:crypto_kem/<set>/vec/encrypt.c:94-132) and the reference's batched syndrome GEMM + finalize
+   (mceliece-gpu-bmma/tc_kernels.cuh:97-147, int8_tensor_enc.h:60-102).  e: n vectors of n_bits/8
+   bytes in libmceliece's layout, any weight; ct: n * CIPHERTEXTBYTES.  Needs a public-key handle. */
+enum {
+  TMCE_SYND_SPARSE = 0, /* XOR of the columns of [I|T] at the set bits of e (engine default) */
+  TMCE_SYND_MFMA = 1,   /* dense GF(2) GEMM on INT8 MFMA over 0/1 bytes, int32 accumulate, parity */
+  TMCE_SYND_VALU = 2    /* dense bit-packed GEMM: 32-bit AND + XOR, popcount parity */
+};
+int tmce_syndrome_batch(tmce_key *pk, unsigned char *ct, const unsigned char *e, size_t n, int method);
+int tmce_syndrome_batch_device(tmce_key *pk, unsigned char *d_ct, const unsigned char *d_e, size_t n,
+                               int method, void *stream);
+
+/* tuning knobs per parameter set (set by the autotuner, tensormceliece_amd.autotune):
+   TMCE_TUNE_SYND_MFMA_CFG  tile shape of the MFMA syndrome GEMM (0..3)
+   TMCE_TUNE_GEN_BLOCKS     persistent grid of the sampler (0 = built-in default)
+   TMCE_TUNE_DECODE_BLOCKS  grid of the decoder (0 = built-in default)
+   TMCE_TUNE_FRONT_MFMA     decapsulation front end S = F c: 1 = INT8 MFMA GEMM (default), 0 = VALU
+   TMCE_TUNE_FRONT_CFG      tile shape of the MFMA front end (0..3)
+   tmce_tune_get returns the value (>= 0) or a negative error code. */
+enum {
+  TMCE_TUNE_SYND_MFMA_CFG = 0,
+  TMCE_TUNE_GEN_BLOCKS = 1,
+  TMCE_TUNE_DECODE_BLOCKS = 2,
+  TMCE_TUNE_FRONT_MFMA = 3,
+  TMCE_TUNE_FRONT_CFG = 4
+};
+int tmce_tune_set(int set, int param, int value);
+int tmce_tune_get(int set, int param);
+
 /* select the pipeline used by subsequent batched calls: 0 = engine form (default: fused,
    persistent sampler, row-parallel decoder), 1 = plain one-launch-per-stage form (the baseline
    mirroring the reference pipelines, mceliece-gpu-params/main.cu:213-262,326-370).  Both give

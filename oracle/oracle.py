@@ -44,6 +44,7 @@ def lib():
             ("oracle_dec", [I, P, P, P], I),
             ("oracle_enc_batch", [I, P, P, P, P, P, LL, I], I),
             ("oracle_dec_batch", [I, P, P, P, P, LL, I], I),
+            ("oracle_syndrome_batch", [I, P, P, P, LL, I], I),
             ("oracle_shake256", [P, ctypes.c_longlong, P, ctypes.c_longlong], None),
             ("oracle_gf_mul", [I, ctypes.c_uint16, ctypes.c_uint16], ctypes.c_uint16),
             ("oracle_gf_inv", [I, ctypes.c_uint16], ctypes.c_uint16),
@@ -104,6 +105,17 @@ def dec_batch(set_name, sk, ct, nthreads=0):
     sk = np.ascontiguousarray(sk, np.uint8)
     lib().oracle_dec_batch(SET_IDS[set_name], _p(k), _p(ret), _p(ct), _p(sk), n, nthreads)
     return k, ret
+
+
+def syndrome_batch(set_name, pk, e, nthreads=0):
+    """c_i = e_i[0:mt) + T e_i[mt:n) for rows of e (n/8 bytes each)."""
+    p = params(set_name)
+    e = np.ascontiguousarray(e, np.uint8).reshape(-1, p.n // 8)
+    n = e.shape[0]
+    ct = np.zeros((n, p.synd_bytes), np.uint8)
+    pk = np.ascontiguousarray(pk, np.uint8)
+    lib().oracle_syndrome_batch(SET_IDS[set_name], _p(ct), _p(pk), _p(e), n, nthreads)
+    return ct
 
 
 def kat_checksum(set_name, loops):

@@ -845,7 +845,10 @@ typedef struct {
 static void *job_run(void *arg) {
   job_t *j = (job_t *)arg;
   const orc_params *p = &PARAMS[j->set];
-  if (j->is_dec) {
+  if (j->is_dec == 2) { /* syndrome of given error vectors (seeds holds e) */
+    for (long i = j->lo; i < j->hi; i++)
+      syndrome_pk(j->set, j->ct + (size_t)i * p->synd_bytes, j->pk, j->seeds + (size_t)i * (p->n / 8));
+  } else if (j->is_dec) {
     dec_key K;
     dec_key_load(j->set, j->sk, &K);
     for (long i = j->lo; i < j->hi; i++) {
@@ -890,6 +893,11 @@ static int run_jobs(job_t *proto, long n, int nthreads) {
 int oracle_enc_batch(int set, uint8_t *ct, uint8_t *key, int *ret, const uint8_t *pk,
                      const uint8_t *seeds, long n, int nthreads) {
   job_t j = {set, 0, ct, key, ret, pk, NULL, seeds, NULL, 0, 0};
+  return run_jobs(&j, n, nthreads);
+}
+
+int oracle_syndrome_batch(int set, uint8_t *ct, const uint8_t *pk, const uint8_t *e, long n, int nthreads) {
+  job_t j = {set, 2, ct, NULL, NULL, pk, NULL, e, NULL, 0, 0};
   return run_jobs(&j, n, nthreads);
 }
 

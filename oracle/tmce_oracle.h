@@ -60,6 +60,8 @@ int oracle_enc_batch(int set, uint8_t *ct, uint8_t *key, int *ret, const uint8_t
                      const uint8_t *seeds, long n, int nthreads);
 int oracle_dec_batch(int set, uint8_t *key, int *ret, const uint8_t *ct, const uint8_t *sk,
                      long n, int nthreads);
+/* c_i = e_i[0:mt) + T e_i[mt:n) (LM:crypto_kem/<set>/vec/encrypt.c syndrome), e: n/8 bytes each */
+int oracle_syndrome_batch(int set, uint8_t *ct, const uint8_t *pk, const uint8_t *e, long n, int nthreads);
 
 /* stage-level functions (for stage parity tests) */
 void oracle_shake256(uint8_t *out, long long outlen, const uint8_t *in, long long inlen);

@@ -24,6 +24,10 @@ PARAMS = {
     "8192128": dict(m=13, n=8192, t=128, pk=1357824, sk=14120, ct=208),
 }
 KEY_BYTES = 32
+# syndrome methods (tmce_syndrome_batch)
+SYND_METHODS = {"sparse": 0, "mfma": 1, "valu": 2}
+# tuning parameters (tmce_tune_set / tmce_tune_get)
+TUNE_PARAMS = {"synd_mfma_cfg": 0, "gen_blocks": 1, "decode_blocks": 2, "front_mfma": 3, "front_cfg": 4}
 
 
 class TmceError(RuntimeError):
@@ -62,6 +66,10 @@ def load():
         "tmce_sync": (I, []),
         "tmce_timing_enable": (I, [I]),
         "tmce_set_form": (I, [I]),
+        "tmce_syndrome_batch": (I, [P, P, P, S, I]),
+        "tmce_syndrome_batch_device": (I, [P, P, P, S, I, P]),
+        "tmce_tune_set": (I, [I, I, I]),
+        "tmce_tune_get": (I, [I, I]),
         "tmce_timing_read": (I, [I, P, P, P]),
     }
     for name, (res, args) in sig.items():
@@ -142,6 +150,16 @@ class PublicKey(_Key):
         return ct, k, st
 
 
+    def syndrome(self, e, method="sparse"):
+        """ct_i = H e_i for rows of e (n/8 bytes each, any weight) -- the PKE encryption."""
+        e = np.ascontiguousarray(e, dtype=np.uint8).reshape(-1, self.params["n"] // 8)
+        n = e.shape[0]
+        ct = np.zeros((n, self.params["ct"]), np.uint8)
+        _check(load().tmce_syndrome_batch(self._h, _ptr(ct), _ptr(e), n, SYND_METHODS.get(method, method)),
+               "tmce_syndrome_batch")
+        return ct
+
+
 class SecretKey(_Key):
     """Device-resident secret key; dec(ct) runs batched decapsulation."""
 
@@ -199,6 +217,22 @@ def enc_device(pk, d_ct, d_k, d_status, d_seeds, n, stream=None):
 def dec_device(sk, d_k, d_status, d_ct, n, stream=None):
     _check(load().tmce_dec_batch_device(sk.handle, d_k.ptr, d_status.ptr if d_status else None, d_ct.ptr, n,
                                         stream), "tmce_dec_batch_device")
+
+
+def syndrome_device(pk, d_ct, d_e, n, method="sparse", stream=None):
+    _check(load().tmce_syndrome_batch_device(pk.handle, d_ct.ptr, d_e.ptr, n, SYND_METHODS.get(method, method),
+                                             stream), "tmce_syndrome_batch_device")
+
+
+def tune_set(set_name, param, value):
+    _check(load().tmce_tune_set(SETS[set_name], TUNE_PARAMS[param], int(value)), f"tmce_tune_set({param})")
+
+
+def tune_get(set_name, param):
+    v = load().tmce_tune_get(SETS[set_name], TUNE_PARAMS[param])
+    if v < 0:
+        _check(v, f"tmce_tune_get({param})")
+    return v
 
 
 def set_form(form):

@@ -29,7 +29,10 @@ struct tmce_key {
   bool pad_bad = false;
   uint32_t *cols = nullptr;
   uint32_t *Trows = nullptr;  // plain form: T as MT rows of ceil(K/32) words
+  uint8_t *Tb = nullptr;      // syndrome GEMM operands, built on first use: 0/1 bytes (MFMA) ...
+  uint32_t *Tv = nullptr;     // ... and 32-row blocks of words (VALU)
   uint32_t *F = nullptr;
+  uint8_t *Fsw = nullptr;     // F as 0/1 bytes in MFMA fragment order (INT8 MFMA front end)
   uint16_t *alpha = nullptr, *la = nullptr, *lu = nullptr;
   uint8_t *rej = nullptr;
 };
@@ -109,6 +112,11 @@ struct Engine {
   DevBuf seeds, ct, key, status, ind, msg, S, attempts, queue;
   DevBuf p_erow, p_ens, p_s, p_loc, p_weight;  // plain-form scratch
   int form = 0;                                // 0 = engine form, 1 = plain per-stage form
+  int synd_cfg[NUM_SETS] = {1, 1, 1, 1, 1};    // MFMA syndrome tile shape per set (tunable)
+  int gen_blocks[NUM_SETS] = {0, 0, 0, 0, 0};  // sampler grid per set (0 = built-in default)
+  int decode_blocks[NUM_SETS] = {0, 0, 0, 0, 0};
+  int front_mfma[NUM_SETS] = {1, 1, 1, 1, 1};  // decapsulation front end: 1 = INT8 MFMA, 0 = VALU
+  int front_cfg[NUM_SETS] = {1, 1, 1, 1, 1};   // its MFMA tile shape
   EventTimer timer;
 };
 
@@ -242,7 +250,7 @@ int enc_device(tmce_key *k, uint8_t *d_ct, uint8_t *d_k, int *d_status, const ui
   a.attempts = nullptr;
   a.queue = e->queue.as<unsigned>();
   a.nops = (int)n;
-  a.gen_blocks = 0;
+  a.gen_blocks = e->gen_blocks[k->set];
   HIPCHK(launch_enc_set(k->set, a, s));
   return 0;
 }
@@ -263,6 +271,8 @@ int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, si
   DecArgs a{};
   a.ct = d_ct;
   a.F = k->F;
+  a.Fsw = e->front_mfma[k->set] ? k->Fsw : nullptr;
+  a.front_cfg = e->front_cfg[k->set];
   a.alpha = k->alpha;
   a.la = k->la;
   a.lu = k->lu;
@@ -275,7 +285,7 @@ int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, si
   a.S = e->S.as<uint32_t>();
   a.msg = e->msg.as<uint32_t>();
   a.nops = (int)n;
-  a.decode_blocks = 0;
+  a.decode_blocks = e->decode_blocks[k->set];
   if (e->form == 1) {
     const int NEW2 = 2 * ((si.n + 63) / 64);
     if (int r = e->p_loc.ensure(n * (si.t + 1) * 2)) return r;
@@ -286,6 +296,39 @@ int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, si
     return 0;
   }
   HIPCHK(launch_dec_set(k->set, a, s));
+  return 0;
+}
+
+// operands of the dense syndrome forms, built from the T rows on first use
+int ensure_synd_key(Engine *e, tmce_key *k, int method, hipStream_t s) {
+  const SetInfo &si = set_info(k->set);
+  if (method == TMCE_SYND_MFMA && !k->Tb) {
+    const size_t nb = gf2_msw_bytes(k->set, GF2_SYND);
+    if (hipMalloc((void **)&k->Tb, nb) != hipSuccess) return fail(TMCE_ENOMEM, "hipMalloc Tb");
+    HIPCHK(launch_build_msw(k->set, GF2_SYND, k->Trows, (si.k + 31) / 32, k->Tb, s));
+    k->bytes += nb;
+  }
+  if (method == TMCE_SYND_VALU && !k->Tv) {
+    const size_t nv = (size_t)((si.mt + 31) / 32) * ((si.k + 31) / 32) * 32;
+    if (hipMalloc((void **)&k->Tv, nv * 4) != hipSuccess) return fail(TMCE_ENOMEM, "hipMalloc Tv");
+    HIPCHK(launch_build_tv(k->set, k->Trows, k->Tv, s));
+    k->bytes += nv * 4;
+  }
+  (void)e;
+  return 0;
+}
+
+int synd_device(tmce_key *k, uint8_t *d_ct, const uint8_t *d_e, size_t n, int method, hipStream_t s) {
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  if (!k || k->kind != 1) return fail(TMCE_EINVAL, "the syndrome needs a public-key handle");
+  if (method < TMCE_SYND_SPARSE || method > TMCE_SYND_VALU) return fail(TMCE_EINVAL, "unknown syndrome method");
+  if (n == 0) return 0;
+  if (n > (size_t)INT32_MAX / 2) return fail(TMCE_EINVAL, "batch too large");
+  if (!s) s = e->stream;
+  if (int r = ensure_synd_key(e, k, method, s)) return r;
+  SyndArgs a{d_e, k->Tb, k->Tv, k->cols, d_ct, (int)n};
+  HIPCHK(launch_synd_set(k->set, a, method, e->synd_cfg[k->set], s));
   return 0;
 }
 
@@ -373,11 +416,19 @@ int tmce_key_from_sk(int set, const unsigned char *sk, tmce_key **out) {
   DecKeyHost h;
   build_dec_key(set, sk, h);
   if (int r = upload(&k->F, h.F)) return r;
+  const size_t fsw = gf2_msw_bytes(set, GF2_FRONT);
+  if (hipMalloc((void **)&k->Fsw, fsw) != hipSuccess) return fail(TMCE_ENOMEM, "hipMalloc Fsw");
+  {
+    const SetInfo &si = set_info(set);
+    const int kwp = (((si.synd_bytes + 3) / 4) + 3) / 4 * 4;
+    HIPCHK(launch_build_msw(set, GF2_FRONT, k->F, kwp, k->Fsw, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+  }
   if (int r = upload(&k->alpha, h.alpha)) return r;
   if (int r = upload(&k->la, h.la)) return r;
   if (int r = upload(&k->lu, h.lu)) return r;
   if (int r = upload(&k->rej, h.rej)) return r;
-  k->bytes = h.F.size() * 4 + (h.alpha.size() + h.la.size() + h.lu.size()) * 2 + h.rej.size();
+  k->bytes = h.F.size() * 4 + fsw + (h.alpha.size() + h.la.size() + h.lu.size()) * 2 + h.rej.size();
   *out = k.release();
   return 0;
 }
@@ -390,7 +441,10 @@ void tmce_key_free(tmce_key *k) {
   }
   (void)hipFree(k->cols);
   (void)hipFree(k->Trows);
+  (void)hipFree(k->Tb);
+  (void)hipFree(k->Tv);
   (void)hipFree(k->F);
+  (void)hipFree(k->Fsw);
   (void)hipFree(k->alpha);
   (void)hipFree(k->la);
   (void)hipFree(k->lu);
@@ -466,6 +520,77 @@ int tmce_dec_batch(tmce_key *sk, unsigned char *k, int *status, const unsigned c
   if (status) HIPCHK(hipMemcpyAsync(status, e->status.p, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
   return 0;
+}
+
+int tmce_syndrome_batch_device(tmce_key *pk, unsigned char *d_ct, const unsigned char *d_e, size_t n, int method,
+                               void *stream) {
+  if (!pk || (n && (!d_ct || !d_e))) return fail(TMCE_EINVAL, "tmce_syndrome_batch_device: bad argument");
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  std::lock_guard<std::mutex> lk(e->mu);
+  return synd_device(pk, d_ct, d_e, n, method, static_cast<hipStream_t>(stream));
+}
+
+int tmce_syndrome_batch(tmce_key *pk, unsigned char *ct, const unsigned char *ev, size_t n, int method) {
+  if (!pk || (n && (!ct || !ev))) return fail(TMCE_EINVAL, "tmce_syndrome_batch: bad argument");
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  if (n == 0) return 0;
+  if (pk->kind != 1) return fail(TMCE_EINVAL, "the syndrome needs a public-key handle");
+  std::lock_guard<std::mutex> lk(e->mu);
+  const SetInfo &si = set_info(pk->set);
+  if (int r = e->seeds.ensure(n * (si.n / 8))) return r;  // error vectors
+  if (int r = e->ct.ensure(n * si.synd_bytes)) return r;
+  HIPCHK(hipMemcpyAsync(e->seeds.p, ev, n * (si.n / 8), hipMemcpyHostToDevice, e->stream));
+  if (int r = synd_device(pk, e->ct.as<uint8_t>(), e->seeds.as<uint8_t>(), n, method, e->stream)) return r;
+  HIPCHK(hipMemcpyAsync(ct, e->ct.p, n * si.synd_bytes, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+int tmce_tune_set(int set, int param, int value) {
+  if (!valid_set(set)) return fail(TMCE_EINVAL, "tmce_tune_set: bad set");
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  std::lock_guard<std::mutex> lk(e->mu);
+  switch (param) {
+    case TMCE_TUNE_SYND_MFMA_CFG:
+      if (value < 0 || value >= SYND_MFMA_CFGS) return fail(TMCE_EINVAL, "synd_mfma_cfg out of range");
+      e->synd_cfg[set] = value;
+      return 0;
+    case TMCE_TUNE_GEN_BLOCKS:
+      if (value < 0 || value > (1 << 20)) return fail(TMCE_EINVAL, "gen_blocks out of range");
+      e->gen_blocks[set] = value;
+      return 0;
+    case TMCE_TUNE_DECODE_BLOCKS:
+      if (value < 0 || value > (1 << 20)) return fail(TMCE_EINVAL, "decode_blocks out of range");
+      e->decode_blocks[set] = value;
+      return 0;
+    case TMCE_TUNE_FRONT_MFMA:
+      if (value != 0 && value != 1) return fail(TMCE_EINVAL, "front_mfma must be 0 or 1");
+      e->front_mfma[set] = value;
+      return 0;
+    case TMCE_TUNE_FRONT_CFG:
+      if (value < 0 || value >= SYND_MFMA_CFGS) return fail(TMCE_EINVAL, "front_cfg out of range");
+      e->front_cfg[set] = value;
+      return 0;
+  }
+  return fail(TMCE_EINVAL, "tmce_tune_set: unknown parameter");
+}
+
+int tmce_tune_get(int set, int param) {
+  if (!valid_set(set)) return fail(TMCE_EINVAL, "tmce_tune_get: bad set");
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  std::lock_guard<std::mutex> lk(e->mu);
+  switch (param) {
+    case TMCE_TUNE_SYND_MFMA_CFG: return e->synd_cfg[set];
+    case TMCE_TUNE_GEN_BLOCKS: return e->gen_blocks[set];
+    case TMCE_TUNE_DECODE_BLOCKS: return e->decode_blocks[set];
+    case TMCE_TUNE_FRONT_MFMA: return e->front_mfma[set];
+    case TMCE_TUNE_FRONT_CFG: return e->front_cfg[set];
+  }
+  return fail(TMCE_EINVAL, "tmce_tune_get: unknown parameter");
 }
 
 void *tmce_dev_alloc(size_t bytes) {

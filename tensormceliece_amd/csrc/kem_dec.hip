@@ -400,9 +400,14 @@ template <int SET>
 hipError_t launch_dec(const DecArgs &a, hipStream_t s) {
   using P = Par<SET>;
   if (a.nops <= 0) return hipSuccess;
-  TMCE_TIMED("dec_front", s,
-             k_dec_front<SET><<<ceil_div(a.nops, FRONT_WG), FRONT_WG, 0, s>>>(
-                 a.ct, reinterpret_cast<const uint4 *>(a.F), a.S, a.nops));
+  if (a.Fsw) {
+    const hipError_t e = launch_front_mfma_set(SET, a.ct, a.Fsw, a.S, a.nops, a.front_cfg, s);
+    if (e != hipSuccess) return e;
+  } else {
+    TMCE_TIMED("dec_front", s,
+               k_dec_front<SET><<<ceil_div(a.nops, FRONT_WG), FRONT_WG, 0, s>>>(
+                   a.ct, reinterpret_cast<const uint4 *>(a.F), a.S, a.nops));
+  }
   int grid = a.decode_blocks > 0 ? a.decode_blocks : 256;
   const int maxgrid = ceil_div(a.nops, DEC_WAVES * 4);
   if (grid > maxgrid) grid = maxgrid;
