@@ -140,11 +140,11 @@ int tmce_syndrome_batch_device(tmce_key *pk, unsigned char *d_ct, const unsigned
                                int method, void *stream);
 
 /* tuning knobs per parameter set (set by the autotuner, tensormceliece_amd.autotune):
-   TMCE_TUNE_SYND_MFMA_CFG  tile shape of the MFMA syndrome GEMM (0..3)
+   TMCE_TUNE_SYND_MFMA_CFG  tile shape of the MFMA syndrome GEMM (0..5)
    TMCE_TUNE_GEN_BLOCKS     persistent grid of the sampler (0 = built-in default)
    TMCE_TUNE_DECODE_BLOCKS  grid of the decoder (0 = built-in default)
    TMCE_TUNE_FRONT_MFMA     decapsulation front end S = F c: 1 = INT8 MFMA GEMM (default), 0 = VALU
-   TMCE_TUNE_FRONT_CFG      tile shape of the MFMA front end (0..3)
+   TMCE_TUNE_FRONT_CFG      tile shape of the MFMA front end (0..5)
    tmce_tune_get returns the value (>= 0) or a negative error code. */
 enum {
   TMCE_TUNE_SYND_MFMA_CFG = 0,

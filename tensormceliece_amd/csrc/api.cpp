@@ -111,6 +111,7 @@ struct Engine {
   SetTables st[NUM_SETS];
   DevBuf seeds, ct, key, status, ind, msg, S, attempts, queue;
   DevBuf p_erow, p_ens, p_s, p_loc, p_weight;  // plain-form scratch
+  DevBuf xscratch;                             // MFMA products: x rows repacked to aligned words
   int form = 0;                                // 0 = engine form, 1 = plain per-stage form
   int synd_cfg[NUM_SETS] = {1, 1, 1, 1, 1};    // MFMA syndrome tile shape per set (tunable)
   int gen_blocks[NUM_SETS] = {0, 0, 0, 0, 0};  // sampler grid per set (0 = built-in default)
@@ -273,6 +274,14 @@ int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, si
   a.F = k->F;
   a.Fsw = e->front_mfma[k->set] ? k->Fsw : nullptr;
   a.front_cfg = e->front_cfg[k->set];
+  a.xscratch = nullptr;
+  if (a.Fsw) {
+    const size_t xb = gf2_scratch_bytes(k->set, GF2_FRONT, n);
+    if (xb) {
+      if (int r = e->xscratch.ensure(xb)) return r;
+      a.xscratch = e->xscratch.as<uint32_t>();
+    }
+  }
   a.alpha = k->alpha;
   a.la = k->la;
   a.lu = k->lu;
@@ -327,7 +336,15 @@ int synd_device(tmce_key *k, uint8_t *d_ct, const uint8_t *d_e, size_t n, int me
   if (n > (size_t)INT32_MAX / 2) return fail(TMCE_EINVAL, "batch too large");
   if (!s) s = e->stream;
   if (int r = ensure_synd_key(e, k, method, s)) return r;
-  SyndArgs a{d_e, k->Tb, k->Tv, k->cols, d_ct, (int)n};
+  uint32_t *xs = nullptr;
+  if (method == TMCE_SYND_MFMA) {
+    const size_t xb = gf2_scratch_bytes(k->set, GF2_SYND, n);
+    if (xb) {
+      if (int r = e->xscratch.ensure(xb)) return r;
+      xs = e->xscratch.as<uint32_t>();
+    }
+  }
+  SyndArgs a{d_e, k->Tb, k->Tv, k->cols, d_ct, xs, (int)n};
   HIPCHK(launch_synd_set(k->set, a, method, e->synd_cfg[k->set], s));
   return 0;
 }

@@ -401,7 +401,7 @@ hipError_t launch_dec(const DecArgs &a, hipStream_t s) {
   using P = Par<SET>;
   if (a.nops <= 0) return hipSuccess;
   if (a.Fsw) {
-    const hipError_t e = launch_front_mfma_set(SET, a.ct, a.Fsw, a.S, a.nops, a.front_cfg, s);
+    const hipError_t e = launch_front_mfma_set(SET, a.ct, a.xscratch, a.Fsw, a.S, a.nops, a.front_cfg, s);
     if (e != hipSuccess) return e;
   } else {
     TMCE_TIMED("dec_front", s,

@@ -39,6 +39,7 @@ struct DecArgs {
   const uint32_t *F;     // key: [R][KWP] front-end matrix (plane-major rows)
   const uint8_t *Fsw;    // key: F as 0/1 bytes in MFMA fragment order (null: VALU front end)
   int front_cfg;         // MFMA tile shape of the front end
+  uint32_t *xscratch;    // gf2_scratch_bytes(set, GF2_FRONT, n) bytes (MFMA front end; null if 0)
   const uint16_t *alpha; // key: [N] support
   const uint16_t *la;    // key: [N] log alpha (0xFFFF for alpha = 0)
   const uint16_t *lu;    // key: [N] log 1/g(alpha)^2 (0xFFFF for 0)
@@ -78,11 +79,12 @@ struct SyndArgs {
   const uint32_t *Tv;    // key: T per 32-row block as [word][row] (VALU form), or null
   const uint32_t *cols;  // key: column-major [I | T] (sparse form)
   uint8_t *ct;           // [n][SYND_BYTES]
+  uint32_t *xscratch;    // gf2_scratch_bytes(set, GF2_SYND, n) bytes (MFMA form; null if 0)
   int nops;
 };
 // method: 0 = sparse column gather, 1 = INT8 MFMA GEMM (cfg = tile shape index), 2 = VALU GEMM
 hipError_t launch_synd_set(int set, const SyndArgs &a, int method, int cfg, hipStream_t s);
-constexpr int SYND_MFMA_CFGS = 4;
+constexpr int SYND_MFMA_CFGS = 6;
 // Swizzled 0/1-byte operands of the INT8 MFMA GF(2) products: GF2_SYND = the public key T
 // (MT x K), GF2_FRONT = the decapsulation front-end matrix F (R x KBITS).
 enum { GF2_SYND = 0, GF2_FRONT = 1 };
@@ -92,8 +94,10 @@ hipError_t launch_build_msw(int set, int which, const uint32_t *rows, int rw, ui
 // VALU operand Tv (ceil(MT/32) * ceil(K/32) * 32 words) from the T rows
 hipError_t launch_build_tv(int set, const uint32_t *Trows, uint32_t *Tv, hipStream_t s);
 // decapsulation front end S = F c on INT8 MFMA (same output layout as the VALU front end)
-hipError_t launch_front_mfma_set(int set, const uint8_t *ct, const uint8_t *Fsw, uint32_t *S, int nops, int cfg,
-                                 hipStream_t s);
+hipError_t launch_front_mfma_set(int set, const uint8_t *ct, uint32_t *scratch, const uint8_t *Fsw, uint32_t *S,
+                                 int nops, int cfg, hipStream_t s);
+// bytes of scratch the MFMA products need for n ops (x rows repacked to aligned words), 0 if none
+size_t gf2_scratch_bytes(int set, int which, size_t n);
 
 hipError_t launch_enc_set(int set, const EncArgs &a, hipStream_t s);
 hipError_t launch_plain_enc_set(int set, const PlainEncArgs &a, hipStream_t s);

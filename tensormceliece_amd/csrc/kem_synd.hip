@@ -75,13 +75,15 @@ __device__ __forceinline__ uint32_t e_head16(const uint8_t *__restrict__ erow, i
 //
 // Workgroup tile: BM = 16*OPW*WO ops x BN = 16*RTW*WR matrix rows, WO x WR waves, each wave
 // OPW x RTW 16x16 int32 tiles.  The k loop runs in super-steps of KSUB MFMA steps (64 deep each)
-// over a three-slot LDS ring:
-//   B (matrix): BN/16 fragments per step, copied HBM/L2 -> LDS by global_load_lds_dwordx4 (no
-//               registers), issued two super-steps ahead;
-//   A (x bits): BM x 64 bits per step, one 32-bit word per thread and step through a register,
-//               one super-step ahead, expanded to 0/1 bytes on the way into the MFMA A operand.
-// One barrier per super-step.  A wave reads its B fragments with ds_read_b128 (conflict-free: 16
-// contiguous bytes per lane) and its x slices as 16-bit words.
+// over a three-slot LDS ring filled two super-steps ahead by global_load_lds (HBM/L2 -> LDS with no
+// registers):
+//   B (matrix): BN/16 fragments (1 KB each) per step, 16 bytes per lane;
+//   A (x bits): BM x 64 bits per step, one aligned 32-bit word per lane, expanded to 0/1 bytes on
+//               the way into the MFMA A operand.
+// The copies are issued from inline assembly with explicit vmcnt waits (the compiler, which cannot
+// tell the ring slots apart, would otherwise wait for every outstanding copy before any LDS read).
+// Before the barrier ending super-step q a wave waits until only the copies it issued for q+2 are
+// outstanding, so slot q+1 is complete for every wave after the barrier.
 // Output: the D tile gives lane l column l%16 (a matrix row) of rows 4*(l/16)+v (ops); a wave ballot
 // of bit 0 per v gives, in 16-bit chunk g, 16 consecutive output bits of op 4g+v, stored as one
 // 16-bit word.
@@ -96,18 +98,34 @@ struct GemmShape {
   static constexpr int APT = 2 * BM / NT;           // x words per thread per step
   static constexpr int BSTEP = NF * 1024, ASTEP = BM * 8;
   static constexpr int SLOT = KSUB * (BSTEP + ASTEP);
+  static constexpr int NI = KSUB * (FPW + APT);     // copies one wave issues per super-step
   static constexpr size_t LDS = (size_t)3 * SLOT;
   static_assert(FPW >= 1 && FPW * WO * WR == NF, "fragments must split evenly over the waves");
   static_assert(APT >= 1 && APT * NT == 2 * BM, "x words must split evenly over the threads");
   static_assert(ceil_div(G::ROWS, BN) * NF <= G::RTP, "row blocks exceed the padded matrix");
+  static_assert(NI < 64, "vmcnt range");
 };
 
-__device__ __forceinline__ void lds_dma16(const void *g, void *l) {
-  __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)l, 16, 0, 0);
+__device__ __forceinline__ uint32_t lds_off(const void *p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
+}
+// LDS DMA: lane l's 16 (4) bytes at g land at LDS address m0 + 16 l (4 l); m0 wave-uniform.
+__device__ __forceinline__ void dma16(const void *g, uint32_t m0) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(m0) : "memory", "m0");
+}
+__device__ __forceinline__ void dma4(const void *g, uint32_t m0) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off" ::"v"(g), "s"(m0) : "memory", "m0");
+}
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 
+// xw: x as aligned 32-bit words, row stride xstride words, xvalid words per row (reads past it are
+// clamped to the last word: the matrix is zero there).  head: the original input rows (G::head16).
 template <class G, int OPW, int RTW, int WO, int WR, int KSUB>
-__global__ __launch_bounds__(64 * WO * WR) void k_gf2_mfma(const uint8_t *__restrict__ x,
+__global__ __launch_bounds__(64 * WO * WR) void k_gf2_mfma(const uint32_t *__restrict__ xw, int xstride, int xvalid,
+                                                          const uint8_t *__restrict__ head,
                                                           const uint4 *__restrict__ Msw,
                                                           uint8_t *__restrict__ out, int nops) {
   using S = GemmShape<G, OPW, RTW, WO, WR, KSUB>;
@@ -117,56 +135,41 @@ __global__ __launch_bounds__(64 * WO * WR) void k_gf2_mfma(const uint8_t *__rest
   const int wo = wave % WO, wr = wave / WO;
   const int op0 = blockIdx.x * S::BM;
   const int rt0 = blockIdx.y * S::NF;  // first row tile of the block
+  const uint32_t lds0 = lds_off(lds);
 
-  // B fragments of super-step q into a slot: the wave copies fragments wave*FPW .. +FPW (1 KB each)
-  auto issue_b = [&](int q, int slot) {
-#pragma unroll
-    for (int u = 0; u < KSUB; ++u) {
-      const int ks = q * KSUB + u;
-      if (ks < S::KS) {
-#pragma unroll
-        for (int f = 0; f < S::FPW; ++f) {
-          const int fr = wave * S::FPW + f;
-          lds_dma16(Msw + ((size_t)ks * G::RTP + rt0 + fr) * 64 + lane,
-                    lds + slot * S::SLOT + u * S::BSTEP + fr * 1024);
-        }
-      }
-    }
-  };
-  // x words of step ks: word i (i = tid + NT*p) is bits [64 ks + 32 (i%2), +32) of op i/2
-  const uint8_t *xrow[S::APT];
+  // x word i = tid + NT*p of a step is word 2 ks + i%2 of op i/2 (rows past the batch: last op)
+  const uint32_t *xrow[S::APT];
 #pragma unroll
   for (int p = 0; p < S::APT; ++p) {
     const int op = op0 + (tid + S::NT * p) / 2;
-    xrow[p] = x + (size_t)(op < nops ? op : nops - 1) * G::IN_BYTES;
+    xrow[p] = xw + (size_t)(op < nops ? op : nops - 1) * xstride;
   }
-  auto load_a = [&](int q, uint32_t (&r)[KSUB][S::APT]) {
+  auto issue = [&](int q, int slot) {
 #pragma unroll
     for (int u = 0; u < KSUB; ++u) {
-      const int ks = q * KSUB + u;
+      const int ks = min(q * KSUB + u, S::KS - 1);  // a short last super-step repeats a valid step
+      const uint32_t sb = lds0 + slot * S::SLOT + u * S::BSTEP;
 #pragma unroll
-      for (int p = 0; p < S::APT; ++p)
-        r[u][p] = ks < S::KS ? G::x_word(xrow[p], 2 * ks + ((tid + S::NT * p) & 1)) : 0u;
-    }
-  };
-  auto store_a = [&](int slot, const uint32_t (&r)[KSUB][S::APT]) {
+      for (int f = 0; f < S::FPW; ++f) {
+        const int fr = wave * S::FPW + f;
+        dma16(Msw + ((size_t)ks * G::RTP + rt0 + fr) * 64 + lane, __builtin_amdgcn_readfirstlane(sb + fr * 1024));
+      }
+      const uint32_t sa = lds0 + slot * S::SLOT + KSUB * S::BSTEP + u * S::ASTEP;
 #pragma unroll
-    for (int u = 0; u < KSUB; ++u) {
-      uint32_t *a = reinterpret_cast<uint32_t *>(lds + slot * S::SLOT + KSUB * S::BSTEP + u * S::ASTEP);
-#pragma unroll
-      for (int p = 0; p < S::APT; ++p) a[tid + S::NT * p] = r[u][p];
+      for (int p = 0; p < S::APT; ++p) {
+        const int w = min(2 * ks + ((tid + S::NT * p) & 1), xvalid - 1);
+        dma4(xrow[p] + w, __builtin_amdgcn_readfirstlane(sa + (wave * 64 + S::NT * p) * 4));
+      }
     }
   };
 
-  uint32_t areg[KSUB][S::APT];
-  issue_b(0, 0);
-  load_a(0, areg);
-  store_a(0, areg);
+  issue(0, 0);
   if (S::QS > 1) {
-    issue_b(1, 1);
-    load_a(1, areg);
+    issue(1, 1);
+    wait_vmcnt<S::NI>();
+  } else {
+    wait_vmcnt<0>();
   }
-  __builtin_amdgcn_s_waitcnt(0);  // slot 0 complete (prologue only: waits for the step-1 copies too)
   __syncthreads();
 
   v4i acc[OPW][RTW];
@@ -179,7 +182,8 @@ __global__ __launch_bounds__(64 * WO * WR) void k_gf2_mfma(const uint8_t *__rest
 #pragma unroll 1
   for (int q = 0; q < S::QS; ++q) {
     const int s1 = slot == 2 ? 0 : slot + 1, s2 = s1 == 2 ? 0 : s1 + 1;
-    if (q + 2 < S::QS) issue_b(q + 2, s2);
+    const bool ahead = q + 2 < S::QS;
+    if (ahead) issue(q + 2, s2);
 #pragma unroll
     for (int u = 0; u < KSUB; ++u) {
       if (KSUB == 1 || q * KSUB + u < S::KS) {
@@ -199,40 +203,50 @@ __global__ __launch_bounds__(64 * WO * WR) void k_gf2_mfma(const uint8_t *__rest
         }
       }
     }
-    if (q + 1 < S::QS) {
-      store_a(s1, areg);  // waits for this super-step's x loads, and with them the older B copies of q+1
-      if (q + 2 < S::QS) load_a(q + 2, areg);
-    }
+    if (ahead) wait_vmcnt<S::NI>();
+    else wait_vmcnt<0>();
     __syncthreads();
     slot = s1;
   }
 
-  // parity bits -> 16-bit output words
+  // parity bits -> 16-bit words in LDS ([op][row tile] per wave; the ring is free after the last
+  // barrier), then one op per lane: XOR the head bits and store the wave's words of the op's row
+  uint16_t *ep = reinterpret_cast<uint16_t *>(lds) + wave * (16 * OPW) * RTW;
 #pragma unroll
-  for (int j = 0; j < RTW; ++j) {
-    const int r0 = (rt0 + RTW * wr + j) * 16;
+  for (int j = 0; j < RTW; ++j)
 #pragma unroll
     for (int i = 0; i < OPW; ++i)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const uint64_t m = __ballot(acc[i][j][v] & 1);
-        const int op = op0 + 16 * OPW * wo + 16 * i + 4 * lg + v;
-        if (li == 0 && r0 < G::OUT_BITS && op < nops) {
-          uint32_t bits = (uint32_t)(m >> (16 * lg)) & 0xFFFFu;
-          bits ^= G::head16(x + (size_t)op * G::IN_BYTES, r0 / 16);
-          reinterpret_cast<uint16_t *>(out + (size_t)op * G::OUT_BYTES)[r0 / 16] = (uint16_t)bits;
-        }
+        if (li == 0) ep[(16 * i + 4 * lg + v) * RTW + j] = (uint16_t)(m >> (16 * lg));
       }
+  wave_lds_sync();
+#pragma unroll
+  for (int o = lane; o < 16 * OPW; o += 64) {
+    const int op = op0 + 16 * OPW * wo + o;
+    if (op < nops) {
+      const uint8_t *hrow = head + (size_t)op * G::IN_BYTES;
+      uint16_t *orow = reinterpret_cast<uint16_t *>(out + (size_t)op * G::OUT_BYTES);
+#pragma unroll
+      for (int j = 0; j < RTW; ++j) {
+        const int h = rt0 + RTW * wr + j;  // 16-bit word index of the output row
+        if (16 * h < G::OUT_BITS) orow[h] = (uint16_t)(ep[o * RTW + j] ^ G::head16(hrow, h));
+      }
+    }
   }
 }
 
-// Syndrome: rows = MT, x = e bits [MT, N), out = ciphertext (c = e[0:MT) + s).
+// Syndrome: rows = MT, x = e bits [MT, N), out = ciphertext (c = e[0:MT) + s).  When MT and N are
+// multiples of 32 the x words are read in place; otherwise (6960119) they are first repacked.
 template <int SET>
 struct GemmSynd {
   using P = Par<SET>;
   static constexpr int ROWS = P::MT, KB = P::K, KP = round_up(KB, 64);
   static constexpr int RTP = round_up(ceil_div(ROWS, 16), 16);
   static constexpr int IN_BYTES = P::E_BYTES, OUT_BYTES = P::SYND_BYTES, OUT_BITS = 8 * P::SYND_BYTES;
+  static constexpr bool DIRECT = P::MT % 32 == 0 && P::N % 32 == 0;
+  static constexpr int XOFF_WORDS = P::MT / 32, XSTRIDE = P::E_BYTES / 4, XVALID = P::K / 32;
   __device__ static uint32_t x_word(const uint8_t *row, int w) { return e_ns_word<P>(row, w); }
   __device__ static uint32_t head16(const uint8_t *row, int h) { return e_head16<P>(row, h); }
 };
@@ -245,6 +259,8 @@ struct GemmFront {
   static constexpr int ROWS = P::R, KB = P::KBITS, KP = round_up(KB, 64);
   static constexpr int RTP = round_up(ceil_div(ROWS, 16), 16);
   static constexpr int IN_BYTES = P::SYND_BYTES, OUT_BYTES = 4 * P::RW, OUT_BITS = P::R;
+  static constexpr bool DIRECT = P::SYND_BYTES % 4 == 0;
+  static constexpr int XOFF_WORDS = 0, XSTRIDE = P::SYND_BYTES / 4, XVALID = P::SYND_BYTES / 4;
   __device__ static uint32_t x_word(const uint8_t *row, int w) {  // ciphertext rows are 2-byte aligned
     const uint16_t *h = reinterpret_cast<const uint16_t *>(row);
     uint32_t v = 0;
@@ -255,6 +271,16 @@ struct GemmFront {
   __device__ static uint32_t head16(const uint8_t *, int) { return 0u; }
 };
 
+// x rows -> aligned words [n][KP/32] (for the sets where they are not aligned in place)
+template <class G>
+__global__ void k_repack_x(const uint8_t *__restrict__ x, uint32_t *__restrict__ xw, int nops) {
+  constexpr int W = G::KP / 32;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nops * W) return;
+  const int op = (int)(i / W), w = (int)(i % W);
+  xw[i] = G::x_word(x + (size_t)op * G::IN_BYTES, w);
+}
+
 template <class G>
 struct MswDims {
   static constexpr int KS = G::KP / 64;
@@ -262,8 +288,8 @@ struct MswDims {
 };
 
 template <class G, int OPW, int RTW, int WO, int WR, int KSUB>
-hipError_t launch_gf2_cfg(const uint8_t *x, const uint8_t *Msw, uint8_t *out, int nops, const char *name,
-                          hipStream_t s) {
+hipError_t launch_gf2_cfg(const uint32_t *xw, int xstride, int xvalid, const uint8_t *x, const uint8_t *Msw,
+                          uint8_t *out, int nops, const char *name, hipStream_t s) {
   using S = GemmShape<G, OPW, RTW, WO, WR, KSUB>;
   static const hipError_t attr = hipFuncSetAttribute(
       reinterpret_cast<const void *>(&k_gf2_mfma<G, OPW, RTW, WO, WR, KSUB>),
@@ -272,21 +298,39 @@ hipError_t launch_gf2_cfg(const uint8_t *x, const uint8_t *Msw, uint8_t *out, in
   dim3 grid(ceil_div(nops, S::BM), ceil_div(G::ROWS, S::BN));
   TMCE_TIMED(name, s,
              (k_gf2_mfma<G, OPW, RTW, WO, WR, KSUB><<<grid, S::NT, S::LDS, s>>>(
-                 x, reinterpret_cast<const uint4 *>(Msw), out, nops)));
+                 xw, xstride, xvalid, x, reinterpret_cast<const uint4 *>(Msw), out, nops)));
   return hipGetLastError();
 }
 
 // Tile shapes offered to the tuner: (OPW, RTW, WO, WR, KSUB) -> BM x BN workgroup tiles
 //   0: 256 x 128, 8 waves, 1 step per barrier     1: 256 x 128, 8 waves, 2 steps per barrier
-//   2: 256 x 256, 8 waves, 2 steps per barrier    3: 128 x 128, 4 waves, 2 steps per barrier
+//   2: 256 x 256, 8 waves, 2 steps per barrier    3: 256 x 256, 8 waves (128 ops each), 2 steps
+//   4: 256 x 128, 4 waves (128 ops each), 2 steps 5: 512 x 128, 8 waves (128 ops each), 2 steps
+// x: input rows (IN_BYTES each); scratch: n * KP/8 bytes when the x bits are not word aligned in
+// place (G::DIRECT false), else unused.
 template <class G>
-hipError_t launch_gf2(const uint8_t *x, const uint8_t *Msw, uint8_t *out, int nops, int cfg, const char *name,
-                      hipStream_t s) {
+hipError_t launch_gf2(const uint8_t *x, uint32_t *scratch, const uint8_t *Msw, uint8_t *out, int nops, int cfg,
+                      const char *name, hipStream_t s) {
+  const uint32_t *xw;
+  int xstride, xvalid;
+  if constexpr (G::DIRECT) {
+    xw = reinterpret_cast<const uint32_t *>(x) + G::XOFF_WORDS;
+    xstride = G::XSTRIDE;
+    xvalid = G::XVALID;
+  } else {
+    if (!scratch) return hipErrorInvalidValue;
+    const size_t nw = (size_t)nops * (G::KP / 32);
+    TMCE_TIMED("gf2_repack", s, k_repack_x<G><<<(unsigned)((nw + 255) / 256), 256, 0, s>>>(x, scratch, nops));
+    xw = scratch;
+    xstride = xvalid = G::KP / 32;
+  }
   switch (cfg) {
-    case 0: return launch_gf2_cfg<G, 4, 4, 4, 2, 1>(x, Msw, out, nops, name, s);
-    case 1: return launch_gf2_cfg<G, 4, 4, 4, 2, 2>(x, Msw, out, nops, name, s);
-    case 2: return launch_gf2_cfg<G, 4, 8, 4, 2, 2>(x, Msw, out, nops, name, s);
-    case 3: return launch_gf2_cfg<G, 4, 4, 2, 2, 2>(x, Msw, out, nops, name, s);
+    case 0: return launch_gf2_cfg<G, 4, 4, 4, 2, 1>(xw, xstride, xvalid, x, Msw, out, nops, name, s);
+    case 1: return launch_gf2_cfg<G, 4, 4, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, name, s);
+    case 2: return launch_gf2_cfg<G, 4, 8, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, name, s);
+    case 3: return launch_gf2_cfg<G, 8, 4, 2, 4, 2>(xw, xstride, xvalid, x, Msw, out, nops, name, s);
+    case 4: return launch_gf2_cfg<G, 8, 4, 2, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, name, s);
+    case 5: return launch_gf2_cfg<G, 8, 4, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, name, s);
   }
   return hipErrorInvalidValue;
 }
@@ -412,7 +456,7 @@ hipError_t launch_synd(const SyndArgs &a, int method, int cfg, hipStream_t s) {
                                                                        a.ct, a.nops));
       return hipGetLastError();
     case 1:
-      return launch_gf2<GemmSynd<SET>>(a.e, a.Tb, a.ct, a.nops, cfg, "synd_mfma", s);
+      return launch_gf2<GemmSynd<SET>>(a.e, a.xscratch, a.Tb, a.ct, a.nops, cfg, "synd_mfma", s);
     case 2: {
       using D = SyndValuDims<P>;
       const size_t lds = (size_t)D::OPS * D::ES * 4;
@@ -485,20 +529,33 @@ hipError_t launch_build_tv(int set, const uint32_t *Trows, uint32_t *Tv, hipStre
 }
 
 template <int SET>
-hipError_t front_mfma(const uint8_t *ct, const uint8_t *Fsw, uint32_t *S, int nops, int cfg, hipStream_t s) {
+hipError_t front_mfma(const uint8_t *ct, uint32_t *scratch, const uint8_t *Fsw, uint32_t *S, int nops, int cfg,
+                      hipStream_t s) {
   if (nops <= 0) return hipSuccess;
-  return launch_gf2<GemmFront<SET>>(ct, Fsw, reinterpret_cast<uint8_t *>(S), nops, cfg, "dec_front_mfma", s);
+  return launch_gf2<GemmFront<SET>>(ct, scratch, Fsw, reinterpret_cast<uint8_t *>(S), nops, cfg, "dec_front_mfma", s);
 }
-hipError_t launch_front_mfma_set(int set, const uint8_t *ct, const uint8_t *Fsw, uint32_t *S, int nops, int cfg,
-                                 hipStream_t s) {
+hipError_t launch_front_mfma_set(int set, const uint8_t *ct, uint32_t *scratch, const uint8_t *Fsw, uint32_t *S,
+                                 int nops, int cfg, hipStream_t s) {
   switch (set) {
-    case S348864: return front_mfma<S348864>(ct, Fsw, S, nops, cfg, s);
-    case S460896: return front_mfma<S460896>(ct, Fsw, S, nops, cfg, s);
-    case S6688128: return front_mfma<S6688128>(ct, Fsw, S, nops, cfg, s);
-    case S6960119: return front_mfma<S6960119>(ct, Fsw, S, nops, cfg, s);
-    case S8192128: return front_mfma<S8192128>(ct, Fsw, S, nops, cfg, s);
+    case S348864: return front_mfma<S348864>(ct, scratch, Fsw, S, nops, cfg, s);
+    case S460896: return front_mfma<S460896>(ct, scratch, Fsw, S, nops, cfg, s);
+    case S6688128: return front_mfma<S6688128>(ct, scratch, Fsw, S, nops, cfg, s);
+    case S6960119: return front_mfma<S6960119>(ct, scratch, Fsw, S, nops, cfg, s);
+    case S8192128: return front_mfma<S8192128>(ct, scratch, Fsw, S, nops, cfg, s);
   }
   return hipErrorInvalidValue;
+}
+
+size_t gf2_scratch_bytes(int set, int which, size_t n) {
+  switch (set) {
+#define TMCE_SCR(S)                                                                                   \
+  case S:                                                                                             \
+    if (which == GF2_SYND) return GemmSynd<S>::DIRECT ? 0 : n * (GemmSynd<S>::KP / 8);                \
+    return GemmFront<S>::DIRECT ? 0 : n * (GemmFront<S>::KP / 8);
+    TMCE_SCR(S348864) TMCE_SCR(S460896) TMCE_SCR(S6688128) TMCE_SCR(S6960119) TMCE_SCR(S8192128)
+#undef TMCE_SCR
+  }
+  return 0;
 }
 
 hipError_t launch_synd_set(int set, const SyndArgs &a, int method, int cfg, hipStream_t s) {
