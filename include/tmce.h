@@ -156,6 +156,33 @@ enum {
 int tmce_tune_set(int set, int param, int value);
 int tmce_tune_get(int set, int param);
 
+/* ---------------------------------------------------------------- multi-batch pipelines
+   Sub-batches of mixed sizes, under up to 16 distinct keys of one parameter set, run through one
+   set of launches (no counterpart in the reference, whose harness launches one fixed-size batch at
+   a time, mceliece-gpu-params/main.cu:213-262,326-370).  A pipeline has a fixed set, direction
+   (0 = encapsulation, 1 = decapsulation) and capacity in ops (each distinct key's ops are padded to
+   a multiple of 512); with use_graph the launch sequence is captured into a HIP graph at the second
+   run and replayed afterwards.  All buffers are device buffers; runs are asynchronous on the
+   pipeline's stream (its own non-blocking stream when `stream` is NULL); a run may start while the
+   previous one is still executing on that stream (they serialise on it). */
+typedef struct tmce_sub_batch {
+  tmce_key *key;            /* public key (encapsulation) or secret key (decapsulation) handle */
+  const unsigned char *in;  /* seeds [n*32] (enc) or ciphertexts [n*CIPHERTEXTBYTES] (dec) */
+  unsigned char *ct;        /* enc: ciphertexts out [n*CIPHERTEXTBYTES]; dec: unused */
+  unsigned char *k;         /* session keys out [n*32] */
+  int *status;              /* per-op status out [n], or NULL */
+  size_t n;
+} tmce_sub_batch;
+typedef struct tmce_pipeline tmce_pipeline;
+int tmce_pipeline_create(int set, int direction, size_t capacity, size_t max_sub_batches, int use_graph,
+                         void *stream, tmce_pipeline **out);
+int tmce_pipeline_run(tmce_pipeline *p, const tmce_sub_batch *subs, size_t nsub);
+int tmce_pipeline_sync(tmce_pipeline *p);
+void tmce_pipeline_free(tmce_pipeline *p);
+size_t tmce_pipeline_device_bytes(const tmce_pipeline *p); /* device memory it holds */
+size_t tmce_pipeline_capacity(const tmce_pipeline *p);     /* ops (rounded up to 512) */
+void *tmce_pipeline_stream(tmce_pipeline *p);              /* its hipStream_t */
+
 /* select the pipeline used by subsequent batched calls: 0 = engine form (default: fused,
    persistent sampler, row-parallel decoder), 1 = plain one-launch-per-stage form (the baseline
    mirroring the reference pipelines, mceliece-gpu-params/main.cu:213-262,326-370).  Both give

@@ -70,6 +70,13 @@ def load():
         "tmce_syndrome_batch_device": (I, [P, P, P, S, I, P]),
         "tmce_tune_set": (I, [I, I, I]),
         "tmce_tune_get": (I, [I, I]),
+        "tmce_pipeline_create": (I, [I, I, S, S, I, P, ctypes.POINTER(P)]),
+        "tmce_pipeline_run": (I, [P, P, S]),
+        "tmce_pipeline_sync": (I, [P]),
+        "tmce_pipeline_free": (V, [P]),
+        "tmce_pipeline_device_bytes": (S, [P]),
+        "tmce_pipeline_capacity": (S, [P]),
+        "tmce_pipeline_stream": (P, [P]),
         "tmce_timing_read": (I, [I, P, P, P]),
     }
     for name, (res, args) in sig.items():
@@ -233,6 +240,60 @@ def tune_get(set_name, param):
     if v < 0:
         _check(v, f"tmce_tune_get({param})")
     return v
+
+
+class SubBatchC(ctypes.Structure):
+    """tmce_sub_batch (include/tmce.h)."""
+    _fields_ = [("key", ctypes.c_void_p), ("inp", ctypes.c_void_p), ("ct", ctypes.c_void_p),
+                ("k", ctypes.c_void_p), ("status", ctypes.c_void_p), ("n", ctypes.c_size_t)]
+
+
+class Pipeline:
+    """Multi-batch pipeline (tmce_pipeline_*): sub-batches of mixed sizes under up to 16 keys of one
+    parameter set per run, one set of launches, optionally replayed from a captured HIP graph.
+    direction: "enc" or "dec".  Runs are asynchronous on the pipeline's stream."""
+
+    def __init__(self, set_name, direction, capacity, max_sub=4096, use_graph=True, stream=None):
+        self.set_name = set_name
+        self.direction = {"enc": 0, "dec": 1}[direction]
+        self._h = ctypes.c_void_p()
+        _check(load().tmce_pipeline_create(SETS[set_name], self.direction, int(capacity), int(max_sub),
+                                           1 if use_graph else 0, stream, ctypes.byref(self._h)),
+               "tmce_pipeline_create")
+
+    @property
+    def capacity(self):
+        return load().tmce_pipeline_capacity(self._h)
+
+    def device_bytes(self):
+        return load().tmce_pipeline_device_bytes(self._h)
+
+    @property
+    def stream(self):
+        return load().tmce_pipeline_stream(self._h)
+
+    def run(self, subs):
+        """subs: iterable of (key handle object, in_ptr, ct_ptr or None, k_ptr, status_ptr or None, n)
+        with device addresses as ints."""
+        subs = list(subs)
+        arr = (SubBatchC * max(1, len(subs)))()
+        for i, (key, inp, ct, k, st, n) in enumerate(subs):
+            arr[i] = SubBatchC(key.handle, inp, ct, k, st, n)
+        _check(load().tmce_pipeline_run(self._h, ctypes.cast(arr, ctypes.c_void_p), len(subs)), "tmce_pipeline_run")
+
+    def sync(self):
+        _check(load().tmce_pipeline_sync(self._h), "tmce_pipeline_sync")
+
+    def close(self):
+        if self._h:
+            load().tmce_pipeline_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def set_form(form):

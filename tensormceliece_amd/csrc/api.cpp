@@ -12,9 +12,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/tmce.h"
-#include "host_math.h"
-#include "kem_kernels.h"
+#include "engine.h"
 
 namespace tmce {
 KernelTimer *g_ktimer = nullptr;
@@ -22,115 +20,14 @@ KernelTimer *g_ktimer = nullptr;
 
 using namespace tmce;
 
-struct tmce_key {
-  int set = -1;
-  int kind = 0;  // 1 = public (encapsulation), 2 = secret (decapsulation)
-  size_t bytes = 0;
-  bool pad_bad = false;
-  uint32_t *cols = nullptr;
-  uint32_t *Trows = nullptr;  // plain form: T as MT rows of ceil(K/32) words
-  uint8_t *Tb = nullptr;      // syndrome GEMM operands, built on first use: 0/1 bytes (MFMA) ...
-  uint32_t *Tv = nullptr;     // ... and 32-row blocks of words (VALU)
-  uint32_t *F = nullptr;
-  uint8_t *Fsw = nullptr;     // F as 0/1 bytes in MFMA fragment order (INT8 MFMA front end)
-  uint16_t *alpha = nullptr, *la = nullptr, *lu = nullptr;
-  uint8_t *rej = nullptr;
-};
-
-namespace {
-
+namespace tmce {
 thread_local std::string g_err;
-
 int fail(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
-
-#define HIPCHK(expr)                                                                          \
-  do {                                                                                        \
-    hipError_t e_ = (expr);                                                                   \
-    if (e_ != hipSuccess) return fail(TMCE_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-struct DevBuf {
-  void *p = nullptr;
-  size_t cap = 0;
-  int ensure(size_t bytes) {
-    if (bytes <= cap) return 0;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-    if (hipMalloc(&p, bytes) != hipSuccess) return fail(TMCE_ENOMEM, "hipMalloc scratch");
-    cap = bytes;
-    return 0;
-  }
-  template <class T> T *as() const { return static_cast<T *>(p); }
-};
-
-struct EventTimer : KernelTimer {
-  struct Rec { std::string name; hipEvent_t a, b; };
-  std::vector<Rec> pending;
-  std::vector<hipEvent_t> pool;
-  std::map<std::string, std::pair<long, double>> acc;
-  hipEvent_t get() {
-    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
-    hipEvent_t e;
-    (void)hipEventCreate(&e);
-    return e;
-  }
-  void begin(const char *name, hipStream_t s) override {
-    Rec r{name, get(), get()};
-    (void)hipEventRecord(r.a, s);
-    pending.push_back(r);
-  }
-  void end(hipStream_t s) override { (void)hipEventRecord(pending.back().b, s); }
-  void drain() {
-    for (auto &r : pending) {
-      (void)hipEventSynchronize(r.b);
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, r.a, r.b);
-      auto &x = acc[r.name];
-      x.first += 1;
-      x.second += ms;
-      pool.push_back(r.a);
-      pool.push_back(r.b);
-    }
-    pending.clear();
-  }
-};
-
-struct SetTables {
-  uint16_t *twl = nullptr, *plo = nullptr, *phi = nullptr, *ptlo = nullptr, *pthi = nullptr;
-};
-
-struct Engine {
-  int device = -1;
-  hipStream_t stream = nullptr;
-  std::mutex mu;
-  uint16_t *lg[2] = {nullptr, nullptr}, *ex[2] = {nullptr, nullptr};
-  SetTables st[NUM_SETS];
-  DevBuf seeds, ct, key, status, ind, msg, S, attempts, queue;
-  DevBuf p_erow, p_ens, p_s, p_loc, p_weight;  // plain-form scratch
-  DevBuf xscratch;                             // MFMA products: x rows repacked to aligned words
-  int form = 0;                                // 0 = engine form, 1 = plain per-stage form
-  int synd_cfg[NUM_SETS] = {1, 1, 1, 1, 1};    // MFMA syndrome tile shape per set (tunable)
-  int gen_blocks[NUM_SETS] = {0, 0, 0, 0, 0};  // sampler grid per set (0 = built-in default)
-  int decode_blocks[NUM_SETS] = {0, 0, 0, 0, 0};
-  int front_mfma[NUM_SETS] = {1, 1, 1, 1, 1};  // decapsulation front end: 1 = INT8 MFMA, 0 = VALU
-  int front_cfg[NUM_SETS] = {1, 1, 1, 1, 1};   // its MFMA tile shape
-  EventTimer timer;
-};
-
 Engine *g_engine = nullptr;
-std::mutex g_engine_mu;
-
-template <class T>
-int upload(T **dst, const std::vector<T> &src) {
-  if (hipMalloc((void **)dst, src.size() * sizeof(T)) != hipSuccess) return fail(TMCE_ENOMEM, "hipMalloc table");
-  if (hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-    return fail(TMCE_EDEVICE, "hipMemcpy table");
-  return 0;
-}
+static std::mutex g_engine_mu;
 
 int engine_init(int device) {
   std::lock_guard<std::mutex> lk(g_engine_mu);
@@ -176,7 +73,10 @@ Engine *engine() {
   return g_engine;
 }
 
-bool valid_set(int set) { return set >= 0 && set < NUM_SETS; }
+}  // namespace tmce
+
+namespace {
+
 
 // ---------------------------------------------------------------- randomness
 tmce_randombytes_fn g_randombytes = nullptr;
@@ -272,10 +172,11 @@ int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, si
   DecArgs a{};
   a.ct = d_ct;
   a.F = k->F;
-  a.Fsw = e->front_mfma[k->set] ? k->Fsw : nullptr;
+  a.Fsw = k->Fsw;
+  a.front_mfma = e->front_mfma[k->set];
   a.front_cfg = e->front_cfg[k->set];
   a.xscratch = nullptr;
-  if (a.Fsw) {
+  if (a.front_mfma) {
     const size_t xb = gf2_scratch_bytes(k->set, GF2_FRONT, n);
     if (xb) {
       if (int r = e->xscratch.ensure(xb)) return r;

@@ -27,10 +27,16 @@ constexpr int DEC_WAVES = DEC_WG / 64;
 // ------------------------------------------------------------------ front end
 template <int SET>
 __global__ __launch_bounds__(FRONT_WG) void k_dec_front(const uint8_t *__restrict__ ct, const uint4 *__restrict__ F,
-                                                        uint32_t *__restrict__ S, int nops) {
+                                                        uint32_t *__restrict__ S, int nops,
+                                                        const PlanDev *__restrict__ plan) {
   using P = Par<SET>;
   constexpr int Q4 = P::KWP / 4;
   __shared__ uint4 rows[32 * Q4];
+  if (plan) {  // key groups are PLAN_ALIGN-aligned: one key per workgroup
+    nops = plan->total;
+    if ((int)(blockIdx.x * FRONT_WG) >= nops) return;
+    F = reinterpret_cast<const uint4 *>(plan->keys[plan_group(plan, blockIdx.x * FRONT_WG)].F);
+  }
   const int tid = threadIdx.x;
   const int op = blockIdx.x * FRONT_WG + tid;
   const bool valid = op < nops;
@@ -131,6 +137,7 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
   for (int i = tid; i < NPHI; i += DEC_WG) phi_s[i] = a.ft.phi[i];
   for (int i = tid; i < NR; i += DEC_WG) pthi_s[i] = P::TOP_TERM ? a.ft.pthi[i] : 0;
   __syncthreads();
+  const int nops = a.plan ? a.plan->total : a.nops;
   const int wave = tid >> 6, lane = tid & 63, row = lane >> 4, l16 = lane & 15;
   uint32_t plo[KD];
 #pragma unroll
@@ -138,14 +145,14 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
   const uint32_t ptlo = P::TOP_TERM ? a.ft.ptlo[lane] : 0u;
   DecWave<SET> &w = Wv[wave];
 
-  for (int base = (blockIdx.x * DEC_WAVES + wave) * 4; base < a.nops; base += gridDim.x * DEC_WAVES * 4) {
+  for (int base = (blockIdx.x * DEC_WAVES + wave) * 4; base < nops; base += gridDim.x * DEC_WAVES * 4) {
     // ---- 1. syndromes of the four ops from their bit planes
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
       const int op = base + q;
       for (int j = lane; j < P::TT; j += 64) {
         uint32_t v = 0;
-        if (op < a.nops) {
+        if (op < nops) {
           const uint32_t *Srow = a.S + (size_t)op * P::RW;
 #pragma unroll
           for (int b = 0; b < M; ++b) v |= ((Srow[b * (P::TT / 32) + (j >> 5)] >> (j & 31)) & 1u) << b;
@@ -219,7 +226,17 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
       const int op = base + q;
-      if (op >= a.nops) break;
+      if (op >= nops) break;
+      // key material of the op (one key, or its key group under a plan)
+      const uint16_t *k_alpha = a.alpha, *k_la = a.la, *k_lu = a.lu;
+      const uint8_t *k_rej = a.rej;
+      if (a.plan) {
+        const KeyDev &kd = a.plan->keys[plan_group(a.plan, op)];
+        k_alpha = kd.alpha;
+        k_la = kd.la;
+        k_lu = kd.lu;
+        k_rej = kd.rej;
+      }
       uint16_t *fq = w.f[q];
       // additive FFT, top-down on the D coefficients: twist, Taylor at x^2+x, even/odd split
 #pragma unroll 1
@@ -305,7 +322,7 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
       for (int it = 0; it < ceil_div(P::N, 64); ++it) {
         const int i = it * 64 + lane;
         const bool valid = i < P::N;
-        const uint32_t av = valid ? a.alpha[i] : 0u;
+        const uint32_t av = valid ? k_alpha[i] : 0u;
         const bool ei = valid && ((w.Z[av >> 5] >> (av & 31)) & 1u);
         const uint64_t mask = __ballot(ei);
         if (ei) {
@@ -326,7 +343,7 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
         for (int k = 0; k < JS; ++k) acc[k] = 0;
         for (int k = 0; k < T; ++k) {
           const int p = w.pos[k];
-          const uint32_t la = a.la[p], lu = a.lu[p];
+          const uint32_t la = k_la[p], lu = k_lu[p];
           if (lu == 0xFFFFu) continue;
           if (la == 0xFFFFu) {
             if (lane == 0) acc[0] ^= ex_s[lu];
@@ -361,7 +378,7 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
       for (int k = lane; k < P::MSG_WORDS; k += 64) {
         uint32_t v = 0;
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq) v |= msg_byte<SET>(4 * k + qq, ok, ebytes, a.rej, ctrow) << (8 * qq);
+        for (int qq = 0; qq < 4; ++qq) v |= msg_byte<SET>(4 * k + qq, ok, ebytes, k_rej, ctrow) << (8 * qq);
         mrow[k] = v;
       }
       wave_lds_sync();
@@ -371,10 +388,13 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
 
 template <int SET>
 __global__ __launch_bounds__(256) void k_hash_msg(const uint32_t *__restrict__ msg, const uint8_t *__restrict__ ct,
-                                                  uint8_t *__restrict__ key, int *__restrict__ status, int nops) {
+                                                  uint8_t *__restrict__ key, int *__restrict__ status, int nops,
+                                                  const PlanDev *__restrict__ plan) {
   using P = Par<SET>;
   // every lane stays active through the hash (round constants via v_readlane)
   const KeccakRC rc;
+  if (plan) nops = plan->total;
+  if ((int)(blockIdx.x * blockDim.x) >= nops) return;  // whole workgroup past the batch (uniform)
   const int gop = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = gop < nops;
   const int op = live ? gop : nops - 1;
@@ -400,32 +420,32 @@ template <int SET>
 hipError_t launch_dec(const DecArgs &a, hipStream_t s) {
   using P = Par<SET>;
   if (a.nops <= 0) return hipSuccess;
-  if (a.Fsw) {
-    const hipError_t e = launch_front_mfma_set(SET, a.ct, a.xscratch, a.Fsw, a.S, a.nops, a.front_cfg, s);
+  if (a.front_mfma) {
+    const hipError_t e = launch_front_mfma_set(SET, a.ct, a.xscratch, a.Fsw, a.S, a.nops, a.front_cfg, a.plan, s);
     if (e != hipSuccess) return e;
   } else {
     TMCE_TIMED("dec_front", s,
                k_dec_front<SET><<<ceil_div(a.nops, FRONT_WG), FRONT_WG, 0, s>>>(
-                   a.ct, reinterpret_cast<const uint4 *>(a.F), a.S, a.nops));
+                   a.ct, reinterpret_cast<const uint4 *>(a.F), a.S, a.nops, a.plan));
   }
   int grid = a.decode_blocks > 0 ? a.decode_blocks : 256;
   const int maxgrid = ceil_div(a.nops, DEC_WAVES * 4);
   if (grid > maxgrid) grid = maxgrid;
   TMCE_TIMED("dec_decode", s, k_dec_decode<SET><<<grid, DEC_WG, 0, s>>>(a));
   TMCE_TIMED("dec_hash", s,
-             k_hash_msg<SET><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, a.ct, a.key, a.status, a.nops));
+             k_hash_msg<SET><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, a.ct, a.key, a.status, a.nops, a.plan));
   return hipGetLastError();
 }
 
 template <int SET>
 hipError_t front_only(const DecArgs &a, hipStream_t s) {
   k_dec_front<SET><<<ceil_div(a.nops, FRONT_WG), FRONT_WG, 0, s>>>(a.ct, reinterpret_cast<const uint4 *>(a.F), a.S,
-                                                                    a.nops);
+                                                                    a.nops, nullptr);
   return hipGetLastError();
 }
 template <int SET>
 hipError_t hash_only(const DecArgs &a, hipStream_t s) {
-  k_hash_msg<SET><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, a.ct, a.key, a.status, a.nops);
+  k_hash_msg<SET><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, a.ct, a.key, a.status, a.nops, nullptr);
   return hipGetLastError();
 }
 hipError_t launch_dec_front_only(int set, const DecArgs &a, hipStream_t s) {

@@ -30,7 +30,8 @@ constexpr int GEN_WAVES = GEN_WG / 64;
 template <int SET>
 __global__ __launch_bounds__(GEN_WG) void k_gen_e(const uint8_t *__restrict__ seeds, uint16_t *__restrict__ ind_out,
                                                   uint32_t *__restrict__ msg, uint32_t *__restrict__ attempts,
-                                                  int nops, unsigned *__restrict__ queue) {
+                                                  int nops, unsigned *__restrict__ queue,
+                                                  const PlanDev *__restrict__ plan) {
   using P = Par<SET>;
   constexpr uint32_t GFMASK = (1u << P::M) - 1u;
   constexpr int NEW = ceil_div(P::N, 32);  // 32-bit words of e
@@ -43,6 +44,7 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e(const uint8_t *__restrict__ se
   uint32_t *bm = bmp[wave][grp];
   for (int i = lane; i < 4 * (NEW + 1); i += 64) bmp[wave][i / (NEW + 1)][i % (NEW + 1)] = 0;
   const KeccakRC rc;
+  if (plan) nops = plan->total;
 
   int op = -1;
   bool done = false;
@@ -187,15 +189,24 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e(const uint8_t *__restrict__ se
 template <int SET>
 __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__ ind, const uint4 *__restrict__ cols,
                                                     uint32_t *__restrict__ msg, uint8_t *__restrict__ ct,
-                                                    uint8_t *__restrict__ key, int *__restrict__ status, int nops) {
+                                                    uint8_t *__restrict__ key, int *__restrict__ status, int nops,
+                                                    const PlanDev *__restrict__ plan) {
   using P = Par<SET>;
   constexpr int C4 = P::CWP / 4;
   // Every lane stays active through the hash (the round constants are read with v_readlane):
   // lanes past the batch redo the last op and store nothing.
   const KeccakRC rc;
+  if (plan) nops = plan->total;
+  if ((int)(blockIdx.x * blockDim.x) >= nops) return;  // whole workgroup past the batch (uniform)
   const int gop = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = gop < nops;
   const int op = live ? gop : nops - 1;
+  bool pad_bad = false;  // LM:crypto_kem/6960119/vec/kem_enc.c: ct and k cleared, return -1
+  if (plan) {
+    const KeyDev &kd = plan->keys[plan_group(plan, op)];
+    cols = reinterpret_cast<const uint4 *>(kd.cols);
+    pad_bad = kd.pad_bad != 0;
+  }
 
   // ---- c = sum of columns over the support of e
   uint4 c[C4];
@@ -254,10 +265,11 @@ __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__
   shake256_blocks_32(reinterpret_cast<const uint64_t *>(mrow), P::MSG_BLOCKS, out, rc);
   uint64_t *kp = reinterpret_cast<uint64_t *>(key + 32 * (size_t)op);
   if (live) {
+    const bool clear = abandoned || pad_bad;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) kp[i] = abandoned ? 0ull : out[i];
-    if (abandoned) for (int h = 0; h < P::SYND_BYTES / 2; ++h) ctp[h] = 0;
-    if (status) status[op] = abandoned ? -2 : 0;
+    for (int i = 0; i < 4; ++i) kp[i] = clear ? 0ull : out[i];
+    if (clear) for (int h = 0; h < P::SYND_BYTES / 2; ++h) ctp[h] = 0;
+    if (status) status[op] = pad_bad ? -1 : abandoned ? -2 : 0;
   }
 }
 
@@ -283,19 +295,63 @@ hipError_t launch_fill_i32(int *p, int v, size_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ---- multi-batch gather / scatter (one thread per combined op)
+__global__ void k_plan_gather(const PlanDev *__restrict__ plan, uint8_t *__restrict__ flat, int row_bytes) {
+  const int op = blockIdx.x * blockDim.x + threadIdx.x;
+  if (op >= plan->total) return;
+  const int sb = plan_sub(plan, op);
+  uint16_t *dst = reinterpret_cast<uint16_t *>(flat + (size_t)op * row_bytes);  // row_bytes even
+  if (sb < 0) {
+    for (int h = 0; h < row_bytes / 2; ++h) dst[h] = 0;
+    return;
+  }
+  const SubDev &d = plan->subs[sb];
+  const uint16_t *src = reinterpret_cast<const uint16_t *>(d.in + (size_t)(op - d.start) * row_bytes);
+  for (int h = 0; h < row_bytes / 2; ++h) dst[h] = src[h];
+}
+__global__ void k_plan_scatter(const PlanDev *__restrict__ plan, const uint8_t *__restrict__ ct, int ct_bytes,
+                               const uint8_t *__restrict__ key, const int *__restrict__ status) {
+  const int op = blockIdx.x * blockDim.x + threadIdx.x;
+  if (op >= plan->total) return;
+  const int sb = plan_sub(plan, op);
+  if (sb < 0) return;
+  const SubDev &d = plan->subs[sb];
+  const int i = op - d.start;
+  if (ct_bytes) {
+    const uint16_t *c = reinterpret_cast<const uint16_t *>(ct + (size_t)op * ct_bytes);
+    uint16_t *o = reinterpret_cast<uint16_t *>(d.ct + (size_t)i * ct_bytes);
+    for (int h = 0; h < ct_bytes / 2; ++h) o[h] = c[h];
+  }
+  const uint4 *k = reinterpret_cast<const uint4 *>(key + 32 * (size_t)op);
+  uint4 *ko = reinterpret_cast<uint4 *>(d.key + 32 * (size_t)i);
+  ko[0] = k[0];
+  ko[1] = k[1];
+  if (d.status) d.status[i] = status[op];
+}
+hipError_t launch_plan_gather(const PlanDev *plan, uint8_t *flat, int row_bytes, int cap, hipStream_t s) {
+  if (cap > 0) k_plan_gather<<<ceil_div(cap, 256), 256, 0, s>>>(plan, flat, row_bytes);
+  return hipGetLastError();
+}
+hipError_t launch_plan_scatter(const PlanDev *plan, const uint8_t *ct, int ct_bytes, const uint8_t *key,
+                               const int *status, int cap, hipStream_t s) {
+  if (cap > 0) k_plan_scatter<<<ceil_div(cap, 256), 256, 0, s>>>(plan, ct, ct_bytes, key, status);
+  return hipGetLastError();
+}
+
 template <int SET>
 hipError_t launch_enc(const EncArgs &a, hipStream_t s) {
   using P = Par<SET>;
   if (a.nops <= 0) return hipSuccess;
-  k_fill_u32<<<1, 64, 0, s>>>(a.queue, 0u, 1);
+  k_fill_u32<<<1, 64, 0, s>>>(a.queue, 0u, 1);  // a kernel, not a memset: the pipeline may be graph-captured
   // persistent grid: enough resident waves to fill the chip, lanes pull ops from the queue
   int grid = a.gen_blocks > 0 ? a.gen_blocks : 1024;
   const int maxgrid = ceil_div(a.nops, GEN_WG);
   if (grid > maxgrid) grid = maxgrid;
-  TMCE_TIMED("gen_e", s, k_gen_e<SET><<<grid, GEN_WG, 0, s>>>(a.seeds, a.ind, a.msg, a.attempts, a.nops, a.queue));
+  TMCE_TIMED("gen_e", s,
+             k_gen_e<SET><<<grid, GEN_WG, 0, s>>>(a.seeds, a.ind, a.msg, a.attempts, a.nops, a.queue, a.plan));
   TMCE_TIMED("enc_finish", s,
              k_enc_finish<SET><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.ind, reinterpret_cast<const uint4 *>(a.cols),
-                                                                     a.msg, a.ct, a.key, a.status, a.nops));
+                                                                     a.msg, a.ct, a.key, a.status, a.nops, a.plan));
   return hipGetLastError();
 }
 

@@ -7,6 +7,58 @@
 
 namespace tmce {
 
+// ---- multi-batch plan (device memory)
+// Sub-batches of one parameter set, possibly under different keys, mapped onto one combined op
+// index space: key group g owns combined ops [gstart[g], gstart[g+1]) (PLAN_ALIGN-aligned, so an
+// MFMA workgroup tile never spans two keys); its sub-batches sit contiguously from gstart[g] and the
+// rest of the group is padding (dead ops, computed and discarded).  Stage kernels given a plan take
+// their op count (total) and per-op key material from it; gather/scatter kernels move the sub-batch
+// inputs and outputs between the callers' buffers and the flat pipeline buffers.
+constexpr int PLAN_MAX_GROUPS = 16;
+constexpr int PLAN_ALIGN = 512;
+struct KeyDev {
+  const uint32_t *cols;   // encapsulation: column-major [I | T]
+  const uint32_t *F;      // decapsulation: front-end matrix (VALU form)
+  const uint8_t *Fsw;     // decapsulation: front-end matrix (MFMA form)
+  const uint16_t *alpha, *la, *lu;
+  const uint8_t *rej;
+  int pad_bad;            // 6960119 public key with nonzero padding bits
+  int reserved;
+};
+struct SubDev {
+  const uint8_t *in;      // seeds [n][32] (enc) or ciphertexts [n][SYND_BYTES] (dec)
+  uint8_t *ct;            // enc: ciphertexts out
+  uint8_t *key;           // session keys out
+  int *status;            // per-op status out (optional)
+  int start, n;           // combined index of the first op, op count
+};
+struct PlanDev {
+  int total;              // combined ops (live + padding)
+  int ngroups, nsub, reserved;
+  int gstart[PLAN_MAX_GROUPS + 1];
+  KeyDev keys[PLAN_MAX_GROUPS];
+  const SubDev *subs;     // nsub entries sorted by start
+};
+__device__ __forceinline__ int plan_group(const PlanDev *p, int op) {
+  int g = 0;
+  while (g + 1 < p->ngroups && op >= p->gstart[g + 1]) ++g;
+  return g;
+}
+// sub-batch holding combined op `op`, or -1 for padding
+__device__ __forceinline__ int plan_sub(const PlanDev *p, int op) {
+  int lo = 0, hi = p->nsub - 1, r = -1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    if (p->subs[mid].start <= op) { r = mid; lo = mid + 1; } else hi = mid - 1;
+  }
+  return (r >= 0 && op < p->subs[r].start + p->subs[r].n) ? r : -1;
+}
+// gather sub-batch inputs into the flat buffer (row_bytes per op; padding rows zero) and scatter
+// the flat outputs back (enc: ct_bytes > 0 also scatters ciphertexts)
+hipError_t launch_plan_gather(const PlanDev *plan, uint8_t *flat, int row_bytes, int cap, hipStream_t s);
+hipError_t launch_plan_scatter(const PlanDev *plan, const uint8_t *ct, int ct_bytes, const uint8_t *key,
+                               const int *status, int cap, hipStream_t s);
+
 // Device buffers of one encapsulation batch (all pointers are device pointers).
 struct EncArgs {
   const uint8_t *seeds;  // [n][32]
@@ -18,8 +70,9 @@ struct EncArgs {
   uint32_t *msg;         // scratch [n][MSG_WORDS] session-key message
   uint32_t *attempts;    // optional [n] gen_e attempts
   unsigned *queue;       // scratch work-queue counter
-  int nops;
+  int nops;              // op count, or with a plan the capacity (the live count is plan->total)
   int gen_blocks;        // persistent grid size of the sampler (0 = default)
+  const PlanDev *plan;   // multi-batch plan (per-group keys; null: one key, `cols`)
 };
 
 // Device-resident tables of one GF(2^m) field and of the additive FFT used by decapsulation.
@@ -37,7 +90,8 @@ struct FieldTables {
 struct DecArgs {
   const uint8_t *ct;     // [n][SYND_BYTES]
   const uint32_t *F;     // key: [R][KWP] front-end matrix (plane-major rows)
-  const uint8_t *Fsw;    // key: F as 0/1 bytes in MFMA fragment order (null: VALU front end)
+  const uint8_t *Fsw;    // key: F as 0/1 bytes in MFMA fragment order
+  int front_mfma;        // front end on INT8 MFMA (1) or on the VALU (0)
   int front_cfg;         // MFMA tile shape of the front end
   uint32_t *xscratch;    // gf2_scratch_bytes(set, GF2_FRONT, n) bytes (MFMA front end; null if 0)
   const uint16_t *alpha; // key: [N] support
@@ -49,8 +103,9 @@ struct DecArgs {
   int *status;           // [n] 0, or -1 for a ciphertext with nonzero padding bits (6960119)
   uint32_t *S;           // scratch [n][RW] syndrome bit planes
   uint32_t *msg;         // scratch [n][MSG_WORDS]
-  int nops;
+  int nops;              // op count, or with a plan the capacity (the live count is plan->total)
   int decode_blocks;     // grid of the decoder (0 = default)
+  const PlanDev *plan;   // multi-batch plan (per-group keys; null: one key)
 };
 
 // Plain one-launch-per-stage form (baseline).
@@ -95,7 +150,7 @@ hipError_t launch_build_msw(int set, int which, const uint32_t *rows, int rw, ui
 hipError_t launch_build_tv(int set, const uint32_t *Trows, uint32_t *Tv, hipStream_t s);
 // decapsulation front end S = F c on INT8 MFMA (same output layout as the VALU front end)
 hipError_t launch_front_mfma_set(int set, const uint8_t *ct, uint32_t *scratch, const uint8_t *Fsw, uint32_t *S,
-                                 int nops, int cfg, hipStream_t s);
+                                 int nops, int cfg, const PlanDev *plan, hipStream_t s);
 // bytes of scratch the MFMA products need for n ops (x rows repacked to aligned words), 0 if none
 size_t gf2_scratch_bytes(int set, int which, size_t n);
 

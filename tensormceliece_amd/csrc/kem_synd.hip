@@ -104,6 +104,7 @@ struct GemmShape {
   static_assert(APT >= 1 && APT * NT == 2 * BM, "x words must split evenly over the threads");
   static_assert(ceil_div(G::ROWS, BN) * NF <= G::RTP, "row blocks exceed the padded matrix");
   static_assert(NI < 64, "vmcnt range");
+  static_assert(PLAN_ALIGN % BM == 0, "plan key groups must hold whole workgroup tiles");
 };
 
 __device__ __forceinline__ uint32_t lds_off(const void *p) {
@@ -127,13 +128,19 @@ template <class G, int OPW, int RTW, int WO, int WR, int KSUB>
 __global__ __launch_bounds__(64 * WO * WR) void k_gf2_mfma(const uint32_t *__restrict__ xw, int xstride, int xvalid,
                                                           const uint8_t *__restrict__ head,
                                                           const uint4 *__restrict__ Msw,
-                                                          uint8_t *__restrict__ out, int nops) {
+                                                          uint8_t *__restrict__ out, int nops,
+                                                          const PlanDev *__restrict__ plan) {
   using S = GemmShape<G, OPW, RTW, WO, WR, KSUB>;
   extern __shared__ __align__(16) uint8_t lds[];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
   const int wo = wave % WO, wr = wave / WO;
   const int op0 = blockIdx.x * S::BM;
+  if (plan) {  // key groups are PLAN_ALIGN-aligned (a multiple of BM): one key per workgroup
+    nops = plan->total;
+    if (op0 >= nops) return;
+    Msw = reinterpret_cast<const uint4 *>(G::plan_matrix(plan->keys[plan_group(plan, op0)]));
+  }
   const int rt0 = blockIdx.y * S::NF;  // first row tile of the block
   const uint32_t lds0 = lds_off(lds);
 
@@ -249,6 +256,7 @@ struct GemmSynd {
   static constexpr int XOFF_WORDS = P::MT / 32, XSTRIDE = P::E_BYTES / 4, XVALID = P::K / 32;
   __device__ static uint32_t x_word(const uint8_t *row, int w) { return e_ns_word<P>(row, w); }
   __device__ static uint32_t head16(const uint8_t *row, int h) { return e_head16<P>(row, h); }
+  __device__ static const void *plan_matrix(const KeyDev &) { return nullptr; }  // no plan form
 };
 
 // Decapsulation front end: rows = R (bit b of syndrome j at row b*TT + j), x = ciphertext bits
@@ -269,12 +277,15 @@ struct GemmFront {
     return v;
   }
   __device__ static uint32_t head16(const uint8_t *, int) { return 0u; }
+  __device__ static const void *plan_matrix(const KeyDev &k) { return k.Fsw; }
 };
 
 // x rows -> aligned words [n][KP/32] (for the sets where they are not aligned in place)
 template <class G>
-__global__ void k_repack_x(const uint8_t *__restrict__ x, uint32_t *__restrict__ xw, int nops) {
+__global__ void k_repack_x(const uint8_t *__restrict__ x, uint32_t *__restrict__ xw, int nops,
+                           const PlanDev *__restrict__ plan) {
   constexpr int W = G::KP / 32;
+  if (plan) nops = plan->total;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (size_t)nops * W) return;
   const int op = (int)(i / W), w = (int)(i % W);
@@ -289,7 +300,7 @@ struct MswDims {
 
 template <class G, int OPW, int RTW, int WO, int WR, int KSUB>
 hipError_t launch_gf2_cfg(const uint32_t *xw, int xstride, int xvalid, const uint8_t *x, const uint8_t *Msw,
-                          uint8_t *out, int nops, const char *name, hipStream_t s) {
+                          uint8_t *out, int nops, const PlanDev *plan, const char *name, hipStream_t s) {
   using S = GemmShape<G, OPW, RTW, WO, WR, KSUB>;
   static const hipError_t attr = hipFuncSetAttribute(
       reinterpret_cast<const void *>(&k_gf2_mfma<G, OPW, RTW, WO, WR, KSUB>),
@@ -298,7 +309,7 @@ hipError_t launch_gf2_cfg(const uint32_t *xw, int xstride, int xvalid, const uin
   dim3 grid(ceil_div(nops, S::BM), ceil_div(G::ROWS, S::BN));
   TMCE_TIMED(name, s,
              (k_gf2_mfma<G, OPW, RTW, WO, WR, KSUB><<<grid, S::NT, S::LDS, s>>>(
-                 xw, xstride, xvalid, x, reinterpret_cast<const uint4 *>(Msw), out, nops)));
+                 xw, xstride, xvalid, x, reinterpret_cast<const uint4 *>(Msw), out, nops, plan)));
   return hipGetLastError();
 }
 
@@ -310,7 +321,7 @@ hipError_t launch_gf2_cfg(const uint32_t *xw, int xstride, int xvalid, const uin
 // place (G::DIRECT false), else unused.
 template <class G>
 hipError_t launch_gf2(const uint8_t *x, uint32_t *scratch, const uint8_t *Msw, uint8_t *out, int nops, int cfg,
-                      const char *name, hipStream_t s) {
+                      const PlanDev *plan, const char *name, hipStream_t s) {
   const uint32_t *xw;
   int xstride, xvalid;
   if constexpr (G::DIRECT) {
@@ -320,17 +331,17 @@ hipError_t launch_gf2(const uint8_t *x, uint32_t *scratch, const uint8_t *Msw, u
   } else {
     if (!scratch) return hipErrorInvalidValue;
     const size_t nw = (size_t)nops * (G::KP / 32);
-    TMCE_TIMED("gf2_repack", s, k_repack_x<G><<<(unsigned)((nw + 255) / 256), 256, 0, s>>>(x, scratch, nops));
+    TMCE_TIMED("gf2_repack", s, k_repack_x<G><<<(unsigned)((nw + 255) / 256), 256, 0, s>>>(x, scratch, nops, plan));
     xw = scratch;
     xstride = xvalid = G::KP / 32;
   }
   switch (cfg) {
-    case 0: return launch_gf2_cfg<G, 4, 4, 4, 2, 1>(xw, xstride, xvalid, x, Msw, out, nops, name, s);
-    case 1: return launch_gf2_cfg<G, 4, 4, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, name, s);
-    case 2: return launch_gf2_cfg<G, 4, 8, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, name, s);
-    case 3: return launch_gf2_cfg<G, 8, 4, 2, 4, 2>(xw, xstride, xvalid, x, Msw, out, nops, name, s);
-    case 4: return launch_gf2_cfg<G, 8, 4, 2, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, name, s);
-    case 5: return launch_gf2_cfg<G, 8, 4, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, name, s);
+    case 0: return launch_gf2_cfg<G, 4, 4, 4, 2, 1>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
+    case 1: return launch_gf2_cfg<G, 4, 4, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
+    case 2: return launch_gf2_cfg<G, 4, 8, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
+    case 3: return launch_gf2_cfg<G, 8, 4, 2, 4, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
+    case 4: return launch_gf2_cfg<G, 8, 4, 2, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
+    case 5: return launch_gf2_cfg<G, 8, 4, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
   }
   return hipErrorInvalidValue;
 }
@@ -456,7 +467,7 @@ hipError_t launch_synd(const SyndArgs &a, int method, int cfg, hipStream_t s) {
                                                                        a.ct, a.nops));
       return hipGetLastError();
     case 1:
-      return launch_gf2<GemmSynd<SET>>(a.e, a.xscratch, a.Tb, a.ct, a.nops, cfg, "synd_mfma", s);
+      return launch_gf2<GemmSynd<SET>>(a.e, a.xscratch, a.Tb, a.ct, a.nops, cfg, nullptr, "synd_mfma", s);
     case 2: {
       using D = SyndValuDims<P>;
       const size_t lds = (size_t)D::OPS * D::ES * 4;
@@ -530,18 +541,19 @@ hipError_t launch_build_tv(int set, const uint32_t *Trows, uint32_t *Tv, hipStre
 
 template <int SET>
 hipError_t front_mfma(const uint8_t *ct, uint32_t *scratch, const uint8_t *Fsw, uint32_t *S, int nops, int cfg,
-                      hipStream_t s) {
+                      const PlanDev *plan, hipStream_t s) {
   if (nops <= 0) return hipSuccess;
-  return launch_gf2<GemmFront<SET>>(ct, scratch, Fsw, reinterpret_cast<uint8_t *>(S), nops, cfg, "dec_front_mfma", s);
+  return launch_gf2<GemmFront<SET>>(ct, scratch, Fsw, reinterpret_cast<uint8_t *>(S), nops, cfg, plan,
+                                    "dec_front_mfma", s);
 }
 hipError_t launch_front_mfma_set(int set, const uint8_t *ct, uint32_t *scratch, const uint8_t *Fsw, uint32_t *S,
-                                 int nops, int cfg, hipStream_t s) {
+                                 int nops, int cfg, const PlanDev *plan, hipStream_t s) {
   switch (set) {
-    case S348864: return front_mfma<S348864>(ct, scratch, Fsw, S, nops, cfg, s);
-    case S460896: return front_mfma<S460896>(ct, scratch, Fsw, S, nops, cfg, s);
-    case S6688128: return front_mfma<S6688128>(ct, scratch, Fsw, S, nops, cfg, s);
-    case S6960119: return front_mfma<S6960119>(ct, scratch, Fsw, S, nops, cfg, s);
-    case S8192128: return front_mfma<S8192128>(ct, scratch, Fsw, S, nops, cfg, s);
+    case S348864: return front_mfma<S348864>(ct, scratch, Fsw, S, nops, cfg, plan, s);
+    case S460896: return front_mfma<S460896>(ct, scratch, Fsw, S, nops, cfg, plan, s);
+    case S6688128: return front_mfma<S6688128>(ct, scratch, Fsw, S, nops, cfg, plan, s);
+    case S6960119: return front_mfma<S6960119>(ct, scratch, Fsw, S, nops, cfg, plan, s);
+    case S8192128: return front_mfma<S8192128>(ct, scratch, Fsw, S, nops, cfg, plan, s);
   }
   return hipErrorInvalidValue;
 }
