@@ -161,10 +161,11 @@ int tmce_tune_get(int set, int param);
    set of launches (no counterpart in the reference, whose harness launches one fixed-size batch at
    a time, mceliece-gpu-params/main.cu:213-262,326-370).  A pipeline has a fixed set, direction
    (0 = encapsulation, 1 = decapsulation) and capacity in ops (each distinct key's ops are padded to
-   a multiple of 512); with use_graph the launch sequence is captured into a HIP graph at the second
+   a multiple of 256); with use_graph the launch sequence is captured into a HIP graph at the second
    run and replayed afterwards.  All buffers are device buffers; runs are asynchronous on the
    pipeline's stream (its own non-blocking stream when `stream` is NULL); a run may start while the
-   previous one is still executing on that stream (they serialise on it). */
+   previous one is still executing on that stream (they serialise on it).  Alignment: k 16 bytes,
+   status 4 bytes, in and ct 2 bytes. */
 typedef struct tmce_sub_batch {
   tmce_key *key;            /* public key (encapsulation) or secret key (decapsulation) handle */
   const unsigned char *in;  /* seeds [n*32] (enc) or ciphertexts [n*CIPHERTEXTBYTES] (dec) */
@@ -180,8 +181,40 @@ int tmce_pipeline_run(tmce_pipeline *p, const tmce_sub_batch *subs, size_t nsub)
 int tmce_pipeline_sync(tmce_pipeline *p);
 void tmce_pipeline_free(tmce_pipeline *p);
 size_t tmce_pipeline_device_bytes(const tmce_pipeline *p); /* device memory it holds */
-size_t tmce_pipeline_capacity(const tmce_pipeline *p);     /* ops (rounded up to 512) */
+size_t tmce_pipeline_capacity(const tmce_pipeline *p);     /* ops (rounded up to 256) */
 void *tmce_pipeline_stream(tmce_pipeline *p);              /* its hipStream_t */
+
+/* ---------------------------------------------------------------- serving loop
+   Continuous batching of encapsulation / decapsulation requests of one parameter set (no
+   counterpart in the reference).  A worker thread keeps `slots_per_direction` pipelines per
+   direction busy, each on its own stream (so batches overlap on the device); pending requests join
+   the next free slot FIFO while the batch fits the slot (ops after each key's padding to 256) and
+   its device footprint fits `hbm_budget` bytes beside the batches in flight (0 = no limit).
+   Requests use host buffers and key handles, which the caller keeps valid (and the keys
+   unfreed) until the request is collected.
+   tmce_server_wait blocks until a request completes and collects it; tmce_server_poll collects up
+   to `max` completed requests.  Latency = completion - submission (host steady clock). */
+typedef struct tmce_server tmce_server;
+typedef struct tmce_request {
+  int direction;            /* 0 = encapsulation, 1 = decapsulation */
+  tmce_key *key;            /* public (enc) or secret (dec) key handle of the server's set */
+  size_t n;                 /* ops: 1 .. slot_capacity - 255 */
+  const unsigned char *in;  /* seeds [n*32] (enc) or ciphertexts [n*CIPHERTEXTBYTES] (dec) */
+  unsigned char *ct;        /* enc: ciphertexts out; dec: unused */
+  unsigned char *k;         /* session keys out [n*32] */
+  int *status;              /* per-op status out [n], or NULL */
+} tmce_request;
+typedef struct tmce_server_stats {
+  unsigned long long batches, requests, ops, pending;
+  size_t in_flight_bytes, peak_in_flight_bytes, enc_bytes_per_op, dec_bytes_per_op;
+} tmce_server_stats;
+int tmce_server_create(int set, size_t slot_capacity, int slots_per_direction, size_t hbm_budget,
+                       tmce_server **out);
+int tmce_server_submit(tmce_server *sv, const tmce_request *rq, unsigned long long *id);
+int tmce_server_wait(tmce_server *sv, unsigned long long id, double *latency_s);
+int tmce_server_poll(tmce_server *sv, unsigned long long *ids, double *latency_s, int max);
+int tmce_server_get_stats(tmce_server *sv, tmce_server_stats *st);
+void tmce_server_free(tmce_server *sv);
 
 /* select the pipeline used by subsequent batched calls: 0 = engine form (default: fused,
    persistent sampler, row-parallel decoder), 1 = plain one-launch-per-stage form (the baseline

@@ -77,6 +77,12 @@ def load():
         "tmce_pipeline_device_bytes": (S, [P]),
         "tmce_pipeline_capacity": (S, [P]),
         "tmce_pipeline_stream": (P, [P]),
+        "tmce_server_create": (I, [I, S, I, S, ctypes.POINTER(P)]),
+        "tmce_server_submit": (I, [P, P, ctypes.POINTER(ctypes.c_ulonglong)]),
+        "tmce_server_wait": (I, [P, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_double)]),
+        "tmce_server_poll": (I, [P, P, P, I]),
+        "tmce_server_get_stats": (I, [P, P]),
+        "tmce_server_free": (V, [P]),
         "tmce_timing_read": (I, [I, P, P, P]),
     }
     for name, (res, args) in sig.items():
@@ -288,6 +294,84 @@ class Pipeline:
         if self._h:
             load().tmce_pipeline_free(self._h)
             self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RequestC(ctypes.Structure):
+    """tmce_request (include/tmce.h)."""
+    _fields_ = [("direction", ctypes.c_int), ("key", ctypes.c_void_p), ("n", ctypes.c_size_t),
+                ("inp", ctypes.c_void_p), ("ct", ctypes.c_void_p), ("k", ctypes.c_void_p),
+                ("status", ctypes.c_void_p)]
+
+
+class ServerStatsC(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_ulonglong), ("requests", ctypes.c_ulonglong), ("ops", ctypes.c_ulonglong),
+                ("pending", ctypes.c_ulonglong), ("in_flight_bytes", ctypes.c_size_t),
+                ("peak_in_flight_bytes", ctypes.c_size_t), ("enc_bytes_per_op", ctypes.c_size_t),
+                ("dec_bytes_per_op", ctypes.c_size_t)]
+
+
+class Server:
+    """Serving loop (tmce_server_*): continuous batching of enc/dec requests of one parameter set.
+    submit() takes host numpy arrays; they are kept alive here until the request is collected."""
+
+    def __init__(self, set_name, slot_capacity=16384, slots_per_direction=2, hbm_budget=0):
+        self.set_name = set_name
+        self.params = PARAMS[set_name]
+        self._h = ctypes.c_void_p()
+        self._live = {}
+        _check(load().tmce_server_create(SETS[set_name], int(slot_capacity), int(slots_per_direction),
+                                         int(hbm_budget), ctypes.byref(self._h)), "tmce_server_create")
+
+    def submit(self, direction, key, inp, ct=None, k=None, status=None):
+        """direction "enc" (inp = seeds [n][32]) or "dec" (inp = ciphertexts); returns (id, outputs)."""
+        d = {"enc": 0, "dec": 1}[direction]
+        inp = np.ascontiguousarray(inp, np.uint8)
+        n = inp.shape[0]
+        if k is None:
+            k = np.zeros((n, KEY_BYTES), np.uint8)
+        if status is None:
+            status = np.zeros(n, np.int32)
+        if d == 0 and ct is None:
+            ct = np.zeros((n, self.params["ct"]), np.uint8)
+        rq = RequestC(d, key.handle.value, n, inp.ctypes.data, ct.ctypes.data if ct is not None else None,
+                      k.ctypes.data, status.ctypes.data)
+        rid = ctypes.c_ulonglong()
+        _check(load().tmce_server_submit(self._h, ctypes.byref(rq), ctypes.byref(rid)), "tmce_server_submit")
+        self._live[rid.value] = (key, inp, ct, k, status)  # the key handle must outlive the request
+        return rid.value, (ct, k, status)
+
+    def wait(self, rid):
+        lat = ctypes.c_double()
+        _check(load().tmce_server_wait(self._h, rid, ctypes.byref(lat)), "tmce_server_wait")
+        self._live.pop(rid, None)
+        return lat.value
+
+    def poll(self, max_items=4096):
+        ids = np.zeros(max_items, np.uint64)
+        lats = np.zeros(max_items, np.float64)
+        n = load().tmce_server_poll(self._h, _ptr(ids), _ptr(lats), max_items)
+        if n < 0:
+            _check(n, "tmce_server_poll")
+        for i in ids[:n]:
+            self._live.pop(int(i), None)
+        return ids[:n].tolist(), lats[:n].tolist()
+
+    def stats(self):
+        st = ServerStatsC()
+        _check(load().tmce_server_get_stats(self._h, ctypes.byref(st)), "tmce_server_get_stats")
+        return {f: getattr(st, f) for f, _ in ServerStatsC._fields_}
+
+    def close(self):
+        if self._h:
+            load().tmce_server_free(self._h)
+            self._h = ctypes.c_void_p()
+            self._live.clear()
 
     def __del__(self):
         try:

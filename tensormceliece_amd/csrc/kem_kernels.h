@@ -15,7 +15,7 @@ namespace tmce {
 // their op count (total) and per-op key material from it; gather/scatter kernels move the sub-batch
 // inputs and outputs between the callers' buffers and the flat pipeline buffers.
 constexpr int PLAN_MAX_GROUPS = 16;
-constexpr int PLAN_ALIGN = 512;
+constexpr int PLAN_ALIGN = 256;
 struct KeyDev {
   const uint32_t *cols;   // encapsulation: column-major [I | T]
   const uint32_t *F;      // decapsulation: front-end matrix (VALU form)

@@ -316,7 +316,7 @@ hipError_t launch_gf2_cfg(const uint32_t *xw, int xstride, int xvalid, const uin
 // Tile shapes offered to the tuner: (OPW, RTW, WO, WR, KSUB) -> BM x BN workgroup tiles
 //   0: 256 x 128, 8 waves, 1 step per barrier     1: 256 x 128, 8 waves, 2 steps per barrier
 //   2: 256 x 256, 8 waves, 2 steps per barrier    3: 256 x 256, 8 waves (128 ops each), 2 steps
-//   4: 256 x 128, 4 waves (128 ops each), 2 steps 5: 512 x 128, 8 waves (128 ops each), 2 steps
+//   4: 256 x 128, 4 waves (128 ops each), 2 steps 5: 256 x 256, 8 waves, 1 step per barrier
 // x: input rows (IN_BYTES each); scratch: n * KP/8 bytes when the x bits are not word aligned in
 // place (G::DIRECT false), else unused.
 template <class G>
@@ -341,7 +341,7 @@ hipError_t launch_gf2(const uint8_t *x, uint32_t *scratch, const uint8_t *Msw, u
     case 2: return launch_gf2_cfg<G, 4, 8, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
     case 3: return launch_gf2_cfg<G, 8, 4, 2, 4, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
     case 4: return launch_gf2_cfg<G, 8, 4, 2, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
-    case 5: return launch_gf2_cfg<G, 8, 4, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
+    case 5: return launch_gf2_cfg<G, 4, 8, 4, 2, 1>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
   }
   return hipErrorInvalidValue;
 }

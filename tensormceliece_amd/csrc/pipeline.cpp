@@ -193,6 +193,8 @@ int tmce_pipeline_run(tmce_pipeline *p, const tmce_sub_batch *subs, size_t nsub)
       return fail(TMCE_EINVAL, "tmce_pipeline_run: sub-batch key of the wrong set or kind");
     if (b.n && (!b.in || !b.k || (p->dir == 0 && !b.ct)))
       return fail(TMCE_EINVAL, "tmce_pipeline_run: sub-batch without buffers");
+    if (((uintptr_t)b.k & 15) || ((uintptr_t)b.status & 3) || ((uintptr_t)b.in & 1) || ((uintptr_t)b.ct & 1))
+      return fail(TMCE_EINVAL, "tmce_pipeline_run: misaligned buffer (k: 16 bytes, status: 4, in/ct: 2)");
     auto it = gidx.find(b.key);
     if (it == gidx.end()) {
       if ((int)keys.size() == PLAN_MAX_GROUPS)

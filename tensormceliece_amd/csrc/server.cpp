@@ -1,0 +1,351 @@
+// server.cpp -- serving loop: continuous batching of encapsulation / decapsulation requests against
+// device-resident keys of one parameter set.
+//
+// Requests (host buffers, any size up to a slot's capacity, any registered key handle) are queued
+// per direction.  A worker thread keeps every slot busy: a slot is a multi-batch pipeline on its own
+// stream plus pinned staging; when one is free and requests are pending, the worker forms a batch
+// FIFO -- a request joins if the batch still fits the slot (ops after each key's padding to 256) and
+// its device footprint fits the HBM budget left by the batches in flight -- stages the inputs,
+// uploads them, runs the pipeline and queues the download, all asynchronously on the slot's stream.
+// Slots run concurrently on the device (enc and dec slots, and several of each); requests that
+// arrive while batches are in flight are admitted into the next free slot.  When a slot's event
+// completes the worker copies each request's outputs back and stamps its completion time.
+//
+// Latency = completion - submission (host steady clock), including queueing, transfers and compute.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+
+#include "engine.h"
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Req {
+  unsigned long long id = 0;
+  int dir = 0;
+  tmce_key *key = nullptr;
+  size_t n = 0;
+  const uint8_t *in = nullptr;
+  uint8_t *ct = nullptr, *k = nullptr;
+  int *status = nullptr;
+  size_t footprint = 0;
+  double t_submit = 0, t_start = 0, t_done = 0;
+  bool done = false;
+};
+
+struct Slot {
+  int dir = 0;
+  tmce_pipeline *pipe = nullptr;
+  hipStream_t stream = nullptr;
+  size_t cap = 0, in_row = 0, out_row = 0;  // ops; bytes per op of input / output staging
+  uint8_t *h_in = nullptr, *h_out = nullptr;  // pinned staging
+  uint8_t *d_io = nullptr;                    // device: inputs [cap][in_row], outputs [cap][out_row]
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  size_t footprint = 0;
+  std::vector<Req *> reqs;
+};
+
+}  // namespace
+
+struct tmce_server {
+  int set = -1;
+  size_t budget = 0, in_flight = 0, peak_in_flight = 0;
+  std::vector<Slot> slots;
+  std::deque<Req *> pending[2];
+  std::unordered_map<unsigned long long, Req *> reqs;
+  std::deque<unsigned long long> completed;
+  unsigned long long next_id = 1;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::thread worker;
+  bool stop = false;
+  int error = 0;  // first failure of the worker (guarded by mu)
+  std::string error_msg;
+  // statistics
+  unsigned long long batches = 0, ops = 0, requests = 0;
+  size_t per_op_bytes[2] = {0, 0};
+};
+
+namespace tmce {
+namespace {
+
+size_t round_up_sz(size_t a, size_t b) { return (a + b - 1) / b * b; }
+
+// Fills `b` from the FIFO of direction `dir`; returns the requests taken (the caller holds mu).
+std::vector<Req *> form_batch(tmce_server *sv, Slot &sl) {
+  std::vector<Req *> out;
+  std::unordered_map<tmce_key *, size_t> group;  // key -> ops in this batch
+  size_t padded = 0, foot = 0;
+  auto &q = sv->pending[sl.dir];
+  while (!q.empty()) {
+    Req *r = q.front();
+    const size_t before = group.count(r->key) ? round_up_sz(group[r->key], PLAN_ALIGN) : 0;
+    if (!group.count(r->key) && group.size() == (size_t)PLAN_MAX_GROUPS) break;
+    const size_t after = round_up_sz((group.count(r->key) ? group[r->key] : 0) + r->n, PLAN_ALIGN);
+    if (padded - before + after > sl.cap) break;
+    if (sv->in_flight + foot + r->footprint > sv->budget && !(out.empty() && sv->in_flight == 0)) break;
+    q.pop_front();
+    group[r->key] += r->n;
+    padded = padded - before + after;
+    foot += r->footprint;
+    out.push_back(r);
+  }
+  sl.footprint = foot;
+  return out;
+}
+
+int launch_slot(tmce_server *sv, Slot &sl, std::vector<Req *> &batch) {
+  std::vector<tmce_sub_batch> subs;
+  size_t pos = 0;
+  uint8_t *d_in = sl.d_io, *d_out = sl.d_io + sl.cap * sl.in_row;
+  const double t = now_s();
+  for (Req *r : batch) {
+    std::memcpy(sl.h_in + pos * sl.in_row, r->in, r->n * sl.in_row);
+    tmce_sub_batch b{};
+    b.key = r->key;
+    b.in = d_in + pos * sl.in_row;
+    // outputs of the request: [k][status][ct (enc)] (k 16-byte aligned: out_row is a multiple of 16)
+    uint8_t *o = d_out + pos * sl.out_row;
+    b.k = o;
+    b.status = reinterpret_cast<int *>(o + r->n * 32);
+    b.ct = sl.dir == 0 ? o + r->n * 36 : nullptr;
+    b.n = r->n;
+    subs.push_back(b);
+    r->t_start = t;
+    pos += r->n;
+  }
+  HIPCHK(hipMemcpyAsync(d_in, sl.h_in, pos * sl.in_row, hipMemcpyHostToDevice, sl.stream));
+  if (int rc = tmce_pipeline_run(sl.pipe, subs.data(), subs.size())) return rc;
+  HIPCHK(hipMemcpyAsync(sl.h_out, d_out, pos * sl.out_row, hipMemcpyDeviceToHost, sl.stream));
+  HIPCHK(hipEventRecord(sl.done, sl.stream));
+  sl.reqs = batch;
+  sl.busy = true;
+  sv->batches += 1;
+  sv->requests += batch.size();
+  sv->ops += pos;
+  return 0;
+}
+
+void finish_slot(tmce_server *sv, Slot &sl) {
+  const SetInfo &si = set_info(sv->set);
+  const size_t out_ct = sl.dir == 0 ? (size_t)si.synd_bytes : 0;
+  size_t pos = 0;
+  const double t = now_s();
+  for (Req *r : sl.reqs) {
+    const uint8_t *o = sl.h_out + pos * sl.out_row;
+    std::memcpy(r->k, o, r->n * 32);
+    if (r->status) std::memcpy(r->status, o + r->n * 32, r->n * sizeof(int));
+    if (sl.dir == 0) std::memcpy(r->ct, o + r->n * 36, r->n * out_ct);
+    r->t_done = t;
+    r->done = true;
+    sv->completed.push_back(r->id);
+    pos += r->n;
+  }
+  sv->in_flight -= sl.footprint;
+  sl.footprint = 0;
+  sl.reqs.clear();
+  sl.busy = false;
+}
+
+void worker_main(tmce_server *sv) {
+  (void)hipSetDevice(g_engine->device);
+  std::unique_lock<std::mutex> lk(sv->mu);
+  while (!sv->stop) {
+    bool progressed = false, any_busy = false;
+    for (Slot &sl : sv->slots) {
+      if (sl.busy && hipEventQuery(sl.done) == hipSuccess) {
+        finish_slot(sv, sl);
+        progressed = true;
+      }
+    }
+    if (progressed) sv->cv_done.notify_all();
+    for (Slot &sl : sv->slots) {
+      if (!sl.busy && !sv->pending[sl.dir].empty()) {
+        std::vector<Req *> batch = form_batch(sv, sl);
+        if (batch.empty()) continue;
+        sv->in_flight += sl.footprint;
+        if (sv->in_flight > sv->peak_in_flight) sv->peak_in_flight = sv->in_flight;
+        if (int rc = launch_slot(sv, sl, batch)) {
+          sv->error = rc;
+          sv->error_msg = tmce_last_error();
+          sv->stop = true;
+          break;
+        }
+      }
+      any_busy |= sl.busy;
+    }
+    if (sv->stop) break;
+    if (any_busy) {
+      // completions are polled: a short sleep outside the lock keeps submissions flowing
+      lk.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+      lk.lock();
+    } else if (sv->pending[0].empty() && sv->pending[1].empty()) {
+      sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending[0].empty() || !sv->pending[1].empty(); });
+    }
+  }
+  sv->cv_done.notify_all();
+}
+
+void destroy_server(tmce_server *sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+  }
+  sv->cv_work.notify_all();
+  if (sv->worker.joinable()) sv->worker.join();
+  for (Slot &sl : sv->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.pipe) tmce_pipeline_free(sl.pipe);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.h_in) (void)hipHostFree(sl.h_in);
+    if (sl.h_out) (void)hipHostFree(sl.h_out);
+    if (sl.d_io) (void)hipFree(sl.d_io);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
+  for (auto &kv : sv->reqs) delete kv.second;
+  delete sv;
+}
+
+}  // namespace
+}  // namespace tmce
+
+using namespace tmce;
+
+extern "C" {
+
+int tmce_server_create(int set, size_t slot_capacity, int slots_per_direction, size_t hbm_budget,
+                       tmce_server **out) {
+  if (!valid_set(set) || !out || slot_capacity == 0 || slots_per_direction < 1 || slots_per_direction > 8)
+    return fail(TMCE_EINVAL, "tmce_server_create: bad argument");
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  auto *sv = new tmce_server();
+  sv->set = set;
+  sv->budget = hbm_budget ? hbm_budget : (size_t)-1;
+  const SetInfo &si = set_info(set);
+  auto bail = [&](int r) {
+    destroy_server(sv);
+    return r;
+  };
+  for (int dir = 0; dir < 2; ++dir)
+    for (int i = 0; i < slots_per_direction; ++i) {
+      sv->slots.emplace_back();
+      Slot &sl = sv->slots.back();
+      sl.dir = dir;
+      if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(TMCE_EDEVICE, "hipStreamCreate"));
+      if (int r = tmce_pipeline_create(set, dir, slot_capacity, PLAN_MAX_GROUPS * 64 + 1024, 1, sl.stream, &sl.pipe))
+        return bail(r);
+      sl.cap = tmce_pipeline_capacity(sl.pipe);
+      sl.in_row = dir == 0 ? 32 : (size_t)si.synd_bytes;
+      sl.out_row = round_up_sz((dir == 0 ? (size_t)si.synd_bytes : 0) + 32 + sizeof(int), 16);
+      if (hipHostMalloc((void **)&sl.h_in, sl.cap * sl.in_row, hipHostMallocDefault) != hipSuccess ||
+          hipHostMalloc((void **)&sl.h_out, sl.cap * sl.out_row, hipHostMallocDefault) != hipSuccess)
+        return bail(fail(TMCE_ENOMEM, "hipHostMalloc staging"));
+      if (hipMalloc((void **)&sl.d_io, sl.cap * (sl.in_row + sl.out_row)) != hipSuccess)
+        return bail(fail(TMCE_ENOMEM, "hipMalloc slot buffers"));
+      if (hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(TMCE_EDEVICE, "hipEventCreate"));
+      // per-op device footprint of this direction: pipeline buffers + slot I/O, per op of capacity
+      const size_t per_op = (tmce_pipeline_device_bytes(sl.pipe) + sl.cap * (sl.in_row + sl.out_row)) / sl.cap;
+      sv->per_op_bytes[dir] = per_op;
+    }
+  sv->worker = std::thread(worker_main, sv);
+  *out = sv;
+  return 0;
+}
+
+int tmce_server_submit(tmce_server *sv, const tmce_request *rq, unsigned long long *id) {
+  if (!sv || !rq || !id || !rq->key || (rq->direction != 0 && rq->direction != 1))
+    return fail(TMCE_EINVAL, "tmce_server_submit: bad argument");
+  if (rq->key->set != sv->set || rq->key->kind != (rq->direction == 0 ? 1 : 2))
+    return fail(TMCE_EINVAL, "tmce_server_submit: key of the wrong set or kind");
+  if (rq->n == 0 || rq->n + PLAN_ALIGN - 1 > sv->slots[0].cap || !rq->in || !rq->k || (rq->direction == 0 && !rq->ct))
+    return fail(TMCE_EINVAL, "tmce_server_submit: empty request, request larger than a slot, or missing buffers");
+  auto *r = new Req();
+  r->dir = rq->direction;
+  r->key = rq->key;
+  r->n = rq->n;
+  r->in = rq->in;
+  r->ct = rq->ct;
+  r->k = rq->k;
+  r->status = rq->status;
+  r->footprint = rq->n * sv->per_op_bytes[r->dir];
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->stop) {
+      delete r;
+      return fail(sv->error ? sv->error : TMCE_EINVAL, sv->error ? sv->error_msg : "server stopped");
+    }
+    r->id = sv->next_id++;
+    r->t_submit = now_s();
+    sv->reqs[r->id] = r;
+    sv->pending[r->dir].push_back(r);
+  }
+  sv->cv_work.notify_one();
+  *id = r->id;
+  return 0;
+}
+
+int tmce_server_wait(tmce_server *sv, unsigned long long id, double *latency_s) {
+  if (!sv) return fail(TMCE_EINVAL, "tmce_server_wait: null server");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(id);
+  if (it == sv->reqs.end()) return fail(TMCE_EINVAL, "tmce_server_wait: unknown request (or already collected)");
+  Req *r = it->second;
+  sv->cv_done.wait(lk, [&] { return r->done || sv->stop; });
+  if (!r->done) return fail(sv->error ? sv->error : TMCE_EDEVICE, sv->error ? sv->error_msg : "server stopped");
+  if (latency_s) *latency_s = r->t_done - r->t_submit;
+  sv->reqs.erase(it);
+  delete r;
+  return 0;
+}
+
+int tmce_server_poll(tmce_server *sv, unsigned long long *ids, double *latency_s, int max) {
+  if (!sv || (max > 0 && !ids)) return fail(TMCE_EINVAL, "tmce_server_poll: bad argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (sv->error) return fail(sv->error, sv->error_msg);
+  int n = 0;
+  while (n < max && !sv->completed.empty()) {
+    const unsigned long long id = sv->completed.front();
+    sv->completed.pop_front();
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) continue;  // already collected by tmce_server_wait
+    ids[n] = id;
+    if (latency_s) latency_s[n] = it->second->t_done - it->second->t_submit;
+    delete it->second;
+    sv->reqs.erase(it);
+    ++n;
+  }
+  return n;
+}
+
+int tmce_server_get_stats(tmce_server *sv, tmce_server_stats *st) {
+  if (!sv || !st) return fail(TMCE_EINVAL, "tmce_server_get_stats: bad argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  st->batches = sv->batches;
+  st->requests = sv->requests;
+  st->ops = sv->ops;
+  st->pending = sv->pending[0].size() + sv->pending[1].size();
+  st->in_flight_bytes = sv->in_flight;
+  st->peak_in_flight_bytes = sv->peak_in_flight;
+  st->enc_bytes_per_op = sv->per_op_bytes[0];
+  st->dec_bytes_per_op = sv->per_op_bytes[1];
+  return 0;
+}
+
+void tmce_server_free(tmce_server *sv) { destroy_server(sv); }
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+"""Serving workload driver: Poisson arrivals of encapsulation / decapsulation requests against
+device-resident keys, through the library's serving loop (tmce_server_*), with latency percentiles.
+
+The workload follows the serving configuration of the reference's scope: request sizes log-uniform in
+[1, 4096] ops, 50/50 enc/dec, key chosen uniformly among the resident keypairs, exponential
+inter-arrival times at a given request rate.  Decapsulation requests take ciphertexts from a pool
+made by the engine beforehand (valid ciphertexts of the chosen key).
+"""
+import threading
+import time
+
+import numpy as np
+
+import tensormceliece_amd as tm
+
+
+def log_uniform_sizes(rng, n, lo=1, hi=4096):
+    return np.minimum(hi, np.maximum(lo, np.exp(rng.uniform(np.log(lo), np.log(hi + 1), n)).astype(np.int64)))
+
+
+class Workload:
+    """Resident keys (public and secret handles) and per-key ciphertext pools for decapsulation."""
+
+    def __init__(self, set_name, keypairs, pool_per_key=4096, seed=1):
+        self.set_name = set_name
+        self.pk = [tm.PublicKey(set_name, pk) for pk, _ in keypairs]
+        self.sk = [tm.SecretKey(set_name, sk) for _, sk in keypairs]
+        rng = np.random.default_rng(seed)
+        self.pool_seeds, self.pool_ct, self.pool_k = [], [], []
+        for K in self.pk:
+            sd = rng.integers(0, 256, (pool_per_key, 32), dtype=np.uint8)
+            ct, k, st = K.enc(sd)
+            assert not st.any()
+            self.pool_seeds.append(sd)
+            self.pool_ct.append(ct)
+            self.pool_k.append(k)
+
+    def device_bytes(self):
+        return sum(k.device_bytes() for k in self.pk + self.sk)
+
+
+def make_requests(wl, n_requests, seed=2, lo=1, hi=4096):
+    """(direction, key index, inputs, pool offset) per request; enc seeds are fresh random bytes."""
+    rng = np.random.default_rng(seed)
+    sizes = log_uniform_sizes(rng, n_requests, lo, hi)
+    dirs = rng.integers(0, 2, n_requests)
+    keys = rng.integers(0, len(wl.pk), n_requests)
+    reqs = []
+    pool = wl.pool_ct[0].shape[0]
+    for n, d, k in zip(sizes, dirs, keys):
+        n = int(n)
+        if d == 0:
+            reqs.append(("enc", int(k), rng.integers(0, 256, (n, 32), dtype=np.uint8), None))
+        else:
+            off = int(rng.integers(0, pool - n + 1))
+            reqs.append(("dec", int(k), wl.pool_ct[k][off:off + n], off))
+    return reqs
+
+
+def run_poisson(server, wl, reqs, rate, keep_outputs=False):
+    """Submits `reqs` at Poisson arrival times (`rate` requests/s; rate <= 0: all at once) and
+    collects them.  Returns dict with latencies (s), throughput (ops/s) and per-request outputs."""
+    rng = np.random.default_rng(3)
+    gaps = rng.exponential(1.0 / rate, len(reqs)) if rate > 0 else np.zeros(len(reqs))
+    arrivals = np.cumsum(gaps)
+    ids = [None] * len(reqs)
+    outputs = [None] * len(reqs)
+    err = []
+    t0 = time.perf_counter()
+
+    def submitter():
+        try:
+            for i, (d, k, inp, _) in enumerate(reqs):
+                delay = arrivals[i] - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                key = wl.pk[k] if d == "enc" else wl.sk[k]
+                ids[i], outs = server.submit(d, key, inp)
+                if keep_outputs:
+                    outputs[i] = outs
+        except Exception as e:  # surfaced by the caller
+            err.append(e)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat = {}
+    while len(lat) < len(reqs):
+        got, lats = server.poll()
+        for i, l in zip(got, lats):
+            lat[i] = l
+        if err:
+            break
+        if not got:
+            time.sleep(0.0002)
+    th.join()
+    if err:
+        raise err[0]
+    t1 = time.perf_counter()
+    lat_s = np.array([lat[i] for i in ids])
+    ops = int(sum(r[2].shape[0] for r in reqs))
+    return {"latency_s": lat_s, "elapsed_s": t1 - t0, "ops": ops, "ops_per_s": ops / (t1 - t0),
+            "p50_ms": float(np.percentile(lat_s, 50) * 1e3), "p99_ms": float(np.percentile(lat_s, 99) * 1e3),
+            "outputs": outputs}

@@ -122,7 +122,7 @@ def test_mixed_keys_enc_dec(tm, orc):
             runs.append((owner[start], np.arange(start, i)))
             start = i
     rng.shuffle(runs)
-    enc_pipe = tm.Pipeline(s, "enc", n + 3 * 512, use_graph=True)
+    enc_pipe = tm.Pipeline(s, "enc", n + 3 * 256, use_graph=True)
     for rep in range(2):
         ct, k, st = run_pipeline(tm, enc_pipe, [(PKs[o], idx) for o, idx in runs], seeds, 32, p.synd_bytes)
         assert not st.any() and np.array_equal(ct, ct_ref) and np.array_equal(k, k_ref), rep
@@ -137,7 +137,7 @@ def test_mixed_keys_enc_dec(tm, orc):
     for i in range(3):
         idx = np.nonzero(owner == i)[0]
         k_dref[idx], st_dref[idx] = SKs[i].dec(cts[idx])
-    dec_pipe = tm.Pipeline(s, "dec", n + 3 * 512, use_graph=True)
+    dec_pipe = tm.Pipeline(s, "dec", n + 3 * 256, use_graph=True)
     for rep in range(2):
         _, k2, st2 = run_pipeline(tm, dec_pipe, [(SKs[o], idx) for o, idx in runs], cts, p.synd_bytes, 0)
         assert np.array_equal(k2, k_dref) and np.array_equal(st2, st_dref), rep
@@ -171,10 +171,10 @@ def test_6960119_bad_public_key_in_mix(tm, orc):
 def test_pipeline_argument_errors(tm, orc):
     s = "348864"
     pk, sk = keys(orc, s)
-    pipe = tm.Pipeline(s, "enc", 512, max_sub=4, use_graph=False)
+    pipe = tm.Pipeline(s, "enc", 256, max_sub=4, use_graph=False)
     d = tm.DeviceBuffer(4096)
     with pytest.raises(tm.TmceError):  # capacity exceeded (two keys -> two padded groups)
-        pipe.run([(tm.PublicKey(s, pk), d.ptr, d.ptr, d.ptr, None, 300),
+        pipe.run([(tm.PublicKey(s, pk), d.ptr, d.ptr, d.ptr, None, 200),
                   (tm.PublicKey(s, keys(orc, s, 1)[0]), d.ptr, d.ptr, d.ptr, None, 10)])
     with pytest.raises(tm.TmceError):  # secret key in an encapsulation pipeline
         pipe.run([(tm.SecretKey(s, sk), d.ptr, d.ptr, d.ptr, None, 1)])
