@@ -1,18 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the batched Classic McEliece KEM engine on one MI355X.
 
-One step = one batch of N encapsulations (seeded, inputs resident in HBM) followed by one batch of
-N decapsulations of the resulting ciphertexts, for one parameter set (default mceliece348864,
-N = 65536, the reference's largest batch).  Prints ONE JSON line (see the driver contract):
-value = KEM operations per second over the timed steps (N encaps + N decaps per step), with
-encaps/s and decaps/s, per-kernel HIP-event timings, a roofline object for the dominant kernel and
-an all-core CPU baseline (the C oracle, bounded sample) beside it.
+Headline step = one batch of N seeded encapsulations (inputs resident in HBM) followed by one batch
+of N decapsulations of the resulting ciphertexts, for one parameter set (default mceliece348864,
+N = 65536, the reference's largest batch).  Prints ONE JSON line: value = KEM operations per second
+over the timed steps (N encaps + N decaps per step), with encaps/s and decaps/s, per-kernel HIP-event
+timings, the plain one-launch-per-stage GPU form on the same inputs, a roofline object for the
+dominant kernel and an all-core CPU baseline (the C oracle, bounded sample).
+
+Unless --quick, the line also carries the other workloads of the scope (key "suite"):
+  syndrome  mceliece348864 syndrome alone (PKE encryption of given weight-64 errors), N = 8192 and
+            65536: INT8-MFMA 0/1-byte GEMM vs bit-packed AND+popcount VALU GEMM vs sparse column
+            gather, with the MFMA fraction of the INT8 peak;
+  per_set   every parameter set at N = 65536: engine vs plain GPU form vs CPU oracle, enc and dec;
+  c1        mceliece348864 KEM at N = 256 (latency of one batch);
+  c3        mceliece460896 and mceliece6688128 decapsulation at N = 8192;
+  c4        mceliece6960119 uneven sub-batches (sizes log-uniform in [1, 8192], total 65536) in one
+            graph-captured multi-batch pipeline run per direction, autotuned launch shapes;
+  c5        mceliece8192128 serving loop: 16 device-resident keypairs, Poisson arrivals of requests
+            of log-uniform size in [1, 4096], 50/50 enc/dec, HBM-footprint admission; throughput
+            at saturation and p50/p99 latency at two offered loads.
 """
 import argparse
 import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -20,6 +34,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+INT8_PEAK_TOPS = 5000.0  # MI355X dense INT8 MFMA
+SETS = ["348864", "460896", "6688128", "6960119", "8192128"]
 
 
 def xorshift_seeds(n, state=0x9E3779B97F4A7C15):
@@ -39,14 +55,350 @@ def algorithmic_bytes(kernel, p, n):
     """HBM bytes one launch must move at minimum (inputs once, outputs once), per kernel."""
     ct, e8 = p["ct"], p["n"] // 8
     msg = ((1 + e8 + ct) // 136 + 1) * 136
+    splanes = 4 * p["m"] * ((2 * p["t"] + 31) // 32)
     per_op = {
         "gen_e": 32 + 2 * p["t"] + msg,           # seed in; support list + message row out
         "enc_finish": 2 * p["t"] + msg + ct + 32,  # support + message in; ct + key out
-        "dec_front": ct + 4 * p["m"] * ((2 * p["t"] + 31) // 32),
-        "dec_decode": 4 * p["m"] * ((2 * p["t"] + 31) // 32) + ct + msg,
+        "dec_front": ct + splanes,
+        "dec_front_mfma": ct + splanes,
+        "dec_decode": splanes + ct + msg,
         "dec_hash": msg + ct + 32 + 4,
     }[kernel]
     return per_op * n
+
+
+def timed_kernels(tm, fn, reps):
+    tm.sync()
+    tm.timing_enable(True)
+    for _ in range(reps):
+        fn()
+    tm.sync()
+    kt = tm.timing_read()
+    tm.timing_enable(False)
+    return {k: {"launches": c, "avg_ms": tot / c} for k, (c, tot) in kt.items() if c}
+
+
+def wall(tm, fn, reps):
+    fn()
+    tm.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    tm.sync()
+    return (time.perf_counter() - t0) / reps
+
+
+def keypairs_parallel(tm, s, n, state=0x51ED):
+    seeds, _ = xorshift_seeds(n, state)
+    with ThreadPoolExecutor(max_workers=min(16, n)) as ex:
+        return list(ex.map(lambda i: tm.keypair_seeded(s, seeds[i].tobytes()), range(n)))
+
+
+def fixed_weight(p, n, rng):
+    e = np.zeros((n, p["n"] // 8), np.uint8)
+    for c0 in range(0, n, 4096):
+        c1 = min(n, c0 + 4096)
+        pos = np.argpartition(rng.random((c1 - c0, p["n"]), dtype=np.float32), p["t"], axis=1)[:, :p["t"]]
+        rows = np.repeat(np.arange(c0, c1), p["t"])
+        np.bitwise_or.at(e, (rows, pos.ravel() >> 3), (1 << (pos.ravel() & 7)).astype(np.uint8))
+    return e
+
+
+class DevKem:
+    """Device-resident inputs/outputs of one set and batch size, engine and plain form timings."""
+
+    def __init__(self, tm, s, n, keypair, seeds):
+        self.tm, self.s, self.n = tm, s, n
+        self.p = tm.PARAMS[s]
+        self.pk, self.sk = keypair
+        self.PK, self.SK = tm.PublicKey(s, self.pk), tm.SecretKey(s, self.sk)
+        self.seeds = seeds
+        self.d_seeds = tm.DeviceBuffer(n * 32)
+        self.d_ct = tm.DeviceBuffer(n * self.p["ct"])
+        self.d_k = tm.DeviceBuffer(n * 32)
+        self.d_st = tm.DeviceBuffer(n * 4)
+        self.d_k2 = tm.DeviceBuffer(n * 32)
+        self.d_st2 = tm.DeviceBuffer(n * 4)
+        self.d_seeds.upload(seeds)
+
+    def enc(self):
+        self.tm.enc_device(self.PK, self.d_ct, self.d_k, self.d_st, self.d_seeds, self.n)
+
+    def dec(self):
+        self.tm.dec_device(self.SK, self.d_k2, self.d_st2, self.d_ct, self.n)
+
+    def rates(self, reps):
+        te, td = wall(self.tm, self.enc, reps), wall(self.tm, self.dec, reps)
+        return self.n / te, self.n / td
+
+    def outputs(self):
+        n, p = self.n, self.p
+        return (self.d_ct.download((n, p["ct"])), self.d_k.download((n, 32)), self.d_k2.download((n, 32)),
+                self.d_st.download(n, np.int32), self.d_st2.download(n, np.int32))
+
+    def plain(self, reps):
+        tm = self.tm
+        tm.set_form("plain")
+        try:
+            e, d = self.rates(reps)
+            kern = timed_kernels(tm, lambda: (self.enc(), self.dec()), 1)
+            out = self.outputs()
+        finally:
+            tm.set_form("engine")
+        return e, d, kern, out
+
+    def free(self):
+        for b in (self.d_seeds, self.d_ct, self.d_k, self.d_st, self.d_k2, self.d_st2):
+            b.free()
+
+
+def cpu_rates(orc, s, pk, sk, seeds, cores, seconds):
+    """Oracle enc and dec on all host cores over a bounded sample (about `seconds` in total)."""
+    ne = max(cores, 32)
+    t0 = time.perf_counter()
+    ct_c, k_c, _ = orc.enc_batch(s, pk, seeds[:ne], cores)
+    te = time.perf_counter() - t0
+    ne2 = int(min(len(seeds), max(ne, ne * (seconds / 2) / max(te, 1e-3))))
+    t0 = time.perf_counter()
+    ct_c, k_c, _ = orc.enc_batch(s, pk, seeds[:ne2], cores)
+    te = time.perf_counter() - t0
+    nd = max(cores, 32)
+    t0 = time.perf_counter()
+    orc.dec_batch(s, sk, ct_c[:nd], cores)
+    td = time.perf_counter() - t0
+    nd2 = int(min(ne2, max(nd, nd * (seconds / 2) / max(td, 1e-3))))
+    t0 = time.perf_counter()
+    k_d, _ = orc.dec_batch(s, sk, ct_c[:nd2], cores)
+    td = time.perf_counter() - t0
+    assert np.array_equal(k_d, k_c[:nd2])
+    return ne2 / te, nd2 / td, ne2, nd2, ct_c, k_c
+
+
+# ------------------------------------------------------------------------------- suite sections
+def section_syndrome(tm, orc, log):
+    s = "348864"
+    p = tm.PARAMS[s]
+    pk, _ = tm.keypair_seeded(s, bytes(range(32)))
+    K = tm.PublicKey(s, pk)
+    out = {}
+    rng = np.random.default_rng(4)
+    for n in (8192, 65536):
+        e = fixed_weight(p, n, rng)
+        d_e, d_ct = tm.DeviceBuffer(e.nbytes), tm.DeviceBuffer(n * p["ct"])
+        d_e.upload(e)
+        ops = 2 * p["m"] * p["t"] * (p["n"] - p["m"] * p["t"]) * n  # int8 MAC-ops of the dense product
+        res, outs = {}, {}
+        for meth in ("sparse", "valu"):
+            kt = timed_kernels(tm, lambda: tm.syndrome_device(K, d_ct, d_e, n, meth), 5)
+            res[meth] = sum(v["avg_ms"] for v in kt.values())
+            outs[meth] = d_ct.download((n, p["ct"]))
+        best_cfg, best_ms = None, None
+        for cfg in range(6):
+            tm.tune_set(s, "synd_mfma_cfg", cfg)
+            kt = timed_kernels(tm, lambda: tm.syndrome_device(K, d_ct, d_e, n, "mfma"), 5)
+            ms = sum(v["avg_ms"] for v in kt.values())
+            if best_ms is None or ms < best_ms:
+                best_cfg, best_ms = cfg, ms
+        tm.tune_set(s, "synd_mfma_cfg", best_cfg)
+        tm.syndrome_device(K, d_ct, d_e, n, "mfma")
+        tm.sync()
+        res["mfma"] = best_ms
+        outs["mfma"] = d_ct.download((n, p["ct"]))
+        same = all(np.array_equal(outs["sparse"], outs[m]) for m in ("mfma", "valu"))
+        idx = rng.choice(n, 32, replace=False)
+        same = same and bool(np.array_equal(outs["mfma"][idx], orc.syndrome_batch(s, pk, e[idx])))
+        tops = ops / (best_ms * 1e-3) / 1e12
+        out[str(n)] = {
+            "ms": {k: round(v, 4) for k, v in res.items()},
+            "columns_per_s": {k: round(n / (v * 1e-3), 1) for k, v in res.items()},
+            "mfma_cfg": best_cfg, "bit_exact": same,
+            "mfma_roofline": {"bound": "mfma", "achieved": round(tops, 1), "peak": INT8_PEAK_TOPS, "unit": "TOPS",
+                              "frac": round(tops / INT8_PEAK_TOPS, 4),
+                              "work": "2*768*2720 int8 ops per column (0/1 bytes, int32 accumulate)"},
+            "valu_lane_ops_per_column": 768 * 85 * 3 // 2,
+        }
+        log(f"syndrome n={n}: {out[str(n)]['ms']}")
+        d_e.free()
+        d_ct.free()
+    return out
+
+
+def section_per_set(tm, orc, keys, cores, cpu_seconds, log, n=65536, reps=3):
+    out = {}
+    for s in SETS:
+        pk, sk = keys[s]
+        seeds = np.random.default_rng(int(s)).integers(0, 256, (n, 32), dtype=np.uint8)
+        dk = DevKem(tm, s, n, (pk, sk), seeds)
+        ee, ed = dk.rates(reps)
+        ct, k1, k2, st1, st2 = dk.outputs()
+        pe, pd, _, pout = dk.plain(max(1, reps - 1))
+        exact = bool(np.array_equal(k1, k2) and not st1.any() and not st2.any()
+                     and all(np.array_equal(a, b) for a, b in zip((ct, k1, k2), pout[:3])))
+        idx = np.random.default_rng(1).choice(n, 16, replace=False)
+        ct_o, k_o, _ = orc.enc_batch(s, pk, seeds[idx])
+        exact = exact and bool(np.array_equal(ct[idx], ct_o) and np.array_equal(k1[idx], k_o))
+        ce, cd, ne, nd, _, _ = cpu_rates(orc, s, pk, sk, seeds, cores, cpu_seconds)
+        dk.free()
+        out[s] = {"engine": {"encaps_per_s": round(ee, 1), "decaps_per_s": round(ed, 1)},
+                  "plain_gpu_form": {"encaps_per_s": round(pe, 1), "decaps_per_s": round(pd, 1)},
+                  "cpu_oracle": {"encaps_per_s": round(ce, 1), "decaps_per_s": round(cd, 1), "cores": cores,
+                                 "sample": f"{ne} enc + {nd} dec"},
+                  "speedup_vs_plain": {"encaps": round(ee / pe, 2), "decaps": round(ed / pd, 2)},
+                  "speedup_vs_cpu": {"encaps": round(ee / ce, 1), "decaps": round(ed / cd, 1)},
+                  "bit_exact": exact}
+        log(f"per_set {s}: {out[s]['engine']} plain {out[s]['plain_gpu_form']} cpu {out[s]['cpu_oracle']}")
+    return out
+
+
+def section_c1(tm, orc, keys, cores, log):
+    s, n = "348864", 256
+    pk, sk = keys[s]
+    seeds = xorshift_seeds(n, 0xC1)[0]
+    dk = DevKem(tm, s, n, (pk, sk), seeds)
+    te = wall(tm, dk.enc, 20)
+    td = wall(tm, dk.dec, 20)
+    ct, k1, k2, st1, st2 = dk.outputs()
+    tm.set_form("plain")
+    try:
+        pte, ptd = wall(tm, dk.enc, 10), wall(tm, dk.dec, 10)
+    finally:
+        tm.set_form("engine")
+    ct_o, k_o, _ = orc.enc_batch(s, pk, seeds, cores)
+    t0 = time.perf_counter()
+    orc.enc_batch(s, pk, seeds, cores)
+    cte = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    k_d, _ = orc.dec_batch(s, sk, ct_o, cores)
+    ctd = time.perf_counter() - t0
+    exact = bool(np.array_equal(ct, ct_o) and np.array_equal(k1, k_o) and np.array_equal(k2, k_d)
+                 and not st1.any() and not st2.any())
+    dk.free()
+    out = {"batch_ms": {"engine_enc": round(te * 1e3, 4), "engine_dec": round(td * 1e3, 4),
+                        "plain_enc": round(pte * 1e3, 4), "plain_dec": round(ptd * 1e3, 4),
+                        "cpu_enc": round(cte * 1e3, 3), "cpu_dec": round(ctd * 1e3, 3)},
+           "cpu_cores": cores, "bit_exact_vs_oracle": exact}
+    log(f"c1: {out}")
+    return out
+
+
+def section_c3(tm, orc, keys, log, n=8192):
+    out = {}
+    for s in ("460896", "6688128"):
+        pk, sk = keys[s]
+        seeds = np.random.default_rng(33).integers(0, 256, (n, 32), dtype=np.uint8)
+        dk = DevKem(tm, s, n, (pk, sk), seeds)
+        dk.enc()
+        td = wall(tm, dk.dec, 5)
+        kern = timed_kernels(tm, dk.dec, 3)
+        ct, k1, k2, st1, st2 = dk.outputs()
+        tm.set_form("plain")
+        try:
+            ptd = wall(tm, dk.dec, 3)
+        finally:
+            tm.set_form("engine")
+        idx = np.random.default_rng(2).choice(n, 16, replace=False)
+        k_o, _ = orc.dec_batch(s, sk, ct[idx])
+        exact = bool(np.array_equal(k1, k2) and not st2.any() and np.array_equal(k2[idx], k_o))
+        dk.free()
+        out[s] = {"decaps_per_s": round(n / td, 1), "plain_decaps_per_s": round(n / ptd, 1),
+                  "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kern.items()}, "bit_exact": exact}
+        log(f"c3 {s}: {out[s]}")
+    return out
+
+
+def section_c4(tm, keys, log, total=65536):
+    from tensormceliece_amd import autotune
+    s = "6960119"
+    p = tm.PARAMS[s]
+    pk, sk = keys[s]
+    best, _ = autotune.tune(s, total, reps=2, keypair=(pk, sk))
+    PK, SK = tm.PublicKey(s, pk), tm.SecretKey(s, sk)
+    rng = np.random.default_rng(44)
+    sizes = []
+    while sum(sizes) < total:
+        v = int(np.exp(rng.uniform(0, np.log(8193))))
+        sizes.append(max(1, min(8192, v, total - sum(sizes))))
+    seeds = rng.integers(0, 256, (total, 32), dtype=np.uint8)
+    d_seeds, d_ct = tm.DeviceBuffer(total * 32), tm.DeviceBuffer(total * p["ct"])
+    d_k, d_st = tm.DeviceBuffer(total * 32), tm.DeviceBuffer(total * 4)
+    d_k2, d_st2 = tm.DeviceBuffer(total * 32), tm.DeviceBuffer(total * 4)
+    d_seeds.upload(seeds)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    enc_subs = [(PK, d_seeds.ptr + 32 * o, d_ct.ptr + p["ct"] * o, d_k.ptr + 32 * o, d_st.ptr + 4 * o, m)
+                for o, m in zip(offs, sizes)]
+    dec_subs = [(SK, d_ct.ptr + p["ct"] * o, None, d_k2.ptr + 32 * o, d_st2.ptr + 4 * o, m)
+                for o, m in zip(offs, sizes)]
+    ep = tm.Pipeline(s, "enc", total, max_sub=len(sizes), use_graph=True)
+    dp = tm.Pipeline(s, "dec", total, max_sub=len(sizes), use_graph=True)
+
+    def run(pipe, subs):
+        pipe.run(subs)
+        pipe.sync()
+
+    for _ in range(2):  # eager run, then capture
+        run(ep, enc_subs)
+        run(dp, dec_subs)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run(ep, enc_subs)
+    te = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run(dp, dec_subs)
+    td = (time.perf_counter() - t0) / reps
+    ct = d_ct.download((total, p["ct"]))
+    k1, k2 = d_k.download((total, 32)), d_k2.download((total, 32))
+    st1, st2 = d_st.download(total, np.int32), d_st2.download(total, np.int32)
+    # the same ops as one flat engine batch, and as the plain per-stage form
+    dk = DevKem(tm, s, total, (pk, sk), seeds)
+    fe, fd = dk.rates(3)
+    fct, fk1, fk2, _, _ = dk.outputs()
+    pe, pd, _, _ = dk.plain(2)
+    dk.free()
+    exact = bool(np.array_equal(ct, fct) and np.array_equal(k1, fk1) and np.array_equal(k2, fk2)
+                 and np.array_equal(k1, k2) and not st1.any() and not st2.any())
+    for b in (d_seeds, d_ct, d_k, d_st, d_k2, d_st2):
+        b.free()
+    out = {"sub_batches": len(sizes), "total_ops": total, "tuned": best,
+           "pipeline_graph": {"encaps_per_s": round(total / te, 1), "decaps_per_s": round(total / td, 1)},
+           "flat_engine_batch": {"encaps_per_s": round(fe, 1), "decaps_per_s": round(fd, 1)},
+           "plain_gpu_form": {"encaps_per_s": round(pe, 1), "decaps_per_s": round(pd, 1)},
+           "speedup_vs_plain": {"encaps": round(total / te / pe, 2), "decaps": round(total / td / pd, 2)},
+           "bit_exact_vs_flat_engine": exact}
+    log(f"c4: {out}")
+    return out
+
+
+def section_c5(tm, log, nkeys=16, n_requests=6000):
+    from tensormceliece_amd import serving
+    s = "8192128"
+    kps = keypairs_parallel(tm, s, nkeys)
+    wl = serving.Workload(s, kps, pool_per_key=4096)
+    server = tm.Server(s, slot_capacity=32768, slots_per_direction=2, hbm_budget=8 << 30)
+    try:
+        reqs = serving.make_requests(wl, n_requests, seed=5)
+        serving.run_poisson(server, wl, reqs[:200], 0.0)  # warm-up (first runs, graph capture)
+        sat = serving.run_poisson(server, wl, reqs, 0.0)
+        mean_ops = float(np.mean([r[2].shape[0] for r in reqs]))
+        req_rate_sat = sat["ops_per_s"] / mean_ops
+        loads = {}
+        for frac in (0.5, 0.8):
+            r = serving.run_poisson(server, wl, reqs, frac * req_rate_sat)
+            loads[f"{int(frac * 100)}%"] = {"offered_requests_per_s": round(frac * req_rate_sat, 1),
+                                            "ops_per_s": round(r["ops_per_s"], 1), "p50_ms": round(r["p50_ms"], 3),
+                                            "p99_ms": round(r["p99_ms"], 3)}
+        st = server.stats()
+    finally:
+        server.close()
+    out = {"keys": nkeys, "requests_per_run": n_requests, "mean_request_ops": round(mean_ops, 1),
+           "key_device_bytes": wl.device_bytes(),
+           "saturation": {"ops_per_s": round(sat["ops_per_s"], 1), "p50_ms": round(sat["p50_ms"], 3),
+                          "p99_ms": round(sat["p99_ms"], 3)},
+           "poisson": loads, "batches": st["batches"], "requests": st["requests"],
+           "peak_in_flight_bytes": st["peak_in_flight_bytes"], "hbm_budget_bytes": 8 << 30}
+    log(f"c5: {out}")
+    return out
 
 
 def main():
@@ -60,6 +412,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-stage GPU form")
+    ap.add_argument("--quick", action="store_true", help="headline only (no suite)")
+    ap.add_argument("--sections", default="syndrome,per_set,c1,c3,c4,c5")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -69,64 +423,40 @@ def main():
               "(every op is independent: N GPUs would each run this step on N/G ops)", file=sys.stderr)
         sys.exit(2)
 
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
     import tensormceliece_amd as tm
+    from oracle import oracle as orc
     tm.init(0)
     s, n = args.set, args.n
     p = tm.PARAMS[s]
+    cores = len(os.sched_getaffinity(0))
 
     kseed, st = xorshift_seeds(1)
     pk, sk = tm.keypair_seeded(s, kseed[0].tobytes())
     seeds, _ = xorshift_seeds(n, st)
-    PK, SK = tm.PublicKey(s, pk), tm.SecretKey(s, sk)
-
-    d_seeds = tm.DeviceBuffer(n * 32)
-    d_ct = tm.DeviceBuffer(n * p["ct"])
-    d_k = tm.DeviceBuffer(n * 32)
-    d_st = tm.DeviceBuffer(n * 4)
-    d_k2 = tm.DeviceBuffer(n * 32)
-    d_st2 = tm.DeviceBuffer(n * 4)
-    d_seeds.upload(seeds)
-
-    def enc():
-        tm.enc_device(PK, d_ct, d_k, d_st, d_seeds, n)
-
-    def dec():
-        tm.dec_device(SK, d_k2, d_st2, d_ct, n)
+    dk = DevKem(tm, s, n, (pk, sk), seeds)
 
     for _ in range(args.warmup):
-        enc()
-        dec()
+        dk.enc()
+        dk.dec()
     tm.sync()
 
     # ---- timed region: K steps of (N encaps + N decaps), device-resident inputs
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        enc()
-        dec()
+        dk.enc()
+        dk.dec()
     tm.sync()
     t_step = (time.perf_counter() - t0) / args.steps
+    log(f"headline {s} n={n}: {t_step * 1e3:.3f} ms per step")
 
     # ---- per-phase and per-kernel timing (HIP events on the engine stream)
     reps = max(3, args.steps // 2)
-    tm.sync()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        enc()
-    tm.sync()
-    t_enc = (time.perf_counter() - t0) / reps
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        dec()
-    tm.sync()
-    t_dec = (time.perf_counter() - t0) / reps
-    tm.timing_enable(True)
-    for _ in range(reps):
-        enc()
-        dec()
-    tm.sync()
-    kt = tm.timing_read()
-    tm.timing_enable(False)
-    kernels = {k: {"launches": c, "avg_ms": tot / c} for k, (c, tot) in kt.items() if c}
+    t_enc = wall(tm, dk.enc, reps)
+    t_dec = wall(tm, dk.dec, reps)
+    kernels = timed_kernels(tm, lambda: (dk.enc(), dk.dec()), reps)
 
     if args.calib_copy:
         nb = args.calib_copy << 20
@@ -136,59 +466,21 @@ def main():
         src.free()
         dst.free()
 
-    # ---- the plain one-launch-per-stage form on the same device-resident inputs
+    ct, k1, k2, st1, st2 = dk.outputs()
     plain = None
     if not args.no_plain:
-        tm.set_form("plain")
-        enc()
-        dec()
-        tm.sync()
-        preps = max(2, reps // 2)
-        t0 = time.perf_counter()
-        for _ in range(preps):
-            enc()
-        tm.sync()
-        pt_enc = (time.perf_counter() - t0) / preps
-        t0 = time.perf_counter()
-        for _ in range(preps):
-            dec()
-        tm.sync()
-        pt_dec = (time.perf_counter() - t0) / preps
-        tm.timing_enable(True)
-        enc()
-        dec()
-        tm.sync()
-        pkt = tm.timing_read()
-        tm.timing_enable(False)
-        plain_ct = d_ct.download((n, p["ct"]))
-        plain_k = d_k.download((n, 32))
-        plain_k2 = d_k2.download((n, 32))
-        tm.set_form("engine")
-        enc()
-        dec()
-        tm.sync()
-        same = bool(np.array_equal(plain_ct, d_ct.download((n, p["ct"]))) and
-                    np.array_equal(plain_k, d_k.download((n, 32))) and
-                    np.array_equal(plain_k2, d_k2.download((n, 32))))
-        plain = {"encaps_per_s": round(n / pt_enc, 1), "decaps_per_s": round(n / pt_dec, 1),
-                 "kem_ops_per_s": round(2 * n / (pt_enc + pt_dec), 1),
-                 "identical_to_engine": same,
-                 "kernels": {k: {"launches": c, "avg_ms": tot / c} for k, (c, tot) in pkt.items() if c}}
+        pe, pd, pkern, pout = dk.plain(max(2, reps // 2))
+        same = all(np.array_equal(a, b) for a, b in zip((ct, k1, k2), pout[:3]))
+        plain = {"encaps_per_s": round(pe, 1), "decaps_per_s": round(pd, 1),
+                 "kem_ops_per_s": round(2 / (1 / pe + 1 / pd), 1), "identical_to_engine": same,
+                 "kernels": pkern}
 
-    # ---- correctness of the measured batch
     verified = None
     if not args.no_verify:
-        ct = d_ct.download((n, p["ct"]))
-        k1 = d_k.download((n, 32))
-        k2 = d_k2.download((n, 32))
-        st1 = d_st.download(n, np.int32)
-        st2 = d_st2.download(n, np.int32)
         ok = bool(np.array_equal(k1, k2) and not st1.any() and not st2.any())
-        from oracle import oracle as orc
         idx = np.random.default_rng(0).choice(n, min(n, 256), replace=False)
         ct_ref, k_ref, _ = orc.enc_batch(s, pk, seeds[idx])
-        ok = ok and bool(np.array_equal(ct[idx], ct_ref) and np.array_equal(k1[idx], k_ref))
-        verified = ok
+        verified = ok and bool(np.array_equal(ct[idx], ct_ref) and np.array_equal(k1[idx], k_ref))
 
     # ---- roofline of the dominant kernel (HBM; algorithmic bytes per launch / launch time)
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
@@ -205,34 +497,36 @@ def main():
                 "note": "the decoder is LDS/VALU bound (table lookups, BM recurrences); HBM traffic is at "
                         "the algorithmic minimum"}
 
-    # ---- CPU baseline: the oracle on all host cores, bounded sample of the same workload
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as orc
-        cores = len(os.sched_getaffinity(0))
-        ne = max(cores, 64)
-        t0 = time.perf_counter()
-        ct_c, k_c, _ = orc.enc_batch(s, pk, seeds[:ne], cores)
-        te = time.perf_counter() - t0
-        # scale the sample to ~cpu_seconds/2 for each of enc and dec
-        ne2 = int(min(n, max(ne, ne * (args.cpu_seconds / 2) / max(te, 1e-3))))
-        t0 = time.perf_counter()
-        ct_c, k_c, _ = orc.enc_batch(s, pk, seeds[:ne2], cores)
-        te = time.perf_counter() - t0
-        nd = max(cores, 64)
-        t0 = time.perf_counter()
-        orc.dec_batch(s, sk, ct_c[:nd], cores)
-        td = time.perf_counter() - t0
-        nd2 = int(min(ne2, max(nd, nd * (args.cpu_seconds / 2) / max(td, 1e-3))))
-        t0 = time.perf_counter()
-        k_d, _ = orc.dec_batch(s, sk, ct_c[:nd2], cores)
-        td = time.perf_counter() - t0
-        assert np.array_equal(k_d, k_c[:nd2])
-        enc_s, dec_s = ne2 / te, nd2 / td
-        cpu = {"value": round(2.0 / (1.0 / enc_s + 1.0 / dec_s), 1), "unit": "KEM ops/s", "cores": cores,
-               "kind": "port", "encaps_per_s": round(enc_s, 1), "decaps_per_s": round(dec_s, 1),
-               "sample": f"oracle C restatement, {ne2} encaps + {nd2} decaps of the same seeded {s} inputs, "
+        ce, cd, ne, nd, _, _ = cpu_rates(orc, s, pk, sk, seeds, cores, args.cpu_seconds)
+        cpu = {"value": round(2.0 / (1.0 / ce + 1.0 / cd), 1), "unit": "KEM ops/s", "cores": cores,
+               "kind": "port", "encaps_per_s": round(ce, 1), "decaps_per_s": round(cd, 1),
+               "sample": f"oracle C restatement, {ne} encaps + {nd} decaps of the same seeded {s} inputs, "
                          f"{cores} threads"}
+    dk.free()
+
+    suite = None
+    if not args.quick:
+        sections = args.sections.split(",")
+        suite = {}
+        keys = {}
+        if {"per_set", "c1", "c3", "c4"} & set(sections):
+            with ThreadPoolExecutor(max_workers=5) as ex:
+                kp = list(ex.map(lambda x: tm.keypair_seeded(x, xorshift_seeds(1)[0][0].tobytes()), SETS))
+            keys = dict(zip(SETS, kp))
+        if "syndrome" in sections:
+            suite["syndrome_348864"] = section_syndrome(tm, orc, log)
+        if "per_set" in sections:
+            suite["per_set_n65536"] = section_per_set(tm, orc, keys, cores, min(4.0, args.cpu_seconds), log)
+        if "c1" in sections:
+            suite["c1_348864_n256"] = section_c1(tm, orc, keys, cores, log)
+        if "c3" in sections:
+            suite["c3_decap_n8192"] = section_c3(tm, orc, keys, log)
+        if "c4" in sections:
+            suite["c4_6960119_uneven"] = section_c4(tm, keys, log)
+        if "c5" in sections:
+            suite["c5_8192128_serving"] = section_c5(tm, log)
 
     kem_ops_per_s = 2 * n / t_step
     line = {
@@ -254,6 +548,7 @@ def main():
         "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "suite": suite,
     }
     print(json.dumps(line))
 

@@ -283,8 +283,9 @@ class Pipeline:
         with device addresses as ints."""
         subs = list(subs)
         arr = (SubBatchC * max(1, len(subs)))()
+        addr = lambda x: None if x is None else int(x)
         for i, (key, inp, ct, k, st, n) in enumerate(subs):
-            arr[i] = SubBatchC(key.handle, inp, ct, k, st, n)
+            arr[i] = SubBatchC(key.handle, addr(inp), addr(ct), addr(k), addr(st), int(n))
         _check(load().tmce_pipeline_run(self._h, ctypes.cast(arr, ctypes.c_void_p), len(subs)), "tmce_pipeline_run")
 
     def sync(self):
