@@ -35,7 +35,7 @@ __global__ __launch_bounds__(FRONT_WG) void k_dec_front(const uint8_t *__restric
   if (plan) {  // key groups are PLAN_ALIGN-aligned: one key per workgroup
     nops = plan->total;
     if ((int)(blockIdx.x * FRONT_WG) >= nops) return;
-    F = reinterpret_cast<const uint4 *>(plan->keys[plan_group(plan, blockIdx.x * FRONT_WG)].F);
+    F = uniform_ptr(reinterpret_cast<const uint4 *>(plan->keys[plan_group(plan, blockIdx.x * FRONT_WG)].F));
   }
   const int tid = threadIdx.x;
   const int op = blockIdx.x * FRONT_WG + tid;
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(FRONT_WG) void k_dec_front(const uint8_t *__restric
   }
   for (int rw = 0; rw < P::RW; ++rw) {
     __syncthreads();
-    for (int i = tid; i < 32 * Q4; i += FRONT_WG) rows[i] = F[(size_t)rw * 32 * Q4 + i];
+    for (int i = tid; i < 32 * Q4; i += FRONT_WG) rows[i] = global_load16(F, (size_t)rw * 32 * Q4 + i);
     __syncthreads();
     uint32_t word = 0;
 #pragma unroll 4
@@ -97,7 +97,8 @@ __device__ __forceinline__ uint32_t row_xor16(uint32_t v) {
 }
 
 template <int SET>
-__device__ __forceinline__ uint32_t msg_byte(int b, bool ok, const uint8_t *ebytes, const uint8_t *rej,
+__device__ __forceinline__ uint32_t msg_byte(int b, bool ok, const uint8_t *ebytes,
+                                             const __attribute__((address_space(1))) uint8_t *rej,
                                              const uint8_t *ctrow) {
   using P = Par<SET>;
   constexpr int EB = P::N / 8;
@@ -229,13 +230,13 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
       if (op >= nops) break;
       // key material of the op (one key, or its key group under a plan)
       const uint16_t *k_alpha = a.alpha, *k_la = a.la, *k_lu = a.lu;
-      const uint8_t *k_rej = a.rej;
-      if (a.plan) {
-        const KeyDev &kd = a.plan->keys[plan_group(a.plan, op)];
-        k_alpha = kd.alpha;
-        k_la = kd.la;
-        k_lu = kd.lu;
-        k_rej = kd.rej;
+      const uint8_t *k_rej = a.rej;  // (loads below go through global_ptr: these may come from memory)
+      if (a.plan) {  // the wave's ops share a key group (groups are PLAN_ALIGN-aligned)
+        const KeyDev &kd = a.plan->keys[plan_group(a.plan, __builtin_amdgcn_readfirstlane(op))];
+        k_alpha = uniform_ptr(kd.alpha);
+        k_la = uniform_ptr(kd.la);
+        k_lu = uniform_ptr(kd.lu);
+        k_rej = uniform_ptr(kd.rej);
       }
       uint16_t *fq = w.f[q];
       // additive FFT, top-down on the D coefficients: twist, Taylor at x^2+x, even/odd split
@@ -322,7 +323,7 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
       for (int it = 0; it < ceil_div(P::N, 64); ++it) {
         const int i = it * 64 + lane;
         const bool valid = i < P::N;
-        const uint32_t av = valid ? k_alpha[i] : 0u;
+        const uint32_t av = valid ? global_ptr(k_alpha)[i] : 0u;
         const bool ei = valid && ((w.Z[av >> 5] >> (av & 31)) & 1u);
         const uint64_t mask = __ballot(ei);
         if (ei) {
@@ -343,7 +344,7 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
         for (int k = 0; k < JS; ++k) acc[k] = 0;
         for (int k = 0; k < T; ++k) {
           const int p = w.pos[k];
-          const uint32_t la = k_la[p], lu = k_lu[p];
+          const uint32_t la = global_ptr(k_la)[p], lu = global_ptr(k_lu)[p];
           if (lu == 0xFFFFu) continue;
           if (la == 0xFFFFu) {
             if (lane == 0) acc[0] ^= ex_s[lu];
@@ -378,7 +379,7 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
       for (int k = lane; k < P::MSG_WORDS; k += 64) {
         uint32_t v = 0;
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq) v |= msg_byte<SET>(4 * k + qq, ok, ebytes, k_rej, ctrow) << (8 * qq);
+        for (int qq = 0; qq < 4; ++qq) v |= msg_byte<SET>(4 * k + qq, ok, ebytes, global_ptr(k_rej), ctrow) << (8 * qq);
         mrow[k] = v;
       }
       wave_lds_sync();

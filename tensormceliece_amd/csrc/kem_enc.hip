@@ -202,10 +202,10 @@ __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__
   const bool live = gop < nops;
   const int op = live ? gop : nops - 1;
   bool pad_bad = false;  // LM:crypto_kem/6960119/vec/kem_enc.c: ct and k cleared, return -1
-  if (plan) {
-    const KeyDev &kd = plan->keys[plan_group(plan, op)];
-    cols = reinterpret_cast<const uint4 *>(kd.cols);
-    pad_bad = kd.pad_bad != 0;
+  if (plan) {  // key groups are PLAN_ALIGN-aligned: one key per 256-op workgroup
+    const KeyDev &kd = plan->keys[plan_group(plan, blockIdx.x * blockDim.x)];
+    cols = uniform_ptr(reinterpret_cast<const uint4 *>(kd.cols));
+    pad_bad = __builtin_amdgcn_readfirstlane(kd.pad_bad) != 0;
   }
 
   // ---- c = sum of columns over the support of e
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__
           const uint4 *col = cols + (size_t)p * C4;
 #pragma unroll
           for (int q = 0; q < C4; ++q) {
-            const uint4 x = col[q];
+            const uint4 x = global_load16(col, q);
             c[q].x ^= x.x; c[q].y ^= x.y; c[q].z ^= x.z; c[q].w ^= x.w;
           }
         }

@@ -44,6 +44,27 @@ __device__ __forceinline__ int plan_group(const PlanDev *p, int op) {
   while (g + 1 < p->ngroups && op >= p->gstart[g + 1]) ++g;
   return g;
 }
+// a pointer that is the same in every lane, kept in scalar registers (so address arithmetic on it
+// stays scalar); the caller guarantees uniformity
+template <class T>
+__device__ __forceinline__ T *uniform_ptr(T *p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return reinterpret_cast<T *>(((uint64_t)hi << 32) | lo);
+}
+// the same pointer typed as global memory: loads through it are global_load (a pointer read from
+// memory is otherwise generic and its loads become flat_load)
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *global_ptr(const T *p) {
+  return (const __attribute__((address_space(1))) T *)p;
+}
+typedef unsigned v4u32 __attribute__((ext_vector_type(4)));
+// 16-byte global load of element i
+__device__ __forceinline__ uint4 global_load16(const uint4 *p, size_t i) {
+  const v4u32 v = ((const __attribute__((address_space(1))) v4u32 *)p)[i];
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
 // sub-batch holding combined op `op`, or -1 for padding
 __device__ __forceinline__ int plan_sub(const PlanDev *p, int op) {
   int lo = 0, hi = p->nsub - 1, r = -1;

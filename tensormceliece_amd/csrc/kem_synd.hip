@@ -139,7 +139,7 @@ __global__ __launch_bounds__(64 * WO * WR) void k_gf2_mfma(const uint32_t *__res
   if (plan) {  // key groups are PLAN_ALIGN-aligned (a multiple of BM): one key per workgroup
     nops = plan->total;
     if (op0 >= nops) return;
-    Msw = reinterpret_cast<const uint4 *>(G::plan_matrix(plan->keys[plan_group(plan, op0)]));
+    Msw = uniform_ptr(reinterpret_cast<const uint4 *>(G::plan_matrix(plan->keys[plan_group(plan, op0)])));
   }
   const int rt0 = blockIdx.y * S::NF;  // first row tile of the block
   const uint32_t lds0 = lds_off(lds);
