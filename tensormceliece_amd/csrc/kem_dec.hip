@@ -318,20 +318,46 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
         }
       }
       wave_lds_sync();
-      // roots at the support
+      // roots at the support: lane l takes the 8 consecutive support elements 512 c + 8 l .. +7
+      // (16-byte loads, issued four at a time); their root flags form byte 64 c + l of e
+      constexpr int NC = ceil_div(P::N, 512), CH = 4;
+      static_assert(P::N % 8 == 0, "support read in groups of 8");
       int wt = 0;
-      for (int it = 0; it < ceil_div(P::N, 64); ++it) {
-        const int i = it * 64 + lane;
-        const bool valid = i < P::N;
-        const uint32_t av = valid ? global_ptr(k_alpha)[i] : 0u;
-        const bool ei = valid && ((w.Z[av >> 5] >> (av & 31)) & 1u);
-        const uint64_t mask = __ballot(ei);
-        if (ei) {
-          const int idx = wt + (int)mbcnt64(mask);
-          if (idx < T) w.pos[idx] = (uint16_t)i;
+      uint8_t *eb8 = reinterpret_cast<uint8_t *>(w.ebits);
+#pragma unroll
+      for (int c0 = 0; c0 < NC; c0 += CH) {
+        uint4 av4[CH];
+#pragma unroll
+        for (int cc = 0; cc < CH; ++cc) {
+          const int i0 = 512 * (c0 + cc) + 8 * lane;
+          av4[cc] = (c0 + cc < NC && i0 < P::N) ? global_load16(reinterpret_cast<const uint4 *>(k_alpha + i0), 0)
+                                                : make_uint4(0, 0, 0, 0);
         }
-        if (lane == 0) { w.ebits[2 * it] = (uint32_t)mask; w.ebits[2 * it + 1] = (uint32_t)(mask >> 32); }
-        wt += __popcll(mask);
+#pragma unroll
+        for (int cc = 0; cc < CH; ++cc) {
+          const int c = c0 + cc;
+          if (c >= NC) break;
+          const int i0 = 512 * c + 8 * lane;
+          const uint32_t pr[4] = {av4[cc].x, av4[cc].y, av4[cc].z, av4[cc].w};
+          uint32_t byte = 0;
+#pragma unroll
+          for (int b = 0; b < 8; ++b) {
+            const uint32_t av = (pr[b >> 1] >> (16 * (b & 1))) & 0xFFFFu;
+            byte |= ((w.Z[av >> 5] >> (av & 31)) & 1u) << b;
+          }
+          if (i0 >= P::N) byte = 0;
+          if (64 * c + lane < ceil_div(P::N, 64) * 8) eb8[64 * c + lane] = (uint8_t)byte;
+#pragma unroll
+          for (int b = 0; b < 8; ++b) {
+            const bool ei = (byte >> b) & 1u;
+            const uint64_t mask = __ballot(ei);
+            if (ei) {
+              const int idx = wt + (int)mbcnt64(mask);
+              if (idx < T) w.pos[idx] = (uint16_t)(i0 + b);
+            }
+            wt += __popcll(mask);
+          }
+        }
       }
       wave_lds_sync();
       // re-encryption: syndromes of e (sum over its t positions of alpha^j / g(alpha)^2) must
@@ -342,9 +368,21 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
         uint32_t acc[JS];
 #pragma unroll
         for (int k = 0; k < JS; ++k) acc[k] = 0;
+        // the log tables at the t positions, lane k holding position k (one gather, no
+        // dependent loads in the loop below)
+        constexpr int KP = ceil_div(T, 64);
+        uint32_t la_r[KP], lu_r[KP];
+#pragma unroll
+        for (int kk = 0; kk < KP; ++kk) {
+          const int k = 64 * kk + lane;
+          const int p = k < T ? w.pos[k] : 0;
+          la_r[kk] = global_ptr(k_la)[p];
+          lu_r[kk] = k < T ? (uint32_t)global_ptr(k_lu)[p] : 0xFFFFu;
+        }
+#pragma unroll 1
         for (int k = 0; k < T; ++k) {
-          const int p = w.pos[k];
-          const uint32_t la = global_ptr(k_la)[p], lu = global_ptr(k_lu)[p];
+          const uint32_t la = (uint32_t)__builtin_amdgcn_readlane((int)la_r[k >> 6], k & 63);
+          const uint32_t lu = (uint32_t)__builtin_amdgcn_readlane((int)lu_r[k >> 6], k & 63);
           if (lu == 0xFFFFu) continue;
           if (la == 0xFFFFu) {
             if (lane == 0) acc[0] ^= ex_s[lu];
