@@ -218,13 +218,13 @@ static void build_dec_key_t(const uint8_t *sk, DecKeyHost &o) {
     o.la[i] = F.logv(a);
     o.lu[i] = F.logv(u[i]);
   }
-  // front-end matrix: row b*TT + j, column i (i < KBITS): bit b of alpha_i^j u_i
+  // front-end matrix: row j*M + b, column i (i < KBITS): bit b of alpha_i^j u_i (syndrome-major)
   o.F.assign((size_t)P::R * P::KWP, 0);
   for (int i = 0; i < P::KBITS; ++i) {
     uint16_t v = u[i];
     for (int j = 0; j < 2 * P::T; ++j) {
       for (int b = 0; b < P::M; ++b)
-        if ((v >> b) & 1) o.F[(size_t)(b * P::TT + j) * P::KWP + i / 32] |= 1u << (i % 32);
+        if ((v >> b) & 1) o.F[(size_t)(j * P::M + b) * P::KWP + i / 32] |= 1u << (i % 32);
       v = F.mul(v, o.alpha[i]);
     }
   }

@@ -78,7 +78,7 @@ struct DecWave {
   uint16_t tmp[P::D];
   uint16_t pos[P::IND_STRIDE];    // error positions of the op being finished
   uint32_t Z[(1 << P::M) / 32];   // zero flags of sigma over all field elements
-  uint32_t ebits[ceil_div(P::N, 64) * 2];
+  uint64_t m64[17 * P::MSG_BLOCKS];  // session-key message of the op being finished
 };
 
 template <int CTRL>
@@ -96,20 +96,6 @@ __device__ __forceinline__ uint32_t row_xor16(uint32_t v) {
   return v;
 }
 
-template <int SET>
-__device__ __forceinline__ uint32_t msg_byte(int b, bool ok, const uint8_t *ebytes,
-                                             const __attribute__((address_space(1))) uint8_t *rej,
-                                             const uint8_t *ctrow) {
-  using P = Par<SET>;
-  constexpr int EB = P::N / 8;
-  uint32_t v = 0;
-  if (b == 0) v = ok ? 1u : 0u;
-  else if (b <= EB) v = ok ? ebytes[b - 1] : rej[b - 1];
-  else if (b <= EB + P::SYND_BYTES) v = ctrow[b - 1 - EB];
-  if (b == P::MSG_LEN) v |= 0x1Fu;
-  if (b == 136 * P::MSG_BLOCKS - 1) v |= 0x80u;
-  return v;
-}
 
 // One wavefront decodes four ops at a time: Berlekamp-Massey runs on the four 16-lane rows in
 // parallel (coefficient i of op `row` lives in lane 16*row + i%16, slot i/16; row reductions and
@@ -147,19 +133,12 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
   DecWave<SET> &w = Wv[wave];
 
   for (int base = (blockIdx.x * DEC_WAVES + wave) * 4; base < nops; base += gridDim.x * DEC_WAVES * 4) {
-    // ---- 1. syndromes of the four ops from their bit planes
+    // ---- 1. syndromes of the four ops (syndrome-major bits from the front end)
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
       const int op = base + q;
-      for (int j = lane; j < P::TT; j += 64) {
-        uint32_t v = 0;
-        if (op < nops) {
-          const uint32_t *Srow = a.S + (size_t)op * P::RW;
-#pragma unroll
-          for (int b = 0; b < M; ++b) v |= ((Srow[b * (P::TT / 32) + (j >> 5)] >> (j & 31)) & 1u) << b;
-        }
-        w.s[q][j] = (uint16_t)v;
-      }
+      const uint32_t *Srow = a.S + (size_t)(op < nops ? op : 0) * P::RW;
+      for (int j = lane; j < P::TT; j += 64) w.s[q][j] = (uint16_t)(op < nops ? synd_value<P>(Srow, j) : 0u);
     }
     wave_lds_sync();
 
@@ -238,6 +217,16 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
         k_lu = uniform_ptr(kd.lu);
         k_rej = uniform_ptr(kd.rej);
       }
+      // the ciphertext (for the session-key message), fetched now, stored to LDS after the top-down
+      // FFT
+      constexpr int CH16 = P::SYND_BYTES / 2, CPL = ceil_div(CH16, 64);
+      uint32_t ctw[CPL];
+      {
+        const uint16_t *ct16 = reinterpret_cast<const uint16_t *>(a.ct + (size_t)op * P::SYND_BYTES);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) ctw[c] = 64 * c + lane < CH16 ? (uint32_t)ct16[64 * c + lane] : 0u;
+      }
+      uint8_t *m8 = reinterpret_cast<uint8_t *>(w.m64);
       uint16_t *fq = w.f[q];
       // additive FFT, top-down on the D coefficients: twist, Taylor at x^2+x, even/odd split
 #pragma unroll 1
@@ -266,6 +255,15 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
         wave_lds_sync();
         for (int c = lane; c < Dn; c += 64) fq[c] = w.tmp[c];
         wave_lds_sync();
+      }
+      // ciphertext bytes into the message buffer (before the butterflies need the registers)
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) {
+        const int h = 64 * c + lane;
+        if (h < CH16) {
+          m8[1 + P::N / 8 + 2 * h] = (uint8_t)ctw[c];
+          m8[2 + P::N / 8 + 2 * h] = (uint8_t)(ctw[c] >> 8);
+        }
       }
       // bottom-up butterflies in registers: V2[r] packs point 64r+lane (low half) and point
       // 64(r+NR/2)+lane (high half); the two butterflies of a packed pair share their multiplier
@@ -319,11 +317,11 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
       }
       wave_lds_sync();
       // roots at the support: lane l takes the 8 consecutive support elements 512 c + 8 l .. +7
-      // (16-byte loads, issued four at a time); their root flags form byte 64 c + l of e
+      // (16-byte loads, issued four at a time); their root flags form byte 64 c + l of e, kept at
+      // message byte 1 + 64 c + l
       constexpr int NC = ceil_div(P::N, 512), CH = 4;
       static_assert(P::N % 8 == 0, "support read in groups of 8");
       int wt = 0;
-      uint8_t *eb8 = reinterpret_cast<uint8_t *>(w.ebits);
 #pragma unroll
       for (int c0 = 0; c0 < NC; c0 += CH) {
         uint4 av4[CH];
@@ -346,7 +344,7 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
             byte |= ((w.Z[av >> 5] >> (av & 31)) & 1u) << b;
           }
           if (i0 >= P::N) byte = 0;
-          if (64 * c + lane < ceil_div(P::N, 64) * 8) eb8[64 * c + lane] = (uint8_t)byte;
+          if (64 * c + lane < P::N / 8) m8[1 + 64 * c + lane] = (uint8_t)byte;
 #pragma unroll
           for (int b = 0; b < 8; ++b) {
             const bool ei = (byte >> b) & 1u;
@@ -410,15 +408,23 @@ __global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
         }
         ok = !__any(mism);
       }
-      // session-key message [b][b ? e : s][c][padding]
-      const uint8_t *ebytes = reinterpret_cast<const uint8_t *>(w.ebits);
-      const uint8_t *ctrow = a.ct + (size_t)op * P::SYND_BYTES;
-      uint32_t *mrow = a.msg + (size_t)op * P::MSG_WORDS;
-      for (int k = lane; k < P::MSG_WORDS; k += 64) {
-        uint32_t v = 0;
-#pragma unroll
-        for (int qq = 0; qq < 4; ++qq) v |= msg_byte<SET>(4 * k + qq, ok, ebytes, global_ptr(k_rej), ctrow) << (8 * qq);
-        mrow[k] = v;
+      // session-key message [b][b ? e : s][c][SHAKE padding] assembled in LDS, stored as 8-byte words
+      {
+        constexpr int EB = P::N / 8, MB = 136 * P::MSG_BLOCKS;
+        if (lane == 0) m8[0] = ok ? 1 : 0;
+        if (!ok) {  // implicit rejection: the key's string s replaces e (ok is uniform in the wave)
+          const auto rj = global_ptr(k_rej);
+          for (int b = lane; b < EB; b += 64) m8[1 + b] = rj[b];
+        }
+        for (int b = P::MSG_LEN + lane; b < MB; b += 64) {
+          uint32_t v = 0;
+          if (b == P::MSG_LEN) v |= 0x1Fu;
+          if (b == MB - 1) v |= 0x80u;
+          m8[b] = (uint8_t)v;
+        }
+        wave_lds_sync();
+        uint64_t *mrow = reinterpret_cast<uint64_t *>(a.msg + (size_t)op * P::MSG_WORDS);
+        for (int k = lane; k < MB / 8; k += 64) mrow[k] = w.m64[k];
       }
       wave_lds_sync();
     }

@@ -80,6 +80,16 @@ hipError_t launch_plan_gather(const PlanDev *plan, uint8_t *flat, int row_bytes,
 hipError_t launch_plan_scatter(const PlanDev *plan, const uint8_t *ct, int ct_bytes, const uint8_t *key,
                                const int *status, int cap, hipStream_t s);
 
+// Syndrome bits of one decapsulation op (output of the front end, RW words): syndrome j occupies
+// bits [j*M, j*M + M) (syndrome-major, so one value is at most two words).
+template <class P>
+__device__ __forceinline__ uint32_t synd_value(const uint32_t *Srow, int j) {
+  const int bit = j * P::M, w = bit >> 5, sh = bit & 31;
+  uint64_t v = Srow[w];
+  if (sh + P::M > 32) v |= (uint64_t)Srow[w + 1] << 32;
+  return (uint32_t)(v >> sh) & ((1u << P::M) - 1u);
+}
+
 // Device buffers of one encapsulation batch (all pointers are device pointers).
 struct EncArgs {
   const uint8_t *seeds;  // [n][32]

@@ -188,7 +188,7 @@ __global__ __launch_bounds__(PG_WG) void k_plain_bm(const uint32_t *__restrict__
   const uint32_t *Srow = Sbits + (size_t)op * P::RW;
   for (int j = 0; j < 2 * T; ++j) {
     uint32_t v = 0;
-    for (int b = 0; b < P::M; ++b) v |= ((Srow[b * (P::TT / 32) + (j >> 5)] >> (j & 31)) & 1u) << b;
+    v = synd_value<P>(Srow, j);
     Ss[j * PG_WG + tid] = (uint16_t)v;
   }
   for (int i = 0; i < TS; ++i) { Cs[i * PG_WG + tid] = i == 0; Bs[i * PG_WG + tid] = i == 1; }
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(256) void k_plain_reenc(const uint32_t *__restrict_
       const int j = k * 64 + lane;
       if (j >= 2 * T) continue;
       uint32_t s = 0;
-      for (int b = 0; b < M; ++b) s |= ((Srow[b * (P::TT / 32) + (j >> 5)] >> (j & 31)) & 1u) << b;
+      s = synd_value<P>(Srow, j);
       mism |= acc[k] != s;
     }
     ok = !__any(mism);

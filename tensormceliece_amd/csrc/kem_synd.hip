@@ -259,7 +259,7 @@ struct GemmSynd {
   __device__ static const void *plan_matrix(const KeyDev &) { return nullptr; }  // no plan form
 };
 
-// Decapsulation front end: rows = R (bit b of syndrome j at row b*TT + j), x = ciphertext bits
+// Decapsulation front end: rows = R (bit b of syndrome j at row j*M + b), x = ciphertext bits
 // (KBITS), out = syndrome bit planes [RW] words.
 template <int SET>
 struct GemmFront {
