@@ -102,8 +102,11 @@ __device__ __forceinline__ uint32_t row_xor16(uint32_t v) {
 // the x-shifts of B and of the syndrome window are DPP row operations), then the wave finishes the
 // ops one by one: additive FFT with the 2^m values in registers (point p = 64*r + lane, so every
 // butterfly pairs two registers of one lane), root flags, weight and re-encryption checks, message.
+// m = 12: at most 128 VGPRs, so two 8-wave workgroups share a CU (LDS holds two); m = 13 keeps the
+// FFT's 2^13 values in registers (two waves per SIMD).
 template <int SET>
-__global__ __launch_bounds__(DEC_WG) void k_dec_decode(DecArgs a) {
+__global__ __launch_bounds__(DEC_WG) __attribute__((amdgpu_waves_per_eu(Par<SET>::M == 12 ? 4 : 1)))
+void k_dec_decode(DecArgs a) {
   using P = Par<SET>;
   constexpr int M = P::M, T = P::T, Dn = P::D, KD = P::KD, Q = P::Q;
   constexpr int NPTS = 1 << M;
@@ -473,7 +476,7 @@ hipError_t launch_dec(const DecArgs &a, hipStream_t s) {
                k_dec_front<SET><<<ceil_div(a.nops, FRONT_WG), FRONT_WG, 0, s>>>(
                    a.ct, reinterpret_cast<const uint4 *>(a.F), a.S, a.nops, a.plan));
   }
-  int grid = a.decode_blocks > 0 ? a.decode_blocks : 256;
+  int grid = a.decode_blocks > 0 ? a.decode_blocks : (P::M == 12 ? 512 : 256);
   const int maxgrid = ceil_div(a.nops, DEC_WAVES * 4);
   if (grid > maxgrid) grid = maxgrid;
   TMCE_TIMED("dec_decode", s, k_dec_decode<SET><<<grid, DEC_WG, 0, s>>>(a));
