@@ -19,8 +19,8 @@ CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 CANDIDATES = {
     "synd_mfma_cfg": list(range(6)),
     "front": [(0, 0)] + [(1, c) for c in range(6)],  # (front_mfma, front_cfg)
-    "gen_blocks": [256, 512, 1024, 2048, 4096],
-    "decode_blocks": [128, 256, 512, 1024, 2048],
+    "gen_blocks": [32, 64, 96, 128, 192, 256, 512],
+    "decode_blocks": [128, 256, 384, 512, 768, 1024],
 }
 
 

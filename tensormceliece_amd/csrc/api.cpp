@@ -152,6 +152,7 @@ int enc_device(tmce_key *k, uint8_t *d_ct, uint8_t *d_k, int *d_status, const ui
   a.queue = e->queue.as<unsigned>();
   a.nops = (int)n;
   a.gen_blocks = e->gen_blocks[k->set];
+  a.gen_form = e->gen_form[k->set];
   HIPCHK(launch_enc_set(k->set, a, s));
   return 0;
 }
@@ -492,6 +493,10 @@ int tmce_tune_set(int set, int param, int value) {
       if (value < 0 || value >= SYND_MFMA_CFGS) return fail(TMCE_EINVAL, "front_cfg out of range");
       e->front_cfg[set] = value;
       return 0;
+    case TMCE_TUNE_GEN_FORM:
+      if (value != 0 && value != 1) return fail(TMCE_EINVAL, "gen_form must be 0 or 1");
+      e->gen_form[set] = value;
+      return 0;
   }
   return fail(TMCE_EINVAL, "tmce_tune_set: unknown parameter");
 }
@@ -507,6 +512,7 @@ int tmce_tune_get(int set, int param) {
     case TMCE_TUNE_DECODE_BLOCKS: return e->decode_blocks[set];
     case TMCE_TUNE_FRONT_MFMA: return e->front_mfma[set];
     case TMCE_TUNE_FRONT_CFG: return e->front_cfg[set];
+    case TMCE_TUNE_GEN_FORM: return e->gen_form[set];
   }
   return fail(TMCE_EINVAL, "tmce_tune_get: unknown parameter");
 }

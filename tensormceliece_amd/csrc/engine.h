@@ -106,6 +106,7 @@ struct Engine {
   int decode_blocks[NUM_SETS] = {0, 0, 0, 0, 0};
   int front_mfma[NUM_SETS] = {1, 1, 1, 1, 1};  // decapsulation front end: 1 = INT8 MFMA, 0 = VALU
   int front_cfg[NUM_SETS] = {1, 1, 1, 1, 1};   // its MFMA tile shape
+  int gen_form[NUM_SETS] = {1, 1, 1, 1, 1};    // sampler: 1 = pipelined lane groups, 0 = one op per lane
   EventTimer timer;
 };
 

@@ -186,6 +186,218 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e(const uint8_t *__restrict__ se
   }
 }
 
+// Pipelined sampler.  An op's rejection chain is sequential only through the first SHAKE block of
+// each attempt (attempt k+1's state is the first 32 bytes of attempt k's first block), so a group of
+// G lanes works on one op: group lane 0 (the chain lane) computes first blocks S1_0, S1_1, ... one per
+// step and hands each to an evaluator lane (1 .. B-1 in rotation), which squeezes the attempt's
+// other B-1 blocks over the next B-1 steps and collects the first t draws below n.  Attempts thus
+// finish in order, one per step, B-1 steps after their first block; a finished attempt with t
+// draws goes through the collision check (as in k_gen_e), the first accepted one is the op's e
+// (libmceliece's semantics), later speculative attempts are dropped and the group takes the next op
+// from the queue.  Each step is one Keccak-f per lane.  The longest chain of a batch shrinks from
+// (attempts x B) permutations to (attempts + B - 1).
+template <int SET>
+__global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict__ seeds,
+                                                       uint16_t *__restrict__ ind_out, uint32_t *__restrict__ msg,
+                                                       uint32_t *__restrict__ attempts, int nops,
+                                                       unsigned *__restrict__ queue, const PlanDev *__restrict__ plan) {
+  using P = Par<SET>;
+  constexpr int B = P::DRAW_BLOCKS;
+  constexpr int G = B == 2 ? 2 : 4;  // lanes per op
+  constexpr int NE = B - 1;          // evaluator lanes per op
+  constexpr uint32_t GFMASK = (1u << P::M) - 1u;
+  constexpr int NEW = ceil_div(P::N, 32);
+  constexpr int DPP_GROUP0 = G == 2 ? 0xA0 : 0x00;  // quad_perm: each lane reads its group's lane 0
+  __shared__ uint16_t lst[P::T * GEN_WG];  // [index][thread]
+  __shared__ uint32_t bmp[GEN_WAVES][4][NEW + 1];
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int wave = tid >> 6;
+  const int grp = lane >> 4, g16 = lane & 15;  // collision check: four 16-lane groups
+  const int r = lane % G, gl = lane - r;        // role in the op group (0 chain, 1..NE evaluators)
+  const uint64_t gmask = ((1ull << G) - 1ull) << gl;
+  uint32_t *bm = bmp[wave][grp];
+  for (int i = lane; i < 4 * (NEW + 1); i += 64) bmp[wave][i / (NEW + 1)][i % (NEW + 1)] = 0;
+  const KeccakRC rc;
+  if (plan) nops = plan->total;
+
+  int op = -1;         // the group's op (every lane of the group holds it)
+  bool done = false;   // the queue is exhausted for this group
+  uint32_t st[8];      // chain lane: seed state of the next attempt
+  int knext = 0;       // chain lane: index of the next attempt
+  int att = -1;        // evaluator: attempt being squeezed (-1: idle)
+  int blk = 0;         // evaluator: blocks of `att` collected
+  int count = 0;       // evaluator: draws below N collected
+  uint64_t A[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) A[i] = 0;
+
+  // draws of block `b` (0-based) of the lane's attempt, from the state A
+  auto extract = [&](int b, bool on) {
+#pragma unroll
+    for (int w = 0; w < 17; ++w)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int idx = b * 68 + w * 4 + q;
+        const uint32_t v = (uint32_t)(A[w] >> (16 * q)) & GFMASK;
+        if (on && idx < P::NDRAW && v < (uint32_t)P::N && count < P::T) {
+          lst[count * GEN_WG + tid] = (uint16_t)v;
+          ++count;
+        }
+      }
+  };
+
+  for (;;) {
+    // ---- refill: group leaders without an op take the next ones from the queue
+    {
+      const bool need = r == 0 && op < 0 && !done;
+      const uint64_t needmask = __ballot(need);
+      int newop = op;
+      bool newdone = done;
+      if (needmask) {
+        const int leader = __ffsll((long long)needmask) - 1;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(queue, (unsigned)__popcll(needmask));
+        base = (unsigned)__shfl((int)base, leader);
+        if (need) {
+          const unsigned rr = base + mbcnt64(needmask);
+          if (rr < (unsigned)nops) newop = (int)rr;
+          else newdone = true;
+        }
+      }
+      const int gop = __shfl(newop, gl);
+      done = __shfl((int)newdone, gl) != 0;
+      if (gop >= 0 && gop != op) {  // a new op for this group
+        knext = 0;
+        att = -1;
+        if (r == 0) {
+          const uint32_t *sp = reinterpret_cast<const uint32_t *>(seeds + 32 * (size_t)gop);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) st[i] = sp[i];
+        }
+      }
+      op = gop;
+    }
+    if (!__any(op >= 0)) break;
+
+    // ---- one Keccak-f per lane: the chain lane starts attempt `knext`, evaluators squeeze on
+    const bool chain = r == 0 && op >= 0 && knext < P::MAX_ATTEMPTS;
+    if (chain) {
+      const uint64_t s0 = st[0] | ((uint64_t)st[1] << 32), s1 = st[2] | ((uint64_t)st[3] << 32);
+      const uint64_t s2 = st[4] | ((uint64_t)st[5] << 32), s3 = st[6] | ((uint64_t)st[7] << 32);
+#pragma unroll
+      for (int i = 0; i < 25; ++i) A[i] = 0;
+      A[0] = 0x40ull | (s0 << 8);
+      A[1] = (s0 >> 56) | (s1 << 8);
+      A[2] = (s1 >> 56) | (s2 << 8);
+      A[3] = (s2 >> 56) | (s3 << 8);
+      A[4] = (s3 >> 56) | (0x1Full << 8);
+      A[16] = 0x80ull << 56;
+    }
+    keccak_f1600(A, rc);
+    const bool evaluating = r >= 1 && r <= NE && att >= 0 && op >= 0;
+    extract(blk, evaluating);
+    if (evaluating) ++blk;
+    const bool fin = evaluating && blk == B;
+    wave_lds_sync();
+
+    // ---- collision check of the attempts that finished this step, four at a time
+    uint64_t cand = __ballot(fin && count == P::T);
+    uint64_t okmask = 0;
+    while (cand) {
+      uint64_t c = cand;
+      for (int k = 0; k < grp && c; ++k) c &= c - 1;
+      const bool have = c != 0;
+      const int j = have ? __ffsll((long long)c) - 1 : 0;
+      for (int k = 0; k < 4 && cand; ++k) cand &= cand - 1;
+      const int opj = __shfl(op, j);
+      const uint16_t *lj = lst + (wave * 64 + j);
+      bool coll = false;
+      if (have) {
+#pragma unroll
+        for (int k = 0; k < ceil_div(P::T, 16); ++k) {
+          const int idx = k * 16 + g16;
+          if (idx < P::T) {
+            const uint32_t v = lj[idx * GEN_WG];
+            const uint32_t bit = 1u << (v & 31u);
+            coll |= (atomicOr(bm + (v >> 5), bit) & bit) != 0;
+          }
+        }
+      }
+      const uint64_t cm = __ballot(coll);
+      const bool ok = have && ((cm >> (16 * grp)) & 0xFFFFull) == 0;
+      wave_lds_sync();
+      if (ok) {
+        uint32_t *mrow = msg + (size_t)opj * P::MSG_WORDS;
+        for (int k = g16; k <= NEW; k += 16) {
+          const uint32_t cur = k < NEW ? bm[k] : 0u;
+          const uint32_t prev = k ? bm[k - 1] : 0x01000000u;
+          mrow[k] = (cur << 8) | (prev >> 24);
+        }
+        uint16_t *io = ind_out + (size_t)opj * P::IND_STRIDE;
+        for (int idx = g16; idx < P::T; idx += 16) io[idx] = lj[idx * GEN_WG];
+      }
+      {
+        const uint64_t acc = __ballot(ok && g16 == 0);
+        for (int g = 0; g < 4; ++g)
+          if ((acc >> (16 * g)) & 1ull) okmask |= 1ull << __shfl(j, 16 * g);
+      }
+      wave_lds_sync();
+      if (have) {
+#pragma unroll
+        for (int k = 0; k < ceil_div(P::T, 16); ++k) {
+          const int idx = k * 16 + g16;
+          if (idx < P::T) bm[lj[idx * GEN_WG] >> 5] = 0;
+        }
+      }
+      wave_lds_sync();
+    }
+
+    // ---- accepted or abandoned ops end; the group takes a new op next step
+    const bool acc_me = (okmask >> lane) & 1ull;
+    if (acc_me && attempts) attempts[op] = (uint32_t)(att + 1);
+    const bool aband_me = fin && !acc_me && att == P::MAX_ATTEMPTS - 1;
+    if (aband_me) {
+      ind_out[(size_t)op * P::IND_STRIDE] = 0xFFFFu;  // never expected: status -2
+      if (attempts) attempts[op] = P::MAX_ATTEMPTS;
+    }
+    const uint64_t endmask = okmask | __ballot(aband_me);
+    const bool gend = (endmask & gmask) != 0;
+    if (gend) {
+      op = -1;
+      att = -1;
+    }
+    if (fin) att = -1;  // the evaluator is free (it may receive the new attempt below)
+
+    // ---- hand-off: the chain lane's first block goes to evaluator 1 + knext % NE
+    const int kg = __shfl(knext, gl);
+    const bool cg = __shfl((int)chain, gl) != 0 && !gend;
+    const bool recv = cg && r == 1 + kg % NE;
+#pragma unroll
+    for (int i = 0; i < 25; ++i) {
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)A[i], DPP_GROUP0, 0xF, 0xF, false);
+      const uint32_t hi =
+          (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(A[i] >> 32), DPP_GROUP0, 0xF, 0xF, false);
+      if (recv) A[i] = ((uint64_t)hi << 32) | lo;
+    }
+    if (chain && !gend) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        st[2 * i] = (uint32_t)A[i];
+        st[2 * i + 1] = (uint32_t)(A[i] >> 32);
+      }
+      ++knext;
+    }
+    if (recv) {
+      att = kg;
+      count = 0;
+    }
+    extract(0, recv);
+    if (recv) blk = 1;
+    wave_lds_sync();
+  }
+}
+
 template <int SET>
 __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__ ind, const uint4 *__restrict__ cols,
                                                     uint32_t *__restrict__ msg, uint8_t *__restrict__ ct,
@@ -343,12 +555,22 @@ hipError_t launch_enc(const EncArgs &a, hipStream_t s) {
   using P = Par<SET>;
   if (a.nops <= 0) return hipSuccess;
   k_fill_u32<<<1, 64, 0, s>>>(a.queue, 0u, 1);  // a kernel, not a memset: the pipeline may be graph-captured
-  // persistent grid: enough resident waves to fill the chip, lanes pull ops from the queue
-  int grid = a.gen_blocks > 0 ? a.gen_blocks : 1024;
-  const int maxgrid = ceil_div(a.nops, GEN_WG);
-  if (grid > maxgrid) grid = maxgrid;
-  TMCE_TIMED("gen_e", s,
-             k_gen_e<SET><<<grid, GEN_WG, 0, s>>>(a.seeds, a.ind, a.msg, a.attempts, a.nops, a.queue, a.plan));
+  // persistent grid: enough resident waves to fill the chip, lanes (or lane groups) pull ops from the
+  // queue
+  if (a.gen_form == 1) {
+    constexpr int G = P::DRAW_BLOCKS == 2 ? 2 : 4;
+    int grid = a.gen_blocks > 0 ? a.gen_blocks : 2048;
+    const int maxgrid = ceil_div(a.nops * G, GEN_WG);
+    if (grid > maxgrid) grid = maxgrid;
+    TMCE_TIMED("gen_e", s,
+               k_gen_e_pipe<SET><<<grid, GEN_WG, 0, s>>>(a.seeds, a.ind, a.msg, a.attempts, a.nops, a.queue, a.plan));
+  } else {
+    int grid = a.gen_blocks > 0 ? a.gen_blocks : 1024;
+    const int maxgrid = ceil_div(a.nops, GEN_WG);
+    if (grid > maxgrid) grid = maxgrid;
+    TMCE_TIMED("gen_e", s,
+               k_gen_e<SET><<<grid, GEN_WG, 0, s>>>(a.seeds, a.ind, a.msg, a.attempts, a.nops, a.queue, a.plan));
+  }
   TMCE_TIMED("enc_finish", s,
              k_enc_finish<SET><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.ind, reinterpret_cast<const uint4 *>(a.cols),
                                                                      a.msg, a.ct, a.key, a.status, a.nops, a.plan));

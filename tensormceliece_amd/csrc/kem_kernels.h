@@ -103,6 +103,7 @@ struct EncArgs {
   unsigned *queue;       // scratch work-queue counter
   int nops;              // op count, or with a plan the capacity (the live count is plan->total)
   int gen_blocks;        // persistent grid size of the sampler (0 = default)
+  int gen_form;          // sampler: 1 = pipelined lane groups (k_gen_e_pipe), 0 = one op per lane
   const PlanDev *plan;   // multi-batch plan (per-group keys; null: one key, `cols`)
 };
 

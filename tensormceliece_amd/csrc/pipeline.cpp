@@ -33,7 +33,7 @@ struct tmce_pipeline {
   hipGraphExec_t exec = nullptr;
   bool warm = false;                // one eager run precedes the capture (first-use set-up, e.g.
                                     // kernel attributes, stays out of the captured stream)
-  int tuned[4] = {-1, -1, -1, -1};  // knobs the captured graph was built with
+  int tuned[5] = {-1, -1, -1, -1, -1};  // knobs the captured graph was built with
 };
 
 namespace tmce {
@@ -45,7 +45,7 @@ size_t plan_image_bytes(int max_sub) { return round_up((int)sizeof(PlanDev), 16)
 int msg_words(const SetInfo &si) { return (1 + si.n / 8 + si.synd_bytes) / 136 * 34 + 34; }
 
 // the stage launches of one run (everything after the plan upload)
-int issue(tmce_pipeline *p, const int knobs[4]) {
+int issue(tmce_pipeline *p, const int knobs[5]) {
   Engine *e = g_engine;
   const SetInfo &si = set_info(p->set);
   hipStream_t s = p->stream;
@@ -61,6 +61,7 @@ int issue(tmce_pipeline *p, const int knobs[4]) {
     a.queue = p->queue.as<unsigned>();
     a.nops = p->cap;
     a.gen_blocks = knobs[0];
+    a.gen_form = knobs[4];
     a.plan = p->plan_dev;
     HIPCHK(launch_enc_set(p->set, a, s));
     HIPCHK(launch_plan_scatter(p->plan_dev, p->ct.as<uint8_t>(), si.synd_bytes, p->key.as<uint8_t>(),
@@ -256,13 +257,14 @@ int tmce_pipeline_run(tmce_pipeline *p, const tmce_sub_batch *subs, size_t nsub)
   if (total == 0) return 0;
 
   // ---- launches: replay the captured graph, or capture / issue them
-  int knobs[4];
+  int knobs[5];
   {
     std::lock_guard<std::mutex> lk(e->mu);
     knobs[0] = e->gen_blocks[p->set];
     knobs[1] = e->decode_blocks[p->set];
     knobs[2] = e->front_mfma[p->set];
     knobs[3] = e->front_cfg[p->set];
+    knobs[4] = e->gen_form[p->set];
   }
   if (!p->use_graph || !p->warm) {
     p->warm = true;

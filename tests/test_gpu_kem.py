@@ -149,3 +149,28 @@ def test_plain_form_parity(tm, orc, s):
     assert np.array_equal(ct, ct_ref) and np.array_equal(k, k_ref)
     k2_ref, ret = orc.dec_batch(s, sk, np.concatenate([ct_ref, rnd]))
     assert np.array_equal(k2, k2_ref) and np.array_equal(st2, ret)
+
+
+@pytest.mark.parametrize("s", SETS)
+def test_sampler_forms(tm, orc, s):
+    """Both samplers (one op per lane; pipelined lane groups) give the oracle's bytes, on a ragged
+    batch, and the same bytes as each other on a larger one."""
+    pk, sk, _ = keys(orc, s)
+    rng = np.random.default_rng(17)
+    seeds = rng.integers(0, 256, (3001, 32), dtype=np.uint8)
+    K = tm.PublicKey(s, pk)
+    old = tm.tune_get(s, "gen_form")
+    out = {}
+    try:
+        for form in (0, 1):
+            tm.tune_set(s, "gen_form", form)
+            print(f"sampler form {form} {s}", flush=True)
+            out[form] = K.enc(seeds)
+    finally:
+        tm.tune_set(s, "gen_form", old)
+    for form in (0, 1):
+        assert not out[form][2].any()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    idx = rng.choice(len(seeds), 48, replace=False)
+    ct_ref, k_ref, _ = orc.enc_batch(s, pk, seeds[idx])
+    assert np.array_equal(out[1][0][idx], ct_ref) and np.array_equal(out[1][1][idx], k_ref)
