@@ -1,8 +1,8 @@
 """Autotuner of the engine's launch shapes, per parameter set and batch size.
 
 Knobs (tmce_tune_set): the MFMA tile shape of the syndrome GEMM and of the decapsulation front end,
-whether the front end runs on MFMA or on the VALU, and the grids of the persistent sampler and of the
-decoder.  Each knob drives one kernel, so the tuner times that kernel alone (HIP events around its
+whether the front end runs on MFMA or on the VALU, the sampler's form (one op per lane, or a lane
+group per op with the SHAKE blocks pipelined) with its persistent grid, and the decoder's grid.  Each knob drives one kernel, so the tuner times that kernel alone (HIP events around its
 launches, on device-resident inputs) for every candidate value and keeps the fastest; knobs are
 independent of each other's kernels, so one pass over each suffices.  Results are cached in a JSON
 file keyed by "<set>:<n>" and applied with apply().
@@ -19,7 +19,9 @@ CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 CANDIDATES = {
     "synd_mfma_cfg": list(range(6)),
     "front": [(0, 0)] + [(1, c) for c in range(6)],  # (front_mfma, front_cfg)
-    "gen_blocks": [32, 64, 96, 128, 192, 256, 512],
+    # (gen_form, gen_blocks): the per-lane sampler keeps at most n/256 workgroups busy, the
+    # pipelined one n * (lanes per op) / 256; larger grids are clamped to that
+    "gen": [(0, 64), (0, 128), (0, 256), (0, 512), (1, 256), (1, 512), (1, 1024), (1, 2048)],
     "decode_blocks": [128, 256, 384, 512, 768, 1024],
 }
 
@@ -83,8 +85,11 @@ def tune(set_name, n=65536, reps=3, keypair=None, verbose=False):
 
     fb = sweep("front", CANDIDATES["front"], set_front, dec, ["dec_front", "gf2_repack"])
     best["front_mfma"], best["front_cfg"] = fb
-    best["gen_blocks"] = sweep("gen_blocks", CANDIDATES["gen_blocks"],
-                               lambda v: tm.tune_set(set_name, "gen_blocks", v), enc, ["gen_e"])
+    def set_gen(v):
+        tm.tune_set(set_name, "gen_form", v[0])
+        tm.tune_set(set_name, "gen_blocks", v[1])
+
+    best["gen_form"], best["gen_blocks"] = sweep("gen", CANDIDATES["gen"], set_gen, enc, ["gen_e"])
     best["decode_blocks"] = sweep("decode_blocks", CANDIDATES["decode_blocks"],
                                   lambda v: tm.tune_set(set_name, "decode_blocks", v), dec, ["dec_decode"])
     for b in (d_seeds, d_ct, d_k, d_k2, d_e):
