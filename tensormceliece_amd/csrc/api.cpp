@@ -184,6 +184,7 @@ int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, si
       a.xscratch = e->xscratch.as<uint32_t>();
     }
   }
+  a.Fcol = k->Fcol;
   a.alpha = k->alpha;
   a.la = k->la;
   a.lu = k->lu;
@@ -343,11 +344,12 @@ int tmce_key_from_sk(int set, const unsigned char *sk, tmce_key **out) {
     HIPCHK(launch_build_msw(set, GF2_FRONT, k->F, kwp, k->Fsw, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
   }
+  if (int r = upload(&k->Fcol, h.Fcol)) return r;
   if (int r = upload(&k->alpha, h.alpha)) return r;
   if (int r = upload(&k->la, h.la)) return r;
   if (int r = upload(&k->lu, h.lu)) return r;
   if (int r = upload(&k->rej, h.rej)) return r;
-  k->bytes = h.F.size() * 4 + fsw + (h.alpha.size() + h.la.size() + h.lu.size()) * 2 + h.rej.size();
+  k->bytes = (h.F.size() + h.Fcol.size()) * 4 + fsw + (h.alpha.size() + h.la.size() + h.lu.size()) * 2 + h.rej.size();
   *out = k.release();
   return 0;
 }
@@ -364,6 +366,7 @@ void tmce_key_free(tmce_key *k) {
   (void)hipFree(k->Tv);
   (void)hipFree(k->F);
   (void)hipFree(k->Fsw);
+  (void)hipFree(k->Fcol);
   (void)hipFree(k->alpha);
   (void)hipFree(k->la);
   (void)hipFree(k->lu);

@@ -25,6 +25,7 @@ struct tmce_key {
   uint32_t *Tv = nullptr;     // ... and 32-row blocks of words (VALU)
   uint32_t *F = nullptr;
   uint8_t *Fsw = nullptr;     // F as 0/1 bytes in MFMA fragment order (INT8 MFMA front end)
+  uint32_t *Fcol = nullptr;   // syndrome map over all N positions, [N][RW] (re-encryption check)
   uint16_t *alpha = nullptr, *la = nullptr, *lu = nullptr;
   uint8_t *rej = nullptr;
 };

@@ -39,7 +39,8 @@ FftTables build_fft_tables(int m, int D);
 // Per-key data of decapsulation, derived from sk.
 struct DecKeyHost {
   std::vector<uint16_t> alpha, la, lu;
-  std::vector<uint32_t> F;   // R x KWP words
+  std::vector<uint32_t> F;     // R x KWP words
+  std::vector<uint32_t> Fcol;  // N x RW words: column i = syndromes of unit vector i (row j*M + b)
   std::vector<uint8_t> rej;  // N/8 bytes
 };
 void build_dec_key(int set, const uint8_t *sk, DecKeyHost &out);

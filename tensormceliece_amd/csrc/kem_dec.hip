@@ -211,13 +211,13 @@ void k_dec_decode(DecArgs a) {
       const int op = base + q;
       if (op >= nops) break;
       // key material of the op (one key, or its key group under a plan)
-      const uint16_t *k_alpha = a.alpha, *k_la = a.la, *k_lu = a.lu;
+      const uint16_t *k_alpha = a.alpha;
+      const uint32_t *k_fcol = a.Fcol;
       const uint8_t *k_rej = a.rej;  // (loads below go through global_ptr: these may come from memory)
       if (a.plan) {  // the wave's ops share a key group (groups are PLAN_ALIGN-aligned)
         const KeyDev &kd = a.plan->keys[plan_group(a.plan, __builtin_amdgcn_readfirstlane(op))];
         k_alpha = uniform_ptr(kd.alpha);
-        k_la = uniform_ptr(kd.la);
-        k_lu = uniform_ptr(kd.lu);
+        k_fcol = uniform_ptr(kd.Fcol);
         k_rej = uniform_ptr(kd.rej);
       }
       // the ciphertext (for the session-key message), fetched now, stored to LDS after the top-down
@@ -361,53 +361,35 @@ void k_dec_decode(DecArgs a) {
         }
       }
       wave_lds_sync();
-      // re-encryption: syndromes of e (sum over its t positions of alpha^j / g(alpha)^2) must
-      // equal the ciphertext's
+      // re-encryption: the syndromes of e (the XOR of the syndrome-map columns at its t positions,
+      // L2-resident, rows as in S) must equal the ciphertext's, word for word
       bool ok = (wt == T);
       if (ok) {
-        constexpr int JS = ceil_div(2 * T, 64);
-        uint32_t acc[JS];
-#pragma unroll
-        for (int k = 0; k < JS; ++k) acc[k] = 0;
-        // the log tables at the t positions, lane k holding position k (one gather, no
-        // dependent loads in the loop below)
+        constexpr int RH = P::RW / 2;  // 8-byte words per column (RW is even)
+        static_assert(P::RW % 2 == 0 && RH <= 64, "one 8-byte word per lane");
         constexpr int KP = ceil_div(T, 64);
-        uint32_t la_r[KP], lu_r[KP];
+        uint32_t pos_r[KP];
 #pragma unroll
         for (int kk = 0; kk < KP; ++kk) {
           const int k = 64 * kk + lane;
-          const int p = k < T ? w.pos[k] : 0;
-          la_r[kk] = global_ptr(k_la)[p];
-          lu_r[kk] = k < T ? (uint32_t)global_ptr(k_lu)[p] : 0xFFFFu;
+          pos_r[kk] = k < T ? w.pos[k] : 0u;
         }
-#pragma unroll 1
+        // every lane takes part (lanes past the column re-read its last word): readlane below
+        // reads position k from lane k, so no lane may skip the position load above
+        const uint2 *fc = reinterpret_cast<const uint2 *>(k_fcol);
+        const int li = lane < RH ? lane : RH - 1;
+        uint2 acc = make_uint2(0, 0);
+#pragma unroll 8
         for (int k = 0; k < T; ++k) {
-          const uint32_t la = (uint32_t)__builtin_amdgcn_readlane((int)la_r[k >> 6], k & 63);
-          const uint32_t lu = (uint32_t)__builtin_amdgcn_readlane((int)lu_r[k >> 6], k & 63);
-          if (lu == 0xFFFFu) continue;
-          if (la == 0xFFFFu) {
-            if (lane == 0) acc[0] ^= ex_s[lu];
-            continue;
-          }
-          uint32_t e = (uint32_t)lane * la + lu;
-          e = (e & Q) + (e >> M);
-          e = (e & Q) + (e >> M);
-          e = e >= (uint32_t)Q ? e - Q : e;
-          uint32_t inc = 64u * la;
-          inc = (inc & Q) + (inc >> M);
-          inc = inc >= (uint32_t)Q ? inc - Q : inc;
-#pragma unroll
-          for (int k2 = 0; k2 < JS; ++k2) {
-            acc[k2] ^= ex_s[e];
-            e += inc;
-            e = e >= (uint32_t)Q ? e - Q : e;
-          }
+          const uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)pos_r[k >> 6], k & 63);
+          const uint2 v = global_load8(fc, (size_t)p * RH + li);
+          acc.x ^= v.x;
+          acc.y ^= v.y;
         }
         bool mism = false;
-#pragma unroll
-        for (int k2 = 0; k2 < JS; ++k2) {
-          const int j = k2 * 64 + lane;
-          if (j < 2 * T && acc[k2] != w.s[q][j]) mism = true;
+        if (lane < RH) {
+          const uint2 sv = global_load8(reinterpret_cast<const uint2 *>(a.S + (size_t)op * P::RW), lane);
+          mism = sv.x != acc.x || sv.y != acc.y;
         }
         ok = !__any(mism);
       }

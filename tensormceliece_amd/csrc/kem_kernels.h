@@ -20,6 +20,7 @@ struct KeyDev {
   const uint32_t *cols;   // encapsulation: column-major [I | T]
   const uint32_t *F;      // decapsulation: front-end matrix (VALU form)
   const uint8_t *Fsw;     // decapsulation: front-end matrix (MFMA form)
+  const uint32_t *Fcol;   // decapsulation: syndrome map columns [N][RW]
   const uint16_t *alpha, *la, *lu;
   const uint8_t *rej;
   int pad_bad;            // 6960119 public key with nonzero padding bits
@@ -60,6 +61,12 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T *global_ptr
   return (const __attribute__((address_space(1))) T *)p;
 }
 typedef unsigned v4u32 __attribute__((ext_vector_type(4)));
+typedef unsigned v2u32 __attribute__((ext_vector_type(2)));
+// 8-byte global load of element i
+__device__ __forceinline__ uint2 global_load8(const uint2 *p, size_t i) {
+  const v2u32 v = ((const __attribute__((address_space(1))) v2u32 *)p)[i];
+  return make_uint2(v.x, v.y);
+}
 // 16-byte global load of element i
 __device__ __forceinline__ uint4 global_load16(const uint4 *p, size_t i) {
   const v4u32 v = ((const __attribute__((address_space(1))) v4u32 *)p)[i];
@@ -126,6 +133,7 @@ struct DecArgs {
   int front_mfma;        // front end on INT8 MFMA (1) or on the VALU (0)
   int front_cfg;         // MFMA tile shape of the front end
   uint32_t *xscratch;    // gf2_scratch_bytes(set, GF2_FRONT, n) bytes (MFMA front end; null if 0)
+  const uint32_t *Fcol;  // key: [N][RW] syndromes of each unit vector (rows as in S)
   const uint16_t *alpha; // key: [N] support
   const uint16_t *la;    // key: [N] log alpha (0xFFFF for alpha = 0)
   const uint16_t *lu;    // key: [N] log 1/g(alpha)^2 (0xFFFF for 0)

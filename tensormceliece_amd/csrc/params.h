@@ -54,7 +54,8 @@ template <int SET> struct Par : SetBase<SET> {
   static constexpr int MSG_BLOCKS = MSG_LEN / 136 + 1;
   static constexpr int MSG_WORDS = MSG_BLOCKS * 34;
   // decapsulation front end: x = ciphertext bits (KBITS, incl. padding bits of the last byte);
-  // output rows r = b*TT + j hold bit b of syndrome j (plane-major, TT = 2t rounded up to 32)
+  // output rows r = j*M + b hold bit b of syndrome j (syndrome-major; j < TT = 2t rounded up to 32,
+  // rows of j >= 2t are zero)
   static constexpr int KBITS = 8 * SYND_BYTES;
   static constexpr int KW = ceil_div(SYND_BYTES, 4);
   static constexpr int KWP = round_up(KW, 4);

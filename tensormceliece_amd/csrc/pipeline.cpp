@@ -225,6 +225,7 @@ int tmce_pipeline_run(tmce_pipeline *p, const tmce_sub_batch *subs, size_t nsub)
     kd.cols = k->cols;
     kd.F = k->F;
     kd.Fsw = k->Fsw;
+    kd.Fcol = k->Fcol;
     kd.alpha = k->alpha;
     kd.la = k->la;
     kd.lu = k->lu;
