@@ -73,7 +73,7 @@ __global__ __launch_bounds__(FRONT_WG) void k_dec_front(const uint8_t *__restric
 template <int SET>
 struct DecWave {
   using P = Par<SET>;
-  uint16_t s[4][P::TT];           // syndromes of the wave's four ops
+  uint16_t s[4][P::TT];           // syndromes of the wave's four ops (log form, ZL for 0)
   uint16_t f[4][P::D];            // locator coefficients (FFT input), per op
   uint16_t tmp[P::D];
   uint16_t pos[P::IND_STRIDE];    // error positions of the op being finished
@@ -96,6 +96,69 @@ __device__ __forceinline__ uint32_t row_xor16(uint32_t v) {
   return v;
 }
 
+
+// row_ror:1 without a previous value (every lane of a rotation has a source lane)
+__device__ __forceinline__ uint32_t ror1(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, DPP_ROR1, 0xF, 0xF, true);
+}
+
+// One Berlekamp-Massey step on the wave's four rows, for NA active coefficient slots: slots >= NA
+// hold zero coefficients of C and B in every row (deg C <= L, deg B <= N + 1 - L), so their table
+// lookups are skipped; the syndrome window W moves through every slot.
+template <int NA, int NSB, int T, int Q>
+__device__ __forceinline__ void bm_step(int N, uint32_t lsN, int l16, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
+                                        uint32_t (&Wl)[NSB], int &L, uint32_t &bb, const uint16_t *lg_s,
+                                        const uint16_t *ex_s) {
+  constexpr uint32_t ZL = 2u * Q;
+  uint32_t R[NSB];
+#pragma unroll
+  for (int s = 0; s < NSB; ++s) R[s] = ror1(Wl[s]);
+#pragma unroll
+  for (int s = 0; s < NSB; ++s) Wl[s] = (l16 == 0) ? (s ? R[s > 0 ? s - 1 : 0] : lsN) : R[s];
+  uint32_t lc[NA], dp = 0;
+#pragma unroll
+  for (int s = 0; s < NA; ++s) lc[s] = lg_s[C[s]];
+#pragma unroll
+  for (int s = 0; s < NA; ++s) dp ^= ex_s[lc[s] + Wl[s]];
+  const uint32_t d = row_xor16(dp);
+  if (d != 0) {
+    uint32_t lf = (uint32_t)lg_s[d] + Q - lg_s[bb];
+    lf = lf >= (uint32_t)Q ? lf - Q : lf;
+    const bool upd = (2 * L <= N);
+#pragma unroll
+    for (int s = 0; s < NA; ++s) {
+      const uint32_t nc = C[s] ^ ex_s[lf + Bl[s]];
+      if (upd) Bl[s] = lc[s];
+      C[s] = nc;
+    }
+    if (upd) {
+      L = N + 1 - L;
+      bb = d;
+    }
+  }
+  // B <- x B on the slots that can hold a nonzero coefficient afterwards
+  constexpr int NS = NA < NSB ? NA + 1 : NSB;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) R[s] = ror1(Bl[s]);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const uint32_t v = (l16 == 0) ? (s ? R[s > 0 ? s - 1 : 0] : ZL) : R[s];
+    Bl[s] = (16 * s + l16 <= T) ? v : ZL;
+  }
+}
+// runs fn(integral_constant<K>) for the wave-uniform n in 1..NSB (straight-line code per count)
+template <int NSB, int K = 1, class Fn>
+__device__ __forceinline__ void with_count(int n, Fn &&fn) {
+  if constexpr (K < NSB) {
+    if (n <= K) {
+      fn(std::integral_constant<int, K>{});
+      return;
+    }
+    with_count<NSB, K + 1>(n, fn);
+  } else {
+    fn(std::integral_constant<int, K>{});
+  }
+}
 
 // One wavefront decodes four ops at a time: Berlekamp-Massey runs on the four 16-lane rows in
 // parallel (coefficient i of op `row` lives in lane 16*row + i%16, slot i/16; row reductions and
@@ -141,7 +204,7 @@ void k_dec_decode(DecArgs a) {
     for (int q = 0; q < 4; ++q) {
       const int op = base + q;
       const uint32_t *Srow = a.S + (size_t)(op < nops ? op : 0) * P::RW;
-      for (int j = lane; j < P::TT; j += 64) w.s[q][j] = (uint16_t)(op < nops ? synd_value<P>(Srow, j) : 0u);
+      for (int j = lane; j < P::TT; j += 64) w.s[q][j] = lg_s[op < nops ? synd_value<P>(Srow, j) : 0u];
     }
     wave_lds_sync();
 
@@ -161,38 +224,15 @@ void k_dec_decode(DecArgs a) {
       uint32_t bb = 1;
 #pragma unroll 1
       for (int N = 0; N < 2 * T; ++N) {
-        uint32_t R[NSB];
-#pragma unroll
-        for (int s = 0; s < NSB; ++s) R[s] = dpp<DPP_ROR1>(Wl[s]);
-        const uint32_t lsN = lg_s[sq[N]];
-#pragma unroll
-        for (int s = 0; s < NSB; ++s) Wl[s] = (l16 == 0) ? (s ? R[s > 0 ? s - 1 : 0] : lsN) : R[s];
-        uint32_t dp = 0, lc[NSB];
-#pragma unroll
-        for (int s = 0; s < NSB; ++s) {
-          lc[s] = lg_s[C[s]];
-          dp ^= ex_s[lc[s] + Wl[s]];
-        }
-        const uint32_t d = row_xor16(dp);
-        if (d != 0) {
-          uint32_t lf = (uint32_t)lg_s[d] + Q - lg_s[bb];
-          lf = lf >= (uint32_t)Q ? lf - Q : lf;
-          const bool upd = (2 * L <= N);
-#pragma unroll
-          for (int s = 0; s < NSB; ++s) {
-            const uint32_t nc = C[s] ^ ex_s[lf + Bl[s]];
-            if (upd) Bl[s] = lc[s];
-            C[s] = nc;
-          }
-          if (upd) { L = N + 1 - L; bb = d; }
-        }
-#pragma unroll
-        for (int s = 0; s < NSB; ++s) R[s] = dpp<DPP_ROR1>(Bl[s]);
-#pragma unroll
-        for (int s = 0; s < NSB; ++s) {
-          uint32_t v = (l16 == 0) ? (s ? R[s > 0 ? s - 1 : 0] : ZL) : R[s];
-          Bl[s] = (16 * s + l16 <= T) ? v : ZL;
-        }
+        // degree bounds over the wave's four rows: deg C <= L, deg B <= N + 1 - L
+        const int L0 = __builtin_amdgcn_readlane(L, 0), L1 = __builtin_amdgcn_readlane(L, 16);
+        const int L2 = __builtin_amdgcn_readlane(L, 32), L3 = __builtin_amdgcn_readlane(L, 48);
+        const int Lmax = max(max(L0, L1), max(L2, L3)), Lmin = min(min(L0, L1), min(L2, L3));
+        const int na = (max(Lmax, N + 1 - Lmin) >> 4) + 1;
+        const uint32_t lsN = sq[N];
+        with_count<NSB>(na, [&](auto k) {
+          bm_step<decltype(k)::value, NSB, T, Q>(N, lsN, l16, C, Bl, Wl, L, bb, lg_s, ex_s);
+        });
       }
       // locator sigma_j = C_{t-j} (sigma_t = C_0 = 1 is added from the power table when t = D)
       for (int j = l16; j < Dn; j += 16) w.f[row][j] = 0;
