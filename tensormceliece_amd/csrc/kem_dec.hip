@@ -75,7 +75,6 @@ struct DecWave {
   using P = Par<SET>;
   uint16_t s[4][P::TT];           // syndromes of the wave's four ops (log form, ZL for 0)
   uint16_t f[4][P::D];            // locator coefficients (FFT input), per op
-  uint16_t tmp[P::D];
   uint16_t pos[P::IND_STRIDE];    // error positions of the op being finished
   uint32_t Z[(1 << P::M) / 32];   // zero flags of sigma over all field elements
   uint64_t m64[17 * P::MSG_BLOCKS];  // session-key message of the op being finished
@@ -176,12 +175,15 @@ void k_dec_decode(DecArgs a) {
   constexpr int NR = NPTS / 64;  // registers per lane holding field values (== D)
   constexpr uint32_t ZL = 2u * Q;
   constexpr int NSB = ceil_div(T + 1, 16);  // BM coefficient slots per lane
+  constexpr int LOG2D = Dn == 64 ? 6 : 7;
+  static_assert((1 << LOG2D) == Dn, "D is 64 or 128");
   static_assert(M - KD == 6 && NR == Dn, "FFT layout assumes 2^(m-KD) = 64 points per constant");
   constexpr int NPHI = (1 << (M - 6)) - (1 << (M - 6 - KD));  // sum over levels of 2^(m-7-d)
   __shared__ uint16_t lg_s[NPTS];
   __shared__ uint16_t ex_s[4 * Q + 2];
   __shared__ uint16_t phi_s[NPHI];
   __shared__ uint16_t pthi_s[NR];
+  __shared__ uint16_t twl_s[2 * Dn];
   __shared__ DecWave<SET> Wv[DEC_WAVES];
 
   const int tid = threadIdx.x;
@@ -189,6 +191,7 @@ void k_dec_decode(DecArgs a) {
   for (int i = tid; i < 4 * Q + 2; i += DEC_WG) ex_s[i] = i < 2 * Q ? a.ft.ex[i < Q ? i : i - Q] : 0;
   for (int i = tid; i < NPHI; i += DEC_WG) phi_s[i] = a.ft.phi[i];
   for (int i = tid; i < NR; i += DEC_WG) pthi_s[i] = P::TOP_TERM ? a.ft.pthi[i] : 0;
+  for (int i = tid; i < 2 * Dn; i += DEC_WG) twl_s[i] = a.ft.twl[i];
   __syncthreads();
   const int nops = a.plan ? a.plan->total : a.nops;
   const int wave = tid >> 6, lane = tid & 63, row = lane >> 4, l16 = lane & 15;
@@ -245,7 +248,43 @@ void k_dec_decode(DecArgs a) {
       wave_lds_sync();
     }
 
-    // ---- 3. finish the four ops one after the other
+    // ---- 3. additive FFT, top-down on the D coefficients of the four locators at once (row r:
+    //         op r; the row's syndrome array is free now and serves as the second buffer): twist,
+    //         Taylor expansion at x^2 + x, even/odd split
+    {
+      uint16_t *fr = w.f[row], *tr = w.s[row];
+      static_assert(P::TT >= Dn, "syndrome array doubles as the FFT buffer");
+#pragma unroll 1
+      for (int d = 0; d < KD; ++d) {
+        const int lgL = LOG2D - d, Ld = 1 << lgL;
+        const uint16_t *tw = twl_s + (2 * Dn - 2 * Ld);
+        for (int c = l16; c < Dn; c += 16) fr[c] = ex_s[lg_s[fr[c]] + tw[c & (Ld - 1)]];
+        wave_lds_sync();
+        for (int lq = lgL - 2; lq >= 0; --lq) {  // block length 4 q with q = 2^lq
+          const int q4 = 1 << lq;
+          for (int u = l16; u < Dn / 4; u += 16) {
+            const int bs = (u >> lq) << (lq + 2), i = u & (q4 - 1);
+            fr[bs + 2 * q4 + i] ^= fr[bs + 3 * q4 + i];
+          }
+          wave_lds_sync();
+          for (int u = l16; u < Dn / 4; u += 16) {
+            const int bs = (u >> lq) << (lq + 2), i = u & (q4 - 1);
+            fr[bs + q4 + i] ^= fr[bs + 2 * q4 + i];
+          }
+          wave_lds_sync();
+        }
+        for (int c = l16; c < Dn; c += 16) {
+          const int bs = c & ~(Ld - 1), i = c & (Ld - 1);
+          tr[bs + (i >> 1) + (i & 1) * (Ld >> 1)] = fr[c];
+        }
+        wave_lds_sync();
+        uint16_t *x = fr;
+        fr = tr;
+        tr = x;
+      }
+    }
+
+    // ---- 4. finish the four ops one after the other
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
       const int op = base + q;
@@ -260,8 +299,8 @@ void k_dec_decode(DecArgs a) {
         k_fcol = uniform_ptr(kd.Fcol);
         k_rej = uniform_ptr(kd.rej);
       }
-      // the ciphertext (for the session-key message), fetched now, stored to LDS after the top-down
-      // FFT
+      // the ciphertext (for the session-key message), fetched now, stored to LDS after the message
+      // buffer's previous reader is done
       constexpr int CH16 = P::SYND_BYTES / 2, CPL = ceil_div(CH16, 64);
       uint32_t ctw[CPL];
       {
@@ -270,35 +309,8 @@ void k_dec_decode(DecArgs a) {
         for (int c = 0; c < CPL; ++c) ctw[c] = 64 * c + lane < CH16 ? (uint32_t)ct16[64 * c + lane] : 0u;
       }
       uint8_t *m8 = reinterpret_cast<uint8_t *>(w.m64);
-      uint16_t *fq = w.f[q];
-      // additive FFT, top-down on the D coefficients: twist, Taylor at x^2+x, even/odd split
-#pragma unroll 1
-      for (int d = 0; d < KD; ++d) {
-        const int Ld = Dn >> d;
-        const uint16_t *tw = a.ft.twl + (2 * Dn - 2 * Ld);
-        for (int c = lane; c < Dn; c += 64) fq[c] = ex_s[lg_s[fq[c]] + tw[c & (Ld - 1)]];
-        wave_lds_sync();
-        for (int len = Ld; len >= 4; len >>= 1) {
-          const int q4 = len >> 2;
-          for (int u = lane; u < Dn / 4; u += 64) {
-            const int bs = (u / q4) * len, i = u % q4;
-            fq[bs + 2 * q4 + i] ^= fq[bs + 3 * q4 + i];
-          }
-          wave_lds_sync();
-          for (int u = lane; u < Dn / 4; u += 64) {
-            const int bs = (u / q4) * len, i = u % q4;
-            fq[bs + q4 + i] ^= fq[bs + 2 * q4 + i];
-          }
-          wave_lds_sync();
-        }
-        for (int c = lane; c < Dn; c += 64) {
-          const int bs = c & ~(Ld - 1), i = c & (Ld - 1);
-          w.tmp[bs + (i >> 1) + (i & 1) * (Ld >> 1)] = fq[c];
-        }
-        wave_lds_sync();
-        for (int c = lane; c < Dn; c += 64) fq[c] = w.tmp[c];
-        wave_lds_sync();
-      }
+      // the top-down result: after KD buffer swaps it is in the op's row of f (KD even) or of s
+      const uint16_t *fq = (KD & 1) ? w.s[q] : w.f[q];
       // ciphertext bytes into the message buffer (before the butterflies need the registers)
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
