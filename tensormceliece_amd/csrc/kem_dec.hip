@@ -70,10 +70,16 @@ __global__ __launch_bounds__(FRONT_WG) void k_dec_front(const uint8_t *__restric
 // ------------------------------------------------------------------ decoder
 // LDS field tables: lg[a] = log a with lg[0] = ZL = 2q (sentinel); ex[i] = g^(i mod q) for
 // i < 2q and 0 for 2q <= i <= 4q, so a*b = ex[lg[a] + lg[b]] with no zero test and no reduction.
+// The decoder keeps every field element and every log doubled (a byte offset into these u16
+// tables, which store doubled values): lookups need no address shift, and XOR commutes with the
+// doubling.
+__device__ __forceinline__ uint32_t tab(const uint16_t *t, uint32_t byte_off) {
+  return *reinterpret_cast<const uint16_t *>(reinterpret_cast<const uint8_t *>(t) + byte_off);
+}
 template <int SET>
 struct DecWave {
   using P = Par<SET>;
-  uint16_t s[4][P::TT];           // syndromes of the wave's four ops (log form, ZL for 0)
+  uint16_t s[4][P::TT];           // syndromes of the wave's four ops (doubled logs, 2 ZL for 0)
   uint16_t f[4][P::D];            // locator coefficients (FFT input), per op
   uint16_t pos[P::IND_STRIDE];    // error positions of the op being finished
   uint32_t Z[(1 << P::M) / 32];   // zero flags of sigma over all field elements
@@ -108,7 +114,7 @@ template <int NA, int NSB, int T, int Q>
 __device__ __forceinline__ void bm_step(int N, uint32_t lsN, int l16, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
                                         uint32_t (&Wl)[NSB], int &L, uint32_t &bb, const uint16_t *lg_s,
                                         const uint16_t *ex_s) {
-  constexpr uint32_t ZL = 2u * Q;
+  constexpr uint32_t ZL2 = 4u * Q, Q2 = 2u * Q;  // doubled sentinel and modulus
   uint32_t R[NSB];
 #pragma unroll
   for (int s = 0; s < NSB; ++s) R[s] = ror1(Wl[s]);
@@ -116,17 +122,17 @@ __device__ __forceinline__ void bm_step(int N, uint32_t lsN, int l16, uint32_t (
   for (int s = 0; s < NSB; ++s) Wl[s] = (l16 == 0) ? (s ? R[s > 0 ? s - 1 : 0] : lsN) : R[s];
   uint32_t lc[NA], dp = 0;
 #pragma unroll
-  for (int s = 0; s < NA; ++s) lc[s] = lg_s[C[s]];
+  for (int s = 0; s < NA; ++s) lc[s] = tab(lg_s, C[s]);
 #pragma unroll
-  for (int s = 0; s < NA; ++s) dp ^= ex_s[lc[s] + Wl[s]];
+  for (int s = 0; s < NA; ++s) dp ^= tab(ex_s, lc[s] + Wl[s]);
   const uint32_t d = row_xor16(dp);
   if (d != 0) {
-    uint32_t lf = (uint32_t)lg_s[d] + Q - lg_s[bb];
-    lf = lf >= (uint32_t)Q ? lf - Q : lf;
+    uint32_t lf = tab(lg_s, d) + Q2 - tab(lg_s, bb);
+    lf = lf >= Q2 ? lf - Q2 : lf;
     const bool upd = (2 * L <= N);
 #pragma unroll
     for (int s = 0; s < NA; ++s) {
-      const uint32_t nc = C[s] ^ ex_s[lf + Bl[s]];
+      const uint32_t nc = C[s] ^ tab(ex_s, lf + Bl[s]);
       if (upd) Bl[s] = lc[s];
       C[s] = nc;
     }
@@ -141,8 +147,8 @@ __device__ __forceinline__ void bm_step(int N, uint32_t lsN, int l16, uint32_t (
   for (int s = 0; s < NS; ++s) R[s] = ror1(Bl[s]);
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    const uint32_t v = (l16 == 0) ? (s ? R[s > 0 ? s - 1 : 0] : ZL) : R[s];
-    Bl[s] = (16 * s + l16 <= T) ? v : ZL;
+    const uint32_t v = (l16 == 0) ? (s ? R[s > 0 ? s - 1 : 0] : ZL2) : R[s];
+    Bl[s] = (16 * s + l16 <= T) ? v : ZL2;
   }
 }
 // runs fn(integral_constant<K>) for the wave-uniform n in 1..NSB (straight-line code per count)
@@ -187,18 +193,18 @@ void k_dec_decode(DecArgs a) {
   __shared__ DecWave<SET> Wv[DEC_WAVES];
 
   const int tid = threadIdx.x;
-  for (int i = tid; i < NPTS; i += DEC_WG) lg_s[i] = i ? a.ft.lg[i] : (uint16_t)ZL;
-  for (int i = tid; i < 4 * Q + 2; i += DEC_WG) ex_s[i] = i < 2 * Q ? a.ft.ex[i < Q ? i : i - Q] : 0;
-  for (int i = tid; i < NPHI; i += DEC_WG) phi_s[i] = a.ft.phi[i];
-  for (int i = tid; i < NR; i += DEC_WG) pthi_s[i] = P::TOP_TERM ? a.ft.pthi[i] : 0;
-  for (int i = tid; i < 2 * Dn; i += DEC_WG) twl_s[i] = a.ft.twl[i];
+  for (int i = tid; i < NPTS; i += DEC_WG) lg_s[i] = 2 * (i ? a.ft.lg[i] : ZL);
+  for (int i = tid; i < 4 * Q + 2; i += DEC_WG) ex_s[i] = i < 2 * Q ? 2 * a.ft.ex[i < Q ? i : i - Q] : 0;
+  for (int i = tid; i < NPHI; i += DEC_WG) phi_s[i] = 2 * a.ft.phi[i];
+  for (int i = tid; i < NR; i += DEC_WG) pthi_s[i] = P::TOP_TERM ? 2 * a.ft.pthi[i] : 0;
+  for (int i = tid; i < 2 * Dn; i += DEC_WG) twl_s[i] = 2 * a.ft.twl[i];
   __syncthreads();
   const int nops = a.plan ? a.plan->total : a.nops;
   const int wave = tid >> 6, lane = tid & 63, row = lane >> 4, l16 = lane & 15;
   uint32_t plo[KD];
 #pragma unroll
-  for (int d = 0; d < KD; ++d) plo[d] = a.ft.plo[64 * d + lane];
-  const uint32_t ptlo = P::TOP_TERM ? a.ft.ptlo[lane] : 0u;
+  for (int d = 0; d < KD; ++d) plo[d] = 2u * a.ft.plo[64 * d + lane];
+  const uint32_t ptlo = P::TOP_TERM ? 2u * a.ft.ptlo[lane] : 0u;
   DecWave<SET> &w = Wv[wave];
 
   for (int base = (blockIdx.x * DEC_WAVES + wave) * 4; base < nops; base += gridDim.x * DEC_WAVES * 4) {
@@ -219,12 +225,12 @@ void k_dec_decode(DecArgs a) {
 #pragma unroll
       for (int s = 0; s < NSB; ++s) {
         const int i = 16 * s + l16;
-        C[s] = (i == 0) ? 1u : 0u;
-        Bl[s] = (i == 1) ? 0u : ZL;
-        Wl[s] = ZL;
+        C[s] = (i == 0) ? 2u : 0u;  // doubled values and logs
+        Bl[s] = (i == 1) ? 0u : 2 * ZL;
+        Wl[s] = 2 * ZL;
       }
       int L = 0;
-      uint32_t bb = 1;
+      uint32_t bb = 2;
 #pragma unroll 1
       for (int N = 0; N < 2 * T; ++N) {
         // degree bounds over the wave's four rows: deg C <= L, deg B <= N + 1 - L
@@ -258,7 +264,7 @@ void k_dec_decode(DecArgs a) {
       for (int d = 0; d < KD; ++d) {
         const int lgL = LOG2D - d, Ld = 1 << lgL;
         const uint16_t *tw = twl_s + (2 * Dn - 2 * Ld);
-        for (int c = l16; c < Dn; c += 16) fr[c] = ex_s[lg_s[fr[c]] + tw[c & (Ld - 1)]];
+        for (int c = l16; c < Dn; c += 16) fr[c] = tab(ex_s, tab(lg_s, fr[c]) + tw[c & (Ld - 1)]);
         wave_lds_sync();
         for (int lq = lgL - 2; lq >= 0; --lq) {  // block length 4 q with q = 2^lq
           const int q4 = 1 << lq;
@@ -334,12 +340,12 @@ void k_dec_decode(DecArgs a) {
         const int phoff = (1 << (M - 6)) - (1 << (M - 6 - d));
 #pragma unroll
         for (int j = 0; j < step; ++j) {
-          const uint32_t lp = lg_s[plo[d] ^ phi_s[phoff + j]];
+          const uint32_t lp = tab(lg_s, plo[d] ^ phi_s[phoff + j]);
 #pragma unroll
           for (int r = j; r < NH; r += 2 * step) {
             const uint32_t y2 = V2[r + step];
-            const uint32_t plo_ = ex_s[lg_s[y2 & 0xFFFFu] + lp];
-            const uint32_t phi_ = ex_s[lg_s[y2 >> 16] + lp];
+            const uint32_t plo_ = tab(ex_s, tab(lg_s, y2 & 0xFFFFu) + lp);
+            const uint32_t phi_ = tab(ex_s, tab(lg_s, y2 >> 16) + lp);
             const uint32_t x2 = V2[r] ^ (plo_ | (phi_ << 16));
             V2[r] = x2;
             V2[r + step] = x2 ^ y2;
@@ -349,9 +355,9 @@ void k_dec_decode(DecArgs a) {
       {  // level 0 (half = 2^(m-1)): the pair is (low, high) of one register
 #pragma unroll
         for (int r = 0; r < NH; ++r) {
-          const uint32_t lp = lg_s[plo[0] ^ phi_s[r]];
+          const uint32_t lp = tab(lg_s, plo[0] ^ phi_s[r]);
           const uint32_t x = V2[r] & 0xFFFFu, y = V2[r] >> 16;
-          const uint32_t xn = x ^ ex_s[lg_s[y] + lp];
+          const uint32_t xn = x ^ tab(ex_s, tab(lg_s, y) + lp);
           V2[r] = xn | ((xn ^ y) << 16);
         }
       }
