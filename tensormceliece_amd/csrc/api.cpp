@@ -188,6 +188,8 @@ int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, si
   a.alpha = k->alpha;
   a.la = k->la;
   a.lu = k->lu;
+  a.ainv = k->ainv;
+  a.supp = k->supp;
   a.rej = k->rej;
   const int w = si.m - 12;
   const SetTables &tb = e->st[k->set];
@@ -348,8 +350,11 @@ int tmce_key_from_sk(int set, const unsigned char *sk, tmce_key **out) {
   if (int r = upload(&k->alpha, h.alpha)) return r;
   if (int r = upload(&k->la, h.la)) return r;
   if (int r = upload(&k->lu, h.lu)) return r;
+  if (int r = upload(&k->ainv, h.ainv)) return r;
+  if (int r = upload(&k->supp, h.supp)) return r;
   if (int r = upload(&k->rej, h.rej)) return r;
-  k->bytes = (h.F.size() + h.Fcol.size()) * 4 + fsw + (h.alpha.size() + h.la.size() + h.lu.size()) * 2 + h.rej.size();
+  k->bytes = (h.F.size() + h.Fcol.size() + h.supp.size()) * 4 + fsw +
+             (h.alpha.size() + h.la.size() + h.lu.size() + h.ainv.size()) * 2 + h.rej.size();
   *out = k.release();
   return 0;
 }
@@ -370,6 +375,8 @@ void tmce_key_free(tmce_key *k) {
   (void)hipFree(k->alpha);
   (void)hipFree(k->la);
   (void)hipFree(k->lu);
+  (void)hipFree(k->ainv);
+  (void)hipFree(k->supp);
   (void)hipFree(k->rej);
   delete k;
 }

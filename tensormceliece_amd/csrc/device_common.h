@@ -17,26 +17,6 @@ __constant__ static const uint64_t kKeccakRC[24] = {
     0x8000000000008002ULL, 0x8000000000000080ULL, 0x000000000000800aULL, 0x800000008000000aULL,
     0x8000000080008081ULL, 0x8000000000008080ULL, 0x0000000080000001ULL, 0x8000000080008008ULL};
 
-// 64-bit rotate as two 32-bit funnel shifts (v_alignbit_b32): for 0 < r < 32,
-// hi' = (hi:lo) >> (32-r), lo' = (lo:hi) >> (32-r); r >= 32 swaps the halves first.
-template <int R>
-__device__ __forceinline__ uint64_t rotl64(uint64_t x) {
-  if constexpr (R == 0) {
-    return x;
-  } else {
-    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-    if constexpr (R >= 32) { const uint32_t t = lo; lo = hi; hi = t; }
-    constexpr int r = R & 31;
-    if constexpr (r == 0) {
-      return ((uint64_t)hi << 32) | lo;
-    } else {
-      const uint32_t nhi = __builtin_amdgcn_alignbit(hi, lo, 32 - r);
-      const uint32_t nlo = __builtin_amdgcn_alignbit(lo, hi, 32 - r);
-      return ((uint64_t)nhi << 32) | nlo;
-    }
-  }
-}
-
 // Round constants held in two VGPRs: lane r (r < 24) has RC[r]; read back with v_readlane, so the
 // rolled round loop has no scalar-memory wait per round.
 struct KeccakRC {
@@ -53,58 +33,101 @@ struct KeccakRC {
   }
 };
 
-// One lane computes the whole permutation; the state stays in 50 VGPRs.  Lane (x,y) is A[x+5y].
-// All lanes of the wave must be active when `rc` is read (it is broadcast with v_readlane).
-__device__ __forceinline__ void keccak_f1600(uint64_t (&A)[25], const KeccakRC &rc) {
-#pragma unroll 1
-  for (int round = 0; round < 24; ++round) {
-    uint64_t C0 = A[0] ^ A[5] ^ A[10] ^ A[15] ^ A[20];
-    uint64_t C1 = A[1] ^ A[6] ^ A[11] ^ A[16] ^ A[21];
-    uint64_t C2 = A[2] ^ A[7] ^ A[12] ^ A[17] ^ A[22];
-    uint64_t C3 = A[3] ^ A[8] ^ A[13] ^ A[18] ^ A[23];
-    uint64_t C4 = A[4] ^ A[9] ^ A[14] ^ A[19] ^ A[24];
-    uint64_t D0 = C4 ^ rotl64<1>(C1), D1 = C0 ^ rotl64<1>(C2), D2 = C1 ^ rotl64<1>(C3);
-    uint64_t D3 = C2 ^ rotl64<1>(C4), D4 = C3 ^ rotl64<1>(C0);
-    // theta + rho + pi: B[y + 5*((2x+3y)%5)] = rot(A[x+5y] ^ D[x], r[x][y])
-    uint64_t B[25];
-    B[0]  = rotl64<0>(A[0] ^ D0);
-    B[10] = rotl64<1>(A[1] ^ D1);
-    B[20] = rotl64<62>(A[2] ^ D2);
-    B[5]  = rotl64<28>(A[3] ^ D3);
-    B[15] = rotl64<27>(A[4] ^ D4);
-    B[16] = rotl64<36>(A[5] ^ D0);
-    B[1]  = rotl64<44>(A[6] ^ D1);
-    B[11] = rotl64<6>(A[7] ^ D2);
-    B[21] = rotl64<55>(A[8] ^ D3);
-    B[6]  = rotl64<20>(A[9] ^ D4);
-    B[7]  = rotl64<3>(A[10] ^ D0);
-    B[17] = rotl64<10>(A[11] ^ D1);
-    B[2]  = rotl64<43>(A[12] ^ D2);
-    B[12] = rotl64<25>(A[13] ^ D3);
-    B[22] = rotl64<39>(A[14] ^ D4);
-    B[23] = rotl64<41>(A[15] ^ D0);
-    B[8]  = rotl64<45>(A[16] ^ D1);
-    B[18] = rotl64<15>(A[17] ^ D2);
-    B[3]  = rotl64<21>(A[18] ^ D3);
-    B[13] = rotl64<8>(A[19] ^ D4);
-    B[14] = rotl64<18>(A[20] ^ D0);
-    B[24] = rotl64<2>(A[21] ^ D1);
-    B[9]  = rotl64<61>(A[22] ^ D2);
-    B[19] = rotl64<56>(A[23] ^ D3);
-    B[4]  = rotl64<14>(A[24] ^ D4);
-    // chi + iota
-#pragma unroll
-    for (int y = 0; y < 5; ++y) {
-      uint64_t b0 = B[5 * y], b1 = B[5 * y + 1], b2 = B[5 * y + 2], b3 = B[5 * y + 3], b4 = B[5 * y + 4];
-      A[5 * y + 0] = b0 ^ (~b1 & b2);
-      A[5 * y + 1] = b1 ^ (~b2 & b3);
-      A[5 * y + 2] = b2 ^ (~b3 & b4);
-      A[5 * y + 3] = b3 ^ (~b4 & b0);
-      A[5 * y + 4] = b4 ^ (~b0 & b1);
-    }
-    A[0] ^= rc.get(round);
+// gfx950 three-input bitwise operation (v_bitop3_b32): bit i of `lut` is the result for the input
+// bits (a, b, c) with i = 4a + 2b + c.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+__device__ __forceinline__ uint32_t chi3(uint32_t a, uint32_t b, uint32_t c) {  // a ^ (~b & c)
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0xD2);
+}
+// (hi:lo) rotated left by R into (oh:ol)
+template <int R>
+__device__ __forceinline__ void rotl2(uint32_t lo, uint32_t hi, uint32_t &ol, uint32_t &oh) {
+  if constexpr (R >= 32) {
+    rotl2<R - 32>(hi, lo, ol, oh);
+  } else if constexpr (R == 0) {
+    ol = lo;
+    oh = hi;
+  } else {
+    oh = __builtin_amdgcn_alignbit(hi, lo, 32 - R);
+    ol = __builtin_amdgcn_alignbit(lo, hi, 32 - R);
   }
 }
+
+// Keccak-f[1600] on one lane, the state as 25 (lo, hi) register pairs.  Per round: column parities
+// C[x] by two 3-input XORs per half, theta as A ^= C[x-1] ^ rotl1(C[x+1]) in one 3-input XOR per
+// half, rho + pi as 24 two-word funnel rotations, chi as one 3-input operation per half, iota.
+template <class RC>
+__device__ __forceinline__ void keccak_rounds(uint64_t (&A)[25], const RC &rc) {
+  uint32_t lo[25], hi[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) {
+    lo[i] = (uint32_t)A[i];
+    hi[i] = (uint32_t)(A[i] >> 32);
+  }
+#pragma unroll 1
+  for (int round = 0; round < 24; ++round) {
+    uint32_t cl[5], ch[5], rl[5], rh[5];
+#pragma unroll
+    for (int x = 0; x < 5; ++x) {
+      cl[x] = xor3(xor3(lo[x], lo[x + 5], lo[x + 10]), lo[x + 15], lo[x + 20]);
+      ch[x] = xor3(xor3(hi[x], hi[x + 5], hi[x + 10]), hi[x + 15], hi[x + 20]);
+    }
+#pragma unroll
+    for (int x = 0; x < 5; ++x) rotl2<1>(cl[x], ch[x], rl[x], rh[x]);
+#pragma unroll
+    for (int i = 0; i < 25; ++i) {
+      const int x = i % 5;
+      lo[i] = xor3(lo[i], cl[(x + 4) % 5], rl[(x + 1) % 5]);
+      hi[i] = xor3(hi[i], ch[(x + 4) % 5], rh[(x + 1) % 5]);
+    }
+    // rho + pi: B[y + 5*((2x+3y)%5)] = rot(A[x+5y], r[x][y])
+    uint32_t bl[25], bh[25];
+    rotl2<0>(lo[0], hi[0], bl[0], bh[0]);
+    rotl2<1>(lo[1], hi[1], bl[10], bh[10]);
+    rotl2<62>(lo[2], hi[2], bl[20], bh[20]);
+    rotl2<28>(lo[3], hi[3], bl[5], bh[5]);
+    rotl2<27>(lo[4], hi[4], bl[15], bh[15]);
+    rotl2<36>(lo[5], hi[5], bl[16], bh[16]);
+    rotl2<44>(lo[6], hi[6], bl[1], bh[1]);
+    rotl2<6>(lo[7], hi[7], bl[11], bh[11]);
+    rotl2<55>(lo[8], hi[8], bl[21], bh[21]);
+    rotl2<20>(lo[9], hi[9], bl[6], bh[6]);
+    rotl2<3>(lo[10], hi[10], bl[7], bh[7]);
+    rotl2<10>(lo[11], hi[11], bl[17], bh[17]);
+    rotl2<43>(lo[12], hi[12], bl[2], bh[2]);
+    rotl2<25>(lo[13], hi[13], bl[12], bh[12]);
+    rotl2<39>(lo[14], hi[14], bl[22], bh[22]);
+    rotl2<41>(lo[15], hi[15], bl[23], bh[23]);
+    rotl2<45>(lo[16], hi[16], bl[8], bh[8]);
+    rotl2<15>(lo[17], hi[17], bl[18], bh[18]);
+    rotl2<21>(lo[18], hi[18], bl[3], bh[3]);
+    rotl2<8>(lo[19], hi[19], bl[13], bh[13]);
+    rotl2<18>(lo[20], hi[20], bl[14], bh[14]);
+    rotl2<2>(lo[21], hi[21], bl[24], bh[24]);
+    rotl2<61>(lo[22], hi[22], bl[9], bh[9]);
+    rotl2<56>(lo[23], hi[23], bl[19], bh[19]);
+    rotl2<14>(lo[24], hi[24], bl[4], bh[4]);
+    // chi
+#pragma unroll
+    for (int y = 0; y < 5; ++y)
+#pragma unroll
+      for (int x = 0; x < 5; ++x) {
+        lo[5 * y + x] = chi3(bl[5 * y + x], bl[5 * y + (x + 1) % 5], bl[5 * y + (x + 2) % 5]);
+        hi[5 * y + x] = chi3(bh[5 * y + x], bh[5 * y + (x + 1) % 5], bh[5 * y + (x + 2) % 5]);
+      }
+    // iota
+    const uint64_t c = rc.get(round);
+    lo[0] ^= (uint32_t)c;
+    hi[0] ^= (uint32_t)(c >> 32);
+  }
+#pragma unroll
+  for (int i = 0; i < 25; ++i) A[i] = (uint64_t)lo[i] | ((uint64_t)hi[i] << 32);
+}
+
+// All lanes of the wave must be active when `rc` is read (it is broadcast with v_readlane).
+__device__ __forceinline__ void keccak_f1600(uint64_t (&A)[25], const KeccakRC &rc) { keccak_rounds(A, rc); }
 
 // Same permutation with the round constant read from constant memory: safe in divergent code
 // (the plain per-thread kernels), at the cost of a scalar load per round.
@@ -112,36 +135,7 @@ __device__ __forceinline__ void keccak_f1600_c(uint64_t (&A)[25]) {
   struct ConstRC {
     __device__ __forceinline__ uint64_t get(int r) const { return kKeccakRC[r]; }
   };
-#pragma unroll 1
-  for (int round = 0; round < 24; ++round) {
-    uint64_t C0 = A[0] ^ A[5] ^ A[10] ^ A[15] ^ A[20];
-    uint64_t C1 = A[1] ^ A[6] ^ A[11] ^ A[16] ^ A[21];
-    uint64_t C2 = A[2] ^ A[7] ^ A[12] ^ A[17] ^ A[22];
-    uint64_t C3 = A[3] ^ A[8] ^ A[13] ^ A[18] ^ A[23];
-    uint64_t C4 = A[4] ^ A[9] ^ A[14] ^ A[19] ^ A[24];
-    uint64_t D0 = C4 ^ rotl64<1>(C1), D1 = C0 ^ rotl64<1>(C2), D2 = C1 ^ rotl64<1>(C3);
-    uint64_t D3 = C2 ^ rotl64<1>(C4), D4 = C3 ^ rotl64<1>(C0);
-    uint64_t B[25];
-    B[0] = rotl64<0>(A[0] ^ D0);   B[10] = rotl64<1>(A[1] ^ D1);  B[20] = rotl64<62>(A[2] ^ D2);
-    B[5] = rotl64<28>(A[3] ^ D3);  B[15] = rotl64<27>(A[4] ^ D4); B[16] = rotl64<36>(A[5] ^ D0);
-    B[1] = rotl64<44>(A[6] ^ D1);  B[11] = rotl64<6>(A[7] ^ D2);  B[21] = rotl64<55>(A[8] ^ D3);
-    B[6] = rotl64<20>(A[9] ^ D4);  B[7] = rotl64<3>(A[10] ^ D0);  B[17] = rotl64<10>(A[11] ^ D1);
-    B[2] = rotl64<43>(A[12] ^ D2); B[12] = rotl64<25>(A[13] ^ D3); B[22] = rotl64<39>(A[14] ^ D4);
-    B[23] = rotl64<41>(A[15] ^ D0); B[8] = rotl64<45>(A[16] ^ D1); B[18] = rotl64<15>(A[17] ^ D2);
-    B[3] = rotl64<21>(A[18] ^ D3); B[13] = rotl64<8>(A[19] ^ D4); B[14] = rotl64<18>(A[20] ^ D0);
-    B[24] = rotl64<2>(A[21] ^ D1); B[9] = rotl64<61>(A[22] ^ D2); B[19] = rotl64<56>(A[23] ^ D3);
-    B[4] = rotl64<14>(A[24] ^ D4);
-#pragma unroll
-    for (int y = 0; y < 5; ++y) {
-      uint64_t b0 = B[5 * y], b1 = B[5 * y + 1], b2 = B[5 * y + 2], b3 = B[5 * y + 3], b4 = B[5 * y + 4];
-      A[5 * y + 0] = b0 ^ (~b1 & b2);
-      A[5 * y + 1] = b1 ^ (~b2 & b3);
-      A[5 * y + 2] = b2 ^ (~b3 & b4);
-      A[5 * y + 3] = b3 ^ (~b4 & b0);
-      A[5 * y + 4] = b4 ^ (~b0 & b1);
-    }
-    A[0] ^= ConstRC().get(round);
-  }
+  keccak_rounds(A, ConstRC());
 }
 
 __device__ __forceinline__ void shake256_blocks_32_c(const uint64_t *__restrict__ msg, int nblocks, uint64_t out[4]) {

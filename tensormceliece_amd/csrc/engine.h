@@ -27,6 +27,8 @@ struct tmce_key {
   uint8_t *Fsw = nullptr;     // F as 0/1 bytes in MFMA fragment order (INT8 MFMA front end)
   uint32_t *Fcol = nullptr;   // syndrome map over all N positions, [N][RW] (re-encryption check)
   uint16_t *alpha = nullptr, *la = nullptr, *lu = nullptr;
+  uint16_t *ainv = nullptr;   // field element -> support index (0xFFFF off the support)
+  uint32_t *supp = nullptr;   // support flags over the field (decoder point order)
   uint8_t *rej = nullptr;
 };
 

@@ -218,6 +218,14 @@ static void build_dec_key_t(const uint8_t *sk, DecKeyHost &o) {
     o.la[i] = F.logv(a);
     o.lu[i] = F.logv(u[i]);
   }
+  o.ainv.assign(1u << P::M, 0xFFFFu);
+  o.supp.assign((1u << P::M) / 32, 0u);
+  for (int i = 0; i < P::N; ++i) {
+    o.ainv[o.alpha[i]] = (uint16_t)i;
+    // decoder point order: field element a = 64 g + l is bit g % 32 of word l * (2^m / 2048) + g / 32
+    const uint32_t av = o.alpha[i], g = av >> 6, l = av & 63;
+    o.supp[l * ((1u << P::M) / 2048) + g / 32] |= 1u << (g % 32);
+  }
   // front-end matrix: row j*M + b, column i (i < KBITS): bit b of alpha_i^j u_i (syndrome-major)
   // and the same map over all N positions, column-major (the decoder's re-encryption check)
   o.F.assign((size_t)P::R * P::KWP, 0);

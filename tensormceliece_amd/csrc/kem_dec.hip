@@ -82,9 +82,16 @@ struct DecWave {
   uint16_t s[4][P::TT];           // syndromes of the wave's four ops (doubled logs, 2 ZL for 0)
   uint16_t f[4][P::D];            // locator coefficients (FFT input), per op
   uint16_t pos[P::IND_STRIDE];    // error positions of the op being finished
-  uint32_t Z[(1 << P::M) / 32];   // zero flags of sigma over all field elements
   uint64_t m64[17 * P::MSG_BLOCKS];  // session-key message of the op being finished
 };
+
+// 0 as far as the compiler can tell only at run time: an index offset that keeps reads of constant
+// LDS tables inside the loop that uses them (hoisted out, each read would hold a VGPR throughout)
+__device__ __forceinline__ int opaque_zero() {
+  int z = 0;
+  asm volatile("" : "+v"(z));
+  return z;
+}
 
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
@@ -190,6 +197,7 @@ void k_dec_decode(DecArgs a) {
   __shared__ uint16_t phi_s[NPHI];
   __shared__ uint16_t pthi_s[NR];
   __shared__ uint16_t twl_s[2 * Dn];
+  __shared__ uint16_t plo_s[(KD + 1) * 64];  // butterfly multipliers' lane parts [KD][64], then (lane)^t
   __shared__ DecWave<SET> Wv[DEC_WAVES];
 
   const int tid = threadIdx.x;
@@ -198,13 +206,11 @@ void k_dec_decode(DecArgs a) {
   for (int i = tid; i < NPHI; i += DEC_WG) phi_s[i] = 2 * a.ft.phi[i];
   for (int i = tid; i < NR; i += DEC_WG) pthi_s[i] = P::TOP_TERM ? 2 * a.ft.pthi[i] : 0;
   for (int i = tid; i < 2 * Dn; i += DEC_WG) twl_s[i] = 2 * a.ft.twl[i];
+  for (int i = tid; i < (KD + 1) * 64; i += DEC_WG)
+    plo_s[i] = i < KD * 64 ? 2 * a.ft.plo[i] : (P::TOP_TERM ? 2 * a.ft.ptlo[i - KD * 64] : 0);
   __syncthreads();
   const int nops = a.plan ? a.plan->total : a.nops;
   const int wave = tid >> 6, lane = tid & 63, row = lane >> 4, l16 = lane & 15;
-  uint32_t plo[KD];
-#pragma unroll
-  for (int d = 0; d < KD; ++d) plo[d] = 2u * a.ft.plo[64 * d + lane];
-  const uint32_t ptlo = P::TOP_TERM ? 2u * a.ft.ptlo[lane] : 0u;
   DecWave<SET> &w = Wv[wave];
 
   for (int base = (blockIdx.x * DEC_WAVES + wave) * 4; base < nops; base += gridDim.x * DEC_WAVES * 4) {
@@ -263,7 +269,7 @@ void k_dec_decode(DecArgs a) {
 #pragma unroll 1
       for (int d = 0; d < KD; ++d) {
         const int lgL = LOG2D - d, Ld = 1 << lgL;
-        const uint16_t *tw = twl_s + (2 * Dn - 2 * Ld);
+        const uint16_t *tw = twl_s + (2 * Dn - 2 * Ld) + opaque_zero();
         for (int c = l16; c < Dn; c += 16) fr[c] = tab(ex_s, tab(lg_s, fr[c]) + tw[c & (Ld - 1)]);
         wave_lds_sync();
         for (int lq = lgL - 2; lq >= 0; --lq) {  // block length 4 q with q = 2^lq
@@ -296,14 +302,32 @@ void k_dec_decode(DecArgs a) {
       const int op = base + q;
       if (op >= nops) break;
       // key material of the op (one key, or its key group under a plan)
-      const uint16_t *k_alpha = a.alpha;
+      const uint16_t *k_ainv = a.ainv;
+      const uint32_t *k_supp = a.supp;
       const uint32_t *k_fcol = a.Fcol;
       const uint8_t *k_rej = a.rej;  // (loads below go through global_ptr: these may come from memory)
       if (a.plan) {  // the wave's ops share a key group (groups are PLAN_ALIGN-aligned)
         const KeyDev &kd = a.plan->keys[plan_group(a.plan, __builtin_amdgcn_readfirstlane(op))];
-        k_alpha = uniform_ptr(kd.alpha);
+        k_ainv = uniform_ptr(kd.ainv);
+        k_supp = uniform_ptr(kd.supp);
         k_fcol = uniform_ptr(kd.Fcol);
         k_rej = uniform_ptr(kd.rej);
+      }
+      // support flags of this lane's field points (point 64 (32 w + r) + lane is bit r of word w),
+      // fetched ahead
+      constexpr int ZPL = (1 << M) / 32 / 64;  // flag words per lane: 2 (m = 12), 4 (m = 13)
+      static_assert(ZPL == 2 || ZPL == 4, "field of 2^12 or 2^13 points");
+      uint32_t sw[ZPL];
+      if constexpr (ZPL == 2) {
+        const uint2 v = global_load8(reinterpret_cast<const uint2 *>(k_supp), lane);
+        sw[0] = v.x;
+        sw[1] = v.y;
+      } else {
+        const uint4 v = global_load16(reinterpret_cast<const uint4 *>(k_supp), lane);
+        sw[0] = v.x;
+        sw[1] = v.y;
+        sw[2] = v.z;
+        sw[3] = v.w;
       }
       // the ciphertext (for the session-key message), fetched now, stored to LDS after the message
       // buffer's previous reader is done
@@ -315,6 +339,10 @@ void k_dec_decode(DecArgs a) {
         for (int c = 0; c < CPL; ++c) ctw[c] = 64 * c + lane < CH16 ? (uint32_t)ct16[64 * c + lane] : 0u;
       }
       uint8_t *m8 = reinterpret_cast<uint8_t *>(w.m64);
+      uint32_t *m32 = reinterpret_cast<uint32_t *>(w.m64);
+      // [b][e] starts as zero (the bits of e are set at the roots below; b is written last)
+      constexpr int EW = (1 + P::N / 8 + 3) / 4;
+      for (int k = lane; k < EW; k += 64) m32[k] = 0;
       // the top-down result: after KD buffer swaps it is in the op's row of f (KD even) or of s
       const uint16_t *fq = (KD & 1) ? w.s[q] : w.f[q];
       // ciphertext bytes into the message buffer (before the butterflies need the registers)
@@ -329,6 +357,8 @@ void k_dec_decode(DecArgs a) {
       // bottom-up butterflies in registers: V2[r] packs point 64r+lane (low half) and point
       // 64(r+NR/2)+lane (high half); the two butterflies of a packed pair share their multiplier
       constexpr int NH = NR / 2;
+      const int zo = opaque_zero();
+      const uint16_t *phq = phi_s + zo, *ptq = pthi_s + zo, *plq = plo_s + zo;
       uint32_t V2[NH];
 #pragma unroll
       for (int r = 0; r < NH; ++r) V2[r] = (uint32_t)fq[r] | ((uint32_t)fq[r + NH] << 16);
@@ -338,9 +368,10 @@ void k_dec_decode(DecArgs a) {
         (void)dummy;
         const int step = 1 << (M - 7 - d);
         const int phoff = (1 << (M - 6)) - (1 << (M - 6 - d));
+        const uint32_t plod = plq[64 * d + lane];
 #pragma unroll
         for (int j = 0; j < step; ++j) {
-          const uint32_t lp = tab(lg_s, plo[d] ^ phi_s[phoff + j]);
+          const uint32_t lp = tab(lg_s, plod ^ phq[phoff + j]);
 #pragma unroll
           for (int r = j; r < NH; r += 2 * step) {
             const uint32_t y2 = V2[r + step];
@@ -353,68 +384,60 @@ void k_dec_decode(DecArgs a) {
         }
       }
       {  // level 0 (half = 2^(m-1)): the pair is (low, high) of one register
+        const uint32_t plo0 = plq[lane];
 #pragma unroll
         for (int r = 0; r < NH; ++r) {
-          const uint32_t lp = tab(lg_s, plo[0] ^ phi_s[r]);
+          const uint32_t lp = tab(lg_s, plo0 ^ phq[r]);
           const uint32_t x = V2[r] & 0xFFFFu, y = V2[r] >> 16;
           const uint32_t xn = x ^ tab(ex_s, tab(lg_s, y) + lp);
           V2[r] = xn | ((xn ^ y) << 16);
         }
       }
-      // zero flags of sigma (adding the monic x^t term where it lies outside the FFT)
+      // zero flags of sigma (adding the monic x^t term where it lies outside the FFT), transposed:
+      // bit r of zf[w] in lane l flags the point 64 g + l of group g = 32 w + r, where the low
+      // halves of V2 are groups 0..NH-1 and the high halves groups NH..2NH-1; masked by the
+      // support flags in the same layout, then counted (the weight of e is the sum over the wave)
+      static_assert(NH % 32 == 0 && 2 * NH == 32 * ZPL, "mask words per lane");
+      uint32_t zm[ZPL];
 #pragma unroll
-      for (int r = 0; r < NH; ++r) {
-        uint32_t lo = V2[r] & 0xFFFFu, hi = V2[r] >> 16;
-        if (P::TOP_TERM) {
-          lo ^= ptlo ^ pthi_s[r];
-          hi ^= ptlo ^ pthi_s[r + NH];
-        }
-        const uint64_t zl = __ballot(lo == 0);
-        const uint64_t zh = __ballot(hi == 0);
-        if (lane == 0) {
-          w.Z[2 * r] = (uint32_t)zl; w.Z[2 * r + 1] = (uint32_t)(zl >> 32);
-          w.Z[2 * (r + NH)] = (uint32_t)zh; w.Z[2 * (r + NH) + 1] = (uint32_t)(zh >> 32);
+      for (int z = 0; z < ZPL; ++z) zm[z] = 0;
+      {
+        const uint32_t ptlo = plq[64 * KD + lane];
+        const uint32_t ptlo2 = ptlo | (ptlo << 16);
+#pragma unroll
+        for (int r = 0; r < NH; ++r) {
+          uint32_t v = V2[r];
+          if (P::TOP_TERM) v = xor3(v, ptlo2, (uint32_t)ptq[r] | ((uint32_t)ptq[r + NH] << 16));
+          // x - 1 has bit 31 set exactly for x = 0 (field values are below 2^15)
+          zm[r / 32] |= (((v & 0xFFFFu) - 1u) >> 31) << (r % 32);
+          zm[(r + NH) / 32] |= (((v >> 16) - 1u) >> 31) << (r % 32);
         }
       }
-      wave_lds_sync();
-      // roots at the support: lane l takes the 8 consecutive support elements 512 c + 8 l .. +7
-      // (16-byte loads, issued four at a time); their root flags form byte 64 c + l of e, kept at
-      // message byte 1 + 64 c + l
-      constexpr int NC = ceil_div(P::N, 512), CH = 4;
-      static_assert(P::N % 8 == 0, "support read in groups of 8");
-      int wt = 0;
+      uint32_t cnt = 0;
 #pragma unroll
-      for (int c0 = 0; c0 < NC; c0 += CH) {
-        uint4 av4[CH];
+      for (int z = 0; z < ZPL; ++z) {
+        zm[z] &= sw[z];
+        cnt += (uint32_t)__popc(zm[z]);
+      }
+      // roots at the support: an exclusive prefix of the per-lane root counts (one count bit per
+      // ballot) places each lane's roots in the position list (as field elements)
+      int pre = 0, wt = 0;
+      for (int k = 0; __any(cnt != 0u); ++k) {  // at most 8 passes (cnt <= 128)
+        const uint64_t bk = __ballot(cnt & 1u);
+        pre += (int)mbcnt64(bk) << k;
+        wt += __popcll(bk) << k;
+        cnt >>= 1;
+      }
+      {
+        int idx = pre;
 #pragma unroll
-        for (int cc = 0; cc < CH; ++cc) {
-          const int i0 = 512 * (c0 + cc) + 8 * lane;
-          av4[cc] = (c0 + cc < NC && i0 < P::N) ? global_load16(reinterpret_cast<const uint4 *>(k_alpha + i0), 0)
-                                                : make_uint4(0, 0, 0, 0);
-        }
-#pragma unroll
-        for (int cc = 0; cc < CH; ++cc) {
-          const int c = c0 + cc;
-          if (c >= NC) break;
-          const int i0 = 512 * c + 8 * lane;
-          const uint32_t pr[4] = {av4[cc].x, av4[cc].y, av4[cc].z, av4[cc].w};
-          uint32_t byte = 0;
-#pragma unroll
-          for (int b = 0; b < 8; ++b) {
-            const uint32_t av = (pr[b >> 1] >> (16 * (b & 1))) & 0xFFFFu;
-            byte |= ((w.Z[av >> 5] >> (av & 31)) & 1u) << b;
-          }
-          if (i0 >= P::N) byte = 0;
-          if (64 * c + lane < P::N / 8) m8[1 + 64 * c + lane] = (uint8_t)byte;
-#pragma unroll
-          for (int b = 0; b < 8; ++b) {
-            const bool ei = (byte >> b) & 1u;
-            const uint64_t mask = __ballot(ei);
-            if (ei) {
-              const int idx = wt + (int)mbcnt64(mask);
-              if (idx < T) w.pos[idx] = (uint16_t)(i0 + b);
-            }
-            wt += __popcll(mask);
+        for (int z = 0; z < ZPL; ++z) {
+          uint32_t bits = zm[z];
+          while (bits) {
+            const int bi = __builtin_ctz(bits);
+            bits &= bits - 1u;
+            if (idx < T) w.pos[idx] = (uint16_t)(64 * (32 * z + bi) + lane);
+            ++idx;
           }
         }
       }
@@ -426,12 +449,17 @@ void k_dec_decode(DecArgs a) {
         constexpr int RH = P::RW / 2;  // 8-byte words per column (RW is even)
         static_assert(P::RW % 2 == 0 && RH <= 64, "one 8-byte word per lane");
         constexpr int KP = ceil_div(T, 64);
+        // support index of each root (lane k holds root k), and its bit of e
         uint32_t pos_r[KP];
+        const auto inv = global_ptr(k_ainv);
 #pragma unroll
         for (int kk = 0; kk < KP; ++kk) {
           const int k = 64 * kk + lane;
-          pos_r[kk] = k < T ? w.pos[k] : 0u;
+          pos_r[kk] = k < T ? (uint32_t)inv[w.pos[k]] : 0u;
         }
+#pragma unroll
+        for (int kk = 0; kk < KP; ++kk)
+          if (64 * kk + lane < T) atomicOr(&m32[(8u + pos_r[kk]) >> 5], 1u << ((8u + pos_r[kk]) & 31u));
         // every lane takes part (lanes past the column re-read its last word): readlane below
         // reads position k from lane k, so no lane may skip the position load above
         const uint2 *fc = reinterpret_cast<const uint2 *>(k_fcol);

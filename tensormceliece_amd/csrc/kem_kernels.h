@@ -22,6 +22,8 @@ struct KeyDev {
   const uint8_t *Fsw;     // decapsulation: front-end matrix (MFMA form)
   const uint32_t *Fcol;   // decapsulation: syndrome map columns [N][RW]
   const uint16_t *alpha, *la, *lu;
+  const uint16_t *ainv;   // decapsulation: field element -> support index (0xFFFF off the support)
+  const uint32_t *supp;   // decapsulation: support flags over the field (decoder point order)
   const uint8_t *rej;
   int pad_bad;            // 6960119 public key with nonzero padding bits
   int reserved;
@@ -137,6 +139,8 @@ struct DecArgs {
   const uint16_t *alpha; // key: [N] support
   const uint16_t *la;    // key: [N] log alpha (0xFFFF for alpha = 0)
   const uint16_t *lu;    // key: [N] log 1/g(alpha)^2 (0xFFFF for 0)
+  const uint16_t *ainv;  // key: [2^m] support index of each field element (0xFFFF off the support)
+  const uint32_t *supp;  // key: [2^m / 32] support flags: element 64 g + l is bit g % 32 of word l 2^(m-11) + g / 32
   const uint8_t *rej;    // key: [N/8] implicit-rejection string s
   FieldTables ft;
   uint8_t *key;          // [n][32]

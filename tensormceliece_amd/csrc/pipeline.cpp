@@ -229,6 +229,8 @@ int tmce_pipeline_run(tmce_pipeline *p, const tmce_sub_batch *subs, size_t nsub)
     kd.alpha = k->alpha;
     kd.la = k->la;
     kd.lu = k->lu;
+    kd.ainv = k->ainv;
+    kd.supp = k->supp;
     kd.rej = k->rej;
     kd.pad_bad = k->pad_bad ? 1 : 0;
   }
