@@ -79,7 +79,10 @@ __device__ __forceinline__ uint32_t tab(const uint16_t *t, uint32_t byte_off) {
 template <int SET>
 struct DecWave {
   using P = Par<SET>;
-  uint16_t s[4][P::TT];           // syndromes of the wave's four ops (doubled logs, 2 ZL for 0)
+  // syndromes of the wave's four ops (doubled logs, 2 ZL for 0) behind SPAD zero entries, so the
+  // Berlekamp-Massey window s_{N-i} reads zero for i > N
+  static constexpr int SPAD = 16 * ceil_div(P::T + 1, 16);
+  uint16_t s[4][SPAD + P::TT];
   uint16_t f[4][P::D];            // locator coefficients (FFT input), per op
   uint16_t pos[P::IND_STRIDE];    // error positions of the op being finished
   uint64_t m64[17 * P::MSG_BLOCKS];  // session-key message of the op being finished
@@ -115,41 +118,39 @@ __device__ __forceinline__ uint32_t ror1(uint32_t v) {
 }
 
 // One Berlekamp-Massey step on the wave's four rows, for NA active coefficient slots: slots >= NA
-// hold zero coefficients of C and B in every row (deg C <= L, deg B <= N + 1 - L), so their table
-// lookups are skipped; the syndrome window W moves through every slot.
+// hold zero coefficients of C and B in every row (deg C <= L, deg B <= N + 1 - L), so they are
+// skipped.  The window W_i = s_{N-i} of slot s is read from the row's padded syndrome array at
+// wp + 32 (NSB - 1 - s) bytes (wp = the lane's window base, advanced 2 bytes per step).  Straight
+// line: a zero discrepancy turns the update factor into the zero sentinel and the length change
+// into a select, so no lane diverges.  lb = log b (doubled).
 template <int NA, int NSB, int T, int Q>
-__device__ __forceinline__ void bm_step(int N, uint32_t lsN, int l16, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
-                                        uint32_t (&Wl)[NSB], int &L, uint32_t &bb, const uint16_t *lg_s,
-                                        const uint16_t *ex_s) {
+__device__ __forceinline__ void bm_step(int N, const uint8_t *wp, int l16, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
+                                        int &L, uint32_t &lb, const uint16_t *lg_s, const uint16_t *ex_s) {
   constexpr uint32_t ZL2 = 4u * Q, Q2 = 2u * Q;  // doubled sentinel and modulus
-  uint32_t R[NSB];
-#pragma unroll
-  for (int s = 0; s < NSB; ++s) R[s] = ror1(Wl[s]);
-#pragma unroll
-  for (int s = 0; s < NSB; ++s) Wl[s] = (l16 == 0) ? (s ? R[s > 0 ? s - 1 : 0] : lsN) : R[s];
   uint32_t lc[NA], dp = 0;
 #pragma unroll
   for (int s = 0; s < NA; ++s) lc[s] = tab(lg_s, C[s]);
 #pragma unroll
-  for (int s = 0; s < NA; ++s) dp ^= tab(ex_s, lc[s] + Wl[s]);
-  const uint32_t d = row_xor16(dp);
-  if (d != 0) {
-    uint32_t lf = tab(lg_s, d) + Q2 - tab(lg_s, bb);
-    lf = lf >= Q2 ? lf - Q2 : lf;
-    const bool upd = (2 * L <= N);
-#pragma unroll
-    for (int s = 0; s < NA; ++s) {
-      const uint32_t nc = C[s] ^ tab(ex_s, lf + Bl[s]);
-      if (upd) Bl[s] = lc[s];
-      C[s] = nc;
-    }
-    if (upd) {
-      L = N + 1 - L;
-      bb = d;
-    }
+  for (int s = 0; s < NA; ++s) {
+    const uint32_t wv = *reinterpret_cast<const uint16_t *>(wp + 32 * (NSB - 1 - s));
+    dp ^= tab(ex_s, lc[s] + wv);
   }
+  const uint32_t d = row_xor16(dp);
+  const uint32_t ld = tab(lg_s, d);
+  uint32_t lf = ld + Q2 - lb;  // log (d / b)
+  lf = lf >= Q2 ? lf - Q2 : lf;
+  lf = d != 0 ? lf : ZL2;
+  const bool upd = d != 0 && 2 * L <= N;
+#pragma unroll
+  for (int s = 0; s < NA; ++s) {
+    C[s] ^= tab(ex_s, lf + Bl[s]);
+    Bl[s] = upd ? lc[s] : Bl[s];
+  }
+  L = upd ? N + 1 - L : L;
+  lb = upd ? ld : lb;
   // B <- x B on the slots that can hold a nonzero coefficient afterwards
   constexpr int NS = NA < NSB ? NA + 1 : NSB;
+  uint32_t R[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) R[s] = ror1(Bl[s]);
 #pragma unroll
@@ -158,18 +159,26 @@ __device__ __forceinline__ void bm_step(int N, uint32_t lsN, int l16, uint32_t (
     Bl[s] = (16 * s + l16 <= T) ? v : ZL2;
   }
 }
-// runs fn(integral_constant<K>) for the wave-uniform n in 1..NSB (straight-line code per count)
-template <int NSB, int K = 1, class Fn>
-__device__ __forceinline__ void with_count(int n, Fn &&fn) {
-  if constexpr (K < NSB) {
-    if (n <= K) {
-      fn(std::integral_constant<int, K>{});
-      return;
-    }
-    with_count<NSB, K + 1>(n, fn);
-  } else {
-    fn(std::integral_constant<int, K>{});
+// 1 + (max over the wave's four rows of max(L, N + 1 - L)) / 16: the active slot count of step N
+__device__ __forceinline__ int bm_slots(int L, int N) {
+  const int L0 = __builtin_amdgcn_readlane(L, 0), L1 = __builtin_amdgcn_readlane(L, 16);
+  const int L2 = __builtin_amdgcn_readlane(L, 32), L3 = __builtin_amdgcn_readlane(L, 48);
+  const int Lmax = max(max(L0, L1), max(L2, L3)), Lmin = min(min(L0, L1), min(L2, L3));
+  return (max(Lmax, N + 1 - Lmin) >> 4) + 1;
+}
+// Berlekamp-Massey steps N.. while the active slot count is NA, then the next count: the count
+// never decreases (max(L, N + 1 - L) does not, in every row), so each count runs as its own loop
+template <int NA, int NSB, int T, int Q>
+__device__ __forceinline__ void bm_run(int &N, const uint8_t *wl, int l16, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
+                                       int &L, uint32_t &lb, const uint16_t *lg_s, const uint16_t *ex_s) {
+  bool go = N < 2 * T && (NA == NSB || bm_slots(L, N) <= NA);
+#pragma unroll 1
+  while (go) {
+    bm_step<NA, NSB, T, Q>(N, wl + 2 * N, l16, C, Bl, L, lb, lg_s, ex_s);
+    ++N;
+    go = N < 2 * T && (NA == NSB || bm_slots(L, N) <= NA);
   }
+  if constexpr (NA < NSB) bm_run<NA + 1, NSB, T, Q>(N, wl, l16, C, Bl, L, lb, lg_s, ex_s);
 }
 
 // One wavefront decodes four ops at a time: Berlekamp-Massey runs on the four 16-lane rows in
@@ -211,7 +220,9 @@ void k_dec_decode(DecArgs a) {
   __syncthreads();
   const int nops = a.plan ? a.plan->total : a.nops;
   const int wave = tid >> 6, lane = tid & 63, row = lane >> 4, l16 = lane & 15;
+  constexpr int SPAD = DecWave<SET>::SPAD;
   DecWave<SET> &w = Wv[wave];
+  for (int i = lane; i < 4 * SPAD; i += 64) w.s[i / SPAD][i % SPAD] = 2 * ZL;  // window padding (zero)
 
   for (int base = (blockIdx.x * DEC_WAVES + wave) * 4; base < nops; base += gridDim.x * DEC_WAVES * 4) {
     // ---- 1. syndromes of the four ops (syndrome-major bits from the front end)
@@ -219,36 +230,25 @@ void k_dec_decode(DecArgs a) {
     for (int q = 0; q < 4; ++q) {
       const int op = base + q;
       const uint32_t *Srow = a.S + (size_t)(op < nops ? op : 0) * P::RW;
-      for (int j = lane; j < P::TT; j += 64) w.s[q][j] = lg_s[op < nops ? synd_value<P>(Srow, j) : 0u];
+      for (int j = lane; j < P::TT; j += 64) w.s[q][SPAD + j] = lg_s[op < nops ? synd_value<P>(Srow, j) : 0u];
     }
     wave_lds_sync();
 
     // ---- 2. Berlekamp-Massey, one op per row; C in normal form, B and the syndrome window
     //         W_i = s_{N-i} in log form
     {
-      const uint16_t *sq = w.s[row];
-      uint32_t C[NSB], Bl[NSB], Wl[NSB];
+      // the lane's window base: slot s reads s_{N - 16 s - l16} at wl + 2 N + 32 (NSB - 1 - s)
+      const uint8_t *wl = reinterpret_cast<const uint8_t *>(w.s[row] + (SPAD - l16 - 16 * (NSB - 1)));
+      uint32_t C[NSB], Bl[NSB];
 #pragma unroll
       for (int s = 0; s < NSB; ++s) {
         const int i = 16 * s + l16;
         C[s] = (i == 0) ? 2u : 0u;  // doubled values and logs
         Bl[s] = (i == 1) ? 0u : 2 * ZL;
-        Wl[s] = 2 * ZL;
       }
-      int L = 0;
-      uint32_t bb = 2;
-#pragma unroll 1
-      for (int N = 0; N < 2 * T; ++N) {
-        // degree bounds over the wave's four rows: deg C <= L, deg B <= N + 1 - L
-        const int L0 = __builtin_amdgcn_readlane(L, 0), L1 = __builtin_amdgcn_readlane(L, 16);
-        const int L2 = __builtin_amdgcn_readlane(L, 32), L3 = __builtin_amdgcn_readlane(L, 48);
-        const int Lmax = max(max(L0, L1), max(L2, L3)), Lmin = min(min(L0, L1), min(L2, L3));
-        const int na = (max(Lmax, N + 1 - Lmin) >> 4) + 1;
-        const uint32_t lsN = sq[N];
-        with_count<NSB>(na, [&](auto k) {
-          bm_step<decltype(k)::value, NSB, T, Q>(N, lsN, l16, C, Bl, Wl, L, bb, lg_s, ex_s);
-        });
-      }
+      int L = 0, N = 0;
+      uint32_t lb = 0;  // log b, b = 1
+      bm_run<1, NSB, T, Q>(N, wl, l16, C, Bl, L, lb, lg_s, ex_s);
       // locator sigma_j = C_{t-j} (sigma_t = C_0 = 1 is added from the power table when t = D)
       for (int j = l16; j < Dn; j += 16) w.f[row][j] = 0;
       wave_lds_sync();
@@ -264,7 +264,7 @@ void k_dec_decode(DecArgs a) {
     //         op r; the row's syndrome array is free now and serves as the second buffer): twist,
     //         Taylor expansion at x^2 + x, even/odd split
     {
-      uint16_t *fr = w.f[row], *tr = w.s[row];
+      uint16_t *fr = w.f[row], *tr = w.s[row] + SPAD;
       static_assert(P::TT >= Dn, "syndrome array doubles as the FFT buffer");
 #pragma unroll 1
       for (int d = 0; d < KD; ++d) {
@@ -344,7 +344,7 @@ void k_dec_decode(DecArgs a) {
       constexpr int EW = (1 + P::N / 8 + 3) / 4;
       for (int k = lane; k < EW; k += 64) m32[k] = 0;
       // the top-down result: after KD buffer swaps it is in the op's row of f (KD even) or of s
-      const uint16_t *fq = (KD & 1) ? w.s[q] : w.f[q];
+      const uint16_t *fq = (KD & 1) ? w.s[q] + SPAD : w.f[q];
       // ciphertext bytes into the message buffer (before the butterflies need the registers)
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
