@@ -21,7 +21,8 @@ CANDIDATES = {
     "front": [(0, 0)] + [(1, c) for c in range(6)],  # (front_mfma, front_cfg)
     # (gen_form, gen_blocks): the per-lane sampler keeps at most n/256 workgroups busy, the
     # pipelined one n * (lanes per op) / 256; larger grids are clamped to that
-    "gen": [(0, 64), (0, 128), (0, 256), (0, 512), (1, 256), (1, 512), (1, 1024), (1, 2048)],
+    "gen": [(0, 64), (0, 128), (0, 256), (0, 512), (1, 128), (1, 192), (1, 256), (1, 384), (1, 512), (1, 1024),
+            (1, 2048)],
     "decode_blocks": [128, 256, 384, 512, 768, 1024],
 }
 

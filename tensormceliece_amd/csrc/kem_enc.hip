@@ -208,7 +208,7 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
   constexpr uint32_t GFMASK = (1u << P::M) - 1u;
   constexpr int NEW = ceil_div(P::N, 32);
   constexpr int DPP_GROUP0 = G == 2 ? 0xA0 : 0x00;  // quad_perm: each lane reads its group's lane 0
-  __shared__ uint16_t lst[P::T * GEN_WG];  // [index][thread]
+  __shared__ uint16_t lst[(P::T + 1) * GEN_WG];  // [index][thread]; row T takes draws past the first T
   __shared__ uint32_t bmp[GEN_WAVES][4][NEW + 1];
   const int tid = threadIdx.x;
   const int lane = lane_id();
@@ -232,18 +232,23 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
 #pragma unroll
   for (int i = 0; i < 25; ++i) A[i] = 0;
 
-  // draws of block `b` (0-based) of the lane's attempt, from the state A
+  // draws of block `b` (0-based) of the lane's attempt, from the state A, appended to the lane's
+  // list while it holds fewer than T (lanes with `on` false keep their count).  Branch-free: every
+  // draw is written at the current count (row T is the spare) and the count advances only for a
+  // draw below N that the list still takes.
+  uint16_t *const col = lst + tid;
   auto extract = [&](int b, bool on) {
+    const int cap = on ? P::T : 0;
+    const int lim = P::NDRAW - 68 * b;  // draws in block b (the last block is partial)
 #pragma unroll
     for (int w = 0; w < 17; ++w)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int idx = b * 68 + w * 4 + q;
+        const int k = 4 * w + q;
+        const bool in_every_block = k < P::NDRAW - 68 * (B - 1);
         const uint32_t v = (uint32_t)(A[w] >> (16 * q)) & GFMASK;
-        if (on && idx < P::NDRAW && v < (uint32_t)P::N && count < P::T) {
-          lst[count * GEN_WG + tid] = (uint16_t)v;
-          ++count;
-        }
+        col[count * GEN_WG] = (uint16_t)v;
+        count += (v < (uint32_t)P::N && count < cap && (in_every_block || k < lim)) ? 1 : 0;
       }
   };
 
