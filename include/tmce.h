@@ -119,7 +119,10 @@ int tmce_enc_batch(tmce_key *pk, unsigned char *ct, unsigned char *k, int *statu
 int tmce_dec_batch(tmce_key *sk, unsigned char *k, int *status, const unsigned char *ct, size_t n);
 
 /* the same over device buffers, asynchronous on `stream` (a hipStream_t; NULL = the engine's
-   stream); inputs resident in HBM */
+   stream); inputs resident in HBM.  The engine keeps one scratch set for encapsulations (and
+   syndromes) and one for decapsulations: a call on a stream other than the set's previous user
+   first waits for that use on the device, so calls of one direction never overlap, while an
+   encapsulation and a decapsulation on two streams may run concurrently. */
 int tmce_enc_batch_device(tmce_key *pk, unsigned char *d_ct, unsigned char *d_k, int *d_status,
                           const unsigned char *d_seeds, size_t n, void *stream);
 int tmce_dec_batch_device(tmce_key *sk, unsigned char *d_k, int *d_status, const unsigned char *d_ct,
@@ -231,6 +234,17 @@ int tmce_memcpy_h2d(void *d, const void *h, size_t bytes);
 int tmce_memcpy_d2h(void *h, const void *d, size_t bytes);
 int tmce_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int tmce_sync(void);
+
+/* streams and events for the asynchronous device entry points (callers without a HIP runtime
+   binding): streams are non-blocking at the default priority; tmce_stream_wait_event makes later
+   work on `stream` wait for the work recorded in `event` */
+int tmce_stream_create(void **stream);
+int tmce_stream_destroy(void *stream);
+int tmce_stream_sync(void *stream);
+int tmce_event_create(void **event);
+int tmce_event_destroy(void *event);
+int tmce_event_record(void *event, void *stream);
+int tmce_stream_wait_event(void *stream, void *event);
 
 /* kernel timing: while enabled, every kernel launch of the engine is bracketed by HIP events on
    its stream; tmce_timing_read() returns per-kernel launch counts and total milliseconds */

@@ -84,6 +84,13 @@ def load():
         "tmce_server_get_stats": (I, [P, P]),
         "tmce_server_free": (V, [P]),
         "tmce_timing_read": (I, [I, P, P, P]),
+        "tmce_stream_create": (I, [ctypes.POINTER(P)]),
+        "tmce_stream_destroy": (I, [P]),
+        "tmce_stream_sync": (I, [P]),
+        "tmce_event_create": (I, [ctypes.POINTER(P)]),
+        "tmce_event_destroy": (I, [P]),
+        "tmce_event_record": (I, [P, P]),
+        "tmce_stream_wait_event": (I, [P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -388,6 +395,45 @@ def set_form(form):
 
 def sync():
     _check(load().tmce_sync(), "tmce_sync")
+
+
+class Stream:
+    """A non-blocking HIP stream (default priority) for the asynchronous device entry points; pass
+    it as `stream=` (its handle converts implicitly)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(load().tmce_stream_create(ctypes.byref(h)), "tmce_stream_create")
+        self._as_parameter_ = h
+
+    def sync(self):
+        _check(load().tmce_stream_sync(self._as_parameter_), "tmce_stream_sync")
+
+    def wait(self, event):
+        """Later work on this stream waits for the work recorded in `event`."""
+        _check(load().tmce_stream_wait_event(self._as_parameter_, event._as_parameter_), "tmce_stream_wait_event")
+
+    def free(self):
+        if self._as_parameter_:
+            load().tmce_stream_destroy(self._as_parameter_)
+            self._as_parameter_ = None
+
+
+class Event:
+    """A HIP event (no timing) recorded on a stream."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(load().tmce_event_create(ctypes.byref(h)), "tmce_event_create")
+        self._as_parameter_ = h
+
+    def record(self, stream=None):
+        _check(load().tmce_event_record(self._as_parameter_, stream), "tmce_event_record")
+
+    def free(self):
+        if self._as_parameter_:
+            load().tmce_event_destroy(self._as_parameter_)
+            self._as_parameter_ = None
 
 
 def timing_enable(on=True):

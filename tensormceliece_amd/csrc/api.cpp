@@ -42,6 +42,8 @@ int engine_init(int device) {
   auto e = std::make_unique<Engine>();
   e->device = device;
   HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&e->use_enc.done, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&e->use_dec.done, hipEventDisableTiming));
   for (int w = 0; w < 2; ++w) {
     const HostField &F = host_field(12 + w);
     std::vector<uint16_t> ex(F.ex.begin(), F.ex.begin() + F.q);
@@ -110,6 +112,18 @@ void rnd_chain(uint8_t *out, long long len, void *ctx) {
 }
 
 // ---------------------------------------------------------------- batched operations
+// Orders a call's use of a scratch set on stream s after the set's previous use on another stream;
+// scratch_done records the use.
+int scratch_wait(ScratchUse &u, hipStream_t s) {
+  if (u.last && u.last != s) HIPCHK(hipStreamWaitEvent(s, u.done, 0));
+  return 0;
+}
+int scratch_done(ScratchUse &u, hipStream_t s) {
+  HIPCHK(hipEventRecord(u.done, s));
+  u.last = s;
+  return 0;
+}
+
 int enc_device(tmce_key *k, uint8_t *d_ct, uint8_t *d_k, int *d_status, const uint8_t *d_seeds, size_t n,
                hipStream_t s) {
   Engine *e = engine();
@@ -126,19 +140,20 @@ int enc_device(tmce_key *k, uint8_t *d_ct, uint8_t *d_k, int *d_status, const ui
     return 0;
   }
   const int MSGW = (1 + si.n / 8 + si.synd_bytes) / 136 * 34 + 34;
+  if (int r = scratch_wait(e->use_enc, s)) return r;
   if (e->form == 1) {
     const int NEW = (si.n + 31) / 32, KWE = (si.k + 31) / 32, CW = (si.mt + 31) / 32;
-    if (int r = e->p_erow.ensure(n * NEW * 4)) return r;
+    if (int r = e->p_erow_enc.ensure(n * NEW * 4)) return r;
     if (int r = e->p_ens.ensure(n * KWE * 4)) return r;
     if (int r = e->p_s.ensure(n * CW * 4)) return r;
-    if (int r = e->msg.ensure(n * MSGW * sizeof(uint32_t))) return r;
-    PlainEncArgs pa{d_seeds, k->Trows, d_ct, d_k, d_status, e->p_erow.as<uint32_t>(), e->p_ens.as<uint32_t>(),
-                    e->p_s.as<uint32_t>(), e->msg.as<uint32_t>(), (int)n};
+    if (int r = e->msg_enc.ensure(n * MSGW * sizeof(uint32_t))) return r;
+    PlainEncArgs pa{d_seeds, k->Trows, d_ct, d_k, d_status, e->p_erow_enc.as<uint32_t>(), e->p_ens.as<uint32_t>(),
+                    e->p_s.as<uint32_t>(), e->msg_enc.as<uint32_t>(), (int)n};
     HIPCHK(launch_plain_enc_set(k->set, pa, s));
-    return 0;
+    return scratch_done(e->use_enc, s);
   }
   if (int r = e->ind.ensure(n * ((si.t + 1) & ~1) * sizeof(uint16_t))) return r;
-  if (int r = e->msg.ensure(n * MSGW * sizeof(uint32_t))) return r;
+  if (int r = e->msg_enc.ensure(n * MSGW * sizeof(uint32_t))) return r;
   if (int r = e->queue.ensure(64)) return r;
   EncArgs a{};
   a.seeds = d_seeds;
@@ -147,14 +162,14 @@ int enc_device(tmce_key *k, uint8_t *d_ct, uint8_t *d_k, int *d_status, const ui
   a.key = d_k;
   a.status = d_status;
   a.ind = e->ind.as<uint16_t>();
-  a.msg = e->msg.as<uint32_t>();
+  a.msg = e->msg_enc.as<uint32_t>();
   a.attempts = nullptr;
   a.queue = e->queue.as<unsigned>();
   a.nops = (int)n;
   a.gen_blocks = e->gen_blocks[k->set];
   a.gen_form = e->gen_form[k->set];
   HIPCHK(launch_enc_set(k->set, a, s));
-  return 0;
+  return scratch_done(e->use_enc, s);
 }
 
 int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, size_t n, hipStream_t s) {
@@ -168,8 +183,10 @@ int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, si
   const int TT = ((2 * si.t + 31) / 32) * 32;
   const int RW = si.m * TT / 32;
   const int MSGW = (1 + si.n / 8 + si.synd_bytes) / 136 * 34 + 34;
+  if (int r = scratch_wait(e->use_dec, s)) return r;
   if (int r = e->S.ensure(n * RW * sizeof(uint32_t))) return r;
-  if (int r = e->msg.ensure(n * MSGW * sizeof(uint32_t))) return r;
+  if (int r = e->msg_dec.ensure(n * MSGW * sizeof(uint32_t))) return r;
+  if (int r = e->dqueue.ensure(1024)) return r;
   DecArgs a{};
   a.ct = d_ct;
   a.F = k->F;
@@ -180,8 +197,8 @@ int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, si
   if (a.front_mfma) {
     const size_t xb = gf2_scratch_bytes(k->set, GF2_FRONT, n);
     if (xb) {
-      if (int r = e->xscratch.ensure(xb)) return r;
-      a.xscratch = e->xscratch.as<uint32_t>();
+      if (int r = e->xs_dec.ensure(xb)) return r;
+      a.xscratch = e->xs_dec.as<uint32_t>();
     }
   }
   a.Fcol = k->Fcol;
@@ -197,38 +214,45 @@ int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, si
   a.key = d_k;
   a.status = d_status;
   a.S = e->S.as<uint32_t>();
-  a.msg = e->msg.as<uint32_t>();
+  a.msg = e->msg_dec.as<uint32_t>();
+  a.queue = e->dqueue.as<unsigned>();
   a.nops = (int)n;
   a.decode_blocks = e->decode_blocks[k->set];
   if (e->form == 1) {
     const int NEW2 = 2 * ((si.n + 63) / 64);
     if (int r = e->p_loc.ensure(n * (si.t + 1) * 2)) return r;
-    if (int r = e->p_erow.ensure(n * NEW2 * 4)) return r;
+    if (int r = e->p_erow_dec.ensure(n * NEW2 * 4)) return r;
     if (int r = e->p_weight.ensure(n * 4)) return r;
-    PlainDecScratch x{e->p_loc.as<uint16_t>(), e->p_erow.as<uint32_t>(), e->p_weight.as<int>()};
+    PlainDecScratch x{e->p_loc.as<uint16_t>(), e->p_erow_dec.as<uint32_t>(), e->p_weight.as<int>()};
     HIPCHK(launch_plain_dec_set(k->set, a, x, s));
-    return 0;
+    return scratch_done(e->use_dec, s);
   }
   HIPCHK(launch_dec_set(k->set, a, s));
-  return 0;
+  return scratch_done(e->use_dec, s);
 }
 
 // operands of the dense syndrome forms, built from the T rows on first use
 int ensure_synd_key(Engine *e, tmce_key *k, int method, hipStream_t s) {
   const SetInfo &si = set_info(k->set);
+  bool built = false;
   if (method == TMCE_SYND_MFMA && !k->Tb) {
     const size_t nb = gf2_msw_bytes(k->set, GF2_SYND);
     if (hipMalloc((void **)&k->Tb, nb) != hipSuccess) return fail(TMCE_ENOMEM, "hipMalloc Tb");
     HIPCHK(launch_build_msw(k->set, GF2_SYND, k->Trows, (si.k + 31) / 32, k->Tb, s));
     k->bytes += nb;
+    built = true;
   }
   if (method == TMCE_SYND_VALU && !k->Tv) {
     const size_t nv = (size_t)((si.mt + 31) / 32) * ((si.k + 31) / 32) * 32;
     if (hipMalloc((void **)&k->Tv, nv * 4) != hipSuccess) return fail(TMCE_ENOMEM, "hipMalloc Tv");
     HIPCHK(launch_build_tv(k->set, k->Trows, k->Tv, s));
     k->bytes += nv * 4;
+    built = true;
   }
   (void)e;
+  // key material is shared by every stream: a newly built operand is complete before the handle
+  // is used again
+  if (built) HIPCHK(hipStreamSynchronize(s));
   return 0;
 }
 
@@ -241,17 +265,18 @@ int synd_device(tmce_key *k, uint8_t *d_ct, const uint8_t *d_e, size_t n, int me
   if (n > (size_t)INT32_MAX / 2) return fail(TMCE_EINVAL, "batch too large");
   if (!s) s = e->stream;
   if (int r = ensure_synd_key(e, k, method, s)) return r;
+  if (int r = scratch_wait(e->use_enc, s)) return r;
   uint32_t *xs = nullptr;
   if (method == TMCE_SYND_MFMA) {
     const size_t xb = gf2_scratch_bytes(k->set, GF2_SYND, n);
     if (xb) {
-      if (int r = e->xscratch.ensure(xb)) return r;
-      xs = e->xscratch.as<uint32_t>();
+      if (int r = e->xs_synd.ensure(xb)) return r;
+      xs = e->xs_synd.as<uint32_t>();
     }
   }
   SyndArgs a{d_e, k->Tb, k->Tv, k->cols, d_ct, xs, (int)n};
   HIPCHK(launch_synd_set(k->set, a, method, e->synd_cfg[k->set], s));
-  return 0;
+  return scratch_done(e->use_enc, s);
 }
 
 // key cache for the single-operation libmceliece entry points
@@ -554,6 +579,53 @@ int tmce_sync(void) {
   if (!e) return TMCE_ENODEV;
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(hipDeviceSynchronize());
+  return 0;
+}
+
+int tmce_stream_create(void **stream) {
+  if (!stream) return fail(TMCE_EINVAL, "tmce_stream_create: null argument");
+  if (!engine()) return TMCE_ENODEV;
+  hipStream_t s = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return 0;
+}
+int tmce_stream_destroy(void *stream) {
+  if (!stream) return 0;
+  HIPCHK(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return 0;
+}
+int tmce_stream_sync(void *stream) {
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  HIPCHK(hipStreamSynchronize(stream ? static_cast<hipStream_t>(stream) : e->stream));
+  return 0;
+}
+int tmce_event_create(void **event) {
+  if (!event) return fail(TMCE_EINVAL, "tmce_event_create: null argument");
+  if (!engine()) return TMCE_ENODEV;
+  hipEvent_t ev = nullptr;
+  HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  *event = ev;
+  return 0;
+}
+int tmce_event_destroy(void *event) {
+  if (!event) return 0;
+  HIPCHK(hipEventDestroy(static_cast<hipEvent_t>(event)));
+  return 0;
+}
+int tmce_event_record(void *event, void *stream) {
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  if (!event) return fail(TMCE_EINVAL, "tmce_event_record: null event");
+  HIPCHK(hipEventRecord(static_cast<hipEvent_t>(event), stream ? static_cast<hipStream_t>(stream) : e->stream));
+  return 0;
+}
+int tmce_stream_wait_event(void *stream, void *event) {
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  if (!event) return fail(TMCE_EINVAL, "tmce_stream_wait_event: null event");
+  HIPCHK(hipStreamWaitEvent(stream ? static_cast<hipStream_t>(stream) : e->stream, static_cast<hipEvent_t>(event), 0));
   return 0;
 }
 

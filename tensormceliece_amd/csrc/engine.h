@@ -94,15 +94,27 @@ struct SetTables {
   uint16_t *twl = nullptr, *plo = nullptr, *phi = nullptr, *ptlo = nullptr, *pthi = nullptr;
 };
 
+// The last stream that used one set of device scratch, and an event recorded after that use: a
+// call on another stream waits for the event first (hipStreamWaitEvent), so calls that share the
+// set never overlap on it, while calls with separate sets may run concurrently on two streams.
+struct ScratchUse {
+  hipEvent_t done = nullptr;
+  hipStream_t last = nullptr;
+};
+
 struct Engine {
   int device = -1;
   hipStream_t stream = nullptr;
   std::mutex mu;
   uint16_t *lg[2] = {nullptr, nullptr}, *ex[2] = {nullptr, nullptr};
   SetTables st[NUM_SETS];
-  DevBuf seeds, ct, key, status, ind, msg, S, attempts, queue;
-  DevBuf p_erow, p_ens, p_s, p_loc, p_weight;  // plain-form scratch
-  DevBuf xscratch;                             // MFMA products: x rows repacked to aligned words
+  DevBuf seeds, ct, key, status;  // staging of the host-buffer entry points (engine stream)
+  // encapsulation and syndrome scratch
+  DevBuf ind, msg_enc, queue, p_erow_enc, p_ens, p_s, xs_synd;
+  ScratchUse use_enc;
+  // decapsulation scratch
+  DevBuf S, msg_dec, dqueue, xs_dec, p_loc, p_erow_dec, p_weight;
+  ScratchUse use_dec;
   int form = 0;                                // 0 = engine form, 1 = plain per-stage form
   int synd_cfg[NUM_SETS] = {1, 1, 1, 1, 1};    // MFMA syndrome tile shape per set (tunable)
   int gen_blocks[NUM_SETS] = {0, 0, 0, 0, 0};  // sampler grid per set (0 = built-in default)

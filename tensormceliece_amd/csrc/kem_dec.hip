@@ -23,6 +23,8 @@ namespace tmce {
 constexpr int FRONT_WG = 256;
 constexpr int DEC_WG = 512;
 constexpr int DEC_WAVES = DEC_WG / 64;
+// decoder work counters: DEC_QUEUES of them, DEC_QSTRIDE words apart (separate 128-byte lines)
+constexpr int DEC_QUEUES = 8, DEC_QSTRIDE = 32;
 
 // ------------------------------------------------------------------ front end
 template <int SET>
@@ -224,7 +226,24 @@ void k_dec_decode(DecArgs a) {
   DecWave<SET> &w = Wv[wave];
   for (int i = lane; i < 4 * SPAD; i += 64) w.s[i / SPAD][i % SPAD] = 2 * ZL;  // window padding (zero)
 
-  for (int base = (blockIdx.x * DEC_WAVES + wave) * 4; base < nops; base += gridDim.x * DEC_WAVES * 4) {
+  // the wave takes batches of four ops from work counters, one per XCD (workgroups are dispatched
+  // to the 8 XCDs round-robin; counter x hands out batches x, x + 8, x + 16, ...), and from the
+  // other counters once its own is exhausted: workgroups that become resident late -- the GPU
+  // shared with other kernels -- take fewer ops, and each counter sees an eighth of the requests
+  const int nbat = (nops + 3) >> 2;
+  int xq = blockIdx.x & (DEC_QUEUES - 1), tried = 0;
+  for (;;) {
+    int bidx = nbat;
+    while (tried < DEC_QUEUES) {
+      unsigned got = 0;
+      if (lane == 0) got = atomicAdd(a.queue + DEC_QSTRIDE * xq, 1u);
+      bidx = xq + DEC_QUEUES * __builtin_amdgcn_readfirstlane((int)got);
+      if (bidx < nbat) break;
+      xq = (xq + 1) & (DEC_QUEUES - 1);
+      ++tried;
+    }
+    if (bidx >= nbat) break;
+    const int base = 4 * bidx;
     // ---- 1. syndromes of the four ops (syndrome-major bits from the front end)
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
@@ -547,6 +566,10 @@ hipError_t launch_dec(const DecArgs &a, hipStream_t s) {
   int grid = a.decode_blocks > 0 ? a.decode_blocks : (P::M == 12 ? 512 : 256);
   const int maxgrid = ceil_div(a.nops, DEC_WAVES * 4);
   if (grid > maxgrid) grid = maxgrid;
+  {
+    const hipError_t e = launch_fill_u32(a.queue, 0u, DEC_QUEUES * DEC_QSTRIDE, s);  // a kernel, not a memset
+    if (e != hipSuccess) return e;
+  }
   TMCE_TIMED("dec_decode", s, k_dec_decode<SET><<<grid, DEC_WG, 0, s>>>(a));
   TMCE_TIMED("dec_hash", s,
              k_hash_msg<SET><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, a.ct, a.key, a.status, a.nops, a.plan));

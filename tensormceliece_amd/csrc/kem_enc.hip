@@ -495,6 +495,11 @@ __global__ void k_fill_u32(unsigned *p, unsigned v, int n) {
   if (i < n) p[i] = v;
 }
 
+hipError_t launch_fill_u32(unsigned *p, unsigned v, int n, hipStream_t s) {
+  k_fill_u32<<<ceil_div(n, 256), 256, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+
 __global__ void k_fill_u8(uint8_t *p, uint8_t v, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;

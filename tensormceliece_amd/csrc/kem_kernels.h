@@ -147,6 +147,7 @@ struct DecArgs {
   int *status;           // [n] 0, or -1 for a ciphertext with nonzero padding bits (6960119)
   uint32_t *S;           // scratch [n][RW] syndrome bit planes
   uint32_t *msg;         // scratch [n][MSG_WORDS]
+  unsigned *queue;       // scratch: the decoder's work counters, 1024 bytes (cleared by the launcher)
   int nops;              // op count, or with a plan the capacity (the live count is plan->total)
   int decode_blocks;     // grid of the decoder (0 = default)
   const PlanDev *plan;   // multi-batch plan (per-group keys; null: one key)
@@ -205,6 +206,7 @@ hipError_t launch_dec_front_only(int set, const DecArgs &a, hipStream_t s);
 hipError_t launch_dec_hash_only(int set, const DecArgs &a, hipStream_t s);
 hipError_t launch_dec_set(int set, const DecArgs &a, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
+hipError_t launch_fill_u32(unsigned *p, unsigned v, int n, hipStream_t s);
 hipError_t launch_fill_i32(int *p, int v, size_t n, hipStream_t s);
 
 // Optional per-kernel timing: when set, each engine launch is bracketed by events on its stream.

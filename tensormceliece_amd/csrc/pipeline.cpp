@@ -77,6 +77,7 @@ int issue(tmce_pipeline *p, const int knobs[5]) {
     a.status = p->status.as<int>();
     a.S = p->S.as<uint32_t>();
     a.msg = p->msg.as<uint32_t>();
+    a.queue = p->queue.as<unsigned>();
     a.nops = p->cap;
     a.decode_blocks = knobs[1];
     a.front_mfma = knobs[2];
@@ -156,6 +157,7 @@ int tmce_pipeline_create(int set, int direction, size_t capacity, size_t max_sub
   } else {
     need.push_back({&p->in, n * si.synd_bytes});
     need.push_back({&p->S, n * rw * 4});
+    need.push_back({&p->queue, 1024});
     const size_t xb = gf2_scratch_bytes(set, GF2_FRONT, n);
     if (xb) need.push_back({&p->xscratch, xb});
   }
