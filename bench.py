@@ -20,9 +20,11 @@ Unless --quick, the line also carries the other workloads of the scope (key "sui
   c3        mceliece460896 and mceliece6688128 decapsulation at N = 8192;
   c4        mceliece6960119 uneven sub-batches (sizes log-uniform in [1, 8192], total 65536) in one
             graph-captured multi-batch pipeline run per direction, autotuned launch shapes;
-  c5        mceliece8192128 serving loop: 16 device-resident keypairs, Poisson arrivals of requests
-            of log-uniform size in [1, 4096], 50/50 enc/dec, HBM-footprint admission; throughput
-            at saturation and p50/p99 latency at two offered loads.
+  c5        mceliece8192128 serving loop: 16 device-resident keypairs (GPU-batched generation),
+            Poisson arrivals of requests of log-uniform size in [1, 4096], 50/50 enc/dec,
+            HBM-footprint admission; throughput at saturation and p50/p99 latency at two loads;
+  keygen    batched key generation (systematic form on the GPU) vs the host generator on all
+            cores, mceliece348864 and mceliece8192128, byte-compared.
 """
 import argparse
 import json
@@ -91,10 +93,27 @@ def wall(tm, fn, reps):
     return (time.perf_counter() - t0) / reps
 
 
-def keypairs_parallel(tm, s, n, state=0x51ED):
-    seeds, _ = xorshift_seeds(n, state)
-    with ThreadPoolExecutor(max_workers=min(16, n)) as ex:
-        return list(ex.map(lambda i: tm.keypair_seeded(s, seeds[i].tobytes()), range(n)))
+def section_keygen(tm, cores, log):
+    """Batched keypairs (systematic form on the GPU, the rest on host threads) against the host
+    generator on all cores, same seeds, byte-compared."""
+    out = {}
+    for s, n in (("348864", 128), ("8192128", 32)):
+        seeds, _ = xorshift_seeds(n, 0x6B65)
+        tm.keypair_batch(s, seeds[:2])  # warm-up: first launch, buffers
+        t0 = time.perf_counter()
+        pk, sk = tm.keypair_batch(s, seeds)
+        tg = time.perf_counter() - t0
+        nh = min(n, cores)
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=cores) as ex:
+            host = list(ex.map(lambda i: tm.keypair_seeded(s, seeds[i].tobytes()), range(nh)))
+        th = time.perf_counter() - t0
+        exact = all(np.array_equal(pk[i], host[i][0]) and np.array_equal(sk[i], host[i][1]) for i in range(nh))
+        out[s] = {"keys": n, "batch_keypairs_per_s": round(n / tg, 1), "host_keypairs_per_s": round(nh / th, 1),
+                  "host_threads": cores, "host_sample_keys": nh, "speedup": round((n / tg) / (nh / th), 2),
+                  "bit_exact_vs_host": exact}
+        log(f"keygen {s}: {out[s]}")
+    return out
 
 
 def fixed_weight(p, n, rng):
@@ -427,7 +446,9 @@ def section_c4(tm, keys, log, total=65536):
 def section_c5(tm, log, nkeys=16, n_requests=6000):
     from tensormceliece_amd import serving
     s = "8192128"
-    kps = keypairs_parallel(tm, s, nkeys)
+    seeds, _ = xorshift_seeds(nkeys, 0x51ED)
+    pks, sks = tm.keypair_batch(s, seeds)  # GPU-batched key generation
+    kps = list(zip(pks, sks))
     wl = serving.Workload(s, kps, pool_per_key=4096)
     server = tm.Server(s, slot_capacity=32768, slots_per_direction=2, hbm_budget=8 << 30)
     try:
@@ -467,7 +488,7 @@ def main():
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-stage GPU form")
     ap.add_argument("--quick", action="store_true", help="headline only (no suite)")
-    ap.add_argument("--sections", default="syndrome,per_set,c1,c3,c4,c5")
+    ap.add_argument("--sections", default="syndrome,per_set,c1,c3,c4,c5,keygen")
     ap.add_argument("--one-stream", action="store_true",
                     help="headline steps serialised on one stream only (profiling: no concurrent kernels)")
     ap.add_argument("--step-gen-blocks", type=int, default=64,
@@ -615,6 +636,8 @@ def main():
             suite["c4_6960119_uneven"] = section_c4(tm, keys, log)
         if "c5" in sections:
             suite["c5_8192128_serving"] = section_c5(tm, log)
+        if "keygen" in sections:
+            suite["keygen_batch"] = section_keygen(tm, cores, log)
 
     kem_ops_per_s = 2 * n / t_step
     line = {

@@ -109,8 +109,16 @@ void tmce_key_free(tmce_key *key);
 /* bytes of device memory held by a key handle (for admission control) */
 size_t tmce_key_device_bytes(const tmce_key *key);
 
-/* seeded batched keypair: keypair i uses randombytes = the seeded chain above on seeds[32*i] */
+/* seeded keypair (host): the keypair uses randombytes = the seeded chain above on seed[0..32) */
 int tmce_keypair_seeded(int set, unsigned char *pk, unsigned char *sk, const unsigned char *seed);
+
+/* batched seeded keypairs on the GPU: keypair i = tmce_keypair_seeded(set, pk_i, sk_i, seeds[32*i]),
+   byte for byte.  Replaces n calls of libmceliece's crypto_kem_keypair
+   (LM:crypto_kem/<set>/vec/kem_keypair.c; the reference generates its keys with it on the CPU,
+   mceliece-gpu-params/main.cu:17-21,159-161).  pk: n*PUBLICKEYBYTES, sk: n*SECRETKEYBYTES.
+   The systematic-form step (pk_gen.c) runs on the GPU; seed expansion, Goppa polynomial, support
+   and Benes control bits run on host threads.  Synchronous. */
+int tmce_keypair_batch(int set, unsigned char *pk, unsigned char *sk, const unsigned char *seeds, size_t n);
 
 /* batched KEM over host buffers: ct[n*CIPHERTEXTBYTES], k[n*32], status[n] (may be NULL),
    seeds[n*32] (encapsulation seeds, see above); synchronous */

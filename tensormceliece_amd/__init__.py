@@ -54,6 +54,7 @@ def load():
         "tmce_key_free": (V, [P]),
         "tmce_key_device_bytes": (S, [P]),
         "tmce_keypair_seeded": (I, [I, P, P, P]),
+        "tmce_keypair_batch": (I, [I, P, P, P, S]),
         "tmce_enc_batch": (I, [P, P, P, P, P, S]),
         "tmce_dec_batch": (I, [P, P, P, P, S]),
         "tmce_enc_batch_device": (I, [P, P, P, P, P, S, P]),
@@ -122,6 +123,18 @@ def keypair_seeded(set_name, seed):
     sd = np.frombuffer(bytes(seed), np.uint8).copy()
     assert sd.size == 32
     _check(load().tmce_keypair_seeded(SETS[set_name], _ptr(pk), _ptr(sk), _ptr(sd)), "keypair")
+    return pk, sk
+
+
+def keypair_batch(set_name, seeds):
+    """Batched keypairs on the GPU: row i equals keypair_seeded(set_name, seeds[i]) byte for byte.
+    seeds: (n, 32) uint8.  Returns (pk (n, PUBLICKEYBYTES), sk (n, SECRETKEYBYTES))."""
+    p = PARAMS[set_name]
+    sd = np.ascontiguousarray(seeds, dtype=np.uint8).reshape(-1, 32)
+    n = sd.shape[0]
+    pk = np.zeros((n, p["pk"]), np.uint8)
+    sk = np.zeros((n, p["sk"]), np.uint8)
+    _check(load().tmce_keypair_batch(SETS[set_name], _ptr(pk), _ptr(sk), _ptr(sd), n), "tmce_keypair_batch")
     return pk, sk
 
 

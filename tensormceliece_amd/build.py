@@ -8,7 +8,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtmce.so")
-SOURCES = ["kem_enc.hip", "kem_dec.hip", "kem_plain.hip", "kem_synd.hip", "host_math.cpp", "api.cpp", "pipeline.cpp", "server.cpp"]
+SOURCES = ["kem_enc.hip", "kem_dec.hip", "kem_plain.hip", "kem_synd.hip", "kem_keygen.hip", "host_math.cpp", "api.cpp",
+           "keygen.cpp", "pipeline.cpp", "server.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",

@@ -74,29 +74,48 @@ void perm_from_controlbits(int m, const uint8_t *cb, std::vector<uint16_t> &pi) 
   }
 }
 
+// Sort of 32-bit keys: LSD radix (three 11-bit digits) for large arrays, std::sort for small ones;
+// tmp has a.size() entries
+static void sort32(std::vector<uint32_t> &a, std::vector<uint32_t> &tmp) {
+  const size_t n = a.size();
+  if (n < 1024) {
+    std::sort(a.begin(), a.end());
+    return;
+  }
+  uint32_t *src = a.data(), *dst = tmp.data();
+  for (int sh = 0; sh < 32; sh += 11) {
+    uint32_t cnt[2049] = {0};
+    for (size_t i = 0; i < n; ++i) ++cnt[((src[i] >> sh) & 2047u) + 1];
+    for (int d = 0; d < 2048; ++d) cnt[d + 1] += cnt[d];
+    for (size_t i = 0; i < n; ++i) dst[cnt[(src[i] >> sh) & 2047u]++] = src[i];
+    std::swap(src, dst);
+  }
+  if (src != a.data()) std::memcpy(a.data(), src, n * sizeof(uint32_t));
+}
+
 // The recursion of LM:crypto_kem/<set>/vec/controlbits.c (cbrecursion).  Every sort below is on
-// distinct non-negative 32-bit keys, so std::sort reproduces the library's order exactly.
+// distinct non-negative 32-bit keys, so any correct sort reproduces the library's order exactly.
 static void cbrec(uint8_t *out, long long pos, long long step, const uint16_t *pi, int w, size_t n) {
   auto flip = [&](long long p, uint32_t b) { out[p >> 3] ^= (uint8_t)((b & 1u) << (p & 7)); };
   if (w == 1) { flip(pos, pi[0]); return; }
-  std::vector<uint32_t> A(n), B(n);
+  std::vector<uint32_t> A(n), B(n), tmp(n);
   for (size_t x = 0; x < n; ++x) A[x] = ((uint32_t)(pi[x] ^ 1u) << 16) | pi[x ^ 1];
-  std::sort(A.begin(), A.end());
+  sort32(A, tmp);
   for (size_t x = 0; x < n; ++x) {
     const uint32_t px = A[x] & 0xffffu;
     B[x] = (px << 16) | std::min<uint32_t>(px, (uint32_t)x);
   }
   for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 16) | (uint32_t)x;
-  std::sort(A.begin(), A.end());
+  sort32(A, tmp);
   for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 16) + (B[x] >> 16);
-  std::sort(A.begin(), A.end());
+  sort32(A, tmp);
   if (w <= 10) {
     for (size_t x = 0; x < n; ++x) B[x] = ((A[x] & 0xffffu) << 10) | (B[x] & 0x3ffu);
     for (int i = 1; i < w - 1; ++i) {
       for (size_t x = 0; x < n; ++x) A[x] = ((B[x] & ~0x3ffu) << 6) | (uint32_t)x;
-      std::sort(A.begin(), A.end());
+      sort32(A, tmp);
       for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 20) | B[x];
-      std::sort(A.begin(), A.end());
+      sort32(A, tmp);
       for (size_t x = 0; x < n; ++x)
         B[x] = std::min<uint32_t>((A[x] & 0xffc00u) | (B[x] & 0x3ffu), A[x] & 0xfffffu);
     }
@@ -105,20 +124,20 @@ static void cbrec(uint8_t *out, long long pos, long long step, const uint16_t *p
     for (size_t x = 0; x < n; ++x) B[x] = (A[x] << 16) | (B[x] & 0xffffu);
     for (int i = 1; i < w - 1; ++i) {
       for (size_t x = 0; x < n; ++x) A[x] = (B[x] & ~0xffffu) | (uint32_t)x;
-      std::sort(A.begin(), A.end());
+      sort32(A, tmp);
       for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 16) | (B[x] & 0xffffu);
       if (i < w - 2) {
         for (size_t x = 0; x < n; ++x) B[x] = (A[x] & ~0xffffu) | (B[x] >> 16);
-        std::sort(B.begin(), B.end());
+        sort32(B, tmp);
         for (size_t x = 0; x < n; ++x) B[x] = (B[x] << 16) | (A[x] & 0xffffu);
       }
-      std::sort(A.begin(), A.end());
+      sort32(A, tmp);
       for (size_t x = 0; x < n; ++x) B[x] = std::min<uint32_t>(B[x], (B[x] & ~0xffffu) | (A[x] & 0xffffu));
     }
     for (size_t x = 0; x < n; ++x) B[x] &= 0xffffu;
   }
   for (size_t x = 0; x < n; ++x) A[x] = ((uint32_t)pi[x] << 16) + (uint32_t)x;
-  std::sort(A.begin(), A.end());
+  sort32(A, tmp);
   for (size_t j = 0; j < n / 2; ++j) {
     const size_t z = 2 * j;
     const uint32_t fj = B[z] & 1u, Fz = (uint32_t)z + fj;
@@ -127,7 +146,7 @@ static void cbrec(uint8_t *out, long long pos, long long step, const uint16_t *p
     B[z] = (A[z] << 16) | Fz;
     B[z + 1] = (A[z + 1] << 16) | (Fz ^ 1u);
   }
-  std::sort(B.begin(), B.end());
+  sort32(B, tmp);
   pos += (2 * w - 3) * step * (long long)(n / 2);
   for (size_t k = 0; k < n / 2; ++k) {
     const size_t y = 2 * k;
@@ -137,7 +156,7 @@ static void cbrec(uint8_t *out, long long pos, long long step, const uint16_t *p
     A[y] = (Ly << 16) | (B[y] & 0xffffu);
     A[y + 1] = ((Ly ^ 1u) << 16) | (B[y + 1] & 0xffffu);
   }
-  std::sort(A.begin(), A.end());
+  sort32(A, tmp);
   pos -= (2 * w - 2) * step * (long long)(n / 2);
   std::vector<uint16_t> q(n);
   for (size_t j = 0; j < n / 2; ++j) {
@@ -442,28 +461,30 @@ static bool systematic_pk(const HostField &F, const uint16_t *g, const uint16_t 
   return true;
 }
 
-// LM:crypto_kem/<set>/vec/kem_keypair.c: seed expansion by SHAKE256, Goppa polynomial, random
-// support by sorting 32-bit keys, systematic public key, Benes control bits, rejection string.
+// LM:crypto_kem/<set>/vec/kem_keypair.c, split for the batched key generator: one attempt per
+// seed of the chain (seed expansion by SHAKE256, Goppa polynomial, random support by sorting
+// 32-bit keys) up to the systematic-form step, and the secret key once that step has succeeded
+// (Benes control bits, rejection string).
 template <int SET>
-static void keypair_t(uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx) {
+static void candidate_t(uint8_t seed33[33], KeygenCandidate &c) {
   using P = Par<SET>;
   const HostField &F = host_field(P::M);
   constexpr int NQ = 1 << P::M;
-  uint8_t seed[33];
-  seed[0] = 64;
-  rnd(seed + 1, 32, ctx);
   const size_t rlen = P::N / 8 + 4 * (size_t)NQ + 2 * P::T + 32;
   std::vector<uint8_t> r(rlen);
-  std::vector<uint16_t> f(P::T), g(P::T + 1), alpha(P::N), pi(NQ);
+  std::vector<uint16_t> f(P::T);
   std::vector<std::pair<uint32_t, uint16_t>> keys(NQ);
+  c.g.assign(P::T + 1, 0);
+  c.alpha.assign(P::N, 0);
+  c.pi.assign(NQ, 0);
   for (;;) {
-    shake256(r.data(), (long long)rlen, seed, 33);
-    std::memcpy(sk, seed + 1, 32);
-    std::memcpy(seed + 1, &r[rlen - 32], 32);
+    shake256(r.data(), (long long)rlen, seed33, 33);
+    std::memcpy(c.sk_seed, seed33 + 1, 32);
+    std::memcpy(seed33 + 1, &r[rlen - 32], 32);  // the next attempt's seed, whatever happens below
     const uint8_t *fp = &r[rlen - 32 - 2 * P::T];
     for (int i = 0; i < P::T; ++i) f[i] = (uint16_t)((fp[2 * i] | (fp[2 * i + 1] << 8)) & ((1 << P::M) - 1));
-    if (!min_poly<SET>(F, f.data(), g.data())) continue;
-    g[P::T] = 1;
+    if (!min_poly<SET>(F, f.data(), c.g.data())) continue;
+    c.g[P::T] = 1;
     const uint8_t *pp = &r[P::N / 8];
     for (int i = 0; i < NQ; ++i)
       keys[i] = {(uint32_t)pp[4 * i] | ((uint32_t)pp[4 * i + 1] << 8) | ((uint32_t)pp[4 * i + 2] << 16) |
@@ -473,17 +494,60 @@ static void keypair_t(uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx) {
     bool dup = false;
     for (int i = 1; i < NQ && !dup; ++i) dup = keys[i - 1].first == keys[i].first;
     if (dup) continue;
-    for (int i = 0; i < NQ; ++i) pi[i] = keys[i].second;
-    for (int i = 0; i < P::N; ++i) alpha[i] = bitrev(pi[i], P::M);
-    if (!systematic_pk<SET>(F, g.data(), alpha.data(), pk)) continue;
-    uint8_t *skp = sk + 40;
-    for (int i = 0; i < P::T; ++i) { skp[2 * i] = (uint8_t)g[i]; skp[2 * i + 1] = (uint8_t)(g[i] >> 8); }
-    controlbits_from_perm(P::M, pi, skp + P::IRR_BYTES);
-    std::memcpy(skp + P::IRR_BYTES + P::COND_BYTES, r.data(), P::N / 8);
-    const uint8_t piv[8] = {0xFF, 0xFF, 0xFF, 0xFF, 0, 0, 0, 0};
-    std::memcpy(sk + 32, piv, 8);
+    for (int i = 0; i < NQ; ++i) c.pi[i] = keys[i].second;
+    for (int i = 0; i < P::N; ++i) c.alpha[i] = bitrev(c.pi[i], P::M);
+    c.s.assign(r.begin(), r.begin() + P::N / 8);
     return;
   }
+}
+
+template <int SET>
+static void finish_t(const KeygenCandidate &c, uint8_t *sk) {
+  using P = Par<SET>;
+  std::memcpy(sk, c.sk_seed, 32);
+  const uint8_t piv[8] = {0xFF, 0xFF, 0xFF, 0xFF, 0, 0, 0, 0};
+  std::memcpy(sk + 32, piv, 8);
+  uint8_t *skp = sk + 40;
+  for (int i = 0; i < P::T; ++i) { skp[2 * i] = (uint8_t)c.g[i]; skp[2 * i + 1] = (uint8_t)(c.g[i] >> 8); }
+  controlbits_from_perm(P::M, c.pi, skp + P::IRR_BYTES);
+  std::memcpy(skp + P::IRR_BYTES + P::COND_BYTES, c.s.data(), P::N / 8);
+}
+
+template <int SET>
+static void keypair_t(uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx) {
+  using P = Par<SET>;
+  const HostField &F = host_field(P::M);
+  uint8_t seed[33];
+  seed[0] = 64;
+  rnd(seed + 1, 32, ctx);
+  KeygenCandidate c;
+  for (;;) {
+    candidate_t<SET>(seed, c);
+    if (systematic_pk<SET>(F, c.g.data(), c.alpha.data(), pk)) break;
+  }
+  finish_t<SET>(c, sk);
+}
+
+void keygen_candidate(int set, uint8_t seed33[33], KeygenCandidate &c) {
+  switch (set) {
+    case S348864: return candidate_t<S348864>(seed33, c);
+    case S460896: return candidate_t<S460896>(seed33, c);
+    case S6688128: return candidate_t<S6688128>(seed33, c);
+    case S6960119: return candidate_t<S6960119>(seed33, c);
+    case S8192128: return candidate_t<S8192128>(seed33, c);
+  }
+  throw std::invalid_argument("set");
+}
+
+void keygen_finish(int set, const KeygenCandidate &c, uint8_t *sk) {
+  switch (set) {
+    case S348864: return finish_t<S348864>(c, sk);
+    case S460896: return finish_t<S460896>(c, sk);
+    case S6688128: return finish_t<S6688128>(c, sk);
+    case S6960119: return finish_t<S6960119>(c, sk);
+    case S8192128: return finish_t<S8192128>(c, sk);
+  }
+  throw std::invalid_argument("set");
 }
 
 void host_keypair(int set, uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx) {

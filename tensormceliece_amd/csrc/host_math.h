@@ -57,6 +57,21 @@ void build_enc_rows(int set, const uint8_t *pk, std::vector<uint32_t> &rows);
 typedef void (*rand_fn)(uint8_t *out, long long len, void *ctx);
 void host_keypair(int set, uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx);
 
+// The same key generation in two host steps around the systematic-form step (which the batched
+// generator runs on the GPU).  keygen_candidate advances seed33 (64 || 32-byte state) through
+// attempts until one has an irreducible Goppa polynomial and distinct support sort keys; if the
+// systematic form of that candidate fails, calling it again continues the chain (libmceliece's
+// loop).  keygen_finish writes sk once the systematic form has succeeded.
+struct KeygenCandidate {
+  uint8_t sk_seed[32];          // the attempt's seed (sk bytes 0..31)
+  std::vector<uint16_t> g;      // Goppa polynomial, t+1 coefficients (monic)
+  std::vector<uint16_t> alpha;  // support, n field elements
+  std::vector<uint16_t> pi;     // the permutation of 2^m elements behind the support
+  std::vector<uint8_t> s;       // implicit-rejection string, n/8 bytes
+};
+void keygen_candidate(int set, uint8_t seed33[33], KeygenCandidate &c);
+void keygen_finish(int set, const KeygenCandidate &c, uint8_t *sk);
+
 void shake256(uint8_t *out, long long outlen, const uint8_t *in, long long inlen);
 
 }  // namespace tmce

@@ -199,6 +199,20 @@ hipError_t launch_front_mfma_set(int set, const uint8_t *ct, uint32_t *scratch, 
 // bytes of scratch the MFMA products need for n ops (x rows repacked to aligned words), 0 if none
 size_t gf2_scratch_bytes(int set, int which, size_t n);
 
+// Batched key generation, systematic-form step: one candidate (g, alpha) per workgroup.
+struct KeygenArgs {
+  const uint16_t *g;      // [ncand][T+1] Goppa polynomials (monic)
+  const uint16_t *alpha;  // [ncand][N] supports
+  const uint16_t *lg;     // field log table, 2^m entries
+  const uint16_t *ex;     // field exp table, 2^m - 1 entries
+  uint64_t *H;            // scratch [ncand][pk_gen_h_bytes(set) / 8]
+  uint8_t *pk;            // [ncand][PK_BYTES]
+  int *ok;                // [ncand] 1: pk written, 0: the left mt x mt block is singular
+  int ncand;
+};
+hipError_t launch_pk_gen_set(int set, const KeygenArgs &a, hipStream_t s);
+size_t pk_gen_h_bytes(int set);
+
 hipError_t launch_enc_set(int set, const EncArgs &a, hipStream_t s);
 hipError_t launch_plain_enc_set(int set, const PlainEncArgs &a, hipStream_t s);
 hipError_t launch_plain_dec_set(int set, const DecArgs &a, const PlainDecScratch &x, hipStream_t s);

@@ -1,0 +1,165 @@
+// kem_keygen.hip -- batched key generation on gfx950: the systematic-form step.
+//
+// One workgroup per candidate key (Goppa polynomial g, support alpha, from the host part of the
+// generator).  It builds the parity-check matrix H in HBM -- row i*m + b, column j: bit b of
+// alpha_j^i / g(alpha_j) -- with the column bits of 64 columns turned into row words by wave
+// ballots, reduces H to [I | T] by Gauss-Jordan elimination, and writes the public key T.  Row
+// swaps are kept as an index permutation in LDS; each elimination step gives every wave whole
+// rows and every lane one or two of the row's words, so rows without the pivot bit are skipped
+// uniformly.  The reduced form of a full-rank matrix is unique, so the bytes do not depend on the
+// pivot order, and the step fails exactly when the left mt x mt block is singular -- the outcome
+// of LM:crypto_kem/<set>/vec/pk_gen.c for the same candidate.
+#include "kem_kernels.h"
+
+namespace tmce {
+
+constexpr int KG_WG = 512;
+constexpr int KG_WAVES = KG_WG / 64;
+
+template <int SET>
+__global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g_all,
+                                                 const uint16_t *__restrict__ alpha_all,
+                                                 const uint16_t *__restrict__ lg, const uint16_t *__restrict__ ex,
+                                                 uint64_t *__restrict__ H_all, uint8_t *__restrict__ pk_all,
+                                                 int *__restrict__ ok_all) {
+  using P = Par<SET>;
+  constexpr int M = P::M, T = P::T, N = P::N, MT = P::MT, Q = P::Q;
+  constexpr int W = ceil_div(N, 64);  // 64-bit words per row of H
+  static_assert(W <= 128, "two row words per lane");
+  __shared__ uint16_t lg_s[1 << M];
+  __shared__ uint16_t ex_s[2 * Q];   // ex_s[i] = generator^(i mod q): a sum of two logs needs no reduction
+  __shared__ uint16_t rowix[MT];     // logical row -> physical row of H
+  __shared__ uint8_t flag[MT];       // physical rows the current step updates
+  __shared__ int red[KG_WAVES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int cand = blockIdx.x;
+  const uint16_t *g = g_all + (size_t)cand * (T + 1);
+  const uint16_t *alpha = alpha_all + (size_t)cand * N;
+  uint64_t *H = H_all + (size_t)cand * MT * W;
+  for (int i = tid; i < (1 << M); i += KG_WG) lg_s[i] = lg[i];
+  for (int i = tid; i < 2 * Q; i += KG_WG) ex_s[i] = ex[i < Q ? i : i - Q];
+  for (int i = tid; i < MT; i += KG_WG) rowix[i] = (uint16_t)i;
+  __syncthreads();
+  auto mul = [&](uint32_t a, uint32_t b) -> uint32_t { return (a && b) ? (uint32_t)ex_s[lg_s[a] + lg_s[b]] : 0u; };
+
+  // ---- H, 64 columns per wave pass: lane l takes column 64 wi + l (zero past n)
+  for (int wi = wave; wi < W; wi += KG_WAVES) {
+    const int j = 64 * wi + lane;
+    const uint32_t a = j < N ? alpha[j] : 0u;
+    uint32_t v = g[T];
+    for (int k = T - 1; k >= 0; --k) v = mul(v, a) ^ g[k];
+    v = j < N ? (uint32_t)ex_s[Q - lg_s[v]] : 0u;  // 1 / g(alpha_j) (g has no root in the field)
+    for (int i = 0; i < T; ++i) {
+      uint64_t out = 0;
+#pragma unroll
+      for (int b = 0; b < M; ++b) {
+        const uint64_t bal = __ballot((v >> b) & 1u);
+        out = lane == b ? bal : out;
+      }
+      if (lane < M) H[(size_t)(i * M + lane) * W + wi] = out;
+      v = mul(v, a);
+    }
+  }
+  __syncthreads();
+
+  // ---- Gauss-Jordan elimination to [I | T]
+  for (int c = 0; c < MT; ++c) {
+    const int cw = c >> 6;
+    const uint64_t cm = 1ull << (c & 63);
+    // pivot: the first logical row at or below c with bit c set (the thread's rows loaded together)
+    constexpr int PR = ceil_div(MT, KG_WG);
+    uint64_t pw[PR];
+#pragma unroll
+    for (int u = 0; u < PR; ++u) {
+      const int r = c + tid + u * KG_WG;
+      pw[u] = r < MT ? H[(size_t)rowix[r] * W + cw] : 0ull;
+    }
+    int best = MT;
+#pragma unroll
+    for (int u = PR - 1; u >= 0; --u)
+      if (pw[u] & cm) best = c + tid + u * KG_WG;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) best = min(best, __shfl_xor(best, o));
+    if (lane == 0) red[wave] = best;
+    __syncthreads();
+    best = red[0];
+#pragma unroll
+    for (int w = 1; w < KG_WAVES; ++w) best = min(best, red[w]);
+    if (best == MT) {  // the left mt x mt block is singular (uniform over the workgroup)
+      if (tid == 0) ok_all[cand] = 0;
+      return;
+    }
+    const int pr = rowix[best], pc = rowix[c];
+    // rows to update: every physical row but the pivot row with bit c set
+    for (int r = tid; r < MT; r += KG_WG) flag[r] = (r != pr && (H[(size_t)r * W + cw] & cm)) ? 1 : 0;
+    __syncthreads();  // flags written, rowix and red read by every thread
+    if (tid == 0 && best != c) {
+      rowix[c] = (uint16_t)pr;
+      rowix[best] = (uint16_t)pc;
+    }
+    const uint64_t *prow = H + (size_t)pr * W;
+    const int w0 = cw + lane, w1 = cw + 64 + lane;
+    const uint64_t p0 = w0 < W ? prow[w0] : 0ull, p1 = w1 < W ? prow[w1] : 0ull;
+    // RU of the wave's rows at a time, loaded unconditionally so the loads overlap; flagged rows are
+    // written back
+    constexpr int RU = W > 64 ? 32 : 64;
+    for (int r0 = wave; r0 < MT; r0 += KG_WAVES * RU) {
+      uint64_t x0[RU], x1[RU];
+#pragma unroll
+      for (int u = 0; u < RU; ++u) {
+        const int r = r0 + u * KG_WAVES;
+        const uint64_t *row = H + (size_t)(r < MT ? r : 0) * W;
+        x0[u] = w0 < W ? row[w0] : 0ull;
+        if constexpr (W > 64) x1[u] = w1 < W ? row[w1] : 0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < RU; ++u) {
+        const int r = r0 + u * KG_WAVES;
+        if (r < MT && flag[r]) {  // uniform in the wave
+          uint64_t *row = H + (size_t)r * W;
+          if (w0 < W) row[w0] = x0[u] ^ p0;
+          if constexpr (W > 64)
+            if (w1 < W) row[w1] = x1[u] ^ p1;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- pk row r = columns mt .. n-1 of logical row r (bits past n are zero in H)
+  uint8_t *pk = pk_all + (size_t)cand * P::PK_BYTES;
+  for (int idx = tid; idx < MT * P::PK_ROW_BYTES; idx += KG_WG) {
+    const int r = idx / P::PK_ROW_BYTES, jb = idx - r * P::PK_ROW_BYTES;
+    const uint64_t *row = H + (size_t)rowix[r] * W;
+    const int bit = MT + 8 * jb, wb = bit >> 6, sh = bit & 63;
+    uint64_t v = row[wb] >> sh;
+    if (sh > 56 && wb + 1 < W) v |= row[wb + 1] << (64 - sh);
+    pk[idx] = (uint8_t)v;
+  }
+  if (tid == 0) ok_all[cand] = 1;
+}
+
+template <int SET>
+hipError_t pk_gen(const KeygenArgs &a, hipStream_t s) {
+  if (a.ncand <= 0) return hipSuccess;
+  k_pk_gen<SET><<<a.ncand, KG_WG, 0, s>>>(a.g, a.alpha, a.lg, a.ex, a.H, a.pk, a.ok);
+  return hipGetLastError();
+}
+
+hipError_t launch_pk_gen_set(int set, const KeygenArgs &a, hipStream_t s) {
+  switch (set) {
+    case S348864: return pk_gen<S348864>(a, s);
+    case S460896: return pk_gen<S460896>(a, s);
+    case S6688128: return pk_gen<S6688128>(a, s);
+    case S6960119: return pk_gen<S6960119>(a, s);
+    case S8192128: return pk_gen<S8192128>(a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+size_t pk_gen_h_bytes(int set) {
+  const SetInfo &si = set_info(set);
+  return (size_t)si.mt * ((si.n + 63) / 64) * 8;
+}
+
+}  // namespace tmce

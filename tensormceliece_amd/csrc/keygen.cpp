@@ -1,0 +1,152 @@
+// keygen.cpp -- tmce_keypair_batch: batched seeded key generation.
+//
+// Host threads run each key's attempts up to the systematic-form step (keygen_candidate: seed
+// expansion, Goppa polynomial, support) and write its secret key after that step succeeds
+// (keygen_finish: Benes control bits, rejection string).  The systematic-form step of every pending
+// candidate runs on the GPU, one workgroup per candidate (k_pk_gen).  A key whose candidate is
+// singular continues its seed chain in the next round, as libmceliece's keypair loop does, so key
+// i is byte-identical to tmce_keypair_seeded on seeds[32 i].
+#include <hip/hip_runtime.h>
+#include <sched.h>
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "engine.h"
+
+using namespace tmce;
+
+namespace {
+
+// fn(i) for i < n on the CPUs this process may run on
+template <class Fn>
+void parallel_for(size_t n, Fn fn) {
+  cpu_set_t cs;
+  size_t nt = sched_getaffinity(0, sizeof(cs), &cs) == 0 ? (size_t)CPU_COUNT(&cs) : 1;
+  nt = std::max<size_t>(1, std::min(nt, n));
+  std::atomic<size_t> next{0};
+  auto work = [&] {
+    for (size_t i; (i = next.fetch_add(1)) < n;) fn(i);
+  };
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < nt; ++t) th.emplace_back(work);
+  work();
+  for (auto &x : th) x.join();
+}
+
+// device buffers of one call (freed on every exit path)
+struct KgBuffers {
+  uint16_t *g = nullptr, *alpha = nullptr;
+  uint64_t *H = nullptr;
+  uint8_t *pk = nullptr;
+  int *ok = nullptr;
+  hipStream_t s = nullptr;
+  ~KgBuffers() {
+    if (s) (void)hipStreamSynchronize(s);
+    (void)hipFree(g);
+    (void)hipFree(alpha);
+    (void)hipFree(H);
+    (void)hipFree(pk);
+    (void)hipFree(ok);
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+// candidates per launch (H of a candidate is mt * ceil(n/64) words: 0.34 MB for 348864, 1.7 MB for
+// 8192128), and attempts of a key's chain evaluated per round: only ~29 % of candidates have a
+// non-singular systematic block, so each round tries the next `ahead` attempts of every pending key
+// (enough to fill a launch, 2..8) and keeps the first one in chain order that succeeds --
+// libmceliece's outcome, in fewer rounds.  A launch takes about as long for one candidate per CU
+// as for one candidate in all, so filling it costs little.
+constexpr size_t KG_LAUNCH = 512;
+constexpr size_t KG_AHEAD_MIN = 2, KG_AHEAD_MAX = 8;
+
+}  // namespace
+
+int tmce_keypair_batch(int set, unsigned char *pk, unsigned char *sk, const unsigned char *seeds, size_t n) {
+  if (!valid_set(set) || (n && (!pk || !sk || !seeds))) return fail(TMCE_EINVAL, "tmce_keypair_batch: bad argument");
+  if (n == 0) return 0;
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  const SetInfo &si = set_info(set);
+  const size_t T1 = (size_t)si.t + 1, N = (size_t)si.n, PKB = (size_t)si.pk_bytes, SKB = (size_t)si.sk_bytes;
+  const size_t hw = pk_gen_h_bytes(set) / 8;
+  const int w = si.m - 12;
+
+  // each key's chain: the keypair's seed is randombytes(32) of the seeded chain on seeds[32 i]
+  // (state = 0x40 || seed; out = SHAKE256(state, 32)), preceded by the domain byte 64
+  std::vector<std::array<uint8_t, 33>> chain(n);
+  for (size_t i = 0; i < n; ++i) {
+    uint8_t st[33];
+    st[0] = 0x40;
+    std::memcpy(st + 1, seeds + 32 * i, 32);
+    chain[i][0] = 64;
+    shake256(chain[i].data() + 1, 32, st, 33);
+  }
+
+  KgBuffers b;
+  const size_t cap = std::min(n * KG_AHEAD_MAX, KG_LAUNCH);
+  HIPCHK(hipStreamCreateWithFlags(&b.s, hipStreamNonBlocking));
+  if (hipMalloc((void **)&b.g, cap * T1 * 2) != hipSuccess || hipMalloc((void **)&b.alpha, cap * N * 2) != hipSuccess ||
+      hipMalloc((void **)&b.H, cap * hw * 8) != hipSuccess || hipMalloc((void **)&b.pk, cap * PKB) != hipSuccess ||
+      hipMalloc((void **)&b.ok, cap * sizeof(int)) != hipSuccess)
+    return fail(TMCE_ENOMEM, "hipMalloc key generation buffers");
+
+  // cand[i][k]: attempt k of key i in this round
+  std::vector<std::vector<KeygenCandidate>> cand(n);
+  std::vector<int> win(n, -1);
+  std::vector<size_t> pending(n);
+  for (size_t i = 0; i < n; ++i) pending[i] = i;
+  std::vector<uint16_t> hg(cap * T1), ha(cap * N);
+  std::vector<int> hok(cap);
+  while (!pending.empty()) {
+    const size_t ahead = std::max(KG_AHEAD_MIN, std::min(KG_AHEAD_MAX, KG_LAUNCH / pending.size()));
+    parallel_for(pending.size(), [&](size_t k) {  // a key's attempts follow its chain, in order
+      const size_t i = pending[k];
+      cand[i].resize(ahead);
+      for (size_t a = 0; a < ahead; ++a) keygen_candidate(set, chain[i].data(), cand[i][a]);
+    });
+    const size_t total = pending.size() * ahead;  // candidate c = (key pending[c / ahead], attempt c % ahead)
+    std::vector<int> okv(total, 0);
+    for (size_t off = 0; off < total; off += cap) {
+      const size_t nc = std::min(cap, total - off);
+      for (size_t k = 0; k < nc; ++k) {
+        const size_t c = off + k;
+        const KeygenCandidate &kc = cand[pending[c / ahead]][c % ahead];
+        std::memcpy(&hg[k * T1], kc.g.data(), T1 * 2);
+        std::memcpy(&ha[k * N], kc.alpha.data(), N * 2);
+      }
+      HIPCHK(hipMemcpyAsync(b.g, hg.data(), nc * T1 * 2, hipMemcpyHostToDevice, b.s));
+      HIPCHK(hipMemcpyAsync(b.alpha, ha.data(), nc * N * 2, hipMemcpyHostToDevice, b.s));
+      KeygenArgs a{b.g, b.alpha, e->lg[w], e->ex[w], b.H, b.pk, b.ok, (int)nc};
+      HIPCHK(launch_pk_gen_set(set, a, b.s));
+      HIPCHK(hipMemcpyAsync(hok.data(), b.ok, nc * sizeof(int), hipMemcpyDeviceToHost, b.s));
+      HIPCHK(hipStreamSynchronize(b.s));
+      // the first successful attempt of each key in this launch (earlier launches hold earlier attempts)
+      for (size_t k = 0; k < nc; ++k) {
+        const size_t c = off + k, i = pending[c / ahead];
+        okv[c] = hok[k];
+        bool first = hok[k] != 0;
+        for (size_t c2 = c - c % ahead; c2 < c && first; ++c2) first = !okv[c2];
+        if (first) {
+          win[i] = (int)(c % ahead);
+          HIPCHK(hipMemcpyAsync(pk + i * PKB, b.pk + k * PKB, PKB, hipMemcpyDeviceToHost, b.s));
+        }
+      }
+      HIPCHK(hipStreamSynchronize(b.s));
+    }
+    std::vector<size_t> done, failed;
+    for (size_t i : pending) (win[i] >= 0 ? done : failed).push_back(i);
+    parallel_for(done.size(), [&](size_t k) {
+      const size_t i = done[k];
+      keygen_finish(set, cand[i][win[i]], sk + i * SKB);
+      cand[i].clear();
+    });
+    pending.swap(failed);
+  }
+  return 0;
+}
