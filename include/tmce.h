@@ -2,7 +2,8 @@
  * tmce.h -- C ABI of libtmce.so, the MI355X (gfx950) batched Classic McEliece KEM engine.
  *
  * Two layers:
- *  1. The libmceliece-20250507 crypto_kem interface for the five parameter sets, same names,
+ *  1. The libmceliece-20250507 crypto_kem interface for the five parameter sets and their f, pc
+ *     and pcf variants (LM:api, 20 in all), same names,
  *     argument meaning, sizes and return conventions, so a program linked against libmceliece
  *     can link against this library instead:
  *       mceliece_kem_<set>_keypair / _enc / _dec / _implementation / _compiler
@@ -37,26 +38,88 @@ extern "C" {
 #endif
 
 /* ---------------------------------------------------------------- sizes (LM:api) */
+/* the five parameter sets and their variants: f = semi-systematic key generation (same sizes),
+   pc = plaintext confirmation (32 more ciphertext bytes) */
 #define mceliece_kem_348864_PUBLICKEYBYTES 261120
 #define mceliece_kem_348864_SECRETKEYBYTES 6492
 #define mceliece_kem_348864_CIPHERTEXTBYTES 96
 #define mceliece_kem_348864_BYTES 32
+#define mceliece_kem_348864f_PUBLICKEYBYTES 261120
+#define mceliece_kem_348864f_SECRETKEYBYTES 6492
+#define mceliece_kem_348864f_CIPHERTEXTBYTES 96
+#define mceliece_kem_348864f_BYTES 32
+#define mceliece_kem_348864pc_PUBLICKEYBYTES 261120
+#define mceliece_kem_348864pc_SECRETKEYBYTES 6492
+#define mceliece_kem_348864pc_CIPHERTEXTBYTES 128
+#define mceliece_kem_348864pc_BYTES 32
+#define mceliece_kem_348864pcf_PUBLICKEYBYTES 261120
+#define mceliece_kem_348864pcf_SECRETKEYBYTES 6492
+#define mceliece_kem_348864pcf_CIPHERTEXTBYTES 128
+#define mceliece_kem_348864pcf_BYTES 32
 #define mceliece_kem_460896_PUBLICKEYBYTES 524160
 #define mceliece_kem_460896_SECRETKEYBYTES 13608
 #define mceliece_kem_460896_CIPHERTEXTBYTES 156
 #define mceliece_kem_460896_BYTES 32
+#define mceliece_kem_460896f_PUBLICKEYBYTES 524160
+#define mceliece_kem_460896f_SECRETKEYBYTES 13608
+#define mceliece_kem_460896f_CIPHERTEXTBYTES 156
+#define mceliece_kem_460896f_BYTES 32
+#define mceliece_kem_460896pc_PUBLICKEYBYTES 524160
+#define mceliece_kem_460896pc_SECRETKEYBYTES 13608
+#define mceliece_kem_460896pc_CIPHERTEXTBYTES 188
+#define mceliece_kem_460896pc_BYTES 32
+#define mceliece_kem_460896pcf_PUBLICKEYBYTES 524160
+#define mceliece_kem_460896pcf_SECRETKEYBYTES 13608
+#define mceliece_kem_460896pcf_CIPHERTEXTBYTES 188
+#define mceliece_kem_460896pcf_BYTES 32
 #define mceliece_kem_6688128_PUBLICKEYBYTES 1044992
 #define mceliece_kem_6688128_SECRETKEYBYTES 13932
 #define mceliece_kem_6688128_CIPHERTEXTBYTES 208
 #define mceliece_kem_6688128_BYTES 32
+#define mceliece_kem_6688128f_PUBLICKEYBYTES 1044992
+#define mceliece_kem_6688128f_SECRETKEYBYTES 13932
+#define mceliece_kem_6688128f_CIPHERTEXTBYTES 208
+#define mceliece_kem_6688128f_BYTES 32
+#define mceliece_kem_6688128pc_PUBLICKEYBYTES 1044992
+#define mceliece_kem_6688128pc_SECRETKEYBYTES 13932
+#define mceliece_kem_6688128pc_CIPHERTEXTBYTES 240
+#define mceliece_kem_6688128pc_BYTES 32
+#define mceliece_kem_6688128pcf_PUBLICKEYBYTES 1044992
+#define mceliece_kem_6688128pcf_SECRETKEYBYTES 13932
+#define mceliece_kem_6688128pcf_CIPHERTEXTBYTES 240
+#define mceliece_kem_6688128pcf_BYTES 32
 #define mceliece_kem_6960119_PUBLICKEYBYTES 1047319
 #define mceliece_kem_6960119_SECRETKEYBYTES 13948
 #define mceliece_kem_6960119_CIPHERTEXTBYTES 194
 #define mceliece_kem_6960119_BYTES 32
+#define mceliece_kem_6960119f_PUBLICKEYBYTES 1047319
+#define mceliece_kem_6960119f_SECRETKEYBYTES 13948
+#define mceliece_kem_6960119f_CIPHERTEXTBYTES 194
+#define mceliece_kem_6960119f_BYTES 32
+#define mceliece_kem_6960119pc_PUBLICKEYBYTES 1047319
+#define mceliece_kem_6960119pc_SECRETKEYBYTES 13948
+#define mceliece_kem_6960119pc_CIPHERTEXTBYTES 226
+#define mceliece_kem_6960119pc_BYTES 32
+#define mceliece_kem_6960119pcf_PUBLICKEYBYTES 1047319
+#define mceliece_kem_6960119pcf_SECRETKEYBYTES 13948
+#define mceliece_kem_6960119pcf_CIPHERTEXTBYTES 226
+#define mceliece_kem_6960119pcf_BYTES 32
 #define mceliece_kem_8192128_PUBLICKEYBYTES 1357824
 #define mceliece_kem_8192128_SECRETKEYBYTES 14120
 #define mceliece_kem_8192128_CIPHERTEXTBYTES 208
 #define mceliece_kem_8192128_BYTES 32
+#define mceliece_kem_8192128f_PUBLICKEYBYTES 1357824
+#define mceliece_kem_8192128f_SECRETKEYBYTES 14120
+#define mceliece_kem_8192128f_CIPHERTEXTBYTES 208
+#define mceliece_kem_8192128f_BYTES 32
+#define mceliece_kem_8192128pc_PUBLICKEYBYTES 1357824
+#define mceliece_kem_8192128pc_SECRETKEYBYTES 14120
+#define mceliece_kem_8192128pc_CIPHERTEXTBYTES 240
+#define mceliece_kem_8192128pc_BYTES 32
+#define mceliece_kem_8192128pcf_PUBLICKEYBYTES 1357824
+#define mceliece_kem_8192128pcf_SECRETKEYBYTES 14120
+#define mceliece_kem_8192128pcf_CIPHERTEXTBYTES 240
+#define mceliece_kem_8192128pcf_BYTES 32
 
 /* ---------------------------------------------------------------- layer 1: libmceliece API */
 #define TMCE_DECLARE_SET(S)                                                                     \
@@ -66,12 +129,169 @@ extern "C" {
   const char *mceliece_kem_##S##_implementation(void);                                          \
   const char *mceliece_kem_##S##_compiler(void);
 
-/* LM:doc/api.md:39-116, one block per parameter set */
+/* LM:doc/api.md:39-116, one block per parameter set and variant */
 TMCE_DECLARE_SET(348864)
+TMCE_DECLARE_SET(348864f)
+TMCE_DECLARE_SET(348864pc)
+TMCE_DECLARE_SET(348864pcf)
 TMCE_DECLARE_SET(460896)
+TMCE_DECLARE_SET(460896f)
+TMCE_DECLARE_SET(460896pc)
+TMCE_DECLARE_SET(460896pcf)
 TMCE_DECLARE_SET(6688128)
+TMCE_DECLARE_SET(6688128f)
+TMCE_DECLARE_SET(6688128pc)
+TMCE_DECLARE_SET(6688128pcf)
 TMCE_DECLARE_SET(6960119)
+TMCE_DECLARE_SET(6960119f)
+TMCE_DECLARE_SET(6960119pc)
+TMCE_DECLARE_SET(6960119pcf)
 TMCE_DECLARE_SET(8192128)
+TMCE_DECLARE_SET(8192128f)
+TMCE_DECLARE_SET(8192128pc)
+TMCE_DECLARE_SET(8192128pcf)
+
+/* the short names of LM:hdef:118-140 (mceliece<set>_keypair = mceliece_kem_<set>_keypair, ...) */
+#define mceliece348864_keypair mceliece_kem_348864_keypair
+#define mceliece348864_enc mceliece_kem_348864_enc
+#define mceliece348864_dec mceliece_kem_348864_dec
+#define mceliece348864_SECRETKEYBYTES mceliece_kem_348864_SECRETKEYBYTES
+#define mceliece348864_PUBLICKEYBYTES mceliece_kem_348864_PUBLICKEYBYTES
+#define mceliece348864_CIPHERTEXTBYTES mceliece_kem_348864_CIPHERTEXTBYTES
+#define mceliece348864_BYTES mceliece_kem_348864_BYTES
+#define mceliece348864f_keypair mceliece_kem_348864f_keypair
+#define mceliece348864f_enc mceliece_kem_348864f_enc
+#define mceliece348864f_dec mceliece_kem_348864f_dec
+#define mceliece348864f_SECRETKEYBYTES mceliece_kem_348864f_SECRETKEYBYTES
+#define mceliece348864f_PUBLICKEYBYTES mceliece_kem_348864f_PUBLICKEYBYTES
+#define mceliece348864f_CIPHERTEXTBYTES mceliece_kem_348864f_CIPHERTEXTBYTES
+#define mceliece348864f_BYTES mceliece_kem_348864f_BYTES
+#define mceliece348864pc_keypair mceliece_kem_348864pc_keypair
+#define mceliece348864pc_enc mceliece_kem_348864pc_enc
+#define mceliece348864pc_dec mceliece_kem_348864pc_dec
+#define mceliece348864pc_SECRETKEYBYTES mceliece_kem_348864pc_SECRETKEYBYTES
+#define mceliece348864pc_PUBLICKEYBYTES mceliece_kem_348864pc_PUBLICKEYBYTES
+#define mceliece348864pc_CIPHERTEXTBYTES mceliece_kem_348864pc_CIPHERTEXTBYTES
+#define mceliece348864pc_BYTES mceliece_kem_348864pc_BYTES
+#define mceliece348864pcf_keypair mceliece_kem_348864pcf_keypair
+#define mceliece348864pcf_enc mceliece_kem_348864pcf_enc
+#define mceliece348864pcf_dec mceliece_kem_348864pcf_dec
+#define mceliece348864pcf_SECRETKEYBYTES mceliece_kem_348864pcf_SECRETKEYBYTES
+#define mceliece348864pcf_PUBLICKEYBYTES mceliece_kem_348864pcf_PUBLICKEYBYTES
+#define mceliece348864pcf_CIPHERTEXTBYTES mceliece_kem_348864pcf_CIPHERTEXTBYTES
+#define mceliece348864pcf_BYTES mceliece_kem_348864pcf_BYTES
+#define mceliece460896_keypair mceliece_kem_460896_keypair
+#define mceliece460896_enc mceliece_kem_460896_enc
+#define mceliece460896_dec mceliece_kem_460896_dec
+#define mceliece460896_SECRETKEYBYTES mceliece_kem_460896_SECRETKEYBYTES
+#define mceliece460896_PUBLICKEYBYTES mceliece_kem_460896_PUBLICKEYBYTES
+#define mceliece460896_CIPHERTEXTBYTES mceliece_kem_460896_CIPHERTEXTBYTES
+#define mceliece460896_BYTES mceliece_kem_460896_BYTES
+#define mceliece460896f_keypair mceliece_kem_460896f_keypair
+#define mceliece460896f_enc mceliece_kem_460896f_enc
+#define mceliece460896f_dec mceliece_kem_460896f_dec
+#define mceliece460896f_SECRETKEYBYTES mceliece_kem_460896f_SECRETKEYBYTES
+#define mceliece460896f_PUBLICKEYBYTES mceliece_kem_460896f_PUBLICKEYBYTES
+#define mceliece460896f_CIPHERTEXTBYTES mceliece_kem_460896f_CIPHERTEXTBYTES
+#define mceliece460896f_BYTES mceliece_kem_460896f_BYTES
+#define mceliece460896pc_keypair mceliece_kem_460896pc_keypair
+#define mceliece460896pc_enc mceliece_kem_460896pc_enc
+#define mceliece460896pc_dec mceliece_kem_460896pc_dec
+#define mceliece460896pc_SECRETKEYBYTES mceliece_kem_460896pc_SECRETKEYBYTES
+#define mceliece460896pc_PUBLICKEYBYTES mceliece_kem_460896pc_PUBLICKEYBYTES
+#define mceliece460896pc_CIPHERTEXTBYTES mceliece_kem_460896pc_CIPHERTEXTBYTES
+#define mceliece460896pc_BYTES mceliece_kem_460896pc_BYTES
+#define mceliece460896pcf_keypair mceliece_kem_460896pcf_keypair
+#define mceliece460896pcf_enc mceliece_kem_460896pcf_enc
+#define mceliece460896pcf_dec mceliece_kem_460896pcf_dec
+#define mceliece460896pcf_SECRETKEYBYTES mceliece_kem_460896pcf_SECRETKEYBYTES
+#define mceliece460896pcf_PUBLICKEYBYTES mceliece_kem_460896pcf_PUBLICKEYBYTES
+#define mceliece460896pcf_CIPHERTEXTBYTES mceliece_kem_460896pcf_CIPHERTEXTBYTES
+#define mceliece460896pcf_BYTES mceliece_kem_460896pcf_BYTES
+#define mceliece6688128_keypair mceliece_kem_6688128_keypair
+#define mceliece6688128_enc mceliece_kem_6688128_enc
+#define mceliece6688128_dec mceliece_kem_6688128_dec
+#define mceliece6688128_SECRETKEYBYTES mceliece_kem_6688128_SECRETKEYBYTES
+#define mceliece6688128_PUBLICKEYBYTES mceliece_kem_6688128_PUBLICKEYBYTES
+#define mceliece6688128_CIPHERTEXTBYTES mceliece_kem_6688128_CIPHERTEXTBYTES
+#define mceliece6688128_BYTES mceliece_kem_6688128_BYTES
+#define mceliece6688128f_keypair mceliece_kem_6688128f_keypair
+#define mceliece6688128f_enc mceliece_kem_6688128f_enc
+#define mceliece6688128f_dec mceliece_kem_6688128f_dec
+#define mceliece6688128f_SECRETKEYBYTES mceliece_kem_6688128f_SECRETKEYBYTES
+#define mceliece6688128f_PUBLICKEYBYTES mceliece_kem_6688128f_PUBLICKEYBYTES
+#define mceliece6688128f_CIPHERTEXTBYTES mceliece_kem_6688128f_CIPHERTEXTBYTES
+#define mceliece6688128f_BYTES mceliece_kem_6688128f_BYTES
+#define mceliece6688128pc_keypair mceliece_kem_6688128pc_keypair
+#define mceliece6688128pc_enc mceliece_kem_6688128pc_enc
+#define mceliece6688128pc_dec mceliece_kem_6688128pc_dec
+#define mceliece6688128pc_SECRETKEYBYTES mceliece_kem_6688128pc_SECRETKEYBYTES
+#define mceliece6688128pc_PUBLICKEYBYTES mceliece_kem_6688128pc_PUBLICKEYBYTES
+#define mceliece6688128pc_CIPHERTEXTBYTES mceliece_kem_6688128pc_CIPHERTEXTBYTES
+#define mceliece6688128pc_BYTES mceliece_kem_6688128pc_BYTES
+#define mceliece6688128pcf_keypair mceliece_kem_6688128pcf_keypair
+#define mceliece6688128pcf_enc mceliece_kem_6688128pcf_enc
+#define mceliece6688128pcf_dec mceliece_kem_6688128pcf_dec
+#define mceliece6688128pcf_SECRETKEYBYTES mceliece_kem_6688128pcf_SECRETKEYBYTES
+#define mceliece6688128pcf_PUBLICKEYBYTES mceliece_kem_6688128pcf_PUBLICKEYBYTES
+#define mceliece6688128pcf_CIPHERTEXTBYTES mceliece_kem_6688128pcf_CIPHERTEXTBYTES
+#define mceliece6688128pcf_BYTES mceliece_kem_6688128pcf_BYTES
+#define mceliece6960119_keypair mceliece_kem_6960119_keypair
+#define mceliece6960119_enc mceliece_kem_6960119_enc
+#define mceliece6960119_dec mceliece_kem_6960119_dec
+#define mceliece6960119_SECRETKEYBYTES mceliece_kem_6960119_SECRETKEYBYTES
+#define mceliece6960119_PUBLICKEYBYTES mceliece_kem_6960119_PUBLICKEYBYTES
+#define mceliece6960119_CIPHERTEXTBYTES mceliece_kem_6960119_CIPHERTEXTBYTES
+#define mceliece6960119_BYTES mceliece_kem_6960119_BYTES
+#define mceliece6960119f_keypair mceliece_kem_6960119f_keypair
+#define mceliece6960119f_enc mceliece_kem_6960119f_enc
+#define mceliece6960119f_dec mceliece_kem_6960119f_dec
+#define mceliece6960119f_SECRETKEYBYTES mceliece_kem_6960119f_SECRETKEYBYTES
+#define mceliece6960119f_PUBLICKEYBYTES mceliece_kem_6960119f_PUBLICKEYBYTES
+#define mceliece6960119f_CIPHERTEXTBYTES mceliece_kem_6960119f_CIPHERTEXTBYTES
+#define mceliece6960119f_BYTES mceliece_kem_6960119f_BYTES
+#define mceliece6960119pc_keypair mceliece_kem_6960119pc_keypair
+#define mceliece6960119pc_enc mceliece_kem_6960119pc_enc
+#define mceliece6960119pc_dec mceliece_kem_6960119pc_dec
+#define mceliece6960119pc_SECRETKEYBYTES mceliece_kem_6960119pc_SECRETKEYBYTES
+#define mceliece6960119pc_PUBLICKEYBYTES mceliece_kem_6960119pc_PUBLICKEYBYTES
+#define mceliece6960119pc_CIPHERTEXTBYTES mceliece_kem_6960119pc_CIPHERTEXTBYTES
+#define mceliece6960119pc_BYTES mceliece_kem_6960119pc_BYTES
+#define mceliece6960119pcf_keypair mceliece_kem_6960119pcf_keypair
+#define mceliece6960119pcf_enc mceliece_kem_6960119pcf_enc
+#define mceliece6960119pcf_dec mceliece_kem_6960119pcf_dec
+#define mceliece6960119pcf_SECRETKEYBYTES mceliece_kem_6960119pcf_SECRETKEYBYTES
+#define mceliece6960119pcf_PUBLICKEYBYTES mceliece_kem_6960119pcf_PUBLICKEYBYTES
+#define mceliece6960119pcf_CIPHERTEXTBYTES mceliece_kem_6960119pcf_CIPHERTEXTBYTES
+#define mceliece6960119pcf_BYTES mceliece_kem_6960119pcf_BYTES
+#define mceliece8192128_keypair mceliece_kem_8192128_keypair
+#define mceliece8192128_enc mceliece_kem_8192128_enc
+#define mceliece8192128_dec mceliece_kem_8192128_dec
+#define mceliece8192128_SECRETKEYBYTES mceliece_kem_8192128_SECRETKEYBYTES
+#define mceliece8192128_PUBLICKEYBYTES mceliece_kem_8192128_PUBLICKEYBYTES
+#define mceliece8192128_CIPHERTEXTBYTES mceliece_kem_8192128_CIPHERTEXTBYTES
+#define mceliece8192128_BYTES mceliece_kem_8192128_BYTES
+#define mceliece8192128f_keypair mceliece_kem_8192128f_keypair
+#define mceliece8192128f_enc mceliece_kem_8192128f_enc
+#define mceliece8192128f_dec mceliece_kem_8192128f_dec
+#define mceliece8192128f_SECRETKEYBYTES mceliece_kem_8192128f_SECRETKEYBYTES
+#define mceliece8192128f_PUBLICKEYBYTES mceliece_kem_8192128f_PUBLICKEYBYTES
+#define mceliece8192128f_CIPHERTEXTBYTES mceliece_kem_8192128f_CIPHERTEXTBYTES
+#define mceliece8192128f_BYTES mceliece_kem_8192128f_BYTES
+#define mceliece8192128pc_keypair mceliece_kem_8192128pc_keypair
+#define mceliece8192128pc_enc mceliece_kem_8192128pc_enc
+#define mceliece8192128pc_dec mceliece_kem_8192128pc_dec
+#define mceliece8192128pc_SECRETKEYBYTES mceliece_kem_8192128pc_SECRETKEYBYTES
+#define mceliece8192128pc_PUBLICKEYBYTES mceliece_kem_8192128pc_PUBLICKEYBYTES
+#define mceliece8192128pc_CIPHERTEXTBYTES mceliece_kem_8192128pc_CIPHERTEXTBYTES
+#define mceliece8192128pc_BYTES mceliece_kem_8192128pc_BYTES
+#define mceliece8192128pcf_keypair mceliece_kem_8192128pcf_keypair
+#define mceliece8192128pcf_enc mceliece_kem_8192128pcf_enc
+#define mceliece8192128pcf_dec mceliece_kem_8192128pcf_dec
+#define mceliece8192128pcf_SECRETKEYBYTES mceliece_kem_8192128pcf_SECRETKEYBYTES
+#define mceliece8192128pcf_PUBLICKEYBYTES mceliece_kem_8192128pcf_PUBLICKEYBYTES
+#define mceliece8192128pcf_CIPHERTEXTBYTES mceliece_kem_8192128pcf_CIPHERTEXTBYTES
+#define mceliece8192128pcf_BYTES mceliece_kem_8192128pcf_BYTES
 
 /* randomness source of keypair/enc (replaces librandombytes' randombytes(),
    librandombytes include/randombytes.h:10); NULL restores getrandom() */
@@ -86,6 +306,14 @@ enum {
   TMCE_SET_6960119 = 3,
   TMCE_SET_8192128 = 4
 };
+/* Variant flags, or-ed into a set id wherever a function below takes `set` (key handles, keypairs,
+   pipelines, servers): TMCE_VARIANT_F selects the f variants' semi-systematic key generation
+   (LM:crypto_kem/<set>f/vec/pk_gen.c; encapsulation and decapsulation are the base set's), and
+   TMCE_VARIANT_PC the pc variants' plaintext confirmation (LM:crypto_kem/<set>pc/vec/kem_enc.c,
+   kem_dec.c): ciphertexts are CIPHERTEXTBYTES = SYND_BYTES + 32 bytes.  Example: 6960119pcf is
+   TMCE_SET_6960119 | TMCE_VARIANT_PC | TMCE_VARIANT_F.  The plain per-stage form (tmce_set_form)
+   covers the variants without TMCE_VARIANT_PC. */
+enum { TMCE_VARIANT_F = 8, TMCE_VARIANT_PC = 16 };
 
 enum {
   TMCE_OK = 0,

@@ -11,8 +11,12 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
-SET_IDS = {"348864": 0, "460896": 1, "6688128": 2, "6960119": 3, "8192128": 4}
-SETS = list(SET_IDS)
+BASE_IDS = {"348864": 0, "460896": 1, "6688128": 2, "6960119": 3, "8192128": 4}
+SETS = list(BASE_IDS)
+# variant ids: base | 8 ("f": semi-systematic key generation) | 16 ("pc": plaintext confirmation)
+VARIANT_FLAGS = {"": 0, "f": 8, "pc": 16, "pcf": 24}
+SET_IDS = {b + v: i | f for b, i in BASE_IDS.items() for v, f in VARIANT_FLAGS.items()}
+VARIANTS = list(SET_IDS)
 
 
 class OrcParams(ctypes.Structure):
@@ -38,6 +42,8 @@ def lib():
         P, I, LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_long
         L.oracle_params.restype = ctypes.POINTER(OrcParams)
         L.oracle_params.argtypes = [I]
+        L.oracle_ct_bytes.restype = I
+        L.oracle_ct_bytes.argtypes = [I]
         for name, args, res in [
             ("oracle_keypair_seeded", [I, P, P, P], None),
             ("oracle_enc_seeded", [I, P, P, P, P], I),
@@ -75,6 +81,10 @@ def params(set_name):
     return lib().oracle_params(SET_IDS[set_name]).contents
 
 
+def ct_bytes(set_name):
+    return lib().oracle_ct_bytes(SET_IDS[set_name])
+
+
 def keypair_seeded(set_name, seed):
     p = params(set_name)
     pk = np.zeros(p.pk_bytes, np.uint8)
@@ -88,7 +98,7 @@ def enc_batch(set_name, pk, seeds, nthreads=0):
     p = params(set_name)
     seeds = np.ascontiguousarray(seeds, np.uint8).reshape(-1, 32)
     n = seeds.shape[0]
-    ct = np.zeros((n, p.synd_bytes), np.uint8)
+    ct = np.zeros((n, ct_bytes(set_name)), np.uint8)
     k = np.zeros((n, 32), np.uint8)
     ret = np.zeros(n, np.int32)
     pk = np.ascontiguousarray(pk, np.uint8)
@@ -98,7 +108,7 @@ def enc_batch(set_name, pk, seeds, nthreads=0):
 
 def dec_batch(set_name, sk, ct, nthreads=0):
     p = params(set_name)
-    ct = np.ascontiguousarray(ct, np.uint8).reshape(-1, p.synd_bytes)
+    ct = np.ascontiguousarray(ct, np.uint8).reshape(-1, ct_bytes(set_name))
     n = ct.shape[0]
     k = np.zeros((n, 32), np.uint8)
     ret = np.zeros(n, np.int32)

@@ -35,7 +35,11 @@ static const orc_params PARAMS[5] = {
   P("8192128", 13, 8192, 128),
 };
 
-const orc_params *oracle_params(int set) { return (set >= 0 && set < 5) ? &PARAMS[set] : NULL; }
+/* variant ids: base set (0..4) | ORC_F (semi-systematic key generation) | ORC_PC (plaintext
+   confirmation); LM:crypto_kem/<set>{,f,pc,pcf} */
+static int valid_variant(int v) { return v >= 0 && (v & 7) < 5 && (v & ~(7 | ORC_F | ORC_PC)) == 0; }
+const orc_params *oracle_params(int set) { return valid_variant(set) ? &PARAMS[set & 7] : NULL; }
+int oracle_ct_bytes(int set) { return valid_variant(set) ? PARAMS[set & 7].synd_bytes + ((set & ORC_PC) ? 32 : 0) : -1; }
 
 /* ------------------------------------------------------------------ GF(2^m) */
 /* field polynomials: m=12: z^12+z^3+1; m=13: z^13+z^4+z^3+z+1 (LM:crypto_kem/<set>/vec/gf.c) */
@@ -283,14 +287,14 @@ static uint16_t poly_eval(const gf_tab *T, const uint16_t *f, int deg, uint16_t 
 /* minimal polynomial of f in GF((2^m)^t) = GF(2^m)[y]/F(y); LM:crypto_kem/<set>/vec/sk_gen.c
    (genpoly_gen) and gf_2mt_mul.c (F).  Returns 0 and irr[0..t) on success, -1 if singular. */
 static void ext_mul(int set, const gf_tab *T, uint16_t *out, const uint16_t *a, const uint16_t *b) {
-  const int t = PARAMS[set].t;
+  const int t = PARAMS[set & 7].t;
   uint16_t prod[256];
   memset(prod, 0, sizeof(prod[0]) * (2 * t - 1));
   for (int i = 0; i < t; i++)
     for (int j = 0; j < t; j++) prod[i + j] ^= gmul(T, a[i], b[j]);
   for (int i = 2 * t - 2; i >= t; i--) {
     uint16_t c = prod[i];
-    switch (set) {
+    switch (set & 7) {
       case ORC_348864: /* y^64 = y^3 + y + z */
         prod[i - t + 3] ^= c; prod[i - t + 1] ^= c; prod[i - t] ^= gmul(T, c, 2); break;
       case ORC_460896: /* y^96 = y^10 + y^9 + y^6 + 1 */
@@ -305,7 +309,7 @@ static void ext_mul(int set, const gf_tab *T, uint16_t *out, const uint16_t *a, 
 }
 
 static int genpoly(int set, uint16_t *irr, const uint16_t *f) {
-  const int t = PARAMS[set].t, m = PARAMS[set].m;
+  const int t = PARAMS[set & 7].t, m = PARAMS[set & 7].m;
   const gf_tab *T = tab(m);
   /* pw[j] = f^j, j = 0..t; solve sum_{j<t} x_j f^j = f^t (coefficient-wise linear system) */
   uint16_t *pw = (uint16_t *)calloc((size_t)(t + 1) * t, sizeof(uint16_t));
@@ -473,10 +477,52 @@ void oracle_perm_from_controlbits(int m, const uint8_t *cb, int16_t *pi) {
 }
 
 /* ------------------------------------------------------------------ public key */
+static inline int mat_bit(const uint64_t *mat, int W, int r, int c) { return (int)((mat[(size_t)r * W + c / 64] >> (c % 64)) & 1); }
+static inline void mat_flip(uint64_t *mat, int W, int r, int c) { mat[(size_t)r * W + c / 64] ^= 1ULL << (c % 64); }
+
+/* The "f" variants' column step (LM:crypto_kem/<set>f/vec/pk_gen.c mov_columns :87-153, called at
+   row mt-32 :251-255): when the elimination reaches column c0 = mt-32, rows c0..mt-1 are zero in
+   columns < c0.  The pivot columns of their 32 x 64 window (columns c0..c0+63) in row-echelon
+   form -- pv[i] = lowest column where any of the still-unused rows is nonzero -- give `pivots`
+   (bit pv[i] set) and the column swaps (c0+i <-> c0+pv[i], i = 0..31 in order) applied to the
+   matrix and to the support permutation pi.  -1 if the window has rank < 32. */
+static int f_move_columns(uint64_t *mat, int W, int mt, int16_t *pi, uint64_t *pivots) {
+  const int c0 = mt - 32;
+  uint64_t buf[32];
+  int pv[32];
+  for (int i = 0; i < 32; i++) {
+    buf[i] = 0;
+    for (int c = 0; c < 64; c++) buf[i] |= (uint64_t)mat_bit(mat, W, c0 + i, c0 + c) << c;
+  }
+  *pivots = 0;
+  for (int i = 0; i < 32; i++) {
+    int best = 64, who = -1;
+    for (int j = i; j < 32; j++)
+      if (buf[j] && __builtin_ctzll(buf[j]) < best) { best = __builtin_ctzll(buf[j]); who = j; }
+    if (who < 0) return -1;
+    uint64_t tmp = buf[i]; buf[i] = buf[who]; buf[who] = tmp;
+    for (int j = i + 1; j < 32; j++)
+      if ((buf[j] >> best) & 1) buf[j] ^= buf[i];
+    pv[i] = best;
+    *pivots |= 1ULL << best;
+  }
+  for (int i = 0; i < 32; i++) {
+    const int a = c0 + i, b = c0 + pv[i];
+    if (a == b) continue;
+    int16_t tp = pi[a]; pi[a] = pi[b]; pi[b] = tp;
+    for (int r = 0; r < mt; r++)
+      if (mat_bit(mat, W, r, a) != mat_bit(mat, W, r, b)) { mat_flip(mat, W, r, a); mat_flip(mat, W, r, b); }
+  }
+  return 0;
+}
+
 /* H~ row i*m+b, column j = bit b of alpha_j^i / g(alpha_j); systematic form [I | T];
-   pk row r = T row r, column mt+c at bit c (LM:crypto_kem/<set>/vec/pk_gen.c). */
-static int pk_from_support(int set, uint8_t *pk, const uint16_t *g /* t+1, monic */, const uint16_t *alpha) {
-  const orc_params *p = &PARAMS[set];
+   pk row r = T row r, column mt+c at bit c (LM:crypto_kem/<set>/vec/pk_gen.c).  With pi and
+   pivots given (the "f" variants) the column step above runs at column mt-32; pi is the support
+   permutation (alpha_j = bitrev(pi[j])) and is permuted with the columns. */
+static int pk_from_support(int set, uint8_t *pk, const uint16_t *g /* t+1, monic */, const uint16_t *alpha,
+                           int16_t *pi, uint64_t *pivots) {
+  const orc_params *p = &PARAMS[set & 7];
   const int m = p->m, n = p->n, t = p->t, mt = p->mt;
   const gf_tab *T = tab(m);
   const int W = (n + 63) / 64;
@@ -492,6 +538,7 @@ static int pk_from_support(int set, uint8_t *pk, const uint16_t *g /* t+1, monic
   free(v);
   int ok = 0;
   for (int c = 0; c < mt; c++) {
+    if (pi && c == mt - 32 && f_move_columns(mat, W, mt, pi, pivots)) { ok = -1; break; }
     int piv = -1;
     for (int r = c; r < mt; r++)
       if ((mat[(size_t)r * W + c / 64] >> (c % 64)) & 1) { piv = r; break; }
@@ -529,7 +576,7 @@ static int cmp_perm(const void *a, const void *b) {
 /* ------------------------------------------------------------------ keypair */
 /* LM:crypto_kem/<set>/vec/kem_keypair.c */
 void oracle_keypair(int set, uint8_t *pk, uint8_t *sk) {
-  const orc_params *p = &PARAMS[set];
+  const orc_params *p = &PARAMS[set & 7];
   const int m = p->m, n = p->n, t = p->t, q = 1 << m;
   uint8_t seed[33];
   seed[0] = 64;
@@ -561,7 +608,8 @@ void oracle_keypair(int set, uint8_t *pk, uint8_t *sk) {
     if (dup) continue;
     for (int i = 0; i < q; i++) pi[i] = (int16_t)ent[i].idx;
     for (int i = 0; i < n; i++) alpha[i] = bitrev_m((uint16_t)pi[i], m);
-    if (pk_from_support(set, pk, irr, alpha)) continue;
+    uint64_t pivots = 0xFFFFFFFFULL; /* LM:crypto_kem/<set>/vec/kem_keypair.c stores 0xFFFFFFFF */
+    if (pk_from_support(set, pk, irr, alpha, (set & ORC_F) ? pi : NULL, &pivots)) continue;
     uint8_t *skp = sk + 40;
     for (int i = 0; i < t; i++) {
       skp[2 * i] = (uint8_t)irr[i];
@@ -571,9 +619,8 @@ void oracle_keypair(int set, uint8_t *pk, uint8_t *sk) {
     oracle_controlbits(m, pi, skp);
     skp += p->cond_bytes;
     memcpy(skp, r, (size_t)(n / 8));
-    /* sk[32..40): 64-bit little-endian 0xFFFFFFFF */
-    memset(sk + 32, 0, 8);
-    memset(sk + 32, 0xFF, 4);
+    /* sk[32..40): the 64-bit pivot mask, little-endian (LM:crypto_kem/<set>f/vec/kem_keypair.c:87) */
+    for (int i = 0; i < 8; i++) sk[32 + i] = (uint8_t)(pivots >> (8 * i));
     break;
   }
   free(r); free(f); free(irr); free(ent); free(pi); free(alpha);
@@ -586,7 +633,7 @@ void oracle_keypair_seeded(int set, uint8_t *pk, uint8_t *sk, const uint8_t seed
 
 /* ------------------------------------------------------------------ support from sk */
 void oracle_support(int set, const uint8_t *sk, uint16_t *alpha) {
-  const orc_params *p = &PARAMS[set];
+  const orc_params *p = &PARAMS[set & 7];
   const int m = p->m;
   int16_t *pi = (int16_t *)malloc(sizeof(int16_t) << m);
   oracle_perm_from_controlbits(m, sk + 40 + p->irr_bytes, pi);
@@ -595,13 +642,13 @@ void oracle_support(int set, const uint8_t *sk, uint16_t *alpha) {
 }
 
 int oracle_pk_from_sk(int set, const uint8_t *sk, uint8_t *pk) {
-  const orc_params *p = &PARAMS[set];
+  const orc_params *p = &PARAMS[set & 7];
   uint16_t *alpha = (uint16_t *)malloc(sizeof(uint16_t) * p->n);
   uint16_t *g = (uint16_t *)malloc(sizeof(uint16_t) * (p->t + 1));
   oracle_support(set, sk, alpha);
   for (int i = 0; i < p->t; i++) g[i] = load_gf(sk + 40 + 2 * i, p->m);
   g[p->t] = 1;
-  int r = pk_from_support(set, pk, g, alpha);
+  int r = pk_from_support(set, pk, g, alpha, NULL, NULL);
   free(alpha);
   free(g);
   return r;
@@ -610,9 +657,9 @@ int oracle_pk_from_sk(int set, const uint8_t *sk, uint8_t *pk) {
 /* ------------------------------------------------------------------ encapsulation */
 /* LM:crypto_kem/<set>/vec/encrypt.c gen_e (the 8192128 variant draws t values, no range filter) */
 void oracle_gen_e(int set, uint8_t *e, int *attempts) {
-  const orc_params *p = &PARAMS[set];
+  const orc_params *p = &PARAMS[set & 7];
   const int m = p->m, n = p->n, t = p->t;
-  const int ndraw = (set == ORC_8192128) ? t : 2 * t;
+  const int ndraw = ((set & 7) == ORC_8192128) ? t : 2 * t;
   uint8_t buf[512];
   uint16_t ind[128];
   int tries = 0;
@@ -644,7 +691,7 @@ void oracle_gen_e(int set, uint8_t *e, int *attempts) {
 
 /* c = e[0:mt] xor T e[mt:n]; bits >= mt of the last byte are zero */
 static void syndrome_pk(int set, uint8_t *c, const uint8_t *pk, const uint8_t *e) {
-  const orc_params *p = &PARAMS[set];
+  const orc_params *p = &PARAMS[set & 7];
   memset(c, 0, (size_t)p->synd_bytes);
   for (int i = 0; i < p->mt; i++)
     if ((e[i >> 3] >> (i & 7)) & 1) c[i >> 3] |= (uint8_t)(1 << (i & 7));
@@ -659,27 +706,33 @@ static void syndrome_pk(int set, uint8_t *c, const uint8_t *pk, const uint8_t *e
   }
 }
 
+/* LM:crypto_kem/<set>/vec/kem_enc.c; the pc variants (LM:crypto_kem/<set>pc/vec/kem_enc.c:32-37)
+   append C = SHAKE256(2 || e, 32) to the ciphertext and hash the longer ciphertext into K */
 int oracle_enc(int set, uint8_t *ct, uint8_t *key, const uint8_t *pk) {
-  const orc_params *p = &PARAMS[set];
-  const int n = p->n;
-  uint8_t e[1024];
-  uint8_t pre[1 + 1024 + 208];
+  const orc_params *p = &PARAMS[set & 7];
+  const int n = p->n, clen = oracle_ct_bytes(set);
+  uint8_t e[1 + 1024];
+  uint8_t pre[1 + 1024 + 240];
   int ret = 0;
-  if (set == ORC_6960119) {
+  if ((set & 7) == ORC_6960119) {
     /* LM:crypto_kem/6960119/vec/kem_enc.c check_pk_padding */
     uint8_t b = 0;
     for (int i = 0; i < p->mt; i++) b |= pk[(size_t)i * p->pk_row_bytes + p->pk_row_bytes - 1];
     b >>= (p->k % 8);
     ret = b ? -1 : 0;
   }
-  oracle_gen_e(set, e, NULL);
-  syndrome_pk(set, ct, pk, e);
+  oracle_gen_e(set, e + 1, NULL);
+  syndrome_pk(set, ct, pk, e + 1);
+  if (set & ORC_PC) {
+    e[0] = 2;
+    oracle_shake256(ct + p->synd_bytes, 32, e, 1 + n / 8);
+  }
   pre[0] = 1;
-  memcpy(pre + 1, e, (size_t)(n / 8));
-  memcpy(pre + 1 + n / 8, ct, (size_t)p->synd_bytes);
-  oracle_shake256(key, 32, pre, 1 + n / 8 + p->synd_bytes);
+  memcpy(pre + 1, e + 1, (size_t)(n / 8));
+  memcpy(pre + 1 + n / 8, ct, (size_t)clen);
+  oracle_shake256(key, 32, pre, 1 + n / 8 + clen);
   if (ret) {
-    memset(ct, 0, (size_t)p->synd_bytes);
+    memset(ct, 0, (size_t)clen);
     memset(key, 0, 32);
   }
   return ret;
@@ -698,7 +751,7 @@ typedef struct {
 } dec_key;
 
 static void dec_key_load(int set, const uint8_t *sk, dec_key *K) {
-  const orc_params *p = &PARAMS[set];
+  const orc_params *p = &PARAMS[set & 7];
   const gf_tab *T = tab(p->m);
   K->alpha = (uint16_t *)malloc(sizeof(uint16_t) * p->n);
   K->u = (uint16_t *)malloc(sizeof(uint16_t) * p->n);
@@ -714,7 +767,7 @@ static void dec_key_free(dec_key *K) { free(K->alpha); free(K->u); }
 
 /* s[j] = sum_{i<n} r_i alpha_i^j / g(alpha_i)^2, j < 2t */
 static void synd_bits(int set, const dec_key *K, const uint8_t *r, int rbits, uint16_t *s) {
-  const orc_params *p = &PARAMS[set];
+  const orc_params *p = &PARAMS[set & 7];
   const gf_tab *T = tab(p->m);
   memset(s, 0, sizeof(uint16_t) * 2 * p->t);
   for (int i = 0; i < rbits; i++) {
@@ -731,7 +784,7 @@ static void synd_bits(int set, const dec_key *K, const uint8_t *r, int rbits, ui
    code's inversion-free variant keeps every intermediate polynomial a nonzero scalar multiple of
    this one, so both give the same root set).  locator[i] = C[t-i]. */
 void oracle_bm(int set, const uint16_t *s, uint16_t *locator) {
-  const orc_params *p = &PARAMS[set];
+  const orc_params *p = &PARAMS[set & 7];
   const int t = p->t, m = p->m;
   const gf_tab *T = tab(m);
   uint16_t C[129], B[129], Tm[129];
@@ -762,7 +815,7 @@ void oracle_bm(int set, const uint16_t *s, uint16_t *locator) {
 }
 
 void oracle_synd_ct(int set, const uint8_t *sk, const uint8_t *ct, uint16_t *s) {
-  const orc_params *p = &PARAMS[set];
+  const orc_params *p = &PARAMS[set & 7];
   dec_key K;
   dec_key_load(set, sk, &K);
   synd_bits(set, &K, ct, 8 * p->synd_bytes, s);
@@ -772,7 +825,7 @@ void oracle_synd_ct(int set, const uint8_t *sk, const uint8_t *ct, uint16_t *s) 
 /* LM:crypto_kem/<set>/vec/decrypt.c: r = (ct bits incl. any padding bits of the last byte, 0...);
    e_i = [locator(alpha_i) == 0]; success iff weight(e) == t and synd(e) == synd(r). */
 static int decrypt_with(int set, const dec_key *K, const uint8_t *ct, uint8_t *e) {
-  const orc_params *p = &PARAMS[set];
+  const orc_params *p = &PARAMS[set & 7];
   const int t = p->t, n = p->n;
   const gf_tab *T = tab(p->m);
   uint16_t s[256], s2[256], loc[129];
@@ -800,20 +853,28 @@ int oracle_decrypt(int set, const uint8_t *sk, const uint8_t *ct, uint8_t *e) {
   return r;
 }
 
-/* LM:crypto_kem/<set>/vec/kem_dec.c (+ 6960119 check_c_padding) */
+/* LM:crypto_kem/<set>/vec/kem_dec.c (+ 6960119 check_c_padding); the pc variants
+   (LM:crypto_kem/<set>pc/vec/kem_dec.c:43-59) also fail when SHAKE256(2 || e, 32) differs from
+   the ciphertext's last 32 bytes, and hash the whole ciphertext */
 static int dec_with(int set, const dec_key *K, uint8_t *key, const uint8_t *ct, const uint8_t *sk) {
-  const orc_params *p = &PARAMS[set];
-  const int n = p->n;
-  uint8_t e[1024], pre[1 + 1024 + 208], ctc[208];
-  memcpy(ctc, ct, (size_t)p->synd_bytes); /* ct and key may alias */
+  const orc_params *p = &PARAMS[set & 7];
+  const int n = p->n, clen = oracle_ct_bytes(set);
+  uint8_t e[1 + 1024], pre[1 + 1024 + 240], ctc[240];
+  memcpy(ctc, ct, (size_t)clen); /* ct and key may alias */
   int pad_bad = 0;
-  if (set == ORC_6960119) pad_bad = (ctc[p->synd_bytes - 1] >> (p->mt % 8)) != 0;
-  int fail = decrypt_with(set, K, ctc, e);
+  if ((set & 7) == ORC_6960119) pad_bad = (ctc[p->synd_bytes - 1] >> (p->mt % 8)) != 0;
+  int fail = decrypt_with(set, K, ctc, e + 1);
+  if (set & ORC_PC) {
+    uint8_t conf[32];
+    e[0] = 2;
+    oracle_shake256(conf, 32, e, 1 + n / 8);
+    if (memcmp(conf, ctc + p->synd_bytes, 32) != 0) fail = 1;
+  }
   const uint8_t *s = sk + 40 + p->irr_bytes + p->cond_bytes;
   pre[0] = fail ? 0 : 1;
-  memcpy(pre + 1, fail ? s : e, (size_t)(n / 8));
-  memcpy(pre + 1 + n / 8, ctc, (size_t)p->synd_bytes);
-  oracle_shake256(key, 32, pre, 1 + n / 8 + p->synd_bytes);
+  memcpy(pre + 1, fail ? s : e + 1, (size_t)(n / 8));
+  memcpy(pre + 1 + n / 8, ctc, (size_t)clen);
+  oracle_shake256(key, 32, pre, 1 + n / 8 + clen);
   if (pad_bad) {
     memset(key, 0xFF, 32);
     return -1;
@@ -822,7 +883,7 @@ static int dec_with(int set, const dec_key *K, uint8_t *key, const uint8_t *ct, 
 }
 
 int oracle_dec(int set, uint8_t *key, const uint8_t *ct, const uint8_t *sk) {
-  const orc_params *p = &PARAMS[set];
+  const orc_params *p = &PARAMS[set & 7];
   uint8_t *skc = (uint8_t *)malloc((size_t)p->sk_bytes); /* sk and key may alias */
   memcpy(skc, sk, (size_t)p->sk_bytes);
   dec_key K;
@@ -844,7 +905,8 @@ typedef struct {
 
 static void *job_run(void *arg) {
   job_t *j = (job_t *)arg;
-  const orc_params *p = &PARAMS[j->set];
+  const orc_params *p = &PARAMS[j->set & 7];
+  const long clen = oracle_ct_bytes(j->set);
   if (j->is_dec == 2) { /* syndrome of given error vectors (seeds holds e) */
     for (long i = j->lo; i < j->hi; i++)
       syndrome_pk(j->set, j->ct + (size_t)i * p->synd_bytes, j->pk, j->seeds + (size_t)i * (p->n / 8));
@@ -852,13 +914,13 @@ static void *job_run(void *arg) {
     dec_key K;
     dec_key_load(j->set, j->sk, &K);
     for (long i = j->lo; i < j->hi; i++) {
-      int r = dec_with(j->set, &K, j->key + 32 * i, j->cts + (size_t)i * p->synd_bytes, j->sk);
+      int r = dec_with(j->set, &K, j->key + 32 * i, j->cts + (size_t)i * clen, j->sk);
       if (j->ret) j->ret[i] = r;
     }
     dec_key_free(&K);
   } else {
     for (long i = j->lo; i < j->hi; i++) {
-      int r = oracle_enc_seeded(j->set, j->ct + (size_t)i * p->synd_bytes, j->key + 32 * i, j->pk,
+      int r = oracle_enc_seeded(j->set, j->ct + (size_t)i * clen, j->key + 32 * i, j->pk,
                                 j->seeds + 32 * i);
       if (j->ret) j->ret[i] = r;
     }
@@ -965,14 +1027,14 @@ static uint64_t kat_myrandom(kat_ctx *c) {
 }
 
 void oracle_kat_checksum(int set, int loops, char *hex_out) {
-  const orc_params *p = &PARAMS[set];
+  const orc_params *p = &PARAMS[set & 7];
   kat_ctx c;
   memset(&c, 0, sizeof c);
   oracle_rng_known_reset();
   uint8_t *pk = (uint8_t *)malloc((size_t)p->pk_bytes);
   uint8_t *sk = (uint8_t *)malloc((size_t)p->sk_bytes);
-  uint8_t ct[208], k[32], t[32];
-  const long clen = p->synd_bytes;
+  uint8_t ct[240], k[32], t[32];
+  const long clen = oracle_ct_bytes(set);
   for (int loop = 0; loop < loops; loop++) {
     oracle_keypair(set, pk, sk);
     kat_checksum(&c, pk, p->pk_bytes);

@@ -5,8 +5,9 @@
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, and only as the
  * checker (or as the CPU timing baseline).  It is a plain-C restatement of the algorithm that
  * libmceliece-20250507 (deps/libmceliece-20250507.tar.gz in the reference checkout) implements
- * for the five non-"f", non-"pc" parameter sets, written from the specification and the
- * library's behaviour, favouring obviously-correct code over speed.
+ * for the five parameter sets and their "f" (semi-systematic key generation) and "pc" (plaintext
+ * confirmation) variants, written from the specification and the library's behaviour, favouring
+ * obviously-correct code over speed.
  *
  * Parity pins (see tests/test_oracle.py):
  *   - whole KEM: the KAT checksums of libmceliece's own test driver
@@ -25,21 +26,24 @@
 extern "C" {
 #endif
 
-/* parameter-set ids */
+/* parameter-set ids; a variant id is a set id | ORC_F and/or | ORC_PC (348864f = 0 | ORC_F, ...).
+   Every function below that takes `set` accepts a variant id. */
 enum { ORC_348864 = 0, ORC_460896 = 1, ORC_6688128 = 2, ORC_6960119 = 3, ORC_8192128 = 4 };
+enum { ORC_F = 8, ORC_PC = 16 };
 
 typedef struct {
   const char *name;
   int m, n, t;
   int mt, k;            /* PK_NROWS, PK_NCOLS */
   int pk_row_bytes;
-  int synd_bytes;       /* = ciphertext bytes */
+  int synd_bytes;       /* = ciphertext bytes of the non-pc variants */
   int irr_bytes;
   int cond_bytes;
   long pk_bytes, sk_bytes;
 } orc_params;
 
-const orc_params *oracle_params(int set);
+const orc_params *oracle_params(int set); /* the base set's parameters; NULL for a bad id */
+int oracle_ct_bytes(int set);              /* SYND_BYTES, + 32 for a pc variant */
 
 /* randomness source used by keypair/enc (thread-local) */
 void oracle_rng_seeded(const uint8_t seed[32]); /* state = 0x40||seed; out = SHAKE256(state,len); state[1..32] = out[0..31] */

@@ -14,15 +14,23 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtmce.so")
 
-SETS = {"348864": 0, "460896": 1, "6688128": 2, "6960119": 3, "8192128": 4}
-# (m, n, t) and byte sizes; sizes as libmceliece's api (LM:api)
-PARAMS = {
-    "348864": dict(m=12, n=3488, t=64, pk=261120, sk=6492, ct=96),
-    "460896": dict(m=13, n=4608, t=96, pk=524160, sk=13608, ct=156),
-    "6688128": dict(m=13, n=6688, t=128, pk=1044992, sk=13932, ct=208),
-    "6960119": dict(m=13, n=6960, t=119, pk=1047319, sk=13948, ct=194),
-    "8192128": dict(m=13, n=8192, t=128, pk=1357824, sk=14120, ct=208),
+BASE_SETS = {"348864": 0, "460896": 1, "6688128": 2, "6960119": 3, "8192128": 4}
+# variant flags (TMCE_VARIANT_F / TMCE_VARIANT_PC): f = semi-systematic key generation,
+# pc = plaintext confirmation (ciphertext + 32 bytes); LM:crypto_kem/<set>{,f,pc,pcf}
+VARIANT_F, VARIANT_PC = 8, 16
+_VARIANT_FLAGS = {"": 0, "f": VARIANT_F, "pc": VARIANT_PC, "pcf": VARIANT_PC | VARIANT_F}
+# (m, n, t) and byte sizes (synd = syndrome bytes, ct = ciphertext bytes); sizes as LM:api
+_BASE_PARAMS = {
+    "348864": dict(m=12, n=3488, t=64, pk=261120, sk=6492, synd=96),
+    "460896": dict(m=13, n=4608, t=96, pk=524160, sk=13608, synd=156),
+    "6688128": dict(m=13, n=6688, t=128, pk=1044992, sk=13932, synd=208),
+    "6960119": dict(m=13, n=6960, t=119, pk=1047319, sk=13948, synd=194),
+    "8192128": dict(m=13, n=8192, t=128, pk=1357824, sk=14120, synd=208),
 }
+# every parameter set and variant by libmceliece's name ("348864", "348864f", "348864pc", ...)
+SETS = {b + v: i | fl for b, i in BASE_SETS.items() for v, fl in _VARIANT_FLAGS.items()}
+PARAMS = {b + v: dict(_BASE_PARAMS[b], ct=_BASE_PARAMS[b]["synd"] + (32 if "pc" in v else 0))
+          for b in BASE_SETS for v in _VARIANT_FLAGS}
 KEY_BYTES = 32
 # syndrome methods (tmce_syndrome_batch)
 SYND_METHODS = {"sparse": 0, "mfma": 1, "valu": 2}
@@ -187,7 +195,7 @@ class PublicKey(_Key):
         """ct_i = H e_i for rows of e (n/8 bytes each, any weight) -- the PKE encryption."""
         e = np.ascontiguousarray(e, dtype=np.uint8).reshape(-1, self.params["n"] // 8)
         n = e.shape[0]
-        ct = np.zeros((n, self.params["ct"]), np.uint8)
+        ct = np.zeros((n, self.params["synd"]), np.uint8)
         _check(load().tmce_syndrome_batch(self._h, _ptr(ct), _ptr(e), n, SYND_METHODS.get(method, method)),
                "tmce_syndrome_batch")
         return ct

@@ -133,13 +133,14 @@ int enc_device(tmce_key *k, uint8_t *d_ct, uint8_t *d_k, int *d_status, const ui
   if (n > (size_t)INT32_MAX / 2) return fail(TMCE_EINVAL, "batch too large");
   const SetInfo &si = set_info(k->set);
   if (!s) s = e->stream;
+  if (k->pc && e->form == 1) return fail(TMCE_EINVAL, "the plain form covers the variants without plaintext confirmation");
   if (k->pad_bad) {  // LM:crypto_kem/6960119/vec/kem_enc.c: outputs cleared, return -1
-    HIPCHK(launch_fill_u8(d_ct, 0, n * si.synd_bytes, s));
+    HIPCHK(launch_fill_u8(d_ct, 0, n * key_ct_bytes(k), s));
     HIPCHK(launch_fill_u8(d_k, 0, n * 32, s));
     if (d_status) HIPCHK(launch_fill_i32(d_status, -1, n, s));
     return 0;
   }
-  const int MSGW = (1 + si.n / 8 + si.synd_bytes) / 136 * 34 + 34;
+  const int MSGW = si.msg_words;
   if (int r = scratch_wait(e->use_enc, s)) return r;
   if (e->form == 1) {
     const int NEW = (si.n + 31) / 32, KWE = (si.k + 31) / 32, CW = (si.mt + 31) / 32;
@@ -168,6 +169,7 @@ int enc_device(tmce_key *k, uint8_t *d_ct, uint8_t *d_k, int *d_status, const ui
   a.nops = (int)n;
   a.gen_blocks = e->gen_blocks[k->set];
   a.gen_form = e->gen_form[k->set];
+  a.pc = k->pc ? 1 : 0;
   HIPCHK(launch_enc_set(k->set, a, s));
   return scratch_done(e->use_enc, s);
 }
@@ -182,13 +184,18 @@ int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, si
   if (!s) s = e->stream;
   const int TT = ((2 * si.t + 31) / 32) * 32;
   const int RW = si.m * TT / 32;
-  const int MSGW = (1 + si.n / 8 + si.synd_bytes) / 136 * 34 + 34;
+  const int MSGW = si.msg_words;
+  if (k->pc && e->form == 1) return fail(TMCE_EINVAL, "the plain form covers the variants without plaintext confirmation");
   if (int r = scratch_wait(e->use_dec, s)) return r;
   if (int r = e->S.ensure(n * RW * sizeof(uint32_t))) return r;
+  if (k->pc)
+    if (int r = e->ct_synd.ensure(n * si.synd_bytes)) return r;
   if (int r = e->msg_dec.ensure(n * MSGW * sizeof(uint32_t))) return r;
   if (int r = e->dqueue.ensure(1024)) return r;
   DecArgs a{};
   a.ct = d_ct;
+  a.pc = k->pc ? 1 : 0;
+  a.ct_synd = k->pc ? e->ct_synd.as<uint8_t>() : nullptr;
   a.F = k->F;
   a.Fsw = k->Fsw;
   a.front_mfma = e->front_mfma[k->set];
@@ -287,36 +294,36 @@ struct CachedKey {
 std::mutex g_cache_mu;
 std::map<std::pair<int, int>, CachedKey> g_cache;  // (set, kind) -> last key
 
-tmce_key *cached_key(int set, int kind, const uint8_t *bytes, size_t len) {
-  auto &c = g_cache[{set, kind}];
+tmce_key *cached_key(int variant, int kind, const uint8_t *bytes, size_t len) {
+  auto &c = g_cache[{variant, kind}];
   if (c.key && c.bytes.size() == len && std::memcmp(c.bytes.data(), bytes, len) == 0) return c.key;
   if (c.key) tmce_key_free(c.key);
   c.key = nullptr;
   tmce_key *k = nullptr;
-  int r = kind == 1 ? tmce_key_from_pk(set, bytes, &k) : tmce_key_from_sk(set, bytes, &k);
+  int r = kind == 1 ? tmce_key_from_pk(variant, bytes, &k) : tmce_key_from_sk(variant, bytes, &k);
   if (r) return nullptr;
   c.key = k;
   c.bytes.assign(bytes, bytes + len);
   return k;
 }
 
-int single_enc(int set, unsigned char *c, unsigned char *k, const unsigned char *pk) {
+int single_enc(int variant, unsigned char *c, unsigned char *k, const unsigned char *pk) {
   uint8_t seed[32];
   randombytes(seed, 32);
   std::lock_guard<std::mutex> lk(g_cache_mu);
-  tmce_key *key = cached_key(set, 1, pk, (size_t)set_info(set).pk_bytes);
+  tmce_key *key = cached_key(variant, 1, pk, (size_t)set_info(variant_base(variant)).pk_bytes);
   if (!key) std::abort();
   int st = 0;
   if (tmce_enc_batch(key, c, k, &st, seed, 1) != 0) std::abort();
   return st;
 }
 
-int single_dec(int set, unsigned char *k, const unsigned char *c, const unsigned char *sk) {
+int single_dec(int variant, unsigned char *k, const unsigned char *c, const unsigned char *sk) {
   std::lock_guard<std::mutex> lk(g_cache_mu);
-  const SetInfo &si = set_info(set);
+  const SetInfo &si = set_info(variant_base(variant));
   uint8_t ctc[256];
-  std::memcpy(ctc, c, si.synd_bytes);  // k and c may alias
-  tmce_key *key = cached_key(set, 2, sk, (size_t)si.sk_bytes);
+  std::memcpy(ctc, c, (size_t)variant_ct_bytes(variant));  // k and c may alias
+  tmce_key *key = cached_key(variant, 2, sk, (size_t)si.sk_bytes);
   if (!key) std::abort();
   int st = 0;
   if (tmce_dec_batch(key, k, &st, ctc, 1) != 0) std::abort();
@@ -333,13 +340,16 @@ const char *tmce_last_error(void) { return g_err.c_str(); }
 
 void tmce_set_randombytes(tmce_randombytes_fn fn) { g_randombytes = fn; }
 
-int tmce_key_from_pk(int set, const unsigned char *pk, tmce_key **out) {
-  if (!valid_set(set) || !pk || !out) return fail(TMCE_EINVAL, "tmce_key_from_pk: bad argument");
+int tmce_key_from_pk(int variant, const unsigned char *pk, tmce_key **out) {
+  if (!variant_valid(variant) || !pk || !out) return fail(TMCE_EINVAL, "tmce_key_from_pk: bad argument");
   Engine *e = engine();
   if (!e) return TMCE_ENODEV;
   std::lock_guard<std::mutex> lk(e->mu);
+  const int set = variant_base(variant);
   auto k = std::make_unique<tmce_key>();
   k->set = set;
+  k->variant = variant;
+  k->pc = variant_pc(variant);
   k->kind = 1;
   k->pad_bad = pk_padding_bad(set, pk);
   std::vector<uint32_t> cols, trows;
@@ -352,13 +362,16 @@ int tmce_key_from_pk(int set, const unsigned char *pk, tmce_key **out) {
   return 0;
 }
 
-int tmce_key_from_sk(int set, const unsigned char *sk, tmce_key **out) {
-  if (!valid_set(set) || !sk || !out) return fail(TMCE_EINVAL, "tmce_key_from_sk: bad argument");
+int tmce_key_from_sk(int variant, const unsigned char *sk, tmce_key **out) {
+  if (!variant_valid(variant) || !sk || !out) return fail(TMCE_EINVAL, "tmce_key_from_sk: bad argument");
   Engine *e = engine();
   if (!e) return TMCE_ENODEV;
   std::lock_guard<std::mutex> lk(e->mu);
+  const int set = variant_base(variant);
   auto k = std::make_unique<tmce_key>();
   k->set = set;
+  k->variant = variant;
+  k->pc = variant_pc(variant);
   k->kind = 2;
   DecKeyHost h;
   build_dec_key(set, sk, h);
@@ -408,12 +421,12 @@ void tmce_key_free(tmce_key *k) {
 
 size_t tmce_key_device_bytes(const tmce_key *k) { return k ? k->bytes : 0; }
 
-int tmce_keypair_seeded(int set, unsigned char *pk, unsigned char *sk, const unsigned char *seed) {
-  if (!valid_set(set) || !pk || !sk || !seed) return fail(TMCE_EINVAL, "tmce_keypair_seeded: bad argument");
+int tmce_keypair_seeded(int variant, unsigned char *pk, unsigned char *sk, const unsigned char *seed) {
+  if (!variant_valid(variant) || !pk || !sk || !seed) return fail(TMCE_EINVAL, "tmce_keypair_seeded: bad argument");
   SeedChain c;
   c.state[0] = 0x40;
   std::memcpy(c.state + 1, seed, 32);
-  host_keypair(set, pk, sk, rnd_chain, &c);
+  host_keypair(variant, pk, sk, rnd_chain, &c);
   return 0;
 }
 
@@ -442,16 +455,16 @@ int tmce_enc_batch(tmce_key *pk, unsigned char *ct, unsigned char *k, int *statu
   if (!e) return TMCE_ENODEV;
   if (n == 0) return 0;
   std::lock_guard<std::mutex> lk(e->mu);
-  const SetInfo &si = set_info(pk->set);
+  const size_t ctb = key_ct_bytes(pk);
   if (int r = e->seeds.ensure(n * 32)) return r;
-  if (int r = e->ct.ensure(n * si.synd_bytes)) return r;
+  if (int r = e->ct.ensure(n * ctb)) return r;
   if (int r = e->key.ensure(n * 32)) return r;
   if (int r = e->status.ensure(n * sizeof(int))) return r;
   HIPCHK(hipMemcpyAsync(e->seeds.p, seeds, n * 32, hipMemcpyHostToDevice, e->stream));
   if (int r = enc_device(pk, e->ct.as<uint8_t>(), e->key.as<uint8_t>(), e->status.as<int>(), e->seeds.as<uint8_t>(), n,
                          e->stream))
     return r;
-  HIPCHK(hipMemcpyAsync(ct, e->ct.p, n * si.synd_bytes, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipMemcpyAsync(ct, e->ct.p, n * ctb, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipMemcpyAsync(k, e->key.p, n * 32, hipMemcpyDeviceToHost, e->stream));
   if (status) HIPCHK(hipMemcpyAsync(status, e->status.p, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -464,11 +477,11 @@ int tmce_dec_batch(tmce_key *sk, unsigned char *k, int *status, const unsigned c
   if (!e) return TMCE_ENODEV;
   if (n == 0) return 0;
   std::lock_guard<std::mutex> lk(e->mu);
-  const SetInfo &si = set_info(sk->set);
-  if (int r = e->ct.ensure(n * si.synd_bytes)) return r;
+  const size_t ctb = key_ct_bytes(sk);
+  if (int r = e->ct.ensure(n * ctb)) return r;
   if (int r = e->key.ensure(n * 32)) return r;
   if (int r = e->status.ensure(n * sizeof(int))) return r;
-  HIPCHK(hipMemcpyAsync(e->ct.p, ct, n * si.synd_bytes, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(e->ct.p, ct, n * ctb, hipMemcpyHostToDevice, e->stream));
   if (int r = dec_device(sk, e->key.as<uint8_t>(), e->status.as<int>(), e->ct.as<uint8_t>(), n, e->stream)) return r;
   HIPCHK(hipMemcpyAsync(k, e->key.p, n * 32, hipMemcpyDeviceToHost, e->stream));
   if (status) HIPCHK(hipMemcpyAsync(status, e->status.p, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
@@ -502,8 +515,9 @@ int tmce_syndrome_batch(tmce_key *pk, unsigned char *ct, const unsigned char *ev
   return 0;
 }
 
-int tmce_tune_set(int set, int param, int value) {
-  if (!valid_set(set)) return fail(TMCE_EINVAL, "tmce_tune_set: bad set");
+int tmce_tune_set(int variant, int param, int value) {
+  if (!variant_valid(variant)) return fail(TMCE_EINVAL, "tmce_tune_set: bad set");
+  const int set = variant_base(variant);  // launch shapes are per parameter set
   Engine *e = engine();
   if (!e) return TMCE_ENODEV;
   std::lock_guard<std::mutex> lk(e->mu);
@@ -536,8 +550,9 @@ int tmce_tune_set(int set, int param, int value) {
   return fail(TMCE_EINVAL, "tmce_tune_set: unknown parameter");
 }
 
-int tmce_tune_get(int set, int param) {
-  if (!valid_set(set)) return fail(TMCE_EINVAL, "tmce_tune_get: bad set");
+int tmce_tune_get(int variant, int param) {
+  if (!variant_valid(variant)) return fail(TMCE_EINVAL, "tmce_tune_get: bad set");
+  const int set = variant_base(variant);
   Engine *e = engine();
   if (!e) return TMCE_ENODEV;
   std::lock_guard<std::mutex> lk(e->mu);
@@ -682,9 +697,24 @@ int tmce_timing_read(int max, char *names, long *counts, double *total_ms) {
   const char *mceliece_kem_##S##_compiler(void) { return "hipcc " __VERSION__; }
 
 TMCE_DEFINE_SET(348864, 0)
+TMCE_DEFINE_SET(348864f, 0 | VAR_F)
+TMCE_DEFINE_SET(348864pc, 0 | VAR_PC)
+TMCE_DEFINE_SET(348864pcf, 0 | VAR_PC | VAR_F)
 TMCE_DEFINE_SET(460896, 1)
+TMCE_DEFINE_SET(460896f, 1 | VAR_F)
+TMCE_DEFINE_SET(460896pc, 1 | VAR_PC)
+TMCE_DEFINE_SET(460896pcf, 1 | VAR_PC | VAR_F)
 TMCE_DEFINE_SET(6688128, 2)
+TMCE_DEFINE_SET(6688128f, 2 | VAR_F)
+TMCE_DEFINE_SET(6688128pc, 2 | VAR_PC)
+TMCE_DEFINE_SET(6688128pcf, 2 | VAR_PC | VAR_F)
 TMCE_DEFINE_SET(6960119, 3)
+TMCE_DEFINE_SET(6960119f, 3 | VAR_F)
+TMCE_DEFINE_SET(6960119pc, 3 | VAR_PC)
+TMCE_DEFINE_SET(6960119pcf, 3 | VAR_PC | VAR_F)
 TMCE_DEFINE_SET(8192128, 4)
+TMCE_DEFINE_SET(8192128f, 4 | VAR_F)
+TMCE_DEFINE_SET(8192128pc, 4 | VAR_PC)
+TMCE_DEFINE_SET(8192128pcf, 4 | VAR_PC | VAR_F)
 
 }  // extern "C"

@@ -175,6 +175,44 @@ __device__ __forceinline__ void shake256_blocks_32(const uint64_t *__restrict__ 
   for (int i = 0; i < 4; ++i) out[i] = A[i];
 }
 
+// SHAKE256 over the first LEN bytes of `msg` with byte 0 replaced by `domain` (the bytes from LEN
+// on are ignored; the padding is applied here); writes 32 bytes.  msg must hold whole 136-byte
+// blocks up to the block containing byte LEN.
+template <int LEN>
+__device__ __forceinline__ void shake256_prefix_32(const uint64_t *__restrict__ msg, uint32_t domain, uint64_t out[4],
+                                                   const KeccakRC &rc) {
+  constexpr int NB = LEN / 136 + 1;  // blocks after padding
+  constexpr int LB = 136 * (NB - 1);  // first byte of the last block
+  uint64_t A[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) A[i] = 0;
+  for (int b = 0; b < NB - 1; ++b) {  // whole blocks of message bytes
+#pragma unroll
+    for (int w = 0; w < 17; ++w) {
+      uint64_t v = msg[17 * b + w];
+      if (w == 0 && b == 0) v = (v & ~0xFFull) | domain;
+      A[w] ^= v;
+    }
+    keccak_f1600(A, rc);
+  }
+#pragma unroll
+  for (int w = 0; w < 17; ++w) {  // the last block: message tail, 0x1F, zeros, 0x80
+    const int b0 = LB + 8 * w;
+    uint64_t v = 0;
+    if (b0 < LEN) {
+      v = msg[17 * (NB - 1) + w];
+      if (b0 + 8 > LEN) v &= (1ull << (8 * (LEN - b0))) - 1ull;
+    }
+    if (b0 == 0) v = (v & ~0xFFull) | domain;
+    if (b0 <= LEN && LEN < b0 + 8) v |= 0x1Full << (8 * (LEN - b0));
+    if (w == 16) v |= 0x80ull << 56;
+    A[w] ^= v;
+  }
+  keccak_f1600(A, rc);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) out[i] = A[i];
+}
+
 // ------------------------------------------------------------------ GF(2^m) through LDS tables
 // lg[a] = discrete log of a (lg[0] unused), ex[i] = g^i for 0 <= i < q, q = 2^m - 1.
 struct GfLds {

@@ -15,8 +15,10 @@
 #include "kem_kernels.h"
 
 struct tmce_key {
-  int set = -1;
-  int kind = 0;  // 1 = public (encapsulation), 2 = secret (decapsulation)
+  int set = -1;      // base parameter set (0..4)
+  int variant = -1;  // set | VAR_F | VAR_PC as given at creation
+  bool pc = false;   // plaintext confirmation: ciphertexts are SYND_BYTES + 32 bytes
+  int kind = 0;      // 1 = public (encapsulation), 2 = secret (decapsulation)
   size_t bytes = 0;
   bool pad_bad = false;
   uint32_t *cols = nullptr;
@@ -113,7 +115,7 @@ struct Engine {
   DevBuf ind, msg_enc, queue, p_erow_enc, p_ens, p_s, xs_synd;
   ScratchUse use_enc;
   // decapsulation scratch
-  DevBuf S, msg_dec, dqueue, xs_dec, p_loc, p_erow_dec, p_weight;
+  DevBuf S, msg_dec, dqueue, xs_dec, p_loc, p_erow_dec, p_weight, ct_synd;
   ScratchUse use_dec;
   int form = 0;                                // 0 = engine form, 1 = plain per-stage form
   int synd_cfg[NUM_SETS] = {1, 1, 1, 1, 1};    // MFMA syndrome tile shape per set (tunable)
@@ -140,5 +142,7 @@ int engine_init(int device);
 Engine *engine();
 
 inline bool valid_set(int set) { return set >= 0 && set < NUM_SETS; }
+// ciphertext bytes of a key's variant
+inline size_t key_ct_bytes(const tmce_key *k) { return (size_t)set_info(k->set).synd_bytes + (k->pc ? 32 : 0); }
 
 }  // namespace tmce

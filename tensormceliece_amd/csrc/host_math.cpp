@@ -422,9 +422,55 @@ static bool min_poly(const HostField &F, const uint16_t *f, uint16_t *g) {
   return true;
 }
 
-// Systematic form of the parity-check matrix; false when the left mt x mt block is singular.
+// The f variants' column step (LM:crypto_kem/<set>f/vec/pk_gen.c mov_columns :87-153): at column
+// c0 = mt - 32 the rows c0..mt-1 are zero left of c0; the pivot columns pv[] of their 32 x 64
+// window (columns c0..c0+63) in row-echelon form -- which depend only on the rows' span -- give the
+// pivot mask and the swaps c0+i <-> c0+pv[i], i = 0..31 in order, of the matrix's columns.
+// Returns false when the window has rank < 32.
 template <int SET>
-static bool systematic_pk(const HostField &F, const uint16_t *g, const uint16_t *alpha, uint8_t *pk) {
+static bool f_column_step(std::vector<uint64_t> &H, uint64_t *pivots, int pv[32]) {
+  using P = Par<SET>;
+  constexpr int W = ceil_div(P::N, 64), C0 = P::MT - 32, W0 = C0 / 64, SH = C0 % 64;
+  auto window = [&](int r) -> uint64_t {
+    const uint64_t *row = &H[(size_t)r * W];
+    return SH ? (row[W0] >> SH) | (row[W0 + 1] << (64 - SH)) : row[W0];
+  };
+  uint64_t b[32];
+  for (int i = 0; i < 32; ++i) b[i] = window(C0 + i);
+  *pivots = 0;
+  for (int i = 0; i < 32; ++i) {
+    int best = 64, who = -1;
+    for (int j = i; j < 32; ++j)
+      if (b[j] && __builtin_ctzll(b[j]) < best) { best = __builtin_ctzll(b[j]); who = j; }
+    if (who < 0) return false;
+    std::swap(b[i], b[who]);
+    for (int j = i + 1; j < 32; ++j)
+      if ((b[j] >> best) & 1) b[j] ^= b[i];
+    pv[i] = best;
+    *pivots |= 1ull << best;
+  }
+  for (int r = 0; r < P::MT; ++r) {
+    uint64_t t = window(r);
+    for (int i = 0; i < 32; ++i) {
+      const uint64_t d = ((t >> i) ^ (t >> pv[i])) & 1;
+      t ^= (d << i) | (d << pv[i]);
+    }
+    uint64_t *row = &H[(size_t)r * W];
+    if (SH) {
+      row[W0] = (row[W0] & ((1ull << SH) - 1)) | (t << SH);
+      row[W0 + 1] = (row[W0 + 1] & ~((1ull << SH) - 1)) | (t >> (64 - SH));
+    } else {
+      row[W0] = t;
+    }
+  }
+  return true;
+}
+
+// Systematic form of the parity-check matrix; false when the left mt x mt block is singular.  With
+// `pivots` (the f variants) the column step above runs at column mt - 32 and sets *pivots.
+template <int SET>
+static bool systematic_pk(const HostField &F, const uint16_t *g, const uint16_t *alpha, uint8_t *pk,
+                          uint64_t *pivots = nullptr) {
   using P = Par<SET>;
   constexpr int W = ceil_div(P::N, 64);
   std::vector<uint64_t> H((size_t)P::MT * W, 0);
@@ -439,6 +485,10 @@ static bool systematic_pk(const HostField &F, const uint16_t *g, const uint16_t 
     }
   }
   for (int c = 0; c < P::MT; ++c) {
+    if (pivots && c == P::MT - 32) {
+      int pv[32];
+      if (!f_column_step<SET>(H, pivots, pv)) return false;
+    }
     const int cw = c / 64;
     const uint64_t cm = 1ull << (c % 64);
     int piv = c;
@@ -502,11 +552,26 @@ static void candidate_t(uint8_t seed33[33], KeygenCandidate &c) {
 }
 
 template <int SET>
+static void apply_pivots_t(KeygenCandidate &c, uint64_t pivots) {
+  using P = Par<SET>;
+  constexpr int C0 = P::MT - 32;
+  int i = 0;
+  for (int b = 0; b < 64; ++b)
+    if ((pivots >> b) & 1) {
+      if (b != i) {
+        std::swap(c.pi[C0 + i], c.pi[C0 + b]);
+        std::swap(c.alpha[C0 + i], c.alpha[C0 + b]);
+      }
+      ++i;
+    }
+  c.pivots = pivots;
+}
+
+template <int SET>
 static void finish_t(const KeygenCandidate &c, uint8_t *sk) {
   using P = Par<SET>;
   std::memcpy(sk, c.sk_seed, 32);
-  const uint8_t piv[8] = {0xFF, 0xFF, 0xFF, 0xFF, 0, 0, 0, 0};
-  std::memcpy(sk + 32, piv, 8);
+  for (int i = 0; i < 8; ++i) sk[32 + i] = (uint8_t)(c.pivots >> (8 * i));  // little-endian
   uint8_t *skp = sk + 40;
   for (int i = 0; i < P::T; ++i) { skp[2 * i] = (uint8_t)c.g[i]; skp[2 * i + 1] = (uint8_t)(c.g[i] >> 8); }
   controlbits_from_perm(P::M, c.pi, skp + P::IRR_BYTES);
@@ -514,17 +579,19 @@ static void finish_t(const KeygenCandidate &c, uint8_t *sk) {
 }
 
 template <int SET>
-static void keypair_t(uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx) {
+static void keypair_t(uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx, bool f) {
   using P = Par<SET>;
   const HostField &F = host_field(P::M);
   uint8_t seed[33];
   seed[0] = 64;
   rnd(seed + 1, 32, ctx);
   KeygenCandidate c;
+  uint64_t pivots = 0;
   for (;;) {
     candidate_t<SET>(seed, c);
-    if (systematic_pk<SET>(F, c.g.data(), c.alpha.data(), pk)) break;
+    if (systematic_pk<SET>(F, c.g.data(), c.alpha.data(), pk, f ? &pivots : nullptr)) break;
   }
+  if (f) apply_pivots_t<SET>(c, pivots);
   finish_t<SET>(c, sk);
 }
 
@@ -550,13 +617,25 @@ void keygen_finish(int set, const KeygenCandidate &c, uint8_t *sk) {
   throw std::invalid_argument("set");
 }
 
-void host_keypair(int set, uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx) {
+void keygen_apply_pivots(int set, KeygenCandidate &c, uint64_t pivots) {
   switch (set) {
-    case S348864: return keypair_t<S348864>(pk, sk, rnd, ctx);
-    case S460896: return keypair_t<S460896>(pk, sk, rnd, ctx);
-    case S6688128: return keypair_t<S6688128>(pk, sk, rnd, ctx);
-    case S6960119: return keypair_t<S6960119>(pk, sk, rnd, ctx);
-    case S8192128: return keypair_t<S8192128>(pk, sk, rnd, ctx);
+    case S348864: return apply_pivots_t<S348864>(c, pivots);
+    case S460896: return apply_pivots_t<S460896>(c, pivots);
+    case S6688128: return apply_pivots_t<S6688128>(c, pivots);
+    case S6960119: return apply_pivots_t<S6960119>(c, pivots);
+    case S8192128: return apply_pivots_t<S8192128>(c, pivots);
+  }
+  throw std::invalid_argument("set");
+}
+
+void host_keypair(int variant, uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx) {
+  const bool f = variant_f(variant);
+  switch (variant_base(variant)) {
+    case S348864: return keypair_t<S348864>(pk, sk, rnd, ctx, f);
+    case S460896: return keypair_t<S460896>(pk, sk, rnd, ctx, f);
+    case S6688128: return keypair_t<S6688128>(pk, sk, rnd, ctx, f);
+    case S6960119: return keypair_t<S6960119>(pk, sk, rnd, ctx, f);
+    case S8192128: return keypair_t<S8192128>(pk, sk, rnd, ctx, f);
   }
   throw std::invalid_argument("set");
 }

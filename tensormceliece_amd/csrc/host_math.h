@@ -54,8 +54,10 @@ bool pk_padding_bad(int set, const uint8_t *pk);
 void build_enc_rows(int set, const uint8_t *pk, std::vector<uint32_t> &rows);
 
 // Host key generation with libmceliece's keypair semantics; rand(out,len) supplies randomness.
+// `variant` is a set id, | VAR_F for the semi-systematic form of the f variants (VAR_PC does not
+// change key generation).
 typedef void (*rand_fn)(uint8_t *out, long long len, void *ctx);
-void host_keypair(int set, uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx);
+void host_keypair(int variant, uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx);
 
 // The same key generation in two host steps around the systematic-form step (which the batched
 // generator runs on the GPU).  keygen_candidate advances seed33 (64 || 32-byte state) through
@@ -68,8 +70,13 @@ struct KeygenCandidate {
   std::vector<uint16_t> alpha;  // support, n field elements
   std::vector<uint16_t> pi;     // the permutation of 2^m elements behind the support
   std::vector<uint8_t> s;       // implicit-rejection string, n/8 bytes
+  uint64_t pivots = 0xFFFFFFFFull;  // sk bytes 32..39 (the f variants' pivot mask)
 };
 void keygen_candidate(int set, uint8_t seed33[33], KeygenCandidate &c);
+// the f variants' column step as the systematic form reported it (pivot mask over columns
+// mt-32 .. mt+31): swaps positions mt-32+i and mt-32+pv[i] of the support permutation, i = 0..31
+// in order, pv = the mask's set bits in increasing order
+void keygen_apply_pivots(int set, KeygenCandidate &c, uint64_t pivots);
 void keygen_finish(int set, const KeygenCandidate &c, uint8_t *sk);
 
 void shake256(uint8_t *out, long long outlen, const uint8_t *in, long long inlen);

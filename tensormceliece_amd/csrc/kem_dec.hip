@@ -9,7 +9,8 @@
 //                 all 2^m field elements with the values in registers, root gather at the support,
 //                 weight check, re-encryption check (syndrome of the found error recomputed from the
 //                 key's log tables), session-key message.
-//   k_hash_msg    one op per lane; K = SHAKE256(b || (b ? e : s) || c); 6960119 padding rule.
+//   k_hash_msg    one op per lane; K = SHAKE256(b || (b ? e : s) || c); 6960119 padding rule; the
+//                 pc variants' confirmation check.
 //
 // Semantics: LM:crypto_kem/<set>/vec/decrypt.c (decrypt :135-190, weight_check :101-116) and
 // kem_dec.c; 6960119 ciphertext padding LM:crypto_kem/6960119/vec/kem_dec.c (check_c_padding).
@@ -521,11 +522,15 @@ void k_dec_decode(DecArgs a) {
   }
 }
 
-template <int SET>
-__global__ __launch_bounds__(256) void k_hash_msg(const uint32_t *__restrict__ msg, const uint8_t *__restrict__ ct,
+// The pc variants (LM:crypto_kem/<set>pc/vec/kem_dec.c:43-59): a decoded op (b = 1) whose
+// SHAKE256(2 || e, 32) differs from the ciphertext's last 32 bytes is rejected as well (b = 0, s in
+// place of e), and the message carries the whole ciphertext: its last 32 bytes are appended here.
+template <int SET, bool PC>
+__global__ __launch_bounds__(256) void k_hash_msg(uint32_t *__restrict__ msg, const uint8_t *__restrict__ ct,
                                                   uint8_t *__restrict__ key, int *__restrict__ status, int nops,
-                                                  const PlanDev *__restrict__ plan) {
+                                                  const PlanDev *__restrict__ plan, const uint8_t *__restrict__ rej) {
   using P = Par<SET>;
+  constexpr int CTB = PC ? P::CT_BYTES_PC : P::SYND_BYTES;
   // every lane stays active through the hash (round constants via v_readlane)
   const KeccakRC rc;
   if (plan) nops = plan->total;
@@ -533,11 +538,45 @@ __global__ __launch_bounds__(256) void k_hash_msg(const uint32_t *__restrict__ m
   const int gop = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = gop < nops;
   const int op = live ? gop : nops - 1;
+  uint32_t *mrow = msg + (size_t)op * P::MSG_WORDS;
+  if constexpr (PC) {
+    const uint16_t *tag16 = reinterpret_cast<const uint16_t *>(ct + (size_t)op * CTB + P::SYND_BYTES);
+    uint32_t tw[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) tw[i] = (uint32_t)tag16[2 * i] | ((uint32_t)tag16[2 * i + 1] << 16);
+    uint64_t cf[4];
+    shake256_prefix_32<P::CONF_LEN>(reinterpret_cast<const uint64_t *>(mrow), 2u, cf, rc);
+    bool same = true;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) same = same && cf[i] == ((uint64_t)tw[2 * i] | ((uint64_t)tw[2 * i + 1] << 32));
+    if (live && (mrow[0] & 0xFFu) == 1u && !same) {  // confirmation failed: implicit rejection
+      if (plan) rej = plan->keys[plan_group(plan, blockIdx.x * blockDim.x)].rej;
+      uint8_t *m8 = reinterpret_cast<uint8_t *>(mrow);
+      m8[0] = 0;
+      for (int i = 0; i < P::N / 8; ++i) m8[1 + i] = rej[i];
+    }
+    // the tag at byte MSG_LEN, then SHAKE padding to the end of block MSG_BLOCKS_PC
+    constexpr int KB = P::MSG_LEN / 4, RS = 8 * (P::MSG_LEN % 4);
+    constexpr int PADW = P::MSG_LEN_PC / 4, PADS = 8 * (P::MSG_LEN_PC % 4);
+    const uint32_t keep = RS ? (mrow[KB] & ((1u << RS) - 1u)) : 0u;
+#pragma unroll
+    for (int k = KB; k < 34 * P::MSG_BLOCKS_PC; ++k) {
+      const int j = k - KB;
+      uint32_t v = 0;
+      if (j < 8) v |= tw[j] << RS;
+      if (RS != 0 && j >= 1 && j - 1 < 8) v |= tw[j - 1] >> ((32 - RS) & 31);
+      if (k == KB) v |= keep;
+      if (k == PADW) v |= 0x1Fu << PADS;
+      if (k == 34 * P::MSG_BLOCKS_PC - 1) v |= 0x80u << 24;
+      if (live) mrow[k] = v;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // this lane's message stores complete
+  }
   uint64_t out[4];
-  shake256_blocks_32(reinterpret_cast<const uint64_t *>(msg + (size_t)op * P::MSG_WORDS), P::MSG_BLOCKS, out, rc);
+  shake256_blocks_32(reinterpret_cast<const uint64_t *>(mrow), PC ? P::MSG_BLOCKS_PC : P::MSG_BLOCKS, out, rc);
   int st = 0;
   if (SET == S6960119) {
-    const uint8_t last = ct[(size_t)op * P::SYND_BYTES + P::SYND_BYTES - 1];
+    const uint8_t last = ct[(size_t)op * CTB + P::SYND_BYTES - 1];
     if (last >> (P::MT % 8)) {
       st = -1;
 #pragma unroll
@@ -551,10 +590,29 @@ __global__ __launch_bounds__(256) void k_hash_msg(const uint32_t *__restrict__ m
   if (status) status[op] = st;
 }
 
+// the syndrome part (first SYND_BYTES) of each pc ciphertext, packed densely for the front end and
+// the decoder; one thread per 16-bit half
 template <int SET>
-hipError_t launch_dec(const DecArgs &a, hipStream_t s) {
+__global__ void k_ct_synd(const uint8_t *__restrict__ ct, uint8_t *__restrict__ out, int nops) {
   using P = Par<SET>;
-  if (a.nops <= 0) return hipSuccess;
+  constexpr int H = P::SYND_BYTES / 2;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)nops * H) return;
+  const long op = i / H, h = i - op * H;
+  reinterpret_cast<uint16_t *>(out)[i] = reinterpret_cast<const uint16_t *>(ct + op * P::CT_BYTES_PC)[h];
+}
+
+template <int SET>
+hipError_t launch_dec(const DecArgs &args, hipStream_t s) {
+  using P = Par<SET>;
+  if (args.nops <= 0) return hipSuccess;
+  DecArgs a = args;  // the front end and decoder read the syndrome part of the ciphertexts
+  if (a.pc) {
+    if (!a.ct_synd) return hipErrorInvalidValue;
+    const long halves = (long)a.nops * (P::SYND_BYTES / 2);
+    k_ct_synd<SET><<<(unsigned)((halves + 255) / 256), 256, 0, s>>>(args.ct, a.ct_synd, a.nops);
+    a.ct = a.ct_synd;
+  }
   if (a.front_mfma) {
     const hipError_t e = launch_front_mfma_set(SET, a.ct, a.xscratch, a.Fsw, a.S, a.nops, a.front_cfg, a.plan, s);
     if (e != hipSuccess) return e;
@@ -571,8 +629,14 @@ hipError_t launch_dec(const DecArgs &a, hipStream_t s) {
     if (e != hipSuccess) return e;
   }
   TMCE_TIMED("dec_decode", s, k_dec_decode<SET><<<grid, DEC_WG, 0, s>>>(a));
-  TMCE_TIMED("dec_hash", s,
-             k_hash_msg<SET><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, a.ct, a.key, a.status, a.nops, a.plan));
+  if (a.pc)
+    TMCE_TIMED("dec_hash", s,
+               k_hash_msg<SET, true><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, args.ct, a.key, a.status, a.nops,
+                                                                           a.plan, a.rej));
+  else
+    TMCE_TIMED("dec_hash", s,
+               k_hash_msg<SET, false><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, a.ct, a.key, a.status, a.nops,
+                                                                            a.plan, a.rej));
   return hipGetLastError();
 }
 
@@ -584,7 +648,7 @@ hipError_t front_only(const DecArgs &a, hipStream_t s) {
 }
 template <int SET>
 hipError_t hash_only(const DecArgs &a, hipStream_t s) {
-  k_hash_msg<SET><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, a.ct, a.key, a.status, a.nops, nullptr);
+  k_hash_msg<SET, false><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, a.ct, a.key, a.status, a.nops, nullptr, a.rej);
   return hipGetLastError();
 }
 hipError_t launch_dec_front_only(int set, const DecArgs &a, hipStream_t s) {

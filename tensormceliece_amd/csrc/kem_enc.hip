@@ -7,7 +7,8 @@
 //              wave-cooperatively on an LDS bitmap; e and the support (in draw order -- only its
 //              set matters downstream) leave through coalesced stores.
 //   k_enc_finish one op per lane; c = XOR of t columns of [I | T] (sparse column gather from the
-//              device-resident, column-major public key; L2 resident), then K = SHAKE256(1||e||c).
+//              device-resident, column-major public key; L2 resident), then K = SHAKE256(1||e||c);
+//              the pc variants append C = SHAKE256(2||e) to c first.
 //
 // Semantics: LM:crypto_kem/<set>/vec/encrypt.c (gen_e :41-92, syndrome :94-132) and kem_enc.c,
 // with randombytes() replaced by the per-op seeded chain of the reference harness
@@ -403,7 +404,7 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
   }
 }
 
-template <int SET>
+template <int SET, bool PC>
 __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__ ind, const uint4 *__restrict__ cols,
                                                     uint32_t *__restrict__ msg, uint8_t *__restrict__ ct,
                                                     uint8_t *__restrict__ key, int *__restrict__ status, int nops,
@@ -449,43 +450,70 @@ __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__
       }
     }
   }
-  uint32_t cw[P::CWP];
+  // ---- the ciphertext as words: c (SYND_BYTES; the bits past MT are zero), and for the pc
+  // variants the confirmation C = SHAKE256(2 || e, 32) after it (LM:crypto_kem/<set>pc/vec/kem_enc.c:32)
+  constexpr int CTB = PC ? P::CT_BYTES_PC : P::SYND_BYTES;  // even for every set
+  constexpr int XW = PC ? ceil_div(CTB, 4) : P::CW;
+  uint32_t *mrow = msg + (size_t)op * P::MSG_WORDS;  // [1][e] written by the sampler
+  uint32_t cx[XW];
 #pragma unroll
-  for (int q = 0; q < C4; ++q) { cw[4 * q] = c[q].x; cw[4 * q + 1] = c[q].y; cw[4 * q + 2] = c[q].z; cw[4 * q + 3] = c[q].w; }
-
-  // ---- ciphertext bytes (SYND_BYTES is even for every set)
-  uint16_t *ctp = reinterpret_cast<uint16_t *>(ct + (size_t)op * P::SYND_BYTES);
-  if (live) {
+  for (int q = 0; q < C4; ++q) {
+    const uint32_t w4[4] = {c[q].x, c[q].y, c[q].z, c[q].w};
 #pragma unroll
-    for (int h = 0; h < P::SYND_BYTES / 2; ++h) ctp[h] = (uint16_t)(cw[h / 2] >> (16 * (h & 1)));
+    for (int i = 0; i < 4; ++i)
+      if (4 * q + i < XW && 4 * q + i < P::CW) cx[4 * q + i] = w4[i];
+  }
+  if constexpr (PC) {
+    uint64_t cf[4];
+    shake256_prefix_32<P::CONF_LEN>(reinterpret_cast<const uint64_t *>(mrow), 2u, cf, rc);
+    constexpr int SB4 = P::SYND_BYTES / 4, SO = P::SYND_BYTES % 4;  // SO is 0 or 2
+    static_assert(SO == 0 || SO == 2, "ciphertext length");
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const uint32_t t = (uint32_t)(cf[i / 2] >> (32 * (i & 1)));
+      if constexpr (SO == 0) {
+        cx[SB4 + i] = t;
+      } else {
+        cx[SB4 + i] = (i == 0 ? (cx[SB4] & 0xFFFFu) : cx[SB4 + i]) | (t << 16);
+        cx[SB4 + i + 1] = t >> 16;
+      }
+    }
   }
 
-  // ---- finish the message: c at byte 1+N/8, then SHAKE padding
-  uint32_t *mrow = msg + (size_t)op * P::MSG_WORDS;
+  // ---- ciphertext bytes
+  uint16_t *ctp = reinterpret_cast<uint16_t *>(ct + (size_t)op * CTB);
+  if (live) {
+#pragma unroll
+    for (int h = 0; h < CTB / 2; ++h) ctp[h] = (uint16_t)(cx[h / 2] >> (16 * (h & 1)));
+  }
+
+  // ---- finish the message: the ciphertext at byte 1+N/8, then SHAKE padding
+  constexpr int MLEN = PC ? P::MSG_LEN_PC : P::MSG_LEN;
+  constexpr int MBLK = PC ? P::MSG_BLOCKS_PC : P::MSG_BLOCKS;
   constexpr int OFF = 1 + P::N / 8;
   constexpr int KB = OFF / 4, RS = 8 * (OFF % 4);
-  constexpr int PADW = P::MSG_LEN / 4, PADS = 8 * (P::MSG_LEN % 4);
+  constexpr int PADW = MLEN / 4, PADS = 8 * (MLEN % 4);
   const uint32_t first = mrow[KB];
 #pragma unroll
-  for (int k = KB; k < P::MSG_WORDS; ++k) {
+  for (int k = KB; k < 34 * MBLK; ++k) {
     const int j = k - KB;
     uint32_t v = 0;
-    if (j < P::CW) v |= cw[j] << RS;
-    if (RS != 0 && j >= 1 && j - 1 < P::CW) v |= cw[j - 1] >> ((32 - RS) & 31);
+    if (j < XW) v |= cx[j] << RS;
+    if (RS != 0 && j >= 1 && j - 1 < XW) v |= cx[j - 1] >> ((32 - RS) & 31);
     if (k == KB) v |= first;
     if (k == PADW) v |= 0x1Fu << PADS;
-    if (k == P::MSG_WORDS - 1) v |= 0x80u << 24;
+    if (k == 34 * MBLK - 1) v |= 0x80u << 24;
     if (live) mrow[k] = v;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // this lane's message stores complete
   uint64_t out[4];
-  shake256_blocks_32(reinterpret_cast<const uint64_t *>(mrow), P::MSG_BLOCKS, out, rc);
+  shake256_blocks_32(reinterpret_cast<const uint64_t *>(mrow), MBLK, out, rc);
   uint64_t *kp = reinterpret_cast<uint64_t *>(key + 32 * (size_t)op);
   if (live) {
     const bool clear = abandoned || pad_bad;
 #pragma unroll
     for (int i = 0; i < 4; ++i) kp[i] = clear ? 0ull : out[i];
-    if (clear) for (int h = 0; h < P::SYND_BYTES / 2; ++h) ctp[h] = 0;
+    if (clear) for (int h = 0; h < CTB / 2; ++h) ctp[h] = 0;
     if (status) status[op] = pad_bad ? -1 : abandoned ? -2 : 0;
   }
 }
@@ -581,9 +609,14 @@ hipError_t launch_enc(const EncArgs &a, hipStream_t s) {
     TMCE_TIMED("gen_e", s,
                k_gen_e<SET><<<grid, GEN_WG, 0, s>>>(a.seeds, a.ind, a.msg, a.attempts, a.nops, a.queue, a.plan));
   }
-  TMCE_TIMED("enc_finish", s,
-             k_enc_finish<SET><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.ind, reinterpret_cast<const uint4 *>(a.cols),
-                                                                     a.msg, a.ct, a.key, a.status, a.nops, a.plan));
+  if (a.pc)
+    TMCE_TIMED("enc_finish", s,
+               k_enc_finish<SET, true><<<ceil_div(a.nops, 256), 256, 0, s>>>(
+                   a.ind, reinterpret_cast<const uint4 *>(a.cols), a.msg, a.ct, a.key, a.status, a.nops, a.plan));
+  else
+    TMCE_TIMED("enc_finish", s,
+               k_enc_finish<SET, false><<<ceil_div(a.nops, 256), 256, 0, s>>>(
+                   a.ind, reinterpret_cast<const uint4 *>(a.cols), a.msg, a.ct, a.key, a.status, a.nops, a.plan));
   return hipGetLastError();
 }
 

@@ -103,7 +103,7 @@ __device__ __forceinline__ uint32_t synd_value(const uint32_t *Srow, int j) {
 struct EncArgs {
   const uint8_t *seeds;  // [n][32]
   const uint32_t *cols;  // key: [N][CWP] column-major [I | T]
-  uint8_t *ct;           // [n][SYND_BYTES]
+  uint8_t *ct;           // [n][SYND_BYTES], + 32 bytes per op when pc
   uint8_t *key;          // [n][32]
   int *status;           // [n] 0, or -2 if the sampler gave up (optional)
   uint16_t *ind;         // scratch [n][IND_STRIDE] support of e
@@ -113,6 +113,7 @@ struct EncArgs {
   int nops;              // op count, or with a plan the capacity (the live count is plan->total)
   int gen_blocks;        // persistent grid size of the sampler (0 = default)
   int gen_form;          // sampler: 1 = pipelined lane groups (k_gen_e_pipe), 0 = one op per lane
+  int pc;                // plaintext confirmation (pc variants): ciphertext = c || SHAKE256(2 || e, 32)
   const PlanDev *plan;   // multi-batch plan (per-group keys; null: one key, `cols`)
 };
 
@@ -129,7 +130,9 @@ struct FieldTables {
 
 // Device buffers of one decapsulation batch.
 struct DecArgs {
-  const uint8_t *ct;     // [n][SYND_BYTES]
+  const uint8_t *ct;     // [n][SYND_BYTES], + 32 bytes per op when pc
+  int pc;                // plaintext confirmation (pc variants)
+  uint8_t *ct_synd;      // pc: scratch [n][SYND_BYTES], the syndrome part of each ciphertext
   const uint32_t *F;     // key: [R][KWP] front-end matrix (plane-major rows)
   const uint8_t *Fsw;    // key: F as 0/1 bytes in MFMA fragment order
   int front_mfma;        // front end on INT8 MFMA (1) or on the VALU (0)
@@ -208,6 +211,7 @@ struct KeygenArgs {
   uint64_t *H;            // scratch [ncand][pk_gen_h_bytes(set) / 8]
   uint8_t *pk;            // [ncand][PK_BYTES]
   int *ok;                // [ncand] 1: pk written, 0: the left mt x mt block is singular
+  uint64_t *pivots;       // f variants: [ncand] pivot mask of the column step (null otherwise)
   int ncand;
 };
 hipError_t launch_pk_gen_set(int set, const KeygenArgs &a, hipStream_t s);

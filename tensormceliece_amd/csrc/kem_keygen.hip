@@ -9,6 +9,14 @@
 // uniformly.  The reduced form of a full-rank matrix is unique, so the bytes do not depend on the
 // pivot order, and the step fails exactly when the left mt x mt block is singular -- the outcome
 // of LM:crypto_kem/<set>/vec/pk_gen.c for the same candidate.
+//
+// The f variants (LM:crypto_kem/<set>f/vec/pk_gen.c, mov_columns :87-153 called at row mt-32
+// :251-255) add a column step when the elimination reaches column c0 = mt - 32: the 32 rows still
+// without a pivot are zero left of c0, and the pivot columns pv[0..32) of their 32 x 64 window
+// (columns c0 .. c0+63) in row-echelon form -- pv[i] the lowest column where any row not yet used
+// is nonzero; these depend only on the rows' span -- give the pivot mask and the column swaps
+// c0+i <-> c0+pv[i] (i = 0..31 in order) applied to every row of H.  The elimination then finishes
+// on the permuted matrix; the host applies the same swaps to the support permutation.
 #include "kem_kernels.h"
 
 namespace tmce {
@@ -16,12 +24,12 @@ namespace tmce {
 constexpr int KG_WG = 512;
 constexpr int KG_WAVES = KG_WG / 64;
 
-template <int SET>
+template <int SET, bool F>
 __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g_all,
                                                  const uint16_t *__restrict__ alpha_all,
                                                  const uint16_t *__restrict__ lg, const uint16_t *__restrict__ ex,
                                                  uint64_t *__restrict__ H_all, uint8_t *__restrict__ pk_all,
-                                                 int *__restrict__ ok_all) {
+                                                 int *__restrict__ ok_all, uint64_t *__restrict__ piv_all) {
   using P = Par<SET>;
   constexpr int M = P::M, T = P::T, N = P::N, MT = P::MT, Q = P::Q;
   constexpr int W = ceil_div(N, 64);  // 64-bit words per row of H
@@ -31,6 +39,7 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
   __shared__ uint16_t rowix[MT];     // logical row -> physical row of H
   __shared__ uint8_t flag[MT];       // physical rows the current step updates
   __shared__ int red[KG_WAVES];
+  __shared__ int pvs[32];            // f variants: pivot columns of the window (relative to mt - 32)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cand = blockIdx.x;
   const uint16_t *g = g_all + (size_t)cand * (T + 1);
@@ -64,6 +73,60 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
 
   // ---- Gauss-Jordan elimination to [I | T]
   for (int c = 0; c < MT; ++c) {
+    if constexpr (F) {
+      if (c == MT - 32) {  // the f variants' column step (uniform branch)
+        constexpr int C0 = MT - 32, W0 = C0 >> 6, SH = C0 & 63;
+        auto window = [&](int prow) -> uint64_t {
+          const uint64_t *row = H + (size_t)prow * W;
+          return SH ? (row[W0] >> SH) | (row[W0 + 1] << (64 - SH)) : row[W0];
+        };
+        if (wave == 0) {  // echelon pivots of the 32 rows' windows, lane i holding row i
+          uint64_t b = lane < 32 ? window(rowix[C0 + lane]) : 0ull;
+          bool full = true;
+          for (int i = 0; i < 32; ++i) {
+            const int z = (lane >= i && lane < 32 && b) ? __builtin_ctzll(b) : 64;
+            int m = z;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o));
+            if (m == 64) { full = false; break; }  // uniform
+            const uint64_t who = __ballot(z == m);
+            const int wl = __builtin_ctzll(who);  // a row with the lowest leading column
+            const uint64_t bw = __shfl(b, wl), bi = __shfl(b, i);
+            if (lane == i) b = bw;
+            else if (lane == wl) b = bi;
+            if (lane > i && lane < 32 && ((b >> m) & 1ull)) b ^= bw;
+            if (lane == 0) pvs[i] = m;
+          }
+          if (lane == 0) red[0] = full ? 1 : 0;
+        }
+        __syncthreads();
+        if (red[0] == 0) {  // rank < 32 (uniform over the workgroup)
+          if (tid == 0) ok_all[cand] = 0;
+          return;
+        }
+        uint64_t mask = 0;
+        int pv[32];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) { pv[i] = pvs[i]; mask |= 1ull << pv[i]; }
+        for (int r = tid; r < MT; r += KG_WG) {  // the swaps on every physical row
+          uint64_t t = window(r);
+#pragma unroll
+          for (int i = 0; i < 32; ++i) {
+            const uint64_t d = ((t >> i) ^ (t >> pv[i])) & 1ull;
+            t ^= (d << i) | (d << pv[i]);
+          }
+          uint64_t *row = H + (size_t)r * W;
+          if (SH) {
+            row[W0] = (row[W0] & ((1ull << SH) - 1ull)) | (t << SH);
+            row[W0 + 1] = (row[W0 + 1] & ~((1ull << SH) - 1ull)) | (t >> (64 - SH));
+          } else {
+            row[W0] = t;
+          }
+        }
+        if (tid == 0) piv_all[cand] = mask;
+        __syncthreads();
+      }
+    }
     const int cw = c >> 6;
     const uint64_t cm = 1ull << (c & 63);
     // pivot: the first logical row at or below c with bit c set (the thread's rows loaded together)
@@ -142,7 +205,10 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
 template <int SET>
 hipError_t pk_gen(const KeygenArgs &a, hipStream_t s) {
   if (a.ncand <= 0) return hipSuccess;
-  k_pk_gen<SET><<<a.ncand, KG_WG, 0, s>>>(a.g, a.alpha, a.lg, a.ex, a.H, a.pk, a.ok);
+  if (a.pivots)
+    k_pk_gen<SET, true><<<a.ncand, KG_WG, 0, s>>>(a.g, a.alpha, a.lg, a.ex, a.H, a.pk, a.ok, a.pivots);
+  else
+    k_pk_gen<SET, false><<<a.ncand, KG_WG, 0, s>>>(a.g, a.alpha, a.lg, a.ex, a.H, a.pk, a.ok, nullptr);
   return hipGetLastError();
 }
 

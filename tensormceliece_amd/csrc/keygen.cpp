@@ -5,7 +5,9 @@
 // (keygen_finish: Benes control bits, rejection string).  The systematic-form step of every pending
 // candidate runs on the GPU, one workgroup per candidate (k_pk_gen).  A key whose candidate is
 // singular continues its seed chain in the next round, as libmceliece's keypair loop does, so key
-// i is byte-identical to tmce_keypair_seeded on seeds[32 i].
+// i is byte-identical to tmce_keypair_seeded on seeds[32 i].  For the f variants the kernel also
+// runs the semi-systematic column step and reports its pivot mask, which the host applies to the
+// candidate's support permutation before the control bits.
 #include <hip/hip_runtime.h>
 #include <sched.h>
 
@@ -44,6 +46,7 @@ struct KgBuffers {
   uint64_t *H = nullptr;
   uint8_t *pk = nullptr;
   int *ok = nullptr;
+  uint64_t *piv = nullptr;
   hipStream_t s = nullptr;
   ~KgBuffers() {
     if (s) (void)hipStreamSynchronize(s);
@@ -52,6 +55,7 @@ struct KgBuffers {
     (void)hipFree(H);
     (void)hipFree(pk);
     (void)hipFree(ok);
+    (void)hipFree(piv);
     if (s) (void)hipStreamDestroy(s);
   }
 };
@@ -67,9 +71,12 @@ constexpr size_t KG_AHEAD_MIN = 2, KG_AHEAD_MAX = 8;
 
 }  // namespace
 
-int tmce_keypair_batch(int set, unsigned char *pk, unsigned char *sk, const unsigned char *seeds, size_t n) {
-  if (!valid_set(set) || (n && (!pk || !sk || !seeds))) return fail(TMCE_EINVAL, "tmce_keypair_batch: bad argument");
+int tmce_keypair_batch(int variant, unsigned char *pk, unsigned char *sk, const unsigned char *seeds, size_t n) {
+  if (!variant_valid(variant) || (n && (!pk || !sk || !seeds)))
+    return fail(TMCE_EINVAL, "tmce_keypair_batch: bad argument");
   if (n == 0) return 0;
+  const int set = variant_base(variant);
+  const bool f = variant_f(variant);
   Engine *e = engine();
   if (!e) return TMCE_ENODEV;
   const SetInfo &si = set_info(set);
@@ -93,16 +100,19 @@ int tmce_keypair_batch(int set, unsigned char *pk, unsigned char *sk, const unsi
   HIPCHK(hipStreamCreateWithFlags(&b.s, hipStreamNonBlocking));
   if (hipMalloc((void **)&b.g, cap * T1 * 2) != hipSuccess || hipMalloc((void **)&b.alpha, cap * N * 2) != hipSuccess ||
       hipMalloc((void **)&b.H, cap * hw * 8) != hipSuccess || hipMalloc((void **)&b.pk, cap * PKB) != hipSuccess ||
-      hipMalloc((void **)&b.ok, cap * sizeof(int)) != hipSuccess)
+      hipMalloc((void **)&b.ok, cap * sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&b.piv, cap * sizeof(uint64_t)) != hipSuccess)
     return fail(TMCE_ENOMEM, "hipMalloc key generation buffers");
 
   // cand[i][k]: attempt k of key i in this round
   std::vector<std::vector<KeygenCandidate>> cand(n);
   std::vector<int> win(n, -1);
+  std::vector<uint64_t> win_piv(n, 0);
   std::vector<size_t> pending(n);
   for (size_t i = 0; i < n; ++i) pending[i] = i;
   std::vector<uint16_t> hg(cap * T1), ha(cap * N);
   std::vector<int> hok(cap);
+  std::vector<uint64_t> hpiv(cap);
   while (!pending.empty()) {
     const size_t ahead = std::max(KG_AHEAD_MIN, std::min(KG_AHEAD_MAX, KG_LAUNCH / pending.size()));
     parallel_for(pending.size(), [&](size_t k) {  // a key's attempts follow its chain, in order
@@ -122,9 +132,10 @@ int tmce_keypair_batch(int set, unsigned char *pk, unsigned char *sk, const unsi
       }
       HIPCHK(hipMemcpyAsync(b.g, hg.data(), nc * T1 * 2, hipMemcpyHostToDevice, b.s));
       HIPCHK(hipMemcpyAsync(b.alpha, ha.data(), nc * N * 2, hipMemcpyHostToDevice, b.s));
-      KeygenArgs a{b.g, b.alpha, e->lg[w], e->ex[w], b.H, b.pk, b.ok, (int)nc};
+      KeygenArgs a{b.g, b.alpha, e->lg[w], e->ex[w], b.H, b.pk, b.ok, f ? b.piv : nullptr, (int)nc};
       HIPCHK(launch_pk_gen_set(set, a, b.s));
       HIPCHK(hipMemcpyAsync(hok.data(), b.ok, nc * sizeof(int), hipMemcpyDeviceToHost, b.s));
+      if (f) HIPCHK(hipMemcpyAsync(hpiv.data(), b.piv, nc * sizeof(uint64_t), hipMemcpyDeviceToHost, b.s));
       HIPCHK(hipStreamSynchronize(b.s));
       // the first successful attempt of each key in this launch (earlier launches hold earlier attempts)
       for (size_t k = 0; k < nc; ++k) {
@@ -134,6 +145,7 @@ int tmce_keypair_batch(int set, unsigned char *pk, unsigned char *sk, const unsi
         for (size_t c2 = c - c % ahead; c2 < c && first; ++c2) first = !okv[c2];
         if (first) {
           win[i] = (int)(c % ahead);
+          win_piv[i] = hpiv[k];
           HIPCHK(hipMemcpyAsync(pk + i * PKB, b.pk + k * PKB, PKB, hipMemcpyDeviceToHost, b.s));
         }
       }
@@ -143,6 +155,7 @@ int tmce_keypair_batch(int set, unsigned char *pk, unsigned char *sk, const unsi
     for (size_t i : pending) (win[i] >= 0 ? done : failed).push_back(i);
     parallel_for(done.size(), [&](size_t k) {
       const size_t i = done[k];
+      if (f) keygen_apply_pivots(set, cand[i][win[i]], win_piv[i]);
       keygen_finish(set, cand[i][win[i]], sk + i * SKB);
       cand[i].clear();
     });

@@ -49,10 +49,17 @@ template <int SET> struct Par : SetBase<SET> {
   // the public key T otherwise; CWP 32-bit words per column (multiple of 4 for 16-byte loads)
   static constexpr int CW = ceil_div(MT, 32);
   static constexpr int CWP = round_up(CW, 4);
-  // message buffer of the session-key hash: [prefix][N/8 bytes][SYND_BYTES][SHAKE padding]
+  // message buffer of the session-key hash: [prefix][N/8 bytes][ciphertext][SHAKE padding]; the
+  // ciphertext is SYND_BYTES long, or SYND_BYTES + 32 with plaintext confirmation (the pc
+  // variants, LM:crypto_kem/<set>pc/vec/kem_enc.c), and rows are sized for the longer message
   static constexpr int MSG_LEN = 1 + N / 8 + SYND_BYTES;
   static constexpr int MSG_BLOCKS = MSG_LEN / 136 + 1;
-  static constexpr int MSG_WORDS = MSG_BLOCKS * 34;
+  static constexpr int CT_BYTES_PC = SYND_BYTES + 32;
+  static constexpr int MSG_LEN_PC = MSG_LEN + 32;
+  static constexpr int MSG_BLOCKS_PC = MSG_LEN_PC / 136 + 1;
+  static constexpr int MSG_WORDS = MSG_BLOCKS_PC * 34;
+  // the pc variants' confirmation hash SHAKE256(2 || e) covers CONF_LEN bytes
+  static constexpr int CONF_LEN = 1 + N / 8;
   // decapsulation front end: x = ciphertext bits (KBITS, incl. padding bits of the last byte);
   // output rows r = j*M + b hold bit b of syndrome j (syndrome-major; j < TT = 2t rounded up to 32,
   // rows of j >= 2t are zero)
@@ -74,11 +81,13 @@ struct SetInfo {
   const char *name;
   int m, n, t, mt, k, pk_row_bytes, synd_bytes, cond_bytes;
   long pk_bytes, sk_bytes;
+  int msg_words;  // 32-bit words per session-key message row
 };
 
 template <int SET> constexpr SetInfo make_info(const char *name) {
   using P = Par<SET>;
-  return SetInfo{name, P::M, P::N, P::T, P::MT, P::K, P::PK_ROW_BYTES, P::SYND_BYTES, P::COND_BYTES, P::PK_BYTES, P::SK_BYTES};
+  return SetInfo{name, P::M, P::N, P::T, P::MT, P::K, P::PK_ROW_BYTES, P::SYND_BYTES, P::COND_BYTES, P::PK_BYTES,
+                 P::SK_BYTES, P::MSG_WORDS};
 }
 
 inline const SetInfo &set_info(int set) {
@@ -87,5 +96,15 @@ inline const SetInfo &set_info(int set) {
       make_info<S6960119>("6960119"), make_info<S8192128>("8192128")};
   return tab[set];
 }
+
+// Variant ids (TMCE_VARIANT_F / TMCE_VARIANT_PC of the C ABI): a base set | VAR_F (semi-systematic
+// key generation, LM:crypto_kem/<set>f) | VAR_PC (plaintext confirmation, LM:crypto_kem/<set>pc).
+// Encapsulation and decapsulation of an f variant are those of its base set.
+constexpr int VAR_F = 8, VAR_PC = 16;
+inline bool variant_valid(int v) { return v >= 0 && (v & 7) < NUM_SETS && (v & ~(7 | VAR_F | VAR_PC)) == 0; }
+inline int variant_base(int v) { return v & 7; }
+inline bool variant_f(int v) { return (v & VAR_F) != 0; }
+inline bool variant_pc(int v) { return (v & VAR_PC) != 0; }
+inline int variant_ct_bytes(int v) { return set_info(variant_base(v)).synd_bytes + (variant_pc(v) ? 32 : 0); }
 
 }  // namespace tmce

@@ -20,6 +20,8 @@
 
 struct tmce_pipeline {
   int set = -1, dir = 0;  // dir: 0 = encapsulation, 1 = decapsulation
+  bool pc = false;        // plaintext-confirmation variant: ciphertexts are SYND_BYTES + 32 bytes
+  int ctb = 0;            // ciphertext bytes
   int cap = 0, max_sub = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false, use_graph = false;
@@ -27,7 +29,7 @@ struct tmce_pipeline {
   tmce::PlanDev *plan_dev = nullptr;
   size_t plan_bytes = 0;
   hipEvent_t uploaded = nullptr;  // the pinned image may be rewritten once this has completed
-  tmce::DevBuf in, ct, key, status, ind, msg, S, queue, xscratch;
+  tmce::DevBuf in, ct, key, status, ind, msg, S, queue, xscratch, ct_synd;
   size_t dev_bytes = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -41,8 +43,6 @@ namespace tmce {
 namespace {
 
 size_t plan_image_bytes(int max_sub) { return round_up((int)sizeof(PlanDev), 16) + (size_t)max_sub * sizeof(SubDev); }
-
-int msg_words(const SetInfo &si) { return (1 + si.n / 8 + si.synd_bytes) / 136 * 34 + 34; }
 
 // the stage launches of one run (everything after the plan upload)
 int issue(tmce_pipeline *p, const int knobs[5]) {
@@ -62,14 +62,17 @@ int issue(tmce_pipeline *p, const int knobs[5]) {
     a.nops = p->cap;
     a.gen_blocks = knobs[0];
     a.gen_form = knobs[4];
+    a.pc = p->pc ? 1 : 0;
     a.plan = p->plan_dev;
     HIPCHK(launch_enc_set(p->set, a, s));
-    HIPCHK(launch_plan_scatter(p->plan_dev, p->ct.as<uint8_t>(), si.synd_bytes, p->key.as<uint8_t>(),
+    HIPCHK(launch_plan_scatter(p->plan_dev, p->ct.as<uint8_t>(), p->ctb, p->key.as<uint8_t>(),
                                p->status.as<int>(), p->cap, s));
   } else {
-    HIPCHK(launch_plan_gather(p->plan_dev, p->in.as<uint8_t>(), si.synd_bytes, p->cap, s));
+    HIPCHK(launch_plan_gather(p->plan_dev, p->in.as<uint8_t>(), p->ctb, p->cap, s));
     DecArgs a{};
     a.ct = p->in.as<uint8_t>();
+    a.pc = p->pc ? 1 : 0;
+    a.ct_synd = p->pc ? p->ct_synd.as<uint8_t>() : nullptr;
     const int w = si.m - 12;
     const SetTables &tb = e->st[p->set];
     a.ft = FieldTables{e->lg[w], e->ex[w], tb.twl, tb.plo, tb.phi, tb.ptlo, tb.pthi};
@@ -98,7 +101,7 @@ void destroy(tmce_pipeline *p) {
   if (p->uploaded) (void)hipEventDestroy(p->uploaded);
   if (p->plan_host) (void)hipHostFree(p->plan_host);
   if (p->plan_dev) (void)hipFree(p->plan_dev);
-  for (DevBuf *b : {&p->in, &p->ct, &p->key, &p->status, &p->ind, &p->msg, &p->S, &p->queue, &p->xscratch})
+  for (DevBuf *b : {&p->in, &p->ct, &p->key, &p->status, &p->ind, &p->msg, &p->S, &p->queue, &p->xscratch, &p->ct_synd})
     if (b->p) (void)hipFree(b->p);
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
@@ -112,16 +115,19 @@ using namespace tmce;
 
 extern "C" {
 
-int tmce_pipeline_create(int set, int direction, size_t capacity, size_t max_sub_batches, int use_graph, void *stream,
-                         tmce_pipeline **out) {
-  if (!valid_set(set) || (direction != 0 && direction != 1) || !out || capacity == 0 || max_sub_batches == 0)
+int tmce_pipeline_create(int variant, int direction, size_t capacity, size_t max_sub_batches, int use_graph,
+                         void *stream, tmce_pipeline **out) {
+  if (!variant_valid(variant) || (direction != 0 && direction != 1) || !out || capacity == 0 || max_sub_batches == 0)
     return fail(TMCE_EINVAL, "tmce_pipeline_create: bad argument");
+  const int set = variant_base(variant);
   if (capacity > (size_t)(1 << 28) || max_sub_batches > (size_t)(1 << 24))
     return fail(TMCE_EINVAL, "tmce_pipeline_create: capacity too large");
   Engine *e = engine();
   if (!e) return TMCE_ENODEV;
   auto *p = new tmce_pipeline();
   p->set = set;
+  p->pc = variant_pc(variant);
+  p->ctb = variant_ct_bytes(variant);
   p->dir = direction;
   p->cap = round_up((int)capacity, PLAN_ALIGN);
   p->max_sub = (int)max_sub_batches;
@@ -148,14 +154,15 @@ int tmce_pipeline_create(int set, int direction, size_t capacity, size_t max_sub
   const int ind_stride = (si.t + 1) & ~1;
   const int rw = si.m * (((2 * si.t + 31) / 32) * 32) / 32;
   struct Need { DevBuf *b; size_t bytes; };
-  std::vector<Need> need = {{&p->key, n * 32}, {&p->status, n * 4}, {&p->msg, n * msg_words(si) * 4}};
+  std::vector<Need> need = {{&p->key, n * 32}, {&p->status, n * 4}, {&p->msg, n * si.msg_words * 4}};
   if (direction == 0) {
     need.push_back({&p->in, n * 32});
-    need.push_back({&p->ct, n * si.synd_bytes});
+    need.push_back({&p->ct, n * p->ctb});
     need.push_back({&p->ind, n * ind_stride * 2});
     need.push_back({&p->queue, 64});
   } else {
-    need.push_back({&p->in, n * si.synd_bytes});
+    need.push_back({&p->in, n * p->ctb});
+    if (p->pc) need.push_back({&p->ct_synd, n * si.synd_bytes});
     need.push_back({&p->S, n * rw * 4});
     need.push_back({&p->queue, 1024});
     const size_t xb = gf2_scratch_bytes(set, GF2_FRONT, n);
@@ -192,8 +199,8 @@ int tmce_pipeline_run(tmce_pipeline *p, const tmce_sub_batch *subs, size_t nsub)
   std::vector<long> gcount;
   for (size_t i = 0; i < nsub; ++i) {
     const tmce_sub_batch &b = subs[i];
-    if (!b.key || b.key->set != p->set || b.key->kind != want_kind)
-      return fail(TMCE_EINVAL, "tmce_pipeline_run: sub-batch key of the wrong set or kind");
+    if (!b.key || b.key->set != p->set || b.key->pc != p->pc || b.key->kind != want_kind)
+      return fail(TMCE_EINVAL, "tmce_pipeline_run: sub-batch key of the wrong set, variant or kind");
     if (b.n && (!b.in || !b.k || (p->dir == 0 && !b.ct)))
       return fail(TMCE_EINVAL, "tmce_pipeline_run: sub-batch without buffers");
     if (((uintptr_t)b.k & 15) || ((uintptr_t)b.status & 3) || ((uintptr_t)b.in & 1) || ((uintptr_t)b.ct & 1))

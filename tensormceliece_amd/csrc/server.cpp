@@ -58,7 +58,9 @@ struct Slot {
 }  // namespace
 
 struct tmce_server {
-  int set = -1;
+  int set = -1;      // base parameter set
+  bool pc = false;   // plaintext-confirmation variant
+  size_t ctb = 0;    // ciphertext bytes
   size_t budget = 0, in_flight = 0, peak_in_flight = 0;
   std::vector<Slot> slots;
   std::deque<Req *> pending[2];
@@ -137,8 +139,7 @@ int launch_slot(tmce_server *sv, Slot &sl, std::vector<Req *> &batch) {
 }
 
 void finish_slot(tmce_server *sv, Slot &sl) {
-  const SetInfo &si = set_info(sv->set);
-  const size_t out_ct = sl.dir == 0 ? (size_t)si.synd_bytes : 0;
+  const size_t out_ct = sl.dir == 0 ? sv->ctb : 0;
   size_t pos = 0;
   const double t = now_s();
   for (Req *r : sl.reqs) {
@@ -225,16 +226,17 @@ using namespace tmce;
 
 extern "C" {
 
-int tmce_server_create(int set, size_t slot_capacity, int slots_per_direction, size_t hbm_budget,
+int tmce_server_create(int variant, size_t slot_capacity, int slots_per_direction, size_t hbm_budget,
                        tmce_server **out) {
-  if (!valid_set(set) || !out || slot_capacity == 0 || slots_per_direction < 1 || slots_per_direction > 8)
+  if (!variant_valid(variant) || !out || slot_capacity == 0 || slots_per_direction < 1 || slots_per_direction > 8)
     return fail(TMCE_EINVAL, "tmce_server_create: bad argument");
   Engine *e = engine();
   if (!e) return TMCE_ENODEV;
   auto *sv = new tmce_server();
-  sv->set = set;
+  sv->set = variant_base(variant);
+  sv->pc = variant_pc(variant);
+  sv->ctb = (size_t)variant_ct_bytes(variant);
   sv->budget = hbm_budget ? hbm_budget : (size_t)-1;
-  const SetInfo &si = set_info(set);
   auto bail = [&](int r) {
     destroy_server(sv);
     return r;
@@ -246,11 +248,11 @@ int tmce_server_create(int set, size_t slot_capacity, int slots_per_direction, s
       sl.dir = dir;
       if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(TMCE_EDEVICE, "hipStreamCreate"));
-      if (int r = tmce_pipeline_create(set, dir, slot_capacity, PLAN_MAX_GROUPS * 64 + 1024, 1, sl.stream, &sl.pipe))
+      if (int r = tmce_pipeline_create(variant, dir, slot_capacity, PLAN_MAX_GROUPS * 64 + 1024, 1, sl.stream, &sl.pipe))
         return bail(r);
       sl.cap = tmce_pipeline_capacity(sl.pipe);
-      sl.in_row = dir == 0 ? 32 : (size_t)si.synd_bytes;
-      sl.out_row = round_up_sz((dir == 0 ? (size_t)si.synd_bytes : 0) + 32 + sizeof(int), 16);
+      sl.in_row = dir == 0 ? 32 : sv->ctb;
+      sl.out_row = round_up_sz((dir == 0 ? sv->ctb : 0) + 32 + sizeof(int), 16);
       if (hipHostMalloc((void **)&sl.h_in, sl.cap * sl.in_row, hipHostMallocDefault) != hipSuccess ||
           hipHostMalloc((void **)&sl.h_out, sl.cap * sl.out_row, hipHostMallocDefault) != hipSuccess)
         return bail(fail(TMCE_ENOMEM, "hipHostMalloc staging"));
@@ -270,8 +272,8 @@ int tmce_server_create(int set, size_t slot_capacity, int slots_per_direction, s
 int tmce_server_submit(tmce_server *sv, const tmce_request *rq, unsigned long long *id) {
   if (!sv || !rq || !id || !rq->key || (rq->direction != 0 && rq->direction != 1))
     return fail(TMCE_EINVAL, "tmce_server_submit: bad argument");
-  if (rq->key->set != sv->set || rq->key->kind != (rq->direction == 0 ? 1 : 2))
-    return fail(TMCE_EINVAL, "tmce_server_submit: key of the wrong set or kind");
+  if (rq->key->set != sv->set || rq->key->pc != sv->pc || rq->key->kind != (rq->direction == 0 ? 1 : 2))
+    return fail(TMCE_EINVAL, "tmce_server_submit: key of the wrong set, variant or kind");
   if (rq->n == 0 || rq->n + PLAN_ALIGN - 1 > sv->slots[0].cap || !rq->in || !rq->k || (rq->direction == 0 && !rq->ct))
     return fail(TMCE_EINVAL, "tmce_server_submit: empty request, request larger than a slot, or missing buffers");
   auto *r = new Req();

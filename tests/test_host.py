@@ -9,10 +9,15 @@ import pytest
 from conftest import ROOT, SETS
 
 
+VARIANTS = [s + v for s in SETS for v in ("", "f", "pc", "pcf")]
+
+
 def header_functions():
     txt = open(os.path.join(ROOT, "include", "tmce.h")).read()
     names = set(re.findall(r"\b(tmce_[a-z0-9_]+)\s*\(", txt))
-    for s in SETS:
+    declared = set(re.findall(r"TMCE_DECLARE_SET\(([0-9a-z]+)\)", txt))
+    assert declared == set(VARIANTS), sorted(set(VARIANTS) ^ declared)
+    for s in declared:
         for f in ("keypair", "enc", "dec", "implementation", "compiler"):
             names.add(f"mceliece_kem_{s}_{f}")
     return sorted(names)
@@ -26,16 +31,30 @@ def test_library_exports_header_symbols():
 
 
 def test_header_sizes_match_params(orc):
+    """Size macros of all 20 variants against the oracle (whose sizes the KAT checksums pin), and
+    the short aliases of LM:hdef."""
     txt = open(os.path.join(ROOT, "include", "tmce.h")).read()
-    for s in SETS:
+    for s in VARIANTS:
         p = orc.params(s)
         for macro, val in (("PUBLICKEYBYTES", p.pk_bytes), ("SECRETKEYBYTES", p.sk_bytes),
-                           ("CIPHERTEXTBYTES", p.synd_bytes), ("BYTES", 32)):
+                           ("CIPHERTEXTBYTES", orc.ct_bytes(s)), ("BYTES", 32)):
             m = re.search(rf"#define mceliece_kem_{s}_{macro} (\d+)", txt)
             assert m and int(m.group(1)) == val, (s, macro)
+            assert f"#define mceliece{s}_{macro} mceliece_kem_{s}_{macro}" in txt, (s, macro)
+        for f in ("keypair", "enc", "dec"):
+            assert f"#define mceliece{s}_{f} mceliece_kem_{s}_{f}" in txt, (s, f)
 
 
-@pytest.mark.parametrize("s", SETS)
+def test_python_params_match_oracle(orc):
+    import tensormceliece_amd as tm
+    assert sorted(tm.PARAMS) == sorted(VARIANTS)
+    for s in VARIANTS:
+        p = orc.params(s)
+        assert (tm.PARAMS[s]["pk"], tm.PARAMS[s]["sk"], tm.PARAMS[s]["ct"]) == (p.pk_bytes, p.sk_bytes, orc.ct_bytes(s))
+        assert tm.SETS[s] == orc.SET_IDS[s]
+
+
+@pytest.mark.parametrize("s", SETS + [s + "f" for s in SETS] + ["348864pcf"])
 def test_host_keypair_matches_oracle(orc, s):
     """The library's host keypair (randombytes = seeded chain) is byte-identical to the oracle's."""
     import tensormceliece_amd as tm
@@ -44,3 +63,17 @@ def test_host_keypair_matches_oracle(orc, s):
     pk_ref, sk_ref = orc.keypair_seeded(s, seed[0].tobytes())
     assert np.array_equal(sk, sk_ref)
     assert np.array_equal(pk, pk_ref)
+
+
+def test_host_f_keypairs_with_column_swaps(orc):
+    """f variants: keys whose column step swaps columns (pivot mask other than the identity
+    0xFFFFFFFF) match the oracle byte for byte."""
+    import tensormceliece_amd as tm
+    seeds, _ = orc.xorshift_seeds(6, 0xF00D)
+    swapped = 0
+    for sd in seeds:
+        pk, sk = tm.keypair_seeded("348864f", sd.tobytes())
+        pk_ref, sk_ref = orc.keypair_seeded("348864f", sd.tobytes())
+        assert np.array_equal(sk, sk_ref) and np.array_equal(pk, pk_ref)
+        swapped += int.from_bytes(sk[32:40].tobytes(), "little") != 0xFFFFFFFF
+    assert swapped >= 1

@@ -32,9 +32,58 @@ KAT64 = {  # kem_<set>_checksums[1] (64 loops)
 }
 
 
+# the f (semi-systematic key generation) and pc (plaintext confirmation) variants:
+# kem_<variant>_checksums[0], e.g. kem_348864f :4794-4796, kem_348864pc :5021-5023,
+# kem_348864pcf :5248-5250, kem_6960119pcf :1616-1618
+KAT8_VARIANTS = {
+    "348864f": "5ad76eff6b5ccaedc1615e9795880cfd79b44c4247db4b8a13684c507a5db437",
+    "348864pc": "ddfa65df30bee9c97e153089d24cce158a7979aa01ebdbb7bf374da13bd81feb",
+    "348864pcf": "4dcfc56d5e162873f227befd1e7c51f2b317f9ccf811e931e86e2fcdeea6c10e",
+    "460896f": "1418b90ac6aa9d36bd3c42c7e8c4ba299eafa45e93a903a383041629fb47f043",
+    "460896pc": "e27bb3f53c37eaaef460488d21750a80baf3874b0b26b23e510325e45c31c560",
+    "460896pcf": "1980e3f38d8258eca9d8d4300ac8dd6a6cde2f2c10c5e7b589971972531c8505",
+    "6688128f": "daec95f7a2db91986d2d1ca7312d66314149b066db0f467935ecf027aa1f4403",
+    "6688128pc": "f6d44759e9e52330923b5dc2957cd080c10b31990a96e0540c52ba42017fa849",
+    "6688128pcf": "80e0655938457c4dcd6aa6323d60f2495efda8a2d4a13b1958e8ca8fd2e848b6",
+    "6960119f": "8d0f85c0ffa074aff7cf1c5d2f8a048174de4ee62a74556e04e8da326e8b640d",
+    "6960119pc": "f164d7a16ea6175870f34056b211d4d45a2bc95085759e2f688721f3fbf326c8",
+    "6960119pcf": "873616887e3e83679d5a708b3fbd8eb2301d18ea21f0f7c238db87178596267a",
+    "8192128f": "2d50dd3ea54c764b5f79505d181c08cefc2276a28c791681cce3a45888c687cc",
+    "8192128pc": "a9d56a17b9b870e832cddde5c5787b55ada2189992d03eb75e23e710bcb98ccd",
+    "8192128pcf": "fdf47ca26ac3e1a3ae8c2859619cce789420a4af8533ded9e1104d232bd62f5e",
+}
+KAT64_VARIANTS = {  # kem_<variant>_checksums[1]
+    "348864pcf": "48fffebf6cd92a688b4724d88cf989ab199066040c7c62202dab6f30ef188d68",
+    "6960119pcf": "d709f666d84fe4450ec59759ef20982df8d614afe788db2b7d0f91b0c0e02254",
+}
+
+
 @pytest.mark.parametrize("s", SETS)
 def test_kat_checksum_8_loops(orc, s):
     assert orc.kat_checksum(s, 8) == KAT8[s]
+
+
+@pytest.fixture(scope="module")
+def variant_kats(orc):
+    """The 15 variant checksums, computed on threads (the oracle's randomness state is per thread)."""
+    from concurrent.futures import ThreadPoolExecutor
+    names = list(KAT8_VARIANTS)
+    with ThreadPoolExecutor(max_workers=max(1, min(8, os.cpu_count() or 1))) as ex:
+        return dict(zip(names, ex.map(lambda v: orc.kat_checksum(v, 8), names)))
+
+
+@pytest.mark.parametrize("v", list(KAT8_VARIANTS))
+def test_kat_checksum_8_loops_variant(variant_kats, v):
+    assert variant_kats[v] == KAT8_VARIANTS[v]
+
+
+def test_kat_checksum_64_loops_348864pcf(orc):
+    assert orc.kat_checksum("348864pcf", 64) == KAT64_VARIANTS["348864pcf"]
+
+
+@pytest.mark.skipif(not os.environ.get("TMCE_SLOW_TESTS"), reason="about 25 s; set TMCE_SLOW_TESTS=1")
+def test_kat_checksum_64_loops_6960119pcf(orc):
+    assert orc.kat_checksum("6960119pcf", 64) == KAT64_VARIANTS["6960119pcf"]
 
 
 def test_kat_checksum_64_loops_348864(orc):
