@@ -296,6 +296,45 @@ def section_syndrome(tm, orc, log):
     return out
 
 
+def section_variants(tm, orc, log, n=65536, reps=3):
+    """The pc variants (plaintext confirmation: one more SHAKE256 over e in each direction, 32 more
+    ciphertext bytes) against their base sets at N=65536, and batched key generation of an f variant
+    (semi-systematic form) against its base set.  Outputs round-trip and a sample is byte-compared
+    with the oracle."""
+    out = {}
+    for base in ("348864", "8192128"):
+        kseed, _ = xorshift_seeds(1, 0x7661)
+        pk, sk = tm.keypair_seeded(base, kseed[0].tobytes())  # the same key serves the pc variant
+        seeds = np.random.default_rng(int(base)).integers(0, 256, (n, 32), dtype=np.uint8)
+        row = {}
+        for v in (base, base + "pc"):
+            dk = DevKem(tm, v, n, (pk, sk), seeds)
+            ee, ed = dk.rates(reps)
+            ct, k1, k2, st1, st2 = dk.outputs()
+            idx = np.random.default_rng(2).choice(n, 8, replace=False)
+            ct_o, k_o, _ = orc.enc_batch(v, pk, seeds[idx])
+            exact = bool(np.array_equal(k1, k2) and not st1.any() and not st2.any()
+                         and np.array_equal(ct[idx], ct_o) and np.array_equal(k1[idx], k_o))
+            dk.free()
+            row[v] = {"encaps_per_s": round(ee, 1), "decaps_per_s": round(ed, 1), "bit_exact": exact}
+        row["pc_cost"] = {"encaps": round(row[base]["encaps_per_s"] / row[base + "pc"]["encaps_per_s"], 3),
+                          "decaps": round(row[base]["decaps_per_s"] / row[base + "pc"]["decaps_per_s"], 3)}
+        out[base] = row
+        log(f"variants {base}: {row}")
+    kg = {}
+    for v in ("348864", "348864f"):
+        seeds, _ = xorshift_seeds(64, 0x6B66)
+        tm.keypair_batch(v, seeds[:2])
+        t0 = time.perf_counter()
+        pkb, skb = tm.keypair_batch(v, seeds)
+        kg[v] = {"keys": 64, "batch_keypairs_per_s": round(64 / (time.perf_counter() - t0), 1)}
+    pk_h, sk_h = tm.keypair_seeded("348864f", seeds[0].tobytes())
+    kg["348864f"]["bit_exact_vs_host_first_key"] = bool(np.array_equal(pkb[0], pk_h) and np.array_equal(skb[0], sk_h))
+    out["keygen"] = kg
+    log(f"variants keygen: {kg}")
+    return out
+
+
 def section_per_set(tm, orc, keys, cores, cpu_seconds, log, n=65536, reps=3):
     out = {}
     for s in SETS:
@@ -488,7 +527,7 @@ def main():
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-stage GPU form")
     ap.add_argument("--quick", action="store_true", help="headline only (no suite)")
-    ap.add_argument("--sections", default="syndrome,per_set,c1,c3,c4,c5,keygen")
+    ap.add_argument("--sections", default="syndrome,per_set,c1,c3,c4,c5,keygen,variants")
     ap.add_argument("--one-stream", action="store_true",
                     help="headline steps serialised on one stream only (profiling: no concurrent kernels)")
     ap.add_argument("--step-gen-blocks", type=int, default=64,
@@ -638,6 +677,8 @@ def main():
             suite["c5_8192128_serving"] = section_c5(tm, log)
         if "keygen" in sections:
             suite["keygen_batch"] = section_keygen(tm, cores, log)
+        if "variants" in sections:
+            suite["variants_n65536"] = section_variants(tm, orc, log)
 
     kem_ops_per_s = 2 * n / t_step
     line = {
