@@ -502,14 +502,25 @@ def section_c5(tm, log, nkeys=16, n_requests=6000):
             loads[f"{int(frac * 100)}%"] = {"offered_requests_per_s": round(frac * req_rate_sat, 1),
                                             "ops_per_s": round(r["ops_per_s"], 1), "p50_ms": round(r["p50_ms"], 3),
                                             "p99_ms": round(r["p99_ms"], 3)}
+        # the same requests with inputs and outputs in pinned slabs: served in place (zero-copy)
+        zreqs, inb, outb = serving.to_slab(reqs, tm.PARAMS[s])
+        zsat = serving.run_poisson(server, wl, zreqs, 0.0, slab=outb)
+        z80 = serving.run_poisson(server, wl, zreqs, 0.8 * req_rate_sat, slab=outb)
+        zero_copy = {"saturation": {"ops_per_s": round(zsat["ops_per_s"], 1), "p50_ms": round(zsat["p50_ms"], 3),
+                                    "p99_ms": round(zsat["p99_ms"], 3)},
+                     "poisson_80%": {"ops_per_s": round(z80["ops_per_s"], 1), "p50_ms": round(z80["p50_ms"], 3),
+                                     "p99_ms": round(z80["p99_ms"], 3)}}
         st = server.stats()
+        zero_copy["zero_copy_requests"] = st["zero_copy_requests"]
+        inb.free()
+        outb.free()
     finally:
         server.close()
     out = {"keys": nkeys, "requests_per_run": n_requests, "mean_request_ops": round(mean_ops, 1),
            "key_device_bytes": wl.device_bytes(),
            "saturation": {"ops_per_s": round(sat["ops_per_s"], 1), "p50_ms": round(sat["p50_ms"], 3),
                           "p99_ms": round(sat["p99_ms"], 3)},
-           "poisson": loads, "batches": st["batches"], "requests": st["requests"],
+           "poisson": loads, "zero_copy_slabs": zero_copy, "batches": st["batches"], "requests": st["requests"],
            "peak_in_flight_bytes": st["peak_in_flight_bytes"], "hbm_budget_bytes": 8 << 30}
     log(f"c5: {out}")
     return out

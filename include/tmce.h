@@ -434,7 +434,12 @@ void *tmce_pipeline_stream(tmce_pipeline *p);              /* its hipStream_t */
    Requests use host buffers and key handles, which the caller keeps valid (and the keys
    unfreed) until the request is collected.
    tmce_server_wait blocks until a request completes and collects it; tmce_server_poll collects up
-   to `max` completed requests.  Latency = completion - submission (host steady clock). */
+   to `max` completed requests.  Latency = completion - submission (host steady clock).
+   Zero-copy requests: when every buffer of a request (in, k, status if given, ct for
+   encapsulation) lies inside memory from tmce_host_alloc, aligned as the pipelines require (k 16
+   bytes, status 4, in and ct 2), the GPU reads the inputs and writes the outputs there in place,
+   and the request skips the slot's staging (host copy, upload, download); other requests are
+   staged.  Results are the same either way. */
 typedef struct tmce_server tmce_server;
 typedef struct tmce_request {
   int direction;            /* 0 = encapsulation, 1 = decapsulation */
@@ -448,6 +453,7 @@ typedef struct tmce_request {
 typedef struct tmce_server_stats {
   unsigned long long batches, requests, ops, pending;
   size_t in_flight_bytes, peak_in_flight_bytes, enc_bytes_per_op, dec_bytes_per_op;
+  unsigned long long zero_copy_requests;  /* requests served in place from tmce_host_alloc slabs */
 } tmce_server_stats;
 int tmce_server_create(int set, size_t slot_capacity, int slots_per_direction, size_t hbm_budget,
                        tmce_server **out);
@@ -466,6 +472,10 @@ int tmce_set_form(int form);
 /* device-memory helpers for callers without a HIP runtime binding */
 void *tmce_dev_alloc(size_t bytes);
 void tmce_dev_free(void *p);
+/* pinned host memory mapped into the GPU's address space (a slab for zero-copy serving requests,
+   see the serving loop); NULL on failure.  Free a slab only when no request using it is in flight. */
+void *tmce_host_alloc(size_t bytes);
+int tmce_host_free(void *p);
 int tmce_memcpy_h2d(void *d, const void *h, size_t bytes);
 int tmce_memcpy_d2h(void *h, const void *d, size_t bytes);
 int tmce_memcpy_d2d(void *dst, const void *src, size_t bytes);

@@ -69,6 +69,8 @@ def load():
         "tmce_dec_batch_device": (I, [P, P, P, P, S, P]),
         "tmce_dev_alloc": (P, [S]),
         "tmce_dev_free": (V, [P]),
+        "tmce_host_alloc": (P, [S]),
+        "tmce_host_free": (I, [P]),
         "tmce_memcpy_h2d": (I, [P, P, S]),
         "tmce_memcpy_d2h": (I, [P, P, S]),
         "tmce_memcpy_d2d": (I, [P, P, S]),
@@ -216,6 +218,44 @@ class SecretKey(_Key):
         return k, st
 
 
+class HostBuffer:
+    """Pinned host memory mapped into the GPU's address space (tmce_host_alloc).  Serving requests
+    whose arrays are views of a HostBuffer (array()) are served in place, without staging copies.
+    Keep it alive, and unfreed, while requests using it are in flight."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = load().tmce_host_alloc(self.nbytes)
+        if not self.ptr:
+            raise TmceError(f"tmce_host_alloc failed: {load().tmce_last_error().decode()}")
+        self._off = 0
+
+    def array(self, shape, dtype=np.uint8, align=16):
+        """The next unused bytes of the buffer as a numpy array (aligned to `align` bytes)."""
+        dt = np.dtype(dtype)
+        nb = int(np.prod(shape)) * dt.itemsize
+        off = (self._off + align - 1) // align * align
+        if off + nb > self.nbytes:
+            raise TmceError("HostBuffer exhausted")
+        self._off = off + nb
+        raw = (ctypes.c_uint8 * nb).from_address(self.ptr + off)
+        return np.frombuffer(raw, dtype=dt).reshape(shape)
+
+    def reset(self):
+        self._off = 0
+
+    def free(self):
+        if self.ptr:
+            load().tmce_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class DeviceBuffer:
     """Device memory owned through the library (for benchmarks with inputs resident in HBM)."""
 
@@ -342,7 +382,7 @@ class ServerStatsC(ctypes.Structure):
     _fields_ = [("batches", ctypes.c_ulonglong), ("requests", ctypes.c_ulonglong), ("ops", ctypes.c_ulonglong),
                 ("pending", ctypes.c_ulonglong), ("in_flight_bytes", ctypes.c_size_t),
                 ("peak_in_flight_bytes", ctypes.c_size_t), ("enc_bytes_per_op", ctypes.c_size_t),
-                ("dec_bytes_per_op", ctypes.c_size_t)]
+                ("dec_bytes_per_op", ctypes.c_size_t), ("zero_copy_requests", ctypes.c_ulonglong)]
 
 
 class Server:
@@ -357,17 +397,20 @@ class Server:
         _check(load().tmce_server_create(SETS[set_name], int(slot_capacity), int(slots_per_direction),
                                          int(hbm_budget), ctypes.byref(self._h)), "tmce_server_create")
 
-    def submit(self, direction, key, inp, ct=None, k=None, status=None):
-        """direction "enc" (inp = seeds [n][32]) or "dec" (inp = ciphertexts); returns (id, outputs)."""
+    def submit(self, direction, key, inp, ct=None, k=None, status=None, slab=None):
+        """direction "enc" (inp = seeds [n][32]) or "dec" (inp = ciphertexts); returns (id, outputs).
+        With `slab` (a HostBuffer) the outputs not given are allocated in it; when inp and every
+        output lie in slabs the request is served in place (zero-copy)."""
         d = {"enc": 0, "dec": 1}[direction]
         inp = np.ascontiguousarray(inp, np.uint8)
         n = inp.shape[0]
+        new = (lambda shape, dt: slab.array(shape, dt)) if slab is not None else (lambda shape, dt: np.zeros(shape, dt))
         if k is None:
-            k = np.zeros((n, KEY_BYTES), np.uint8)
+            k = new((n, KEY_BYTES), np.uint8)
         if status is None:
-            status = np.zeros(n, np.int32)
+            status = new(n, np.int32)
         if d == 0 and ct is None:
-            ct = np.zeros((n, self.params["ct"]), np.uint8)
+            ct = new((n, self.params["ct"]), np.uint8)
         rq = RequestC(d, key.handle.value, n, inp.ctypes.data, ct.ctypes.data if ct is not None else None,
                       k.ctypes.data, status.ctypes.data)
         rid = ctypes.c_ulonglong()

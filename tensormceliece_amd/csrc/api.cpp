@@ -75,6 +75,25 @@ Engine *engine() {
   return g_engine;
 }
 
+// pinned host slabs: host range -> device address of its start
+struct Slab {
+  uintptr_t hi;
+  uint8_t *dev;
+};
+static std::mutex g_slab_mu;
+static std::map<uintptr_t, Slab> g_slabs;  // keyed by the slab's first host address
+
+uint8_t *slab_device_ptr(const void *p, size_t len) {
+  if (!p) return nullptr;
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  std::lock_guard<std::mutex> lk(g_slab_mu);
+  auto it = g_slabs.upper_bound(a);
+  if (it == g_slabs.begin()) return nullptr;
+  --it;
+  if (a < it->first || a + len > it->second.hi || a + len < a) return nullptr;
+  return it->second.dev + (a - it->first);
+}
+
 }  // namespace tmce
 
 namespace {
@@ -574,6 +593,36 @@ void *tmce_dev_alloc(size_t bytes) {
   return p;
 }
 void tmce_dev_free(void *p) { (void)hipFree(p); }
+
+void *tmce_host_alloc(size_t bytes) {
+  if (!engine()) return nullptr;
+  void *h = nullptr, *d = nullptr;
+  if (hipHostMalloc(&h, bytes ? bytes : 1, hipHostMallocMapped) != hipSuccess) {
+    fail(TMCE_ENOMEM, "hipHostMalloc slab");
+    return nullptr;
+  }
+  if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess || !d) {
+    (void)hipHostFree(h);
+    fail(TMCE_EDEVICE, "hipHostGetDevicePointer");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_slab_mu);
+  g_slabs[reinterpret_cast<uintptr_t>(h)] = Slab{reinterpret_cast<uintptr_t>(h) + (bytes ? bytes : 1),
+                                                 static_cast<uint8_t *>(d)};
+  return h;
+}
+
+int tmce_host_free(void *p) {
+  if (!p) return 0;
+  {
+    std::lock_guard<std::mutex> lk(g_slab_mu);
+    auto it = g_slabs.find(reinterpret_cast<uintptr_t>(p));
+    if (it == g_slabs.end()) return fail(TMCE_EINVAL, "tmce_host_free: not a slab from tmce_host_alloc");
+    g_slabs.erase(it);
+  }
+  HIPCHK(hipHostFree(p));
+  return 0;
+}
 int tmce_memcpy_h2d(void *d, const void *h, size_t bytes) {
   HIPCHK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
   return 0;

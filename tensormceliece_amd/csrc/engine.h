@@ -137,6 +137,10 @@ int upload(T **dst, const std::vector<T> &src) {
   return 0;
 }
 
+// device address of the host range [p, p + len) when it lies inside one tmce_host_alloc slab, else
+// null (the slab registry is process-wide)
+uint8_t *slab_device_ptr(const void *p, size_t len);
+
 int engine_init(int device);
 // the engine (initialised on device 0 on first use), or null if there is no usable device
 Engine *engine();

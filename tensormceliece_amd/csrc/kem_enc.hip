@@ -545,46 +545,50 @@ hipError_t launch_fill_i32(int *p, int v, size_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
-// ---- multi-batch gather / scatter (one thread per combined op)
+// ---- multi-batch gather / scatter: one thread per 16-bit unit of a row, adjacent threads on
+// adjacent units, so each wave reads (writes) contiguous bytes of a sub-batch's rows -- in HBM, or
+// across the host link when the sub-batch lies in a pinned host slab
 __global__ void k_plan_gather(const PlanDev *__restrict__ plan, uint8_t *__restrict__ flat, int row_bytes) {
-  const int op = blockIdx.x * blockDim.x + threadIdx.x;
-  if (op >= plan->total) return;
+  const int H = row_bytes / 2;  // row_bytes even
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)plan->total * H) return;
+  const int op = (int)(i / H), h = (int)(i - (size_t)op * H);
   const int sb = plan_sub(plan, op);
-  uint16_t *dst = reinterpret_cast<uint16_t *>(flat + (size_t)op * row_bytes);  // row_bytes even
-  if (sb < 0) {
-    for (int h = 0; h < row_bytes / 2; ++h) dst[h] = 0;
-    return;
+  uint16_t v = 0;  // padding rows are zero
+  if (sb >= 0) {
+    const SubDev &d = plan->subs[sb];
+    v = reinterpret_cast<const uint16_t *>(d.in + (size_t)(op - d.start) * row_bytes)[h];
   }
-  const SubDev &d = plan->subs[sb];
-  const uint16_t *src = reinterpret_cast<const uint16_t *>(d.in + (size_t)(op - d.start) * row_bytes);
-  for (int h = 0; h < row_bytes / 2; ++h) dst[h] = src[h];
+  reinterpret_cast<uint16_t *>(flat)[i] = v;
 }
+// units per op: the ciphertext's 16-bit halves (enc), at least two; units 0 and 1 also move the
+// session key's two 16-byte halves, unit 0 the status
 __global__ void k_plan_scatter(const PlanDev *__restrict__ plan, const uint8_t *__restrict__ ct, int ct_bytes,
                                const uint8_t *__restrict__ key, const int *__restrict__ status) {
-  const int op = blockIdx.x * blockDim.x + threadIdx.x;
-  if (op >= plan->total) return;
+  const int H = ct_bytes / 2, U = H > 2 ? H : 2;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)plan->total * U) return;
+  const int op = (int)(i / U), u = (int)(i - (size_t)op * U);
   const int sb = plan_sub(plan, op);
   if (sb < 0) return;
   const SubDev &d = plan->subs[sb];
-  const int i = op - d.start;
-  if (ct_bytes) {
-    const uint16_t *c = reinterpret_cast<const uint16_t *>(ct + (size_t)op * ct_bytes);
-    uint16_t *o = reinterpret_cast<uint16_t *>(d.ct + (size_t)i * ct_bytes);
-    for (int h = 0; h < ct_bytes / 2; ++h) o[h] = c[h];
-  }
-  const uint4 *k = reinterpret_cast<const uint4 *>(key + 32 * (size_t)op);
-  uint4 *ko = reinterpret_cast<uint4 *>(d.key + 32 * (size_t)i);
-  ko[0] = k[0];
-  ko[1] = k[1];
-  if (d.status) d.status[i] = status[op];
+  const int j = op - d.start;
+  if (u < H)
+    reinterpret_cast<uint16_t *>(d.ct + (size_t)j * ct_bytes)[u] =
+        reinterpret_cast<const uint16_t *>(ct + (size_t)op * ct_bytes)[u];
+  if (u < 2)
+    reinterpret_cast<uint4 *>(d.key + 32 * (size_t)j)[u] = reinterpret_cast<const uint4 *>(key + 32 * (size_t)op)[u];
+  if (u == 0 && d.status) d.status[j] = status[op];
 }
 hipError_t launch_plan_gather(const PlanDev *plan, uint8_t *flat, int row_bytes, int cap, hipStream_t s) {
-  if (cap > 0) k_plan_gather<<<ceil_div(cap, 256), 256, 0, s>>>(plan, flat, row_bytes);
+  const size_t units = (size_t)cap * (row_bytes / 2);
+  if (units > 0) k_plan_gather<<<(unsigned)((units + 255) / 256), 256, 0, s>>>(plan, flat, row_bytes);
   return hipGetLastError();
 }
 hipError_t launch_plan_scatter(const PlanDev *plan, const uint8_t *ct, int ct_bytes, const uint8_t *key,
                                const int *status, int cap, hipStream_t s) {
-  if (cap > 0) k_plan_scatter<<<ceil_div(cap, 256), 256, 0, s>>>(plan, ct, ct_bytes, key, status);
+  const size_t units = (size_t)cap * (ct_bytes / 2 > 2 ? ct_bytes / 2 : 2);
+  if (units > 0) k_plan_scatter<<<(unsigned)((units + 255) / 256), 256, 0, s>>>(plan, ct, ct_bytes, key, status);
   return hipGetLastError();
 }
 

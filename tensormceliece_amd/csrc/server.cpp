@@ -11,6 +11,10 @@
 // arrive while batches are in flight are admitted into the next free slot.  When a slot's event
 // completes the worker copies each request's outputs back and stamps its completion time.
 //
+// Requests whose buffers all lie in pinned slabs from tmce_host_alloc skip the staging: their
+// sub-batches point at the slabs' device addresses, so the pipeline's gather kernel reads the
+// inputs and its scatter kernel writes the outputs across the host link directly.
+//
 // Latency = completion - submission (host steady clock), including queueing, transfers and compute.
 #include <hip/hip_runtime.h>
 
@@ -40,6 +44,11 @@ struct Req {
   size_t footprint = 0;
   double t_submit = 0, t_start = 0, t_done = 0;
   bool done = false;
+  // zero-copy: device addresses of the request's buffers inside tmce_host_alloc slabs
+  bool zero_copy = false;
+  const uint8_t *d_in = nullptr;
+  uint8_t *d_ct = nullptr, *d_k = nullptr;
+  int *d_status = nullptr;
 };
 
 struct Slot {
@@ -74,7 +83,7 @@ struct tmce_server {
   int error = 0;  // first failure of the worker (guarded by mu)
   std::string error_msg;
   // statistics
-  unsigned long long batches = 0, ops = 0, requests = 0;
+  unsigned long long batches = 0, ops = 0, requests = 0, zero_copy = 0;
   size_t per_op_bytes[2] = {0, 0};
 };
 
@@ -108,33 +117,41 @@ std::vector<Req *> form_batch(tmce_server *sv, Slot &sl) {
 
 int launch_slot(tmce_server *sv, Slot &sl, std::vector<Req *> &batch) {
   std::vector<tmce_sub_batch> subs;
-  size_t pos = 0;
+  size_t pos = 0, ops = 0;  // pos: staged ops
   uint8_t *d_in = sl.d_io, *d_out = sl.d_io + sl.cap * sl.in_row;
   const double t = now_s();
   for (Req *r : batch) {
-    std::memcpy(sl.h_in + pos * sl.in_row, r->in, r->n * sl.in_row);
     tmce_sub_batch b{};
     b.key = r->key;
-    b.in = d_in + pos * sl.in_row;
-    // outputs of the request: [k][status][ct (enc)] (k 16-byte aligned: out_row is a multiple of 16)
-    uint8_t *o = d_out + pos * sl.out_row;
-    b.k = o;
-    b.status = reinterpret_cast<int *>(o + r->n * 32);
-    b.ct = sl.dir == 0 ? o + r->n * 36 : nullptr;
     b.n = r->n;
+    if (r->zero_copy) {  // the GPU reads and writes the request's slab buffers in place
+      b.in = r->d_in;
+      b.k = r->d_k;
+      b.status = r->d_status;
+      b.ct = r->d_ct;
+    } else {
+      std::memcpy(sl.h_in + pos * sl.in_row, r->in, r->n * sl.in_row);
+      b.in = d_in + pos * sl.in_row;
+      // outputs of the request: [k][status][ct (enc)] (k 16-byte aligned: out_row is a multiple of 16)
+      uint8_t *o = d_out + pos * sl.out_row;
+      b.k = o;
+      b.status = reinterpret_cast<int *>(o + r->n * 32);
+      b.ct = sl.dir == 0 ? o + r->n * 36 : nullptr;
+      pos += r->n;
+    }
     subs.push_back(b);
     r->t_start = t;
-    pos += r->n;
+    ops += r->n;
   }
-  HIPCHK(hipMemcpyAsync(d_in, sl.h_in, pos * sl.in_row, hipMemcpyHostToDevice, sl.stream));
+  if (pos) HIPCHK(hipMemcpyAsync(d_in, sl.h_in, pos * sl.in_row, hipMemcpyHostToDevice, sl.stream));
   if (int rc = tmce_pipeline_run(sl.pipe, subs.data(), subs.size())) return rc;
-  HIPCHK(hipMemcpyAsync(sl.h_out, d_out, pos * sl.out_row, hipMemcpyDeviceToHost, sl.stream));
+  if (pos) HIPCHK(hipMemcpyAsync(sl.h_out, d_out, pos * sl.out_row, hipMemcpyDeviceToHost, sl.stream));
   HIPCHK(hipEventRecord(sl.done, sl.stream));
   sl.reqs = batch;
   sl.busy = true;
   sv->batches += 1;
   sv->requests += batch.size();
-  sv->ops += pos;
+  sv->ops += ops;
   return 0;
 }
 
@@ -143,14 +160,16 @@ void finish_slot(tmce_server *sv, Slot &sl) {
   size_t pos = 0;
   const double t = now_s();
   for (Req *r : sl.reqs) {
-    const uint8_t *o = sl.h_out + pos * sl.out_row;
-    std::memcpy(r->k, o, r->n * 32);
-    if (r->status) std::memcpy(r->status, o + r->n * 32, r->n * sizeof(int));
-    if (sl.dir == 0) std::memcpy(r->ct, o + r->n * 36, r->n * out_ct);
+    if (!r->zero_copy) {
+      const uint8_t *o = sl.h_out + pos * sl.out_row;
+      std::memcpy(r->k, o, r->n * 32);
+      if (r->status) std::memcpy(r->status, o + r->n * 32, r->n * sizeof(int));
+      if (sl.dir == 0) std::memcpy(r->ct, o + r->n * 36, r->n * out_ct);
+      pos += r->n;
+    }
     r->t_done = t;
     r->done = true;
     sv->completed.push_back(r->id);
-    pos += r->n;
   }
   sv->in_flight -= sl.footprint;
   sl.footprint = 0;
@@ -285,6 +304,16 @@ int tmce_server_submit(tmce_server *sv, const tmce_request *rq, unsigned long lo
   r->k = rq->k;
   r->status = rq->status;
   r->footprint = rq->n * sv->per_op_bytes[r->dir];
+  {  // zero-copy when every buffer lies in a slab with the pipelines' alignment
+    const size_t n = rq->n;
+    r->d_in = slab_device_ptr(rq->in, n * (r->dir == 0 ? 32 : sv->ctb));
+    r->d_k = slab_device_ptr(rq->k, n * 32);
+    r->d_status = rq->status ? reinterpret_cast<int *>(slab_device_ptr(rq->status, n * sizeof(int))) : nullptr;
+    r->d_ct = r->dir == 0 ? slab_device_ptr(rq->ct, n * sv->ctb) : nullptr;
+    r->zero_copy = r->d_in && r->d_k && (!rq->status || r->d_status) && (r->dir == 1 || r->d_ct) &&
+                   !((uintptr_t)r->d_k & 15) && !((uintptr_t)r->d_status & 3) && !((uintptr_t)r->d_in & 1) &&
+                   !((uintptr_t)r->d_ct & 1);
+  }
   {
     std::lock_guard<std::mutex> lk(sv->mu);
     if (sv->stop) {
@@ -293,6 +322,7 @@ int tmce_server_submit(tmce_server *sv, const tmce_request *rq, unsigned long lo
     }
     r->id = sv->next_id++;
     r->t_submit = now_s();
+    if (r->zero_copy) sv->zero_copy += 1;
     sv->reqs[r->id] = r;
     sv->pending[r->dir].push_back(r);
   }
@@ -345,6 +375,7 @@ int tmce_server_get_stats(tmce_server *sv, tmce_server_stats *st) {
   st->peak_in_flight_bytes = sv->peak_in_flight;
   st->enc_bytes_per_op = sv->per_op_bytes[0];
   st->dec_bytes_per_op = sv->per_op_bytes[1];
+  st->zero_copy_requests = sv->zero_copy;
   return 0;
 }
 

@@ -94,3 +94,38 @@ def test_request_errors(tm, orc):
         assert server.wait(rid) > 0 and not st.any()
     finally:
         server.close()
+
+
+@pytest.mark.parametrize("s", ["348864", "6960119pc"])
+def test_zero_copy_requests(tm, orc, s):
+    """Requests whose buffers lie in pinned slabs (tmce_host_alloc) are served in place -- the GPU
+    reads the inputs and writes the outputs across the host link -- with the same bytes as staged
+    requests; a slab request with a misaligned output buffer falls back to staging."""
+    from tensormceliece_amd import serving
+    wl = serving.Workload(s, keypairs(orc, s, 3), pool_per_key=1024)
+    reqs = serving.make_requests(wl, 90, seed=13, hi=1024)
+    zreqs, inb, outb = serving.to_slab(reqs, tm.PARAMS[s])
+    server = tm.Server(s, slot_capacity=4096, slots_per_direction=2)
+    try:
+        print(f"zero-copy serving {s}", flush=True)
+        staged = serving.run_poisson(server, wl, reqs, 0.0, keep_outputs=True)
+        zc = serving.run_poisson(server, wl, zreqs, 0.0, keep_outputs=True, slab=outb)
+        st = server.stats()
+        # a slab request with k 8 bytes off the required 16-byte alignment: staged, still exact
+        d, k, inp, _ = zreqs[0]
+        kbuf = outb.array(inp.shape[0] * 32 + 8, np.uint8)[8:].reshape(-1, 32)
+        key = wl.pk[k] if d == "enc" else wl.sk[k]
+        rid, outs = server.submit(d, key, inp, k=kbuf, slab=outb)
+        server.wait(rid)
+        st2 = server.stats()
+    finally:
+        server.close()
+    assert st["zero_copy_requests"] == len(reqs)
+    assert st2["zero_copy_requests"] == len(reqs)  # the misaligned one was staged
+    for a, b in zip(staged["outputs"], zc["outputs"]):
+        for x, y in zip(a, b):
+            assert (x is None and y is None) or np.array_equal(x, y)
+    check_alone(tm, wl, zreqs, zc["outputs"])
+    assert np.array_equal(outs[1], zc["outputs"][0][1])
+    inb.free()
+    outb.free()
