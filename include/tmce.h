@@ -127,7 +127,17 @@ extern "C" {
   int mceliece_kem_##S##_enc(unsigned char *c, unsigned char *k, const unsigned char *pk);     \
   int mceliece_kem_##S##_dec(unsigned char *k, const unsigned char *c, const unsigned char *sk); \
   const char *mceliece_kem_##S##_implementation(void);                                          \
-  const char *mceliece_kem_##S##_compiler(void);
+  const char *mceliece_kem_##S##_compiler(void);                                                \
+  /* implementation dispatch, as libmceliece's scripts-build/dispatch generates it: one         \
+     implementation here; impl < 0 or out of range selects it as well */                         \
+  void (*mceliece_dispatch_kem_##S##_keypair(long long impl))(unsigned char *, unsigned char *);  \
+  int (*mceliece_dispatch_kem_##S##_enc(long long impl))(unsigned char *, unsigned char *,        \
+                                                          const unsigned char *);                \
+  int (*mceliece_dispatch_kem_##S##_dec(long long impl))(unsigned char *, const unsigned char *,  \
+                                                          const unsigned char *);                \
+  const char *mceliece_dispatch_kem_##S##_implementation(long long impl);                       \
+  const char *mceliece_dispatch_kem_##S##_compiler(long long impl);                             \
+  long long mceliece_numimpl_kem_##S(void);
 
 /* LM:doc/api.md:39-116, one block per parameter set and variant */
 TMCE_DECLARE_SET(348864)

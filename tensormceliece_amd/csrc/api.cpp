@@ -743,7 +743,21 @@ int tmce_timing_read(int max, char *names, long *counts, double *total_ms) {
     return single_dec(ID, k, c, sk);                                                              \
   }                                                                                               \
   const char *mceliece_kem_##S##_implementation(void) { return "tmce-gfx950"; }                   \
-  const char *mceliece_kem_##S##_compiler(void) { return "hipcc " __VERSION__; }
+  const char *mceliece_kem_##S##_compiler(void) { return "hipcc " __VERSION__; }                  \
+  void (*mceliece_dispatch_kem_##S##_keypair(long long))(unsigned char *, unsigned char *) {      \
+    return mceliece_kem_##S##_keypair;                                                            \
+  }                                                                                               \
+  int (*mceliece_dispatch_kem_##S##_enc(long long))(unsigned char *, unsigned char *,             \
+                                                    const unsigned char *) {                      \
+    return mceliece_kem_##S##_enc;                                                                \
+  }                                                                                               \
+  int (*mceliece_dispatch_kem_##S##_dec(long long))(unsigned char *, const unsigned char *,       \
+                                                    const unsigned char *) {                      \
+    return mceliece_kem_##S##_dec;                                                                \
+  }                                                                                               \
+  const char *mceliece_dispatch_kem_##S##_implementation(long long) { return "tmce-gfx950"; }     \
+  const char *mceliece_dispatch_kem_##S##_compiler(long long) { return "hipcc " __VERSION__; }    \
+  long long mceliece_numimpl_kem_##S(void) { return 1; }
 
 TMCE_DEFINE_SET(348864, 0)
 TMCE_DEFINE_SET(348864f, 0 | VAR_F)

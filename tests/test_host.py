@@ -20,7 +20,28 @@ def header_functions():
     for s in declared:
         for f in ("keypair", "enc", "dec", "implementation", "compiler"):
             names.add(f"mceliece_kem_{s}_{f}")
+            names.add(f"mceliece_dispatch_kem_{s}_{f}")
+        names.add(f"mceliece_numimpl_kem_{s}")
     return sorted(names)
+
+
+def test_dispatch_entry_points():
+    """libmceliece's implementation dispatch: one implementation, whose function pointers are the
+    entry points themselves (no GPU call)."""
+    import tensormceliece_amd as tm
+    lib = tm.load()
+    for s in VARIANTS:
+        numimpl = getattr(lib, f"mceliece_numimpl_kem_{s}")
+        numimpl.restype = ctypes.c_longlong
+        assert numimpl() == 1
+        for f in ("keypair", "enc", "dec"):
+            disp = getattr(lib, f"mceliece_dispatch_kem_{s}_{f}")
+            disp.restype, disp.argtypes = ctypes.c_void_p, [ctypes.c_longlong]
+            direct = ctypes.cast(getattr(lib, f"mceliece_kem_{s}_{f}"), ctypes.c_void_p).value
+            assert disp(-1) == direct and disp(0) == direct
+        impl = getattr(lib, f"mceliece_dispatch_kem_{s}_implementation")
+        impl.restype, impl.argtypes = ctypes.c_char_p, [ctypes.c_longlong]
+        assert impl(0) == b"tmce-gfx950"
 
 
 def test_library_exports_header_symbols():
