@@ -10,8 +10,10 @@
  *     (LM:doc/api.md:1-115; symbol names from LM:hdef; reference callers
  *     mceliece-gpu-params/main.cu:17-21,159-167 and mceliece-gpu-bmma/main.cu:60,139).
  *     Randomness comes from randombytes() exactly as in libmceliece: keypair draws 32 bytes;
- *     enc draws one 32-byte seed per call and expands it with the seeded chain below.  Install a
- *     source with tmce_set_randombytes() (default: getrandom()).
+ *     enc draws 4t bytes per attempt of gen_e (2t for 8192128; LM:crypto_kem/<set>/vec/
+ *     encrypt.c:41-92) on the host and runs the rest on the GPU, so any randomness source --
+ *     libmceliece's test driver's knownrandombytes included -- gives libmceliece's bytes.  Install
+ *     a source with tmce_set_randombytes() (default: getrandom()).
  *  2. Batched extensions (no counterpart in libmceliece; the reference harness has no exported
  *     API -- its per-stage kernels are launched from mceliece-gpu-params/main.cu:213-262
  *     (encapsulation) and :326-370 (decapsulation)):

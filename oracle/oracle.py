@@ -62,6 +62,7 @@ def lib():
             ("oracle_decrypt", [I, P, P, P], I),
             ("oracle_pk_from_sk", [I, P, P], I),
             ("oracle_kat_checksum", [I, I, ctypes.c_char_p], None),
+            ("oracle_kat_checksum_fns", [I, I, ctypes.c_char_p, P, P, P], None),
             ("oracle_kat_shake_checksum", [I, I, ctypes.c_char_p], None),
             ("oracle_rng_seeded", [P], None),
             ("oracle_gen_e", [I, P, P], None),
@@ -132,6 +133,19 @@ def kat_checksum(set_name, loops):
     buf = ctypes.create_string_buffer(65)
     lib().oracle_kat_checksum(SET_IDS[set_name], loops, buf)
     return buf.value.decode()
+
+
+def kat_checksum_fns(set_name, loops, keypair_fn, enc_fn, dec_fn):
+    """The driver's KAT loop over another implementation's entry points (addresses); that
+    implementation must draw its randomness through randombytes_address()."""
+    buf = ctypes.create_string_buffer(65)
+    lib().oracle_kat_checksum_fns(SET_IDS[set_name], loops, buf, keypair_fn, enc_fn, dec_fn)
+    return buf.value.decode()
+
+
+def randombytes_address():
+    """Address of oracle_randombytes(out, len): the randomness the KAT loop's driver provides."""
+    return ctypes.cast(lib().oracle_randombytes, ctypes.c_void_p).value
 
 
 def shake_checksum(loops, maxtest):

@@ -86,6 +86,14 @@ int oracle_pk_from_sk(int set, const uint8_t *sk, uint8_t *pk);
 
 /* libmceliece test-driver KAT checksum (loops = 8 or 64); writes 64 hex chars + NUL */
 void oracle_kat_checksum(int set, int loops, char *hex_out);
+/* the same driver loop over another implementation's libmceliece-style entry points (e.g. a
+   library under test); the driver's knownrandombytes is served by oracle_randombytes, which the
+   implementation must use as its randombytes() */
+typedef void (*orc_keypair_fn)(uint8_t *pk, uint8_t *sk);
+typedef int (*orc_enc_fn)(uint8_t *ct, uint8_t *key, const uint8_t *pk);
+typedef int (*orc_dec_fn)(uint8_t *key, const uint8_t *ct, const uint8_t *sk);
+void oracle_kat_checksum_fns(int set, int loops, char *hex_out, orc_keypair_fn kp, orc_enc_fn enc,
+                             orc_dec_fn dec);
 /* libmceliece test-driver xof_shake256 checksum (64/128 or 512/16384) */
 void oracle_kat_shake_checksum(int loops, int maxtest, char *hex_out);
 

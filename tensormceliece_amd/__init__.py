@@ -70,6 +70,7 @@ def load():
         "tmce_dev_alloc": (P, [S]),
         "tmce_dev_free": (V, [P]),
         "tmce_host_alloc": (P, [S]),
+        "tmce_set_randombytes": (V, [P]),
         "tmce_host_free": (I, [P]),
         "tmce_memcpy_h2d": (I, [P, P, S]),
         "tmce_memcpy_d2h": (I, [P, P, S]),
@@ -123,6 +124,17 @@ def _ptr(a):
 
 def init(device=0):
     _check(load().tmce_init(device), "tmce_init")
+
+
+def set_randombytes(fn_address=None):
+    """randombytes() of the libmceliece entry points (kem_keypair / kem_enc): the address of a C
+    function void(void *out, long long len), or None for getrandom()."""
+    load().tmce_set_randombytes(fn_address)
+
+
+def entry_address(set_name, fn):
+    """Address of mceliece_kem_<set>_<fn> (keypair, enc, dec) in libtmce.so."""
+    return ctypes.cast(getattr(load(), f"mceliece_kem_{set_name}_{fn}"), ctypes.c_void_p).value
 
 
 def keypair_seeded(set_name, seed):

@@ -305,6 +305,63 @@ int synd_device(tmce_key *k, uint8_t *d_ct, const uint8_t *d_e, size_t n, int me
   return scratch_done(e->use_enc, s);
 }
 
+// Encapsulation of given error supports (host buffers, pos[i*t .. i*t+t)): the rows the sampler
+// would write -- the support and the message prefix [1][e] -- are built here, and the finishing
+// kernel (syndrome by column gather, confirmation, session key) runs on them.
+int enc_support(tmce_key *k, uint8_t *ct, uint8_t *kk, int *status, const uint16_t *pos, size_t n) {
+  Engine *e = engine();
+  if (!e) return TMCE_ENODEV;
+  if (!k || k->kind != 1) return fail(TMCE_EINVAL, "encapsulation needs a public-key handle");
+  if (n == 0) return 0;
+  std::lock_guard<std::mutex> lk(e->mu);
+  const SetInfo &si = set_info(k->set);
+  const size_t ctb = key_ct_bytes(k);
+  if (k->pad_bad) {  // LM:crypto_kem/6960119/vec/kem_enc.c: outputs cleared, return -1
+    std::memset(ct, 0, n * ctb);
+    std::memset(kk, 0, n * 32);
+    if (status)
+      for (size_t i = 0; i < n; ++i) status[i] = -1;
+    return 0;
+  }
+  const size_t IS = (size_t)((si.t + 1) & ~1), MW = (size_t)si.msg_words;
+  std::vector<uint16_t> ind(n * IS, 0);
+  std::vector<uint32_t> msg(n * MW, 0);
+  for (size_t i = 0; i < n; ++i) {
+    uint8_t *m8 = reinterpret_cast<uint8_t *>(&msg[i * MW]);
+    m8[0] = 1;
+    for (int j = 0; j < si.t; ++j) {
+      const uint16_t p = pos[i * si.t + j];
+      ind[i * IS + j] = p;
+      m8[1 + p / 8] |= (uint8_t)(1u << (p % 8));
+    }
+  }
+  hipStream_t s = e->stream;
+  if (int r = scratch_wait(e->use_enc, s)) return r;
+  if (int r = e->ind.ensure(ind.size() * sizeof(uint16_t))) return r;
+  if (int r = e->msg_enc.ensure(msg.size() * sizeof(uint32_t))) return r;
+  if (int r = e->ct.ensure(n * ctb)) return r;
+  if (int r = e->key.ensure(n * 32)) return r;
+  if (int r = e->status.ensure(n * sizeof(int))) return r;
+  HIPCHK(hipMemcpyAsync(e->ind.p, ind.data(), ind.size() * sizeof(uint16_t), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(e->msg_enc.p, msg.data(), msg.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  EncArgs a{};
+  a.cols = k->cols;
+  a.ct = e->ct.as<uint8_t>();
+  a.key = e->key.as<uint8_t>();
+  a.status = e->status.as<int>();
+  a.ind = e->ind.as<uint16_t>();
+  a.msg = e->msg_enc.as<uint32_t>();
+  a.nops = (int)n;
+  a.pc = k->pc ? 1 : 0;
+  HIPCHK(launch_enc_finish_set(k->set, a, s));
+  if (int r = scratch_done(e->use_enc, s)) return r;
+  HIPCHK(hipMemcpyAsync(ct, e->ct.p, n * ctb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(kk, e->key.p, n * 32, hipMemcpyDeviceToHost, s));
+  if (status) HIPCHK(hipMemcpyAsync(status, e->status.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
 // key cache for the single-operation libmceliece entry points
 struct CachedKey {
   std::vector<uint8_t> bytes;
@@ -326,14 +383,18 @@ tmce_key *cached_key(int variant, int kind, const uint8_t *bytes, size_t len) {
   return k;
 }
 
+// libmceliece's enc: the error vector is drawn on the host from randombytes() exactly as
+// libmceliece's gen_e draws it (so any randomness source -- its test driver's included -- gives
+// libmceliece's bytes), the rest runs on the GPU
 int single_enc(int variant, unsigned char *c, unsigned char *k, const unsigned char *pk) {
-  uint8_t seed[32];
-  randombytes(seed, 32);
+  const int set = variant_base(variant);
+  std::vector<uint16_t> pos((size_t)set_info(set).t);
+  host_gen_e(set, rnd_global, nullptr, pos.data());
   std::lock_guard<std::mutex> lk(g_cache_mu);
-  tmce_key *key = cached_key(variant, 1, pk, (size_t)set_info(variant_base(variant)).pk_bytes);
+  tmce_key *key = cached_key(variant, 1, pk, (size_t)set_info(set).pk_bytes);
   if (!key) std::abort();
   int st = 0;
-  if (tmce_enc_batch(key, c, k, &st, seed, 1) != 0) std::abort();
+  if (enc_support(key, c, k, &st, pos.data(), 1) != 0) std::abort();
   return st;
 }
 

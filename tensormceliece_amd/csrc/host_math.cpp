@@ -617,6 +617,24 @@ void keygen_finish(int set, const KeygenCandidate &c, uint8_t *sk) {
   throw std::invalid_argument("set");
 }
 
+void host_gen_e(int set, rand_fn rnd, void *ctx, uint16_t *pos) {
+  const SetInfo &si = set_info(set);
+  const int t = si.t, ndraw = set == S8192128 ? t : 2 * t;
+  const uint16_t mask = (uint16_t)((1u << si.m) - 1u);
+  std::vector<uint8_t> buf(2 * (size_t)ndraw);
+  for (;;) {
+    rnd(buf.data(), (long long)buf.size(), ctx);
+    int count = 0;
+    for (int i = 0; i < ndraw && count < t; ++i) {
+      const uint16_t v = (uint16_t)((buf[2 * i] | (buf[2 * i + 1] << 8)) & mask);
+      if (v < si.n) pos[count++] = v;
+    }
+    if (count < t) continue;
+    std::sort(pos, pos + t);
+    if (std::adjacent_find(pos, pos + t) == pos + t) return;
+  }
+}
+
 void keygen_apply_pivots(int set, KeygenCandidate &c, uint64_t pivots) {
   switch (set) {
     case S348864: return apply_pivots_t<S348864>(c, pivots);

@@ -81,4 +81,10 @@ void keygen_finish(int set, const KeygenCandidate &c, uint8_t *sk);
 
 void shake256(uint8_t *out, long long outlen, const uint8_t *in, long long inlen);
 
+// libmceliece's gen_e (LM:crypto_kem/<set>/vec/encrypt.c gen_e :41-92) drawing from `rnd` exactly
+// as it draws from randombytes(): per attempt 4t bytes (2t for 8192128) read as little-endian
+// m-bit values; the first t below n (8192128: all t); rejected when a value repeats.  pos[0..t) =
+// the error positions, ascending.
+void host_gen_e(int set, rand_fn rnd, void *ctx, uint16_t *pos);
+
 }  // namespace tmce

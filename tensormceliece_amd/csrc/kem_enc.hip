@@ -624,6 +624,29 @@ hipError_t launch_enc(const EncArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int SET>
+hipError_t enc_finish_only(const EncArgs &a, hipStream_t s) {
+  if (a.nops <= 0) return hipSuccess;
+  if (a.pc)
+    k_enc_finish<SET, true><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.ind, reinterpret_cast<const uint4 *>(a.cols),
+                                                                  a.msg, a.ct, a.key, a.status, a.nops, a.plan);
+  else
+    k_enc_finish<SET, false><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.ind, reinterpret_cast<const uint4 *>(a.cols),
+                                                                   a.msg, a.ct, a.key, a.status, a.nops, a.plan);
+  return hipGetLastError();
+}
+
+hipError_t launch_enc_finish_set(int set, const EncArgs &a, hipStream_t s) {
+  switch (set) {
+    case S348864: return enc_finish_only<S348864>(a, s);
+    case S460896: return enc_finish_only<S460896>(a, s);
+    case S6688128: return enc_finish_only<S6688128>(a, s);
+    case S6960119: return enc_finish_only<S6960119>(a, s);
+    case S8192128: return enc_finish_only<S8192128>(a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_enc_set(int set, const EncArgs &a, hipStream_t s) {
   switch (set) {
     case S348864: return launch_enc<S348864>(a, s);

@@ -218,6 +218,8 @@ hipError_t launch_pk_gen_set(int set, const KeygenArgs &a, hipStream_t s);
 size_t pk_gen_h_bytes(int set);
 
 hipError_t launch_enc_set(int set, const EncArgs &a, hipStream_t s);
+// the finishing launch alone, for error supports and message rows [1][e] written by the caller
+hipError_t launch_enc_finish_set(int set, const EncArgs &a, hipStream_t s);
 hipError_t launch_plain_enc_set(int set, const PlainEncArgs &a, hipStream_t s);
 hipError_t launch_plain_dec_set(int set, const DecArgs &a, const PlainDecScratch &x, hipStream_t s);
 hipError_t launch_dec_front_only(int set, const DecArgs &a, hipStream_t s);
