@@ -152,24 +152,34 @@ __device__ __forceinline__ void shake256_blocks_32_c(const uint64_t *__restrict_
 }
 
 // SHAKE256 over a message already laid out in memory as whole 136-byte blocks with the
-// padding applied (0x1F after the message, 0x80 in the last byte); writes 32 bytes.  The next
-// block's 17 words are loaded before the current block's permutation, so their memory latency
-// overlaps it.
+// padding applied (0x1F after the message, 0x80 in the last byte); writes 32 bytes.  With
+// PREFETCH the next block's 17 words are loaded before the current block's permutation, so their
+// memory latency overlaps it (34 more VGPRs live through the permutation).
+template <bool PREFETCH = true>
 __device__ __forceinline__ void shake256_blocks_32(const uint64_t *__restrict__ msg, int nblocks,
                                                    uint64_t out[4], const KeccakRC &rc) {
-  uint64_t A[25], nxt[17];
+  uint64_t A[25];
 #pragma unroll
   for (int i = 0; i < 25; ++i) A[i] = 0;
+  if constexpr (PREFETCH) {
+    uint64_t nxt[17];
 #pragma unroll
-  for (int w = 0; w < 17; ++w) nxt[w] = msg[w];
-  for (int b = 0; b < nblocks; ++b) {
+    for (int w = 0; w < 17; ++w) nxt[w] = msg[w];
+    for (int b = 0; b < nblocks; ++b) {
 #pragma unroll
-    for (int w = 0; w < 17; ++w) A[w] ^= nxt[w];
-    if (b + 1 < nblocks) {
+      for (int w = 0; w < 17; ++w) A[w] ^= nxt[w];
+      if (b + 1 < nblocks) {
 #pragma unroll
-      for (int w = 0; w < 17; ++w) nxt[w] = msg[17 * (b + 1) + w];
+        for (int w = 0; w < 17; ++w) nxt[w] = msg[17 * (b + 1) + w];
+      }
+      keccak_f1600(A, rc);
     }
-    keccak_f1600(A, rc);
+  } else {
+    for (int b = 0; b < nblocks; ++b) {
+#pragma unroll
+      for (int w = 0; w < 17; ++w) A[w] ^= msg[17 * b + w];
+      keccak_f1600(A, rc);
+    }
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) out[i] = A[i];
