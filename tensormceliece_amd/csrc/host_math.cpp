@@ -76,46 +76,63 @@ void perm_from_controlbits(int m, const uint8_t *cb, std::vector<uint16_t> &pi) 
 
 // Sort of 32-bit keys: LSD radix (three 11-bit digits) for large arrays, std::sort for small ones;
 // tmp has a.size() entries
-static void sort32(std::vector<uint32_t> &a, std::vector<uint32_t> &tmp) {
-  const size_t n = a.size();
-  if (n < 1024) {
-    std::sort(a.begin(), a.end());
+// LSD radix sort of n 32-bit keys on 8-bit digits.  With `index_low` the low 16 bits of a[x] are
+// x itself (ascending), so sorting on the high 16 bits alone, stably, gives the full order.
+static void sort32(uint32_t *a, uint32_t *tmp, size_t n, bool index_low) {
+  if (n < 64) {
+    std::sort(a, a + n);
     return;
   }
-  uint32_t *src = a.data(), *dst = tmp.data();
-  for (int sh = 0; sh < 32; sh += 11) {
-    uint32_t cnt[2049] = {0};
-    for (size_t i = 0; i < n; ++i) ++cnt[((src[i] >> sh) & 2047u) + 1];
-    for (int d = 0; d < 2048; ++d) cnt[d + 1] += cnt[d];
-    for (size_t i = 0; i < n; ++i) dst[cnt[(src[i] >> sh) & 2047u]++] = src[i];
+  uint32_t *src = a, *dst = tmp;
+  for (int sh = index_low ? 16 : 0; sh < 32; sh += 8) {
+    uint32_t cnt[257] = {0};
+    for (size_t i = 0; i < n; ++i) ++cnt[((src[i] >> sh) & 255u) + 1];
+    if (cnt[((src[0] >> sh) & 255u) + 1] == n) continue;  // one digit value: already ordered here
+    for (int d = 0; d < 256; ++d) cnt[d + 1] += cnt[d];
+    for (size_t i = 0; i < n; ++i) dst[cnt[(src[i] >> sh) & 255u]++] = src[i];
     std::swap(src, dst);
   }
-  if (src != a.data()) std::memcpy(a.data(), src, n * sizeof(uint32_t));
+  if (src != a) std::memcpy(a, src, n * sizeof(uint32_t));
 }
+
+// Per-level work arrays of the recursion (level w holds n = 2^w entries), kept per thread.
+struct CbWork {
+  std::vector<uint32_t> A[16], B[16], T[16];
+  std::vector<uint16_t> Q[16];
+  void ensure(int w, size_t n) {
+    if (A[w].size() < n) {
+      A[w].resize(n);
+      B[w].resize(n);
+      T[w].resize(n);
+      Q[w].resize(n);
+    }
+  }
+};
 
 // The recursion of LM:crypto_kem/<set>/vec/controlbits.c (cbrecursion).  Every sort below is on
 // distinct non-negative 32-bit keys, so any correct sort reproduces the library's order exactly.
-static void cbrec(uint8_t *out, long long pos, long long step, const uint16_t *pi, int w, size_t n) {
+static void cbrec(uint8_t *out, long long pos, long long step, const uint16_t *pi, int w, size_t n, CbWork &wk) {
   auto flip = [&](long long p, uint32_t b) { out[p >> 3] ^= (uint8_t)((b & 1u) << (p & 7)); };
   if (w == 1) { flip(pos, pi[0]); return; }
-  std::vector<uint32_t> A(n), B(n), tmp(n);
+  wk.ensure(w, n);
+  uint32_t *A = wk.A[w].data(), *B = wk.B[w].data(), *tmp = wk.T[w].data();
   for (size_t x = 0; x < n; ++x) A[x] = ((uint32_t)(pi[x] ^ 1u) << 16) | pi[x ^ 1];
-  sort32(A, tmp);
+  sort32(A, tmp, n, false);
   for (size_t x = 0; x < n; ++x) {
     const uint32_t px = A[x] & 0xffffu;
     B[x] = (px << 16) | std::min<uint32_t>(px, (uint32_t)x);
   }
   for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 16) | (uint32_t)x;
-  sort32(A, tmp);
+  sort32(A, tmp, n, true);
   for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 16) + (B[x] >> 16);
-  sort32(A, tmp);
+  sort32(A, tmp, n, false);
   if (w <= 10) {
     for (size_t x = 0; x < n; ++x) B[x] = ((A[x] & 0xffffu) << 10) | (B[x] & 0x3ffu);
     for (int i = 1; i < w - 1; ++i) {
       for (size_t x = 0; x < n; ++x) A[x] = ((B[x] & ~0x3ffu) << 6) | (uint32_t)x;
-      sort32(A, tmp);
+      sort32(A, tmp, n, true);
       for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 20) | B[x];
-      sort32(A, tmp);
+      sort32(A, tmp, n, false);
       for (size_t x = 0; x < n; ++x)
         B[x] = std::min<uint32_t>((A[x] & 0xffc00u) | (B[x] & 0x3ffu), A[x] & 0xfffffu);
     }
@@ -124,20 +141,20 @@ static void cbrec(uint8_t *out, long long pos, long long step, const uint16_t *p
     for (size_t x = 0; x < n; ++x) B[x] = (A[x] << 16) | (B[x] & 0xffffu);
     for (int i = 1; i < w - 1; ++i) {
       for (size_t x = 0; x < n; ++x) A[x] = (B[x] & ~0xffffu) | (uint32_t)x;
-      sort32(A, tmp);
+      sort32(A, tmp, n, true);
       for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 16) | (B[x] & 0xffffu);
       if (i < w - 2) {
         for (size_t x = 0; x < n; ++x) B[x] = (A[x] & ~0xffffu) | (B[x] >> 16);
-        sort32(B, tmp);
+        sort32(B, tmp, n, false);
         for (size_t x = 0; x < n; ++x) B[x] = (B[x] << 16) | (A[x] & 0xffffu);
       }
-      sort32(A, tmp);
+      sort32(A, tmp, n, false);
       for (size_t x = 0; x < n; ++x) B[x] = std::min<uint32_t>(B[x], (B[x] & ~0xffffu) | (A[x] & 0xffffu));
     }
     for (size_t x = 0; x < n; ++x) B[x] &= 0xffffu;
   }
   for (size_t x = 0; x < n; ++x) A[x] = ((uint32_t)pi[x] << 16) + (uint32_t)x;
-  sort32(A, tmp);
+  sort32(A, tmp, n, true);
   for (size_t j = 0; j < n / 2; ++j) {
     const size_t z = 2 * j;
     const uint32_t fj = B[z] & 1u, Fz = (uint32_t)z + fj;
@@ -146,7 +163,7 @@ static void cbrec(uint8_t *out, long long pos, long long step, const uint16_t *p
     B[z] = (A[z] << 16) | Fz;
     B[z + 1] = (A[z + 1] << 16) | (Fz ^ 1u);
   }
-  sort32(B, tmp);
+  sort32(B, tmp, n, false);
   pos += (2 * w - 3) * step * (long long)(n / 2);
   for (size_t k = 0; k < n / 2; ++k) {
     const size_t y = 2 * k;
@@ -156,21 +173,22 @@ static void cbrec(uint8_t *out, long long pos, long long step, const uint16_t *p
     A[y] = (Ly << 16) | (B[y] & 0xffffu);
     A[y + 1] = ((Ly ^ 1u) << 16) | (B[y + 1] & 0xffffu);
   }
-  sort32(A, tmp);
+  sort32(A, tmp, n, false);
   pos -= (2 * w - 2) * step * (long long)(n / 2);
-  std::vector<uint16_t> q(n);
+  uint16_t *q = wk.Q[w].data();
   for (size_t j = 0; j < n / 2; ++j) {
     q[j] = (uint16_t)((A[2 * j] & 0xffffu) >> 1);
     q[j + n / 2] = (uint16_t)((A[2 * j + 1] & 0xffffu) >> 1);
   }
-  cbrec(out, pos, step * 2, q.data(), w - 1, n / 2);
-  cbrec(out, pos + step, step * 2, q.data() + n / 2, w - 1, n / 2);
+  cbrec(out, pos, step * 2, q, w - 1, n / 2, wk);
+  cbrec(out, pos + step, step * 2, q + n / 2, w - 1, n / 2, wk);
 }
 
 void controlbits_from_perm(int m, const std::vector<uint16_t> &pi, uint8_t *cb) {
   const size_t n = size_t(1) << m;
   std::memset(cb, 0, ((2 * m - 1) * n / 2 + 7) / 8);
-  cbrec(cb, 0, 1, pi.data(), m, n);
+  static thread_local CbWork wk;
+  cbrec(cb, 0, 1, pi.data(), m, n, wk);
 }
 
 // ------------------------------------------------------------------ additive FFT tables
