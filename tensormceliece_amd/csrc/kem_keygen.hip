@@ -34,10 +34,15 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
   constexpr int M = P::M, T = P::T, N = P::N, MT = P::MT, Q = P::Q;
   constexpr int W = ceil_div(N, 64);  // 64-bit words per row of H
   static_assert(W <= 128, "two row words per lane");
-  __shared__ uint16_t lg_s[1 << M];
-  __shared__ uint16_t ex_s[2 * Q];   // ex_s[i] = generator^(i mod q): a sum of two logs needs no reduction
+  // field tables while H is built; afterwards the same LDS stages the panel's pivot rows
+  constexpr int TABW = ceil_div(2 * ((1 << M) + 2 * Q), 8), STAGEW = 64 * 64;
+  __shared__ uint64_t tab_lds[TABW > STAGEW ? TABW : STAGEW];
+  uint16_t *lg_s = reinterpret_cast<uint16_t *>(tab_lds);
+  uint16_t *ex_s = lg_s + (1 << M);  // ex_s[i] = generator^(i mod q): a sum of two logs needs no reduction
+  uint64_t *stage = tab_lds;         // [pivot bit j][64 words] of the current panel
   __shared__ uint16_t rowix[MT];     // logical row -> physical row of H
-  __shared__ uint8_t flag[MT];       // physical rows the current step updates
+  __shared__ uint64_t pw[MT];        // the panel's word of every physical row
+  __shared__ uint64_t mk[MT];        // pivot rows (by bit) XORed into each physical row in this panel
   __shared__ int red[KG_WAVES];
   __shared__ int pvs[32];            // f variants: pivot columns of the window (relative to mt - 32)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -71,8 +76,14 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
   }
   __syncthreads();
 
-  // ---- Gauss-Jordan elimination to [I | T]
-  for (int c = 0; c < MT; ++c) {
+  // ---- Gauss-Jordan elimination to [I | T], one panel at a time.  A panel is a run of pivot
+  // columns inside one 64-bit word cw (it ends at the word's end, at mt, and for the f variants at
+  // mt - 32).  Phase 1 eliminates the panel's columns on the word cw of every row, in LDS, and
+  // records in mk[r] which of the panel's pivot rows (by column bit) row r has absorbed, as
+  // combinations of the pivot rows' contents at the panel's start.  Phase 2 applies those
+  // combinations to the words right of cw, 64 words at a time: each row is read and written once
+  // per panel instead of once per column.
+  for (int c = 0; c < MT;) {
     if constexpr (F) {
       if (c == MT - 32) {  // the f variants' column step (uniform branch)
         constexpr int C0 = MT - 32, W0 = C0 >> 6, SH = C0 & 63;
@@ -128,65 +139,70 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
       }
     }
     const int cw = c >> 6;
-    const uint64_t cm = 1ull << (c & 63);
-    // pivot: the first logical row at or below c with bit c set (the thread's rows loaded together)
-    constexpr int PR = ceil_div(MT, KG_WG);
-    uint64_t pw[PR];
-#pragma unroll
-    for (int u = 0; u < PR; ++u) {
-      const int r = c + tid + u * KG_WG;
-      pw[u] = r < MT ? H[(size_t)rowix[r] * W + cw] : 0ull;
+    int ce = 64 * (cw + 1) < MT ? 64 * (cw + 1) : MT;
+    if (F && c < MT - 32 && ce > MT - 32) ce = MT - 32;
+    for (int r = tid; r < MT; r += KG_WG) {
+      pw[r] = H[(size_t)r * W + cw];
+      mk[r] = 0;
     }
-    int best = MT;
-#pragma unroll
-    for (int u = PR - 1; u >= 0; --u)
-      if (pw[u] & cm) best = c + tid + u * KG_WG;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) best = min(best, __shfl_xor(best, o));
-    if (lane == 0) red[wave] = best;
     __syncthreads();
-    best = red[0];
-#pragma unroll
-    for (int w = 1; w < KG_WAVES; ++w) best = min(best, red[w]);
-    if (best == MT) {  // the left mt x mt block is singular (uniform over the workgroup)
-      if (tid == 0) ok_all[cand] = 0;
-      return;
-    }
-    const int pr = rowix[best], pc = rowix[c];
-    // rows to update: every physical row but the pivot row with bit c set
-    for (int r = tid; r < MT; r += KG_WG) flag[r] = (r != pr && (H[(size_t)r * W + cw] & cm)) ? 1 : 0;
-    __syncthreads();  // flags written, rowix and red read by every thread
-    if (tid == 0 && best != c) {
-      rowix[c] = (uint16_t)pr;
-      rowix[best] = (uint16_t)pc;
-    }
-    const uint64_t *prow = H + (size_t)pr * W;
-    const int w0 = cw + lane, w1 = cw + 64 + lane;
-    const uint64_t p0 = w0 < W ? prow[w0] : 0ull, p1 = w1 < W ? prow[w1] : 0ull;
-    // RU of the wave's rows at a time, loaded unconditionally so the loads overlap; flagged rows are
-    // written back
-    constexpr int RU = W > 64 ? 32 : 64;
-    for (int r0 = wave; r0 < MT; r0 += KG_WAVES * RU) {
-      uint64_t x0[RU], x1[RU];
-#pragma unroll
-      for (int u = 0; u < RU; ++u) {
-        const int r = r0 + u * KG_WAVES;
-        const uint64_t *row = H + (size_t)(r < MT ? r : 0) * W;
-        x0[u] = w0 < W ? row[w0] : 0ull;
-        if constexpr (W > 64) x1[u] = w1 < W ? row[w1] : 0ull;
-      }
-#pragma unroll
-      for (int u = 0; u < RU; ++u) {
-        const int r = r0 + u * KG_WAVES;
-        if (r < MT && flag[r]) {  // uniform in the wave
-          uint64_t *row = H + (size_t)r * W;
-          if (w0 < W) row[w0] = x0[u] ^ p0;
-          if constexpr (W > 64)
-            if (w1 < W) row[w1] = x1[u] ^ p1;
+    // phase 1
+    for (int cc = c; cc < ce; ++cc) {
+      const uint64_t bm = 1ull << (cc & 63);
+      int best = MT;  // first logical row at or below cc with the column bit (rows ascend per thread)
+      for (int l = cc + tid; l < MT; l += KG_WG)
+        if (pw[rowix[l]] & bm) {
+          best = l;
+          break;
         }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) best = min(best, __shfl_xor(best, o));
+      if (lane == 0) red[wave] = best;
+      __syncthreads();
+      best = red[0];
+#pragma unroll
+      for (int w = 1; w < KG_WAVES; ++w) best = min(best, red[w]);
+      if (best == MT) {  // the left mt x mt block is singular (uniform over the workgroup)
+        if (tid == 0) ok_all[cand] = 0;
+        return;
       }
+      const int pr = rowix[best], pc = rowix[cc];
+      const uint64_t ppw = pw[pr], pmk = mk[pr] ^ bm;
+      __syncthreads();  // rowix, red and the pivot row read by every thread
+      if (tid == 0 && best != cc) {
+        rowix[cc] = (uint16_t)pr;
+        rowix[best] = (uint16_t)pc;
+      }
+      for (int r = tid; r < MT; r += KG_WG)
+        if (r != pr && (pw[r] & bm)) {
+          pw[r] ^= ppw;
+          mk[r] ^= pmk;
+        }
+      __syncthreads();
     }
-    __syncthreads();
+    for (int r = tid; r < MT; r += KG_WG) H[(size_t)r * W + cw] = pw[r];
+    // phase 2: the pivot of column 64 cw + j sits at logical row 64 cw + j; its words right of cw
+    // are still as at the panel's start (phase 1 wrote word cw only)
+    for (int w0 = cw + 1; w0 < W; w0 += 64) {
+      for (int i = tid; i < STAGEW; i += KG_WG) {
+        const int j = i >> 6, col = 64 * cw + j, w = w0 + (i & 63);
+        stage[i] = (col >= c && col < ce && w < W) ? H[(size_t)rowix[col] * W + w] : 0ull;
+      }
+      __syncthreads();
+      for (int r = wave; r < MT; r += KG_WAVES) {
+        uint64_t m = mk[r];  // uniform in the wave
+        if (m == 0) continue;
+        uint64_t acc = 0;
+        while (m) {
+          const int j = __builtin_ctzll(m);
+          m &= m - 1;
+          acc ^= stage[64 * j + lane];
+        }
+        if (w0 + lane < W) H[(size_t)r * W + w0 + lane] ^= acc;
+      }
+      __syncthreads();  // the stage is rewritten for the next 64 words
+    }
+    c = ce;
   }
 
   // ---- pk row r = columns mt .. n-1 of logical row r (bits past n are zero in H)
