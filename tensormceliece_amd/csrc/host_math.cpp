@@ -585,14 +585,16 @@ static void apply_pivots_t(KeygenCandidate &c, uint64_t pivots) {
   c.pivots = pivots;
 }
 
+// sk = seed || pivots || g || control bits || s; cb == null: the control bits are computed here
 template <int SET>
-static void finish_t(const KeygenCandidate &c, uint8_t *sk) {
+static void finish_t(const KeygenCandidate &c, const uint8_t *cb, uint8_t *sk) {
   using P = Par<SET>;
   std::memcpy(sk, c.sk_seed, 32);
   for (int i = 0; i < 8; ++i) sk[32 + i] = (uint8_t)(c.pivots >> (8 * i));  // little-endian
   uint8_t *skp = sk + 40;
   for (int i = 0; i < P::T; ++i) { skp[2 * i] = (uint8_t)c.g[i]; skp[2 * i + 1] = (uint8_t)(c.g[i] >> 8); }
-  controlbits_from_perm(P::M, c.pi, skp + P::IRR_BYTES);
+  if (cb) std::memcpy(skp + P::IRR_BYTES, cb, P::COND_BYTES);
+  else controlbits_from_perm(P::M, c.pi, skp + P::IRR_BYTES);
   std::memcpy(skp + P::IRR_BYTES + P::COND_BYTES, c.s.data(), P::N / 8);
 }
 
@@ -610,7 +612,7 @@ static void keypair_t(uint8_t *pk, uint8_t *sk, rand_fn rnd, void *ctx, bool f) 
     if (systematic_pk<SET>(F, c.g.data(), c.alpha.data(), pk, f ? &pivots : nullptr)) break;
   }
   if (f) apply_pivots_t<SET>(c, pivots);
-  finish_t<SET>(c, sk);
+  finish_t<SET>(c, nullptr, sk);
 }
 
 void keygen_candidate(int set, uint8_t seed33[33], KeygenCandidate &c) {
@@ -624,16 +626,18 @@ void keygen_candidate(int set, uint8_t seed33[33], KeygenCandidate &c) {
   throw std::invalid_argument("set");
 }
 
-void keygen_finish(int set, const KeygenCandidate &c, uint8_t *sk) {
+void keygen_finish_with_cb(int set, const KeygenCandidate &c, const uint8_t *cb, uint8_t *sk) {
   switch (set) {
-    case S348864: return finish_t<S348864>(c, sk);
-    case S460896: return finish_t<S460896>(c, sk);
-    case S6688128: return finish_t<S6688128>(c, sk);
-    case S6960119: return finish_t<S6960119>(c, sk);
-    case S8192128: return finish_t<S8192128>(c, sk);
+    case S348864: return finish_t<S348864>(c, cb, sk);
+    case S460896: return finish_t<S460896>(c, cb, sk);
+    case S6688128: return finish_t<S6688128>(c, cb, sk);
+    case S6960119: return finish_t<S6960119>(c, cb, sk);
+    case S8192128: return finish_t<S8192128>(c, cb, sk);
   }
   throw std::invalid_argument("set");
 }
+
+void keygen_finish(int set, const KeygenCandidate &c, uint8_t *sk) { keygen_finish_with_cb(set, c, nullptr, sk); }
 
 void host_gen_e(int set, rand_fn rnd, void *ctx, uint16_t *pos) {
   const SetInfo &si = set_info(set);

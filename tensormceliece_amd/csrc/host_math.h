@@ -78,6 +78,8 @@ void keygen_candidate(int set, uint8_t seed33[33], KeygenCandidate &c);
 // in order, pv = the mask's set bits in increasing order
 void keygen_apply_pivots(int set, KeygenCandidate &c, uint64_t pivots);
 void keygen_finish(int set, const KeygenCandidate &c, uint8_t *sk);
+// the same with the Benes control bits of c.pi computed elsewhere (cb: COND_BYTES bytes)
+void keygen_finish_with_cb(int set, const KeygenCandidate &c, const uint8_t *cb, uint8_t *sk);
 
 void shake256(uint8_t *out, long long outlen, const uint8_t *in, long long inlen);
 

@@ -215,6 +215,9 @@ struct KeygenArgs {
   int ncand;
 };
 hipError_t launch_pk_gen_set(int set, const KeygenArgs &a, hipStream_t s);
+// Benes control bits of n keys' support permutations (pi: [n][2^m] u16) into cb: [n][(2m-1) 2^m / 8]
+// bytes (LM:crypto_kem/<set>/vec/controlbits.c), one workgroup per key
+hipError_t launch_controlbits(int m, const uint16_t *pi, uint8_t *cb, int nkeys, hipStream_t s);
 size_t pk_gen_h_bytes(int set);
 
 hipError_t launch_enc_set(int set, const EncArgs &a, hipStream_t s);

@@ -218,6 +218,158 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
   if (tid == 0) ok_all[cand] = 1;
 }
 
+// ------------------------------------------------------------------ Benes control bits
+// The recursion of LM:crypto_kem/<set>/vec/controlbits.c (cbrecursion), level by level: at level d
+// (w = m - d) the 2^d subproblems are the segments of 2^w entries of the level's permutation array,
+// each a permutation of its own 0 .. 2^w - 1, and every step of the recursion body runs on all
+// segments at once (elementwise, or as a bitonic sort inside each segment -- the keys of a sort are
+// distinct, so any correct sort gives the library's order).  Subproblem s of level d (its choices
+// child 0 / child 1 from the top are the bits of s, most significant first) writes its first-layer
+// bits at d n/2 + rev_d(s) + 2^d j and its last-layer bits at (2m - 2 - d) n/2 + rev_d(s) + 2^d k,
+// rev_d = d-bit reversal, which is where the recursion's position and step arithmetic puts them;
+// its children are its two halves, segments 2s and 2s + 1 of level d + 1.  One workgroup per key.
+constexpr int CB_WG = 1024;
+
+template <int MM>
+__device__ __forceinline__ void cb_sort(uint32_t *v, int w, int tid) {
+  constexpr int NN = 1 << MM;
+  for (int k = 2; k <= (1 << w); k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int p = tid; p < NN / 2; p += CB_WG) {
+        const int i = (p / j) * 2 * j + (p % j), l = i + j;
+        const uint32_t a = v[i], b = v[l];
+        // ascending, except the first halves of the merges below the segment size; the last merge
+        // (k = segment size) is ascending in every segment
+        const bool up = k == (1 << w) || (i & k) == 0;
+        if ((a > b) == up) {
+          v[i] = b;
+          v[l] = a;
+        }
+      }
+      __syncthreads();
+    }
+}
+
+template <int MM>
+__global__ __launch_bounds__(CB_WG) void k_controlbits(const uint16_t *__restrict__ pi_all, uint8_t *__restrict__ cb_all) {
+  constexpr int NN = 1 << MM, HALF = NN / 2;
+  constexpr int CBW = (2 * MM - 1) * HALF / 32;  // control-bit words: (2m - 1) layers of n/2 bits
+  __shared__ uint32_t A[NN], B[NN], cbw[CBW];
+  __shared__ uint16_t PI[NN];
+  const int tid = threadIdx.x;
+  const uint16_t *pi = pi_all + (size_t)blockIdx.x * NN;
+  for (int x = tid; x < NN; x += CB_WG) PI[x] = pi[x];
+  for (int i = tid; i < CBW; i += CB_WG) cbw[i] = 0;
+  __syncthreads();
+  auto flip = [&](int pos, uint32_t b) {
+    if (b & 1u) atomicXor(&cbw[pos >> 5], 1u << (pos & 31));
+  };
+  for (int d = 0; d < MM - 1; ++d) {
+    const int w = MM - d, ns = 1 << w, lm = ns - 1;
+    for (int x = tid; x < NN; x += CB_WG) A[x] = ((uint32_t)(PI[x] ^ 1u) << 16) | PI[x ^ 1];
+    __syncthreads();
+    cb_sort<MM>(A, w, tid);
+    for (int x = tid; x < NN; x += CB_WG) {
+      const uint32_t px = A[x] & 0xffffu, lx = (uint32_t)(x & lm);
+      B[x] = (px << 16) | (px < lx ? px : lx);
+      A[x] = (A[x] << 16) | lx;
+    }
+    __syncthreads();
+    cb_sort<MM>(A, w, tid);
+    for (int x = tid; x < NN; x += CB_WG) A[x] = (A[x] << 16) + (B[x] >> 16);
+    __syncthreads();
+    cb_sort<MM>(A, w, tid);
+    if (w <= 10) {
+      for (int x = tid; x < NN; x += CB_WG) B[x] = ((A[x] & 0xffffu) << 10) | (B[x] & 0x3ffu);
+      __syncthreads();
+      for (int i = 1; i < w - 1; ++i) {
+        for (int x = tid; x < NN; x += CB_WG) A[x] = ((B[x] & ~0x3ffu) << 6) | (uint32_t)(x & lm);
+        __syncthreads();
+        cb_sort<MM>(A, w, tid);
+        for (int x = tid; x < NN; x += CB_WG) A[x] = (A[x] << 20) | B[x];
+        __syncthreads();
+        cb_sort<MM>(A, w, tid);
+        for (int x = tid; x < NN; x += CB_WG) {
+          const uint32_t u = (A[x] & 0xffc00u) | (B[x] & 0x3ffu), v = A[x] & 0xfffffu;
+          B[x] = u < v ? u : v;
+        }
+        __syncthreads();
+      }
+      for (int x = tid; x < NN; x += CB_WG) B[x] &= 0x3ffu;
+    } else {
+      for (int x = tid; x < NN; x += CB_WG) B[x] = (A[x] << 16) | (B[x] & 0xffffu);
+      __syncthreads();
+      for (int i = 1; i < w - 1; ++i) {
+        for (int x = tid; x < NN; x += CB_WG) A[x] = (B[x] & ~0xffffu) | (uint32_t)(x & lm);
+        __syncthreads();
+        cb_sort<MM>(A, w, tid);
+        for (int x = tid; x < NN; x += CB_WG) A[x] = (A[x] << 16) | (B[x] & 0xffffu);
+        __syncthreads();
+        if (i < w - 2) {
+          for (int x = tid; x < NN; x += CB_WG) B[x] = (A[x] & ~0xffffu) | (B[x] >> 16);
+          __syncthreads();
+          cb_sort<MM>(B, w, tid);
+          for (int x = tid; x < NN; x += CB_WG) B[x] = (B[x] << 16) | (A[x] & 0xffffu);
+          __syncthreads();
+        }
+        cb_sort<MM>(A, w, tid);
+        for (int x = tid; x < NN; x += CB_WG) {
+          const uint32_t u = (B[x] & ~0xffffu) | (A[x] & 0xffffu);
+          B[x] = B[x] < u ? B[x] : u;
+        }
+        __syncthreads();
+      }
+      for (int x = tid; x < NN; x += CB_WG) B[x] &= 0xffffu;
+    }
+    __syncthreads();
+    for (int x = tid; x < NN; x += CB_WG) A[x] = ((uint32_t)PI[x] << 16) + (uint32_t)(x & lm);
+    __syncthreads();
+    cb_sort<MM>(A, w, tid);
+    const int rb = d ? 32 - d : 0;  // rev_d(seg) = bit reversal of the segment index in d bits
+    for (int p = tid; p < HALF; p += CB_WG) {  // first layer: pair (2j, 2j + 1) of segment seg
+      const int x = 2 * p, seg = x >> w, j = (x & lm) >> 1;
+      const int rv = d ? (int)(__builtin_bitreverse32((uint32_t)seg) >> rb) : 0;
+      const uint32_t fj = B[x] & 1u, Fz = (uint32_t)(x & lm) + fj;
+      flip(d * HALF + rv + (j << d), fj);
+      B[x] = (A[x] << 16) | Fz;
+      B[x + 1] = (A[x + 1] << 16) | (Fz ^ 1u);
+    }
+    __syncthreads();
+    cb_sort<MM>(B, w, tid);
+    for (int p = tid; p < HALF; p += CB_WG) {  // last layer
+      const int y = 2 * p, seg = y >> w, k = (y & lm) >> 1;
+      const int rv = d ? (int)(__builtin_bitreverse32((uint32_t)seg) >> rb) : 0;
+      const uint32_t lk = B[y] & 1u, Ly = (uint32_t)(y & lm) + lk;
+      flip((2 * MM - 2 - d) * HALF + rv + (k << d), lk);
+      A[y] = (Ly << 16) | (B[y] & 0xffffu);
+      A[y + 1] = ((Ly ^ 1u) << 16) | (B[y + 1] & 0xffffu);
+    }
+    __syncthreads();
+    cb_sort<MM>(A, w, tid);
+    for (int p = tid; p < HALF; p += CB_WG) {  // the children's permutations: the segment's halves
+      const int y = 2 * p, seg = y >> w, j = (y & lm) >> 1, base = seg << w;
+      PI[base + j] = (uint16_t)((A[y] & 0xffffu) >> 1);
+      PI[base + (ns >> 1) + j] = (uint16_t)((A[y + 1] & 0xffffu) >> 1);
+    }
+    __syncthreads();
+  }
+  for (int p = tid; p < HALF; p += CB_WG) {  // w = 1: the middle layer, one bit per segment of 2
+    const int rv = (int)(__builtin_bitreverse32((uint32_t)p) >> (32 - (MM - 1)));
+    flip((MM - 1) * HALF + rv, PI[2 * p]);
+  }
+  __syncthreads();
+  uint32_t *out = reinterpret_cast<uint32_t *>(cb_all + (size_t)blockIdx.x * (CBW * 4));
+  for (int i = tid; i < CBW; i += CB_WG) out[i] = cbw[i];
+}
+
+hipError_t launch_controlbits(int m, const uint16_t *pi, uint8_t *cb, int nkeys, hipStream_t s) {
+  if (nkeys <= 0) return hipSuccess;
+  if (m == 12) k_controlbits<12><<<nkeys, CB_WG, 0, s>>>(pi, cb);
+  else if (m == 13) k_controlbits<13><<<nkeys, CB_WG, 0, s>>>(pi, cb);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 template <int SET>
 hipError_t pk_gen(const KeygenArgs &a, hipStream_t s) {
   if (a.ncand <= 0) return hipSuccess;

@@ -47,6 +47,8 @@ struct KgBuffers {
   uint8_t *pk = nullptr;
   int *ok = nullptr;
   uint64_t *piv = nullptr;
+  uint16_t *pi = nullptr;  // support permutations of the keys whose control bits are computed
+  uint8_t *cb = nullptr;   // their control bits
   hipStream_t s = nullptr;
   ~KgBuffers() {
     if (s) (void)hipStreamSynchronize(s);
@@ -56,6 +58,8 @@ struct KgBuffers {
     (void)hipFree(pk);
     (void)hipFree(ok);
     (void)hipFree(piv);
+    (void)hipFree(pi);
+    (void)hipFree(cb);
     if (s) (void)hipStreamDestroy(s);
   }
 };
@@ -67,6 +71,7 @@ struct KgBuffers {
 // libmceliece's outcome, in fewer rounds.  A launch takes about as long for one candidate per CU
 // as for one candidate in all, so filling it costs little.
 constexpr size_t KG_LAUNCH = 512;
+constexpr size_t KG_CB_KEYS = 1024;  // control-bit workgroups per launch
 constexpr size_t KG_AHEAD_MIN = 2, KG_AHEAD_MAX = 8;
 
 }  // namespace
@@ -103,6 +108,9 @@ int tmce_keypair_batch(int variant, unsigned char *pk, unsigned char *sk, const 
       hipMalloc((void **)&b.ok, cap * sizeof(int)) != hipSuccess ||
       hipMalloc((void **)&b.piv, cap * sizeof(uint64_t)) != hipSuccess)
     return fail(TMCE_ENOMEM, "hipMalloc key generation buffers");
+  const size_t NQ = (size_t)1 << si.m, CBB = (size_t)si.cond_bytes, cbcap = std::min(n, KG_CB_KEYS);
+  if (hipMalloc((void **)&b.pi, cbcap * NQ * 2) != hipSuccess || hipMalloc((void **)&b.cb, cbcap * CBB) != hipSuccess)
+    return fail(TMCE_ENOMEM, "hipMalloc key generation buffers");
 
   // cand[i][k]: attempt k of key i in this round
   std::vector<std::vector<KeygenCandidate>> cand(n);
@@ -113,6 +121,8 @@ int tmce_keypair_batch(int variant, unsigned char *pk, unsigned char *sk, const 
   std::vector<uint16_t> hg(cap * T1), ha(cap * N);
   std::vector<int> hok(cap);
   std::vector<uint64_t> hpiv(cap);
+  std::vector<uint16_t> hpi(cbcap * NQ);
+  std::vector<uint8_t> hcb(cbcap * CBB);
   while (!pending.empty()) {
     const size_t ahead = std::max(KG_AHEAD_MIN, std::min(KG_AHEAD_MAX, KG_LAUNCH / pending.size()));
     parallel_for(pending.size(), [&](size_t k) {  // a key's attempts follow its chain, in order
@@ -153,12 +163,25 @@ int tmce_keypair_batch(int variant, unsigned char *pk, unsigned char *sk, const 
     }
     std::vector<size_t> done, failed;
     for (size_t i : pending) (win[i] >= 0 ? done : failed).push_back(i);
-    parallel_for(done.size(), [&](size_t k) {
-      const size_t i = done[k];
-      if (f) keygen_apply_pivots(set, cand[i][win[i]], win_piv[i]);
-      keygen_finish(set, cand[i][win[i]], sk + i * SKB);
-      cand[i].clear();
-    });
+    // secret keys of the keys done: support permutations (the f variants' swaps applied) on the
+    // host, Benes control bits on the GPU (one workgroup per key), the rest of sk on the host
+    for (size_t off = 0; off < done.size(); off += cbcap) {
+      const size_t nd = std::min(cbcap, done.size() - off);
+      parallel_for(nd, [&](size_t k) {
+        const size_t i = done[off + k];
+        if (f) keygen_apply_pivots(set, cand[i][win[i]], win_piv[i]);
+        std::memcpy(&hpi[k * NQ], cand[i][win[i]].pi.data(), NQ * 2);
+      });
+      HIPCHK(hipMemcpyAsync(b.pi, hpi.data(), nd * NQ * 2, hipMemcpyHostToDevice, b.s));
+      HIPCHK(launch_controlbits(si.m, b.pi, b.cb, (int)nd, b.s));
+      HIPCHK(hipMemcpyAsync(hcb.data(), b.cb, nd * CBB, hipMemcpyDeviceToHost, b.s));
+      HIPCHK(hipStreamSynchronize(b.s));
+      parallel_for(nd, [&](size_t k) {
+        const size_t i = done[off + k];
+        keygen_finish_with_cb(set, cand[i][win[i]], &hcb[k * CBB], sk + i * SKB);
+        cand[i].clear();
+      });
+    }
     pending.swap(failed);
   }
   return 0;
