@@ -570,6 +570,47 @@ static void candidate_t(uint8_t seed33[33], KeygenCandidate &c) {
 }
 
 template <int SET>
+static bool step_t(uint8_t seed33[33], KeygenCandidate &c) {
+  using P = Par<SET>;
+  constexpr int NQ = 1 << P::M;
+  const size_t rlen = P::N / 8 + 4 * (size_t)NQ + 2 * P::T + 32;
+  std::vector<uint8_t> r(rlen);
+  std::vector<std::pair<uint32_t, uint16_t>> keys(NQ);
+  c.f.assign(P::T, 0);
+  c.g.assign(P::T + 1, 0);
+  c.alpha.assign(P::N, 0);
+  c.pi.assign(NQ, 0);
+  shake256(r.data(), (long long)rlen, seed33, 33);
+  std::memcpy(c.sk_seed, seed33 + 1, 32);
+  std::memcpy(seed33 + 1, &r[rlen - 32], 32);  // the next attempt's seed, whatever happens below
+  const uint8_t *fp = &r[rlen - 32 - 2 * P::T];
+  for (int i = 0; i < P::T; ++i) c.f[i] = (uint16_t)((fp[2 * i] | (fp[2 * i + 1] << 8)) & ((1 << P::M) - 1));
+  const uint8_t *pp = &r[P::N / 8];
+  for (int i = 0; i < NQ; ++i)
+    keys[i] = {(uint32_t)pp[4 * i] | ((uint32_t)pp[4 * i + 1] << 8) | ((uint32_t)pp[4 * i + 2] << 16) |
+                   ((uint32_t)pp[4 * i + 3] << 24),
+               (uint16_t)i};
+  std::sort(keys.begin(), keys.end());
+  for (int i = 1; i < NQ; ++i)
+    if (keys[i - 1].first == keys[i].first) return false;
+  for (int i = 0; i < NQ; ++i) c.pi[i] = keys[i].second;
+  for (int i = 0; i < P::N; ++i) c.alpha[i] = bitrev(c.pi[i], P::M);
+  c.s.assign(r.begin(), r.begin() + P::N / 8);
+  return true;
+}
+
+bool keygen_step(int set, uint8_t seed33[33], KeygenCandidate &c) {
+  switch (set) {
+    case S348864: return step_t<S348864>(seed33, c);
+    case S460896: return step_t<S460896>(seed33, c);
+    case S6688128: return step_t<S6688128>(seed33, c);
+    case S6960119: return step_t<S6960119>(seed33, c);
+    case S8192128: return step_t<S8192128>(seed33, c);
+  }
+  throw std::invalid_argument("set");
+}
+
+template <int SET>
 static void apply_pivots_t(KeygenCandidate &c, uint64_t pivots) {
   using P = Par<SET>;
   constexpr int C0 = P::MT - 32;

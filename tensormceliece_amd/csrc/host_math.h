@@ -71,8 +71,13 @@ struct KeygenCandidate {
   std::vector<uint16_t> pi;     // the permutation of 2^m elements behind the support
   std::vector<uint8_t> s;       // implicit-rejection string, n/8 bytes
   uint64_t pivots = 0xFFFFFFFFull;  // sk bytes 32..39 (the f variants' pivot mask)
+  std::vector<uint16_t> f;      // keygen_step: the element of GF((2^m)^t) whose minimal polynomial is g
 };
 void keygen_candidate(int set, uint8_t seed33[33], KeygenCandidate &c);
+// One attempt of the chain without the Goppa polynomial (computed elsewhere from c.f): seed
+// expansion, f, support by sorting; advances seed33.  False when the support's sort keys repeat
+// (the attempt fails whatever g is).
+bool keygen_step(int set, uint8_t seed33[33], KeygenCandidate &c);
 // the f variants' column step as the systematic form reported it (pivot mask over columns
 // mt-32 .. mt+31): swaps positions mt-32+i and mt-32+pv[i] of the support permutation, i = 0..31
 // in order, pv = the mask's set bits in increasing order

@@ -212,9 +212,14 @@ struct KeygenArgs {
   uint8_t *pk;            // [ncand][PK_BYTES]
   int *ok;                // [ncand] 1: pk written, 0: the left mt x mt block is singular
   uint64_t *pivots;       // f variants: [ncand] pivot mask of the column step (null otherwise)
+  const int *valid;       // [ncand] 0: the attempt failed before the systematic form (null: all valid)
   int ncand;
 };
 hipError_t launch_pk_gen_set(int set, const KeygenArgs &a, hipStream_t s);
+// Goppa polynomials (monic, [ncand][T+1]) of candidates f ([ncand][T]); valid[c] is cleared when the
+// powers of f are dependent; candidates with valid[c] == 0 on entry are skipped
+hipError_t launch_genpoly_set(int set, const uint16_t *f, const uint16_t *lg, const uint16_t *ex, uint16_t *g,
+                              int *valid, int ncand, hipStream_t s);
 // Benes control bits of n keys' support permutations (pi: [n][2^m] u16) into cb: [n][(2m-1) 2^m / 8]
 // bytes (LM:crypto_kem/<set>/vec/controlbits.c), one workgroup per key
 hipError_t launch_controlbits(int m, const uint16_t *pi, uint8_t *cb, int nkeys, hipStream_t s);

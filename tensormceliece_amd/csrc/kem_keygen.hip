@@ -29,7 +29,8 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
                                                  const uint16_t *__restrict__ alpha_all,
                                                  const uint16_t *__restrict__ lg, const uint16_t *__restrict__ ex,
                                                  uint64_t *__restrict__ H_all, uint8_t *__restrict__ pk_all,
-                                                 int *__restrict__ ok_all, uint64_t *__restrict__ piv_all) {
+                                                 int *__restrict__ ok_all, uint64_t *__restrict__ piv_all,
+                                                 const int *__restrict__ valid) {
   using P = Par<SET>;
   constexpr int M = P::M, T = P::T, N = P::N, MT = P::MT, Q = P::Q;
   constexpr int W = ceil_div(N, 64);  // 64-bit words per row of H
@@ -47,6 +48,10 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
   __shared__ int pvs[32];            // f variants: pivot columns of the window (relative to mt - 32)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cand = blockIdx.x;
+  if (valid && !valid[cand]) {  // the attempt failed before the systematic form (uniform)
+    if (tid == 0) ok_all[cand] = 0;
+    return;
+  }
   const uint16_t *g = g_all + (size_t)cand * (T + 1);
   const uint16_t *alpha = alpha_all + (size_t)cand * N;
   uint64_t *H = H_all + (size_t)cand * MT * W;
@@ -218,6 +223,154 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
   if (tid == 0) ok_all[cand] = 1;
 }
 
+// ------------------------------------------------------------------ Goppa polynomial
+// g = the minimal polynomial of f over GF(2^m) in GF((2^m)^t) = GF(2^m)[y] / F(y)
+// (LM:crypto_kem/<set>/vec/sk_gen.c genpoly_gen, gf_2mt_mul.c for F): the powers f^0 .. f^t as the
+// columns of a t x (t + 1) matrix, Gauss-Jordan on its first t columns, g_i = the last column
+// (the solution is unique when the system is nonsingular, so the elimination order is free).  One
+// workgroup per candidate; products through log/exp tables in LDS.  y^t = F_low(y): the taps of F
+// below t with their coefficients (all 1, except z = 2 at y^0 for 348864).
+template <int SET> struct ExtTaps;
+template <> struct ExtTaps<S348864> { static constexpr int NT = 3; static constexpr int tap[3] = {3, 1, 0}; static constexpr int coef[3] = {1, 1, 2}; };
+template <> struct ExtTaps<S460896> { static constexpr int NT = 4; static constexpr int tap[4] = {10, 9, 6, 0}; static constexpr int coef[4] = {1, 1, 1, 1}; };
+template <> struct ExtTaps<S6960119> { static constexpr int NT = 2; static constexpr int tap[2] = {8, 0}; static constexpr int coef[2] = {1, 1}; };
+template <> struct ExtTaps<S6688128> { static constexpr int NT = 4; static constexpr int tap[4] = {7, 2, 1, 0}; static constexpr int coef[4] = {1, 1, 1, 1}; };
+template <> struct ExtTaps<S8192128> { static constexpr int NT = 4; static constexpr int tap[4] = {7, 2, 1, 0}; static constexpr int coef[4] = {1, 1, 1, 1}; };
+
+constexpr int GP_WG = 256;
+
+template <int SET>
+__global__ __launch_bounds__(GP_WG) void k_genpoly(const uint16_t *__restrict__ f_all, const uint16_t *__restrict__ lg,
+                                                   const uint16_t *__restrict__ ex, uint16_t *__restrict__ g_all,
+                                                   int *__restrict__ valid) {
+  using P = Par<SET>;
+  using X = ExtTaps<SET>;
+  constexpr int M = P::M, T = P::T, Q = P::Q;
+  constexpr uint32_t ZL = 0xFFFFu;  // log of 0
+  constexpr int C = T + 1;          // matrix columns (powers 0 .. t)
+  static_assert(T >= 2 * 10, "taps stay below t/2");
+  __shared__ uint16_t lg_s[1 << M];
+  __shared__ uint16_t ex_s[2 * Q];  // ex_s[i] = generator^(i mod q)
+  __shared__ uint16_t mat[T * C];   // mat[i C + j] = coefficient i of f^j
+  __shared__ uint16_t lf[T], la[T], fac[T];
+  __shared__ uint16_t prod[2 * T];
+  __shared__ int red[GP_WG / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, cand = blockIdx.x;
+  if (!valid[cand]) return;  // the attempt failed already (support sort keys repeat): uniform
+  for (int i = tid; i < (1 << M); i += GP_WG) lg_s[i] = lg[i];
+  for (int i = tid; i < 2 * Q; i += GP_WG) ex_s[i] = ex[i < Q ? i : i - Q];
+  __syncthreads();
+  auto lgz = [&](uint32_t a) -> uint32_t { return a ? (uint32_t)lg_s[a] : ZL; };
+  auto mul_l = [&](uint32_t la_, uint32_t lb_) -> uint32_t {  // product of two logs
+    return (la_ == ZL || lb_ == ZL) ? 0u : (uint32_t)ex_s[la_ + lb_];
+  };
+  const uint16_t *f = f_all + (size_t)cand * T;
+  for (int i = tid; i < T; i += GP_WG) {
+    const uint32_t fi = f[i];
+    lf[i] = (uint16_t)lgz(fi);
+    mat[i * C + 0] = i == 0 ? 1 : 0;
+    mat[i * C + 1] = (uint16_t)fi;
+  }
+  __syncthreads();
+  // ---- f^j = f^(j-1) f mod F(y), j = 2 .. t
+  for (int j = 2; j <= T; ++j) {
+    for (int i = tid; i < T; i += GP_WG) la[i] = (uint16_t)lgz(mat[i * C + j - 1]);
+    __syncthreads();
+    for (int k = tid; k < 2 * T - 1; k += GP_WG) {
+      uint32_t acc = 0;
+      const int lo = k - T + 1 > 0 ? k - T + 1 : 0, hi = k < T - 1 ? k : T - 1;
+      for (int i = lo; i <= hi; ++i) acc ^= mul_l(la[i], lf[k - i]);
+      prod[k] = (uint16_t)acc;
+    }
+    __syncthreads();
+    // y^i for i >= t: the reduction from the top, in closed form.  Index q in [t, 2t-2] receives
+    // tap contributions only from indices >= 2t - 10, which receive none, so
+    // c_q = p_q + sum_taps coef c_(q + t - tap) with c = p on the right; then the output below t
+    // is p_k + sum_taps coef c_(k + t - tap).
+    for (int q = T + tid; q < 2 * T - 1; q += GP_WG) {
+      uint32_t v = prod[q];
+#pragma unroll
+      for (int u = 0; u < X::NT; ++u) {
+        const int src = q + T - X::tap[u];
+        if (src <= 2 * T - 2) {
+          const uint32_t sv = prod[src];
+          v ^= X::coef[u] == 1 ? sv : mul_l(lgz(sv), lg_s[X::coef[u]]);
+        }
+      }
+      la[q - T] = (uint16_t)v;  // c_q, parked in la (free now) at q - t <= t - 2
+    }
+    __syncthreads();
+    for (int k = tid; k < T; k += GP_WG) {
+      uint32_t v = prod[k];
+#pragma unroll
+      for (int u = 0; u < X::NT; ++u) {
+        const int src = k + T - X::tap[u];
+        if (src >= T && src <= 2 * T - 2) {
+          const uint32_t cv = la[src - T];
+          v ^= X::coef[u] == 1 ? cv : mul_l(lgz(cv), lg_s[X::coef[u]]);
+        }
+      }
+      mat[k * C + j] = (uint16_t)v;
+    }
+    __syncthreads();
+  }
+  // ---- Gauss-Jordan on columns 0 .. t-1; column t is the right-hand side
+  for (int c = 0; c < T; ++c) {
+    int best = T;
+    for (int r = c + tid; r < T; r += GP_WG)
+      if (mat[r * C + c]) {
+        best = r;
+        break;
+      }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) best = min(best, __shfl_xor(best, o));
+    if (lane == 0) red[wave] = best;
+    __syncthreads();
+    best = red[0];
+#pragma unroll
+    for (int w = 1; w < GP_WG / 64; ++w) best = min(best, red[w]);
+    if (best == T) {  // 1, f, ..., f^(t-1) are linearly dependent: the attempt fails (uniform)
+      if (tid == 0) valid[cand] = 0;
+      return;
+    }
+    if (best != c)
+      for (int j = c + tid; j < C; j += GP_WG) {
+        const uint16_t t0 = mat[c * C + j];
+        mat[c * C + j] = mat[best * C + j];
+        mat[best * C + j] = t0;
+      }
+    __syncthreads();
+    const uint32_t linv = (uint32_t)(Q - lg_s[mat[c * C + c]]);  // log of the pivot's inverse (< 2q)
+    __syncthreads();
+    for (int j = c + tid; j < C; j += GP_WG) mat[c * C + j] = (uint16_t)mul_l(lgz(mat[c * C + j]), linv);
+    for (int r = tid; r < T; r += GP_WG) fac[r] = r == c ? 0 : mat[r * C + c];
+    __syncthreads();
+    for (int e = tid; e < T * (C - c); e += GP_WG) {
+      const int r = e / (C - c), j = c + e % (C - c);
+      const uint32_t fr = fac[r];
+      if (fr) mat[r * C + j] ^= (uint16_t)mul_l(lgz(fr), lgz(mat[c * C + j]));
+    }
+    __syncthreads();
+  }
+  uint16_t *g = g_all + (size_t)cand * (T + 1);
+  for (int i = tid; i < T; i += GP_WG) g[i] = mat[i * C + T];
+  if (tid == 0) g[T] = 1;
+}
+
+hipError_t launch_genpoly_set(int set, const uint16_t *f, const uint16_t *lg, const uint16_t *ex, uint16_t *g,
+                              int *valid, int ncand, hipStream_t s) {
+  if (ncand <= 0) return hipSuccess;
+  switch (set) {
+    case S348864: k_genpoly<S348864><<<ncand, GP_WG, 0, s>>>(f, lg, ex, g, valid); break;
+    case S460896: k_genpoly<S460896><<<ncand, GP_WG, 0, s>>>(f, lg, ex, g, valid); break;
+    case S6688128: k_genpoly<S6688128><<<ncand, GP_WG, 0, s>>>(f, lg, ex, g, valid); break;
+    case S6960119: k_genpoly<S6960119><<<ncand, GP_WG, 0, s>>>(f, lg, ex, g, valid); break;
+    case S8192128: k_genpoly<S8192128><<<ncand, GP_WG, 0, s>>>(f, lg, ex, g, valid); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ Benes control bits
 // The recursion of LM:crypto_kem/<set>/vec/controlbits.c (cbrecursion), level by level: at level d
 // (w = m - d) the 2^d subproblems are the segments of 2^w entries of the level's permutation array,
@@ -374,9 +527,9 @@ template <int SET>
 hipError_t pk_gen(const KeygenArgs &a, hipStream_t s) {
   if (a.ncand <= 0) return hipSuccess;
   if (a.pivots)
-    k_pk_gen<SET, true><<<a.ncand, KG_WG, 0, s>>>(a.g, a.alpha, a.lg, a.ex, a.H, a.pk, a.ok, a.pivots);
+    k_pk_gen<SET, true><<<a.ncand, KG_WG, 0, s>>>(a.g, a.alpha, a.lg, a.ex, a.H, a.pk, a.ok, a.pivots, a.valid);
   else
-    k_pk_gen<SET, false><<<a.ncand, KG_WG, 0, s>>>(a.g, a.alpha, a.lg, a.ex, a.H, a.pk, a.ok, nullptr);
+    k_pk_gen<SET, false><<<a.ncand, KG_WG, 0, s>>>(a.g, a.alpha, a.lg, a.ex, a.H, a.pk, a.ok, nullptr, a.valid);
   return hipGetLastError();
 }
 

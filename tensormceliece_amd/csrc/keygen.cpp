@@ -1,9 +1,9 @@
 // keygen.cpp -- tmce_keypair_batch: batched seeded key generation.
 //
-// Host threads run each key's attempts up to the systematic-form step (keygen_candidate: seed
-// expansion, Goppa polynomial, support) and write its secret key after that step succeeds
-// (keygen_finish: Benes control bits, rejection string).  The systematic-form step of every pending
-// candidate runs on the GPU, one workgroup per candidate (k_pk_gen).  A key whose candidate is
+// Host threads run each key's attempts up to the Goppa polynomial (keygen_step: seed expansion, f,
+// support by sorting) and write its secret key after the attempt succeeds.  The GPU computes the
+// Goppa polynomial of every pending attempt (k_genpoly), the systematic form (k_pk_gen) and, for the
+// keys done, the Benes control bits (k_controlbits), one workgroup per attempt or key.  A key whose candidate is
 // singular continues its seed chain in the next round, as libmceliece's keypair loop does, so key
 // i is byte-identical to tmce_keypair_seeded on seeds[32 i].  For the f variants the kernel also
 // runs the semi-systematic column step and reports its pivot mask, which the host applies to the
@@ -42,7 +42,8 @@ void parallel_for(size_t n, Fn fn) {
 
 // device buffers of one call (freed on every exit path)
 struct KgBuffers {
-  uint16_t *g = nullptr, *alpha = nullptr;
+  uint16_t *g = nullptr, *alpha = nullptr, *fv = nullptr;
+  int *valid = nullptr;
   uint64_t *H = nullptr;
   uint8_t *pk = nullptr;
   int *ok = nullptr;
@@ -53,6 +54,8 @@ struct KgBuffers {
   ~KgBuffers() {
     if (s) (void)hipStreamSynchronize(s);
     (void)hipFree(g);
+    (void)hipFree(fv);
+    (void)hipFree(valid);
     (void)hipFree(alpha);
     (void)hipFree(H);
     (void)hipFree(pk);
@@ -106,7 +109,8 @@ int tmce_keypair_batch(int variant, unsigned char *pk, unsigned char *sk, const 
   if (hipMalloc((void **)&b.g, cap * T1 * 2) != hipSuccess || hipMalloc((void **)&b.alpha, cap * N * 2) != hipSuccess ||
       hipMalloc((void **)&b.H, cap * hw * 8) != hipSuccess || hipMalloc((void **)&b.pk, cap * PKB) != hipSuccess ||
       hipMalloc((void **)&b.ok, cap * sizeof(int)) != hipSuccess ||
-      hipMalloc((void **)&b.piv, cap * sizeof(uint64_t)) != hipSuccess)
+      hipMalloc((void **)&b.piv, cap * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc((void **)&b.fv, cap * (T1 - 1) * 2) != hipSuccess || hipMalloc((void **)&b.valid, cap * sizeof(int)) != hipSuccess)
     return fail(TMCE_ENOMEM, "hipMalloc key generation buffers");
   const size_t NQ = (size_t)1 << si.m, CBB = (size_t)si.cond_bytes, cbcap = std::min(n, KG_CB_KEYS);
   if (hipMalloc((void **)&b.pi, cbcap * NQ * 2) != hipSuccess || hipMalloc((void **)&b.cb, cbcap * CBB) != hipSuccess)
@@ -118,7 +122,9 @@ int tmce_keypair_batch(int variant, unsigned char *pk, unsigned char *sk, const 
   std::vector<uint64_t> win_piv(n, 0);
   std::vector<size_t> pending(n);
   for (size_t i = 0; i < n; ++i) pending[i] = i;
-  std::vector<uint16_t> hg(cap * T1), ha(cap * N);
+  std::vector<uint16_t> hf(cap * (T1 - 1)), ha(cap * N);
+  std::vector<int> hvalid(cap);
+  std::vector<std::vector<uint8_t>> sok(n);  // per key and attempt: the support's sort keys are distinct
   std::vector<int> hok(cap);
   std::vector<uint64_t> hpiv(cap);
   std::vector<uint16_t> hpi(cbcap * NQ);
@@ -128,7 +134,8 @@ int tmce_keypair_batch(int variant, unsigned char *pk, unsigned char *sk, const 
     parallel_for(pending.size(), [&](size_t k) {  // a key's attempts follow its chain, in order
       const size_t i = pending[k];
       cand[i].resize(ahead);
-      for (size_t a = 0; a < ahead; ++a) keygen_candidate(set, chain[i].data(), cand[i][a]);
+      sok[i].assign(ahead, 0);
+      for (size_t a = 0; a < ahead; ++a) sok[i][a] = keygen_step(set, chain[i].data(), cand[i][a]) ? 1 : 0;
     });
     const size_t total = pending.size() * ahead;  // candidate c = (key pending[c / ahead], attempt c % ahead)
     std::vector<int> okv(total, 0);
@@ -137,12 +144,16 @@ int tmce_keypair_batch(int variant, unsigned char *pk, unsigned char *sk, const 
       for (size_t k = 0; k < nc; ++k) {
         const size_t c = off + k;
         const KeygenCandidate &kc = cand[pending[c / ahead]][c % ahead];
-        std::memcpy(&hg[k * T1], kc.g.data(), T1 * 2);
+        std::memcpy(&hf[k * (T1 - 1)], kc.f.data(), (T1 - 1) * 2);
         std::memcpy(&ha[k * N], kc.alpha.data(), N * 2);
+        hvalid[k] = sok[pending[c / ahead]][c % ahead];
       }
-      HIPCHK(hipMemcpyAsync(b.g, hg.data(), nc * T1 * 2, hipMemcpyHostToDevice, b.s));
+      HIPCHK(hipMemcpyAsync(b.fv, hf.data(), nc * (T1 - 1) * 2, hipMemcpyHostToDevice, b.s));
       HIPCHK(hipMemcpyAsync(b.alpha, ha.data(), nc * N * 2, hipMemcpyHostToDevice, b.s));
-      KeygenArgs a{b.g, b.alpha, e->lg[w], e->ex[w], b.H, b.pk, b.ok, f ? b.piv : nullptr, (int)nc};
+      HIPCHK(hipMemcpyAsync(b.valid, hvalid.data(), nc * sizeof(int), hipMemcpyHostToDevice, b.s));
+      // Goppa polynomials on the GPU, then the systematic form of the attempts still valid
+      HIPCHK(launch_genpoly_set(set, b.fv, e->lg[w], e->ex[w], b.g, b.valid, (int)nc, b.s));
+      KeygenArgs a{b.g, b.alpha, e->lg[w], e->ex[w], b.H, b.pk, b.ok, f ? b.piv : nullptr, b.valid, (int)nc};
       HIPCHK(launch_pk_gen_set(set, a, b.s));
       HIPCHK(hipMemcpyAsync(hok.data(), b.ok, nc * sizeof(int), hipMemcpyDeviceToHost, b.s));
       if (f) HIPCHK(hipMemcpyAsync(hpiv.data(), b.piv, nc * sizeof(uint64_t), hipMemcpyDeviceToHost, b.s));
@@ -157,6 +168,7 @@ int tmce_keypair_batch(int variant, unsigned char *pk, unsigned char *sk, const 
           win[i] = (int)(c % ahead);
           win_piv[i] = hpiv[k];
           HIPCHK(hipMemcpyAsync(pk + i * PKB, b.pk + k * PKB, PKB, hipMemcpyDeviceToHost, b.s));
+          HIPCHK(hipMemcpyAsync(cand[i][c % ahead].g.data(), b.g + k * T1, T1 * 2, hipMemcpyDeviceToHost, b.s));
         }
       }
       HIPCHK(hipStreamSynchronize(b.s));
