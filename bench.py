@@ -551,8 +551,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(every op is independent: N GPUs would each run this step on N/G ops)", file=sys.stderr)
+        print(f"bench.py: the engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(every op is independent: G GPUs would each run this step on N/G ops)", file=sys.stderr)
         sys.exit(2)
 
     def log(msg):
