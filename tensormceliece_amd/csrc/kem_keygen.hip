@@ -42,6 +42,7 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
   uint16_t *ex_s = lg_s + (1 << M);  // ex_s[i] = generator^(i mod q): a sum of two logs needs no reduction
   uint64_t *stage = tab_lds;         // [pivot bit j][64 words] of the current panel
   __shared__ uint16_t rowix[MT];     // logical row -> physical row of H
+  __shared__ uint16_t lix[MT];       // physical row -> logical row
   __shared__ uint64_t pw[MT];        // the panel's word of every physical row
   __shared__ uint64_t mk[MT];        // pivot rows (by bit) XORed into each physical row in this panel
   __shared__ int red[KG_WAVES];
@@ -57,7 +58,7 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
   uint64_t *H = H_all + (size_t)cand * MT * W;
   for (int i = tid; i < (1 << M); i += KG_WG) lg_s[i] = lg[i];
   for (int i = tid; i < 2 * Q; i += KG_WG) ex_s[i] = ex[i < Q ? i : i - Q];
-  for (int i = tid; i < MT; i += KG_WG) rowix[i] = (uint16_t)i;
+  for (int i = tid; i < MT; i += KG_WG) rowix[i] = lix[i] = (uint16_t)i;
   __syncthreads();
   auto mul = [&](uint32_t a, uint32_t b) -> uint32_t { return (a && b) ? (uint32_t)ex_s[lg_s[a] + lg_s[b]] : 0u; };
 
@@ -151,15 +152,35 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
       mk[r] = 0;
     }
     __syncthreads();
-    // phase 1
-    for (int cc = c; cc < ce; ++cc) {
-      const uint64_t bm = 1ull << (cc & 63);
-      int best = MT;  // first logical row at or below cc with the column bit (rows ascend per thread)
-      for (int l = cc + tid; l < MT; l += KG_WG)
-        if (pw[rowix[l]] & bm) {
-          best = l;
-          break;
+    // phase 1: one pass over each thread's physical rows both applies the previous step's pivot
+    // row and looks for the next pivot (the rows' logical indices in lix; the two rows of the
+    // previous swap, which tid 0 writes meanwhile, by their new indices), so a step costs two
+    // workgroup barriers
+    int ppr = -1, ppc = -1, pbest = -1;  // previous step: pivot row, row it displaces, pivot's old logical index
+    uint64_t pbm = 0, ppw = 0, pmk = 0;  // previous step: column bit, pivot row word and mask
+    for (int cc = c;; ++cc) {
+      const bool last = cc == ce;
+      const uint64_t bm = last ? 0ull : 1ull << (cc & 63);
+      if (tid == 0 && ppr >= 0 && pbest != cc - 1) {  // the previous step's row swap
+        rowix[cc - 1] = (uint16_t)ppr;
+        rowix[pbest] = (uint16_t)ppc;
+        lix[ppr] = (uint16_t)(cc - 1);
+        lix[ppc] = (uint16_t)pbest;
+      }
+      int best = MT;  // first logical row at or below cc with the column bit
+      for (int r = tid; r < MT; r += KG_WG) {
+        uint64_t v = pw[r];
+        if (ppr >= 0 && r != ppr && (v & pbm)) {
+          v ^= ppw;
+          pw[r] = v;
+          mk[r] ^= pmk;
         }
+        if (!last) {
+          const int l = r == ppr ? cc - 1 : r == ppc ? pbest : (int)lix[r];
+          if (l >= cc && (v & bm) && l < best) best = l;
+        }
+      }
+      if (last) break;
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) best = min(best, __shfl_xor(best, o));
       if (lane == 0) red[wave] = best;
@@ -171,20 +192,15 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
         if (tid == 0) ok_all[cand] = 0;
         return;
       }
-      const int pr = rowix[best], pc = rowix[cc];
-      const uint64_t ppw = pw[pr], pmk = mk[pr] ^ bm;
-      __syncthreads();  // rowix, red and the pivot row read by every thread
-      if (tid == 0 && best != cc) {
-        rowix[cc] = (uint16_t)pr;
-        rowix[best] = (uint16_t)pc;
-      }
-      for (int r = tid; r < MT; r += KG_WG)
-        if (r != pr && (pw[r] & bm)) {
-          pw[r] ^= ppw;
-          mk[r] ^= pmk;
-        }
-      __syncthreads();
+      ppr = rowix[best];
+      ppc = rowix[cc];
+      pbest = best;
+      pbm = bm;
+      ppw = pw[ppr];
+      pmk = mk[ppr] ^ bm;
+      __syncthreads();  // red, rowix and the pivot row read by every thread
     }
+    __syncthreads();  // the last step's updates and swap
     for (int r = tid; r < MT; r += KG_WG) H[(size_t)r * W + cw] = pw[r];
     // phase 2: the pivot of column 64 cw + j sits at logical row 64 cw + j; its words right of cw
     // are still as at the panel's start (phase 1 wrote word cw only)
