@@ -210,6 +210,7 @@ void k_dec_decode(DecArgs a) {
   __shared__ uint16_t pthi_s[NR];
   __shared__ uint16_t twl_s[2 * Dn];
   __shared__ uint16_t plo_s[(KD + 1) * 64];  // butterfly multipliers' lane parts [KD][64], then (lane)^t
+  __shared__ uint16_t lpt_s[NPHI * 64];      // logs of the butterfly multipliers [phi index][lane]
   __shared__ DecWave<SET> Wv[DEC_WAVES];
 
   const int tid = threadIdx.x;
@@ -220,6 +221,16 @@ void k_dec_decode(DecArgs a) {
   for (int i = tid; i < 2 * Dn; i += DEC_WG) twl_s[i] = 2 * a.ft.twl[i];
   for (int i = tid; i < (KD + 1) * 64; i += DEC_WG)
     plo_s[i] = i < KD * 64 ? 2 * a.ft.plo[i] : (P::TOP_TERM ? 2 * a.ft.ptlo[i - KD * 64] : 0);
+  __syncthreads();
+  // the bottom-up multiplier of phi index k in lane l is plo[d][l] ^ phi[k], where level d holds the
+  // indices [2^(m-6) - 2^(m-6-d), 2^(m-6) - 2^(m-7-d)); its log is looked up once here, so the
+  // butterflies read it lane-contiguous (no bank conflict) instead of by a table gather
+  for (int i = tid; i < NPHI * 64; i += DEC_WG) {
+    const int k = i >> 6, l = i & 63;
+    int d = 0;
+    while (d + 1 < KD && k >= (1 << (M - 6)) - (1 << (M - 7 - d))) ++d;
+    lpt_s[i] = (uint16_t)tab(lg_s, plo_s[64 * d + l] ^ phi_s[k]);
+  }
   __syncthreads();
   const int nops = a.plan ? a.plan->total : a.nops;
   const int wave = tid >> 6, lane = tid & 63, row = lane >> 4, l16 = lane & 15;
@@ -378,7 +389,7 @@ void k_dec_decode(DecArgs a) {
       // 64(r+NR/2)+lane (high half); the two butterflies of a packed pair share their multiplier
       constexpr int NH = NR / 2;
       const int zo = opaque_zero();
-      const uint16_t *phq = phi_s + zo, *ptq = pthi_s + zo, *plq = plo_s + zo;
+      const uint16_t *lpq = lpt_s + zo + lane, *ptq = pthi_s + zo, *plq = plo_s + zo;
       uint32_t V2[NH];
 #pragma unroll
       for (int r = 0; r < NH; ++r) V2[r] = (uint32_t)fq[r] | ((uint32_t)fq[r + NH] << 16);
@@ -388,10 +399,9 @@ void k_dec_decode(DecArgs a) {
         (void)dummy;
         const int step = 1 << (M - 7 - d);
         const int phoff = (1 << (M - 6)) - (1 << (M - 6 - d));
-        const uint32_t plod = plq[64 * d + lane];
 #pragma unroll
         for (int j = 0; j < step; ++j) {
-          const uint32_t lp = tab(lg_s, plod ^ phq[phoff + j]);
+          const uint32_t lp = lpq[64 * (phoff + j)];
 #pragma unroll
           for (int r = j; r < NH; r += 2 * step) {
             const uint32_t y2 = V2[r + step];
@@ -404,10 +414,9 @@ void k_dec_decode(DecArgs a) {
         }
       }
       {  // level 0 (half = 2^(m-1)): the pair is (low, high) of one register
-        const uint32_t plo0 = plq[lane];
 #pragma unroll
         for (int r = 0; r < NH; ++r) {
-          const uint32_t lp = tab(lg_s, plo0 ^ phq[r]);
+          const uint32_t lp = lpq[64 * r];
           const uint32_t x = V2[r] & 0xFFFFu, y = V2[r] >> 16;
           const uint32_t xn = x ^ tab(ex_s, tab(lg_s, y) + lp);
           V2[r] = xn | ((xn ^ y) << 16);
