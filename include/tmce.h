@@ -356,8 +356,11 @@ int tmce_keypair_seeded(int set, unsigned char *pk, unsigned char *sk, const uns
    byte for byte.  Replaces n calls of libmceliece's crypto_kem_keypair
    (LM:crypto_kem/<set>/vec/kem_keypair.c; the reference generates its keys with it on the CPU,
    mceliece-gpu-params/main.cu:17-21,159-161).  pk: n*PUBLICKEYBYTES, sk: n*SECRETKEYBYTES.
-   The systematic-form step (pk_gen.c) runs on the GPU; seed expansion, Goppa polynomial, support
-   and Benes control bits run on host threads.  Synchronous. */
+   set may be any variant (the f variants' semi-systematic form included).  The Goppa polynomial
+   (sk_gen.c), the systematic form (pk_gen.c) and the Benes control bits (controlbits.c) run on the
+   GPU; seed expansion and the support sort on host threads.  Synchronous; concurrent calls are
+   serialised (they share a 64 MB pinned staging buffer that the library allocates on the first
+   call and keeps). */
 int tmce_keypair_batch(int set, unsigned char *pk, unsigned char *sk, const unsigned char *seeds, size_t n);
 
 /* batched KEM over host buffers: ct[n*CIPHERTEXTBYTES], k[n*32], status[n] (may be NULL),

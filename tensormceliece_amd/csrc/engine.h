@@ -125,6 +125,10 @@ struct Engine {
   int front_cfg[NUM_SETS] = {1, 1, 1, 1, 1};   // its MFMA tile shape
   int gen_form[NUM_SETS] = {1, 1, 1, 1, 1};    // sampler: 1 = pipelined lane groups, 0 = one op per lane
   EventTimer timer;
+  // key generation: pinned host staging for the public-key downloads (allocated on first use and
+  // kept; kg_mu serialises the calls that use it)
+  std::mutex kg_mu;
+  uint8_t *kg_stage = nullptr;
 };
 
 extern Engine *g_engine;

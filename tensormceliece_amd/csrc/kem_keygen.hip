@@ -391,41 +391,46 @@ hipError_t launch_genpoly_set(int set, const uint16_t *f, const uint16_t *lg, co
 // The recursion of LM:crypto_kem/<set>/vec/controlbits.c (cbrecursion), level by level: at level d
 // (w = m - d) the 2^d subproblems are the segments of 2^w entries of the level's permutation array,
 // each a permutation of its own 0 .. 2^w - 1, and every step of the recursion body runs on all
-// segments at once (elementwise, or as a bitonic sort inside each segment -- the keys of a sort are
-// distinct, so any correct sort gives the library's order).  Subproblem s of level d (its choices
+// segments at once (elementwise, or as a sort inside each segment, done as the scatter that the
+// sort's keys make it equal to, below).  Subproblem s of level d (its choices
 // child 0 / child 1 from the top are the bits of s, most significant first) writes its first-layer
 // bits at d n/2 + rev_d(s) + 2^d j and its last-layer bits at (2m - 2 - d) n/2 + rev_d(s) + 2^d k,
 // rev_d = d-bit reversal, which is where the recursion's position and step arithmetic puts them;
 // its children are its two halves, segments 2s and 2s + 1 of level d + 1.  One workgroup per key.
+// Every "sort" of the recursion sorts words whose high field (bits sh and up) runs over a
+// permutation of 0 .. 2^w - 1 inside each segment of 2^w entries, so the sorted segment holds word x
+// at position (its high field): the sort is a scatter into a second array, copied back.  Wave v of
+// the workgroup owns the entries [CH v, CH (v + 1)) (CH = 2^m / 16): the elementwise steps touch
+// only their own entry or its pair partner, and a scatter inside segments no wider than a chunk
+// stays inside the wave's chunk, so those need only the wave's own LDS ordering; a scatter across
+// chunks runs between workgroup barriers.
 constexpr int CB_WG = 1024;
 
 template <int MM>
-__device__ __forceinline__ void cb_sort(uint32_t *v, int w, int tid) {
-  constexpr int NN = 1 << MM;
-  for (int k = 2; k <= (1 << w); k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int p = tid; p < NN / 2; p += CB_WG) {
-        const int i = (p / j) * 2 * j + (p % j), l = i + j;
-        const uint32_t a = v[i], b = v[l];
-        // ascending, except the first halves of the merges below the segment size; the last merge
-        // (k = segment size) is ascending in every segment
-        const bool up = k == (1 << w) || (i & k) == 0;
-        if ((a > b) == up) {
-          v[i] = b;
-          v[l] = a;
-        }
-      }
-      __syncthreads();
-    }
+__device__ __forceinline__ void cb_sort(uint32_t *v, uint32_t *tmp, int w, int sh, int tid) {
+  constexpr int NN = 1 << MM, CH = NN / (CB_WG / 64);
+  const int lane = tid & 63, c0 = (tid >> 6) * CH, lm = (1 << w) - 1;
+  const bool wide = (1 << w) > CH;
+  // wide: every wave is done with tmp (its previous copy-back) before any wave writes it
+  if (wide) __syncthreads();
+  else wave_lds_sync();
+  for (int x = c0 + lane; x < c0 + CH; x += 64) {
+    const uint32_t a = v[x];
+    tmp[(x & ~lm) | ((a >> sh) & lm)] = a;
+  }
+  if (wide) __syncthreads();
+  else wave_lds_sync();
+  for (int x = c0 + lane; x < c0 + CH; x += 64) v[x] = tmp[x];
+  wave_lds_sync();
 }
 
 template <int MM>
 __global__ __launch_bounds__(CB_WG) void k_controlbits(const uint16_t *__restrict__ pi_all, uint8_t *__restrict__ cb_all) {
-  constexpr int NN = 1 << MM, HALF = NN / 2;
+  constexpr int NN = 1 << MM, HALF = NN / 2, CH = NN / (CB_WG / 64);
   constexpr int CBW = (2 * MM - 1) * HALF / 32;  // control-bit words: (2m - 1) layers of n/2 bits
-  __shared__ uint32_t A[NN], B[NN], cbw[CBW];
+  __shared__ uint32_t A[NN], B[NN], C[NN], cbw[CBW];
   __shared__ uint16_t PI[NN];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, c0 = (tid >> 6) * CH, c1 = c0 + CH;
   const uint16_t *pi = pi_all + (size_t)blockIdx.x * NN;
   for (int x = tid; x < NN; x += CB_WG) PI[x] = pi[x];
   for (int i = tid; i < CBW; i += CB_WG) cbw[i] = 0;
@@ -433,69 +438,57 @@ __global__ __launch_bounds__(CB_WG) void k_controlbits(const uint16_t *__restric
   auto flip = [&](int pos, uint32_t b) {
     if (b & 1u) atomicXor(&cbw[pos >> 5], 1u << (pos & 31));
   };
+  // x: the wave's own entries; p: the wave's own pairs (2p, 2p + 1)
+#define CB_OWN(x) for (int x = c0 + lane; x < c1; x += 64)
+#define CB_OWN_PAIRS(p) for (int p = (c0 >> 1) + lane; p < (c1 >> 1); p += 64)
   for (int d = 0; d < MM - 1; ++d) {
     const int w = MM - d, ns = 1 << w, lm = ns - 1;
-    for (int x = tid; x < NN; x += CB_WG) A[x] = ((uint32_t)(PI[x] ^ 1u) << 16) | PI[x ^ 1];
-    __syncthreads();
-    cb_sort<MM>(A, w, tid);
-    for (int x = tid; x < NN; x += CB_WG) {
+    CB_OWN(x) A[x] = ((uint32_t)(PI[x] ^ 1u) << 16) | PI[x ^ 1];
+    cb_sort<MM>(A, C, w, 16, tid);
+    CB_OWN(x) {
       const uint32_t px = A[x] & 0xffffu, lx = (uint32_t)(x & lm);
       B[x] = (px << 16) | (px < lx ? px : lx);
       A[x] = (A[x] << 16) | lx;
     }
-    __syncthreads();
-    cb_sort<MM>(A, w, tid);
-    for (int x = tid; x < NN; x += CB_WG) A[x] = (A[x] << 16) + (B[x] >> 16);
-    __syncthreads();
-    cb_sort<MM>(A, w, tid);
+    cb_sort<MM>(A, C, w, 16, tid);
+    CB_OWN(x) A[x] = (A[x] << 16) + (B[x] >> 16);
+    cb_sort<MM>(A, C, w, 16, tid);
     if (w <= 10) {
-      for (int x = tid; x < NN; x += CB_WG) B[x] = ((A[x] & 0xffffu) << 10) | (B[x] & 0x3ffu);
-      __syncthreads();
+      CB_OWN(x) B[x] = ((A[x] & 0xffffu) << 10) | (B[x] & 0x3ffu);
       for (int i = 1; i < w - 1; ++i) {
-        for (int x = tid; x < NN; x += CB_WG) A[x] = ((B[x] & ~0x3ffu) << 6) | (uint32_t)(x & lm);
-        __syncthreads();
-        cb_sort<MM>(A, w, tid);
-        for (int x = tid; x < NN; x += CB_WG) A[x] = (A[x] << 20) | B[x];
-        __syncthreads();
-        cb_sort<MM>(A, w, tid);
-        for (int x = tid; x < NN; x += CB_WG) {
+        CB_OWN(x) A[x] = ((B[x] & ~0x3ffu) << 6) | (uint32_t)(x & lm);
+        cb_sort<MM>(A, C, w, 16, tid);
+        CB_OWN(x) A[x] = (A[x] << 20) | B[x];
+        cb_sort<MM>(A, C, w, 20, tid);
+        CB_OWN(x) {
           const uint32_t u = (A[x] & 0xffc00u) | (B[x] & 0x3ffu), v = A[x] & 0xfffffu;
           B[x] = u < v ? u : v;
         }
-        __syncthreads();
       }
-      for (int x = tid; x < NN; x += CB_WG) B[x] &= 0x3ffu;
+      CB_OWN(x) B[x] &= 0x3ffu;
     } else {
-      for (int x = tid; x < NN; x += CB_WG) B[x] = (A[x] << 16) | (B[x] & 0xffffu);
-      __syncthreads();
+      CB_OWN(x) B[x] = (A[x] << 16) | (B[x] & 0xffffu);
       for (int i = 1; i < w - 1; ++i) {
-        for (int x = tid; x < NN; x += CB_WG) A[x] = (B[x] & ~0xffffu) | (uint32_t)(x & lm);
-        __syncthreads();
-        cb_sort<MM>(A, w, tid);
-        for (int x = tid; x < NN; x += CB_WG) A[x] = (A[x] << 16) | (B[x] & 0xffffu);
-        __syncthreads();
+        CB_OWN(x) A[x] = (B[x] & ~0xffffu) | (uint32_t)(x & lm);
+        cb_sort<MM>(A, C, w, 16, tid);
+        CB_OWN(x) A[x] = (A[x] << 16) | (B[x] & 0xffffu);
         if (i < w - 2) {
-          for (int x = tid; x < NN; x += CB_WG) B[x] = (A[x] & ~0xffffu) | (B[x] >> 16);
-          __syncthreads();
-          cb_sort<MM>(B, w, tid);
-          for (int x = tid; x < NN; x += CB_WG) B[x] = (B[x] << 16) | (A[x] & 0xffffu);
-          __syncthreads();
+          CB_OWN(x) B[x] = (A[x] & ~0xffffu) | (B[x] >> 16);
+          cb_sort<MM>(B, C, w, 16, tid);
+          CB_OWN(x) B[x] = (B[x] << 16) | (A[x] & 0xffffu);
         }
-        cb_sort<MM>(A, w, tid);
-        for (int x = tid; x < NN; x += CB_WG) {
+        cb_sort<MM>(A, C, w, 16, tid);
+        CB_OWN(x) {
           const uint32_t u = (B[x] & ~0xffffu) | (A[x] & 0xffffu);
           B[x] = B[x] < u ? B[x] : u;
         }
-        __syncthreads();
       }
-      for (int x = tid; x < NN; x += CB_WG) B[x] &= 0xffffu;
+      CB_OWN(x) B[x] &= 0xffffu;
     }
-    __syncthreads();
-    for (int x = tid; x < NN; x += CB_WG) A[x] = ((uint32_t)PI[x] << 16) + (uint32_t)(x & lm);
-    __syncthreads();
-    cb_sort<MM>(A, w, tid);
+    CB_OWN(x) A[x] = ((uint32_t)PI[x] << 16) + (uint32_t)(x & lm);
+    cb_sort<MM>(A, C, w, 16, tid);
     const int rb = d ? 32 - d : 0;  // rev_d(seg) = bit reversal of the segment index in d bits
-    for (int p = tid; p < HALF; p += CB_WG) {  // first layer: pair (2j, 2j + 1) of segment seg
+    CB_OWN_PAIRS(p) {  // first layer: pair (2j, 2j + 1) of segment seg
       const int x = 2 * p, seg = x >> w, j = (x & lm) >> 1;
       const int rv = d ? (int)(__builtin_bitreverse32((uint32_t)seg) >> rb) : 0;
       const uint32_t fj = B[x] & 1u, Fz = (uint32_t)(x & lm) + fj;
@@ -503,9 +496,8 @@ __global__ __launch_bounds__(CB_WG) void k_controlbits(const uint16_t *__restric
       B[x] = (A[x] << 16) | Fz;
       B[x + 1] = (A[x + 1] << 16) | (Fz ^ 1u);
     }
-    __syncthreads();
-    cb_sort<MM>(B, w, tid);
-    for (int p = tid; p < HALF; p += CB_WG) {  // last layer
+    cb_sort<MM>(B, C, w, 16, tid);
+    CB_OWN_PAIRS(p) {  // last layer
       const int y = 2 * p, seg = y >> w, k = (y & lm) >> 1;
       const int rv = d ? (int)(__builtin_bitreverse32((uint32_t)seg) >> rb) : 0;
       const uint32_t lk = B[y] & 1u, Ly = (uint32_t)(y & lm) + lk;
@@ -513,15 +505,18 @@ __global__ __launch_bounds__(CB_WG) void k_controlbits(const uint16_t *__restric
       A[y] = (Ly << 16) | (B[y] & 0xffffu);
       A[y + 1] = ((Ly ^ 1u) << 16) | (B[y + 1] & 0xffffu);
     }
-    __syncthreads();
-    cb_sort<MM>(A, w, tid);
-    for (int p = tid; p < HALF; p += CB_WG) {  // the children's permutations: the segment's halves
+    cb_sort<MM>(A, C, w, 16, tid);
+    // every wave has read this level's PI (the scatters above of a segment wider than a chunk hold
+    // workgroup barriers; a segment within a chunk is read and written by its own wave only)
+    CB_OWN_PAIRS(p) {  // the children's permutations: the segment's halves
       const int y = 2 * p, seg = y >> w, j = (y & lm) >> 1, base = seg << w;
       PI[base + j] = (uint16_t)((A[y] & 0xffffu) >> 1);
       PI[base + (ns >> 1) + j] = (uint16_t)((A[y + 1] & 0xffffu) >> 1);
     }
     __syncthreads();
   }
+#undef CB_OWN
+#undef CB_OWN_PAIRS
   for (int p = tid; p < HALF; p += CB_WG) {  // w = 1: the middle layer, one bit per segment of 2
     const int rv = (int)(__builtin_bitreverse32((uint32_t)p) >> (32 - (MM - 1)));
     flip((MM - 1) * HALF + rv, PI[2 * p]);

@@ -76,6 +76,9 @@ struct KgBuffers {
 constexpr size_t KG_LAUNCH = 512;
 constexpr size_t KG_CB_KEYS = 1024;  // control-bit workgroups per launch
 constexpr size_t KG_AHEAD_MIN = 2, KG_AHEAD_MAX = 8;
+// pinned staging for the winners' public keys: a download into pageable memory runs at a few GB/s
+// (tens of milliseconds for a round's keys), into pinned memory at the link's rate
+constexpr size_t KG_STAGE_BYTES = size_t(64) << 20;
 
 }  // namespace
 
@@ -103,6 +106,13 @@ int tmce_keypair_batch(int variant, unsigned char *pk, unsigned char *sk, const 
     shake256(chain[i].data() + 1, 32, st, 33);
   }
 
+  std::lock_guard<std::mutex> lock(e->kg_mu);
+  if (!e->kg_stage && hipHostMalloc((void **)&e->kg_stage, KG_STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
+    e->kg_stage = nullptr;
+    return fail(TMCE_ENOMEM, "hipHostMalloc key generation staging");
+  }
+  const size_t slots = std::max<size_t>(1, KG_STAGE_BYTES / PKB);
+
   KgBuffers b;
   const size_t cap = std::min(n * KG_AHEAD_MAX, KG_LAUNCH);
   HIPCHK(hipStreamCreateWithFlags(&b.s, hipStreamNonBlocking));
@@ -122,13 +132,20 @@ int tmce_keypair_batch(int variant, unsigned char *pk, unsigned char *sk, const 
   std::vector<uint64_t> win_piv(n, 0);
   std::vector<size_t> pending(n);
   for (size_t i = 0; i < n; ++i) pending[i] = i;
-  std::vector<uint16_t> hf(cap * (T1 - 1)), ha(cap * N);
-  std::vector<int> hvalid(cap);
+  // host sides of the uploads and downloads, in the pinned staging buffer (each phase from its start)
+  auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  uint16_t *ha = reinterpret_cast<uint16_t *>(e->kg_stage);
+  uint16_t *hf = reinterpret_cast<uint16_t *>(e->kg_stage + align(cap * N * 2));
+  int *hvalid = reinterpret_cast<int *>(e->kg_stage + align(cap * N * 2) + align(cap * (T1 - 1) * 2));
+  uint16_t *hpi = reinterpret_cast<uint16_t *>(e->kg_stage);
+  uint8_t *hcb = e->kg_stage + align(cbcap * NQ * 2);
+  static_assert(KG_LAUNCH * (8192 + 128 + 2) * 2 + KG_CB_KEYS * (8192 * 2 + 25 * 4096 / 8) + 1024 <= KG_STAGE_BYTES,
+                "staging holds a launch's inputs and a control-bit launch");
+  std::vector<uint16_t> hg(cap * T1);
+  std::vector<std::pair<size_t, size_t>> wins;  // (key, candidate slot of the launch)
   std::vector<std::vector<uint8_t>> sok(n);  // per key and attempt: the support's sort keys are distinct
   std::vector<int> hok(cap);
   std::vector<uint64_t> hpiv(cap);
-  std::vector<uint16_t> hpi(cbcap * NQ);
-  std::vector<uint8_t> hcb(cbcap * CBB);
   while (!pending.empty()) {
     const size_t ahead = std::max(KG_AHEAD_MIN, std::min(KG_AHEAD_MAX, KG_LAUNCH / pending.size()));
     parallel_for(pending.size(), [&](size_t k) {  // a key's attempts follow its chain, in order
@@ -148,17 +165,19 @@ int tmce_keypair_batch(int variant, unsigned char *pk, unsigned char *sk, const 
         std::memcpy(&ha[k * N], kc.alpha.data(), N * 2);
         hvalid[k] = sok[pending[c / ahead]][c % ahead];
       }
-      HIPCHK(hipMemcpyAsync(b.fv, hf.data(), nc * (T1 - 1) * 2, hipMemcpyHostToDevice, b.s));
-      HIPCHK(hipMemcpyAsync(b.alpha, ha.data(), nc * N * 2, hipMemcpyHostToDevice, b.s));
-      HIPCHK(hipMemcpyAsync(b.valid, hvalid.data(), nc * sizeof(int), hipMemcpyHostToDevice, b.s));
+      HIPCHK(hipMemcpyAsync(b.fv, hf, nc * (T1 - 1) * 2, hipMemcpyHostToDevice, b.s));
+      HIPCHK(hipMemcpyAsync(b.alpha, ha, nc * N * 2, hipMemcpyHostToDevice, b.s));
+      HIPCHK(hipMemcpyAsync(b.valid, hvalid, nc * sizeof(int), hipMemcpyHostToDevice, b.s));
       // Goppa polynomials on the GPU, then the systematic form of the attempts still valid
       HIPCHK(launch_genpoly_set(set, b.fv, e->lg[w], e->ex[w], b.g, b.valid, (int)nc, b.s));
       KeygenArgs a{b.g, b.alpha, e->lg[w], e->ex[w], b.H, b.pk, b.ok, f ? b.piv : nullptr, b.valid, (int)nc};
       HIPCHK(launch_pk_gen_set(set, a, b.s));
       HIPCHK(hipMemcpyAsync(hok.data(), b.ok, nc * sizeof(int), hipMemcpyDeviceToHost, b.s));
+      HIPCHK(hipMemcpyAsync(hg.data(), b.g, nc * T1 * 2, hipMemcpyDeviceToHost, b.s));
       if (f) HIPCHK(hipMemcpyAsync(hpiv.data(), b.piv, nc * sizeof(uint64_t), hipMemcpyDeviceToHost, b.s));
       HIPCHK(hipStreamSynchronize(b.s));
       // the first successful attempt of each key in this launch (earlier launches hold earlier attempts)
+      wins.clear();
       for (size_t k = 0; k < nc; ++k) {
         const size_t c = off + k, i = pending[c / ahead];
         okv[c] = hok[k];
@@ -167,11 +186,18 @@ int tmce_keypair_batch(int variant, unsigned char *pk, unsigned char *sk, const 
         if (first) {
           win[i] = (int)(c % ahead);
           win_piv[i] = hpiv[k];
-          HIPCHK(hipMemcpyAsync(pk + i * PKB, b.pk + k * PKB, PKB, hipMemcpyDeviceToHost, b.s));
-          HIPCHK(hipMemcpyAsync(cand[i][c % ahead].g.data(), b.g + k * T1, T1 * 2, hipMemcpyDeviceToHost, b.s));
+          std::memcpy(cand[i][c % ahead].g.data(), &hg[k * T1], T1 * 2);
+          wins.emplace_back(i, k);
         }
       }
-      HIPCHK(hipStreamSynchronize(b.s));
+      // their public keys, through the pinned staging buffer
+      for (size_t w0 = 0; w0 < wins.size(); w0 += slots) {
+        const size_t nw = std::min(slots, wins.size() - w0);
+        for (size_t q = 0; q < nw; ++q)
+          HIPCHK(hipMemcpyAsync(e->kg_stage + q * PKB, b.pk + wins[w0 + q].second * PKB, PKB, hipMemcpyDeviceToHost, b.s));
+        HIPCHK(hipStreamSynchronize(b.s));
+        parallel_for(nw, [&](size_t q) { std::memcpy(pk + wins[w0 + q].first * PKB, e->kg_stage + q * PKB, PKB); });
+      }
     }
     std::vector<size_t> done, failed;
     for (size_t i : pending) (win[i] >= 0 ? done : failed).push_back(i);
@@ -184,9 +210,9 @@ int tmce_keypair_batch(int variant, unsigned char *pk, unsigned char *sk, const 
         if (f) keygen_apply_pivots(set, cand[i][win[i]], win_piv[i]);
         std::memcpy(&hpi[k * NQ], cand[i][win[i]].pi.data(), NQ * 2);
       });
-      HIPCHK(hipMemcpyAsync(b.pi, hpi.data(), nd * NQ * 2, hipMemcpyHostToDevice, b.s));
+      HIPCHK(hipMemcpyAsync(b.pi, hpi, nd * NQ * 2, hipMemcpyHostToDevice, b.s));
       HIPCHK(launch_controlbits(si.m, b.pi, b.cb, (int)nd, b.s));
-      HIPCHK(hipMemcpyAsync(hcb.data(), b.cb, nd * CBB, hipMemcpyDeviceToHost, b.s));
+      HIPCHK(hipMemcpyAsync(hcb, b.cb, nd * CBB, hipMemcpyDeviceToHost, b.s));
       HIPCHK(hipStreamSynchronize(b.s));
       parallel_for(nd, [&](size_t k) {
         const size_t i = done[off + k];
