@@ -36,11 +36,11 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
   constexpr int W = ceil_div(N, 64);  // 64-bit words per row of H
   static_assert(W <= 128, "two row words per lane");
   // field tables while H is built; afterwards the same LDS stages the panel's pivot rows
-  constexpr int TABW = ceil_div(2 * ((1 << M) + 2 * Q), 8), STAGEW = 64 * 64;
+  constexpr int TABW = ceil_div(2 * ((1 << M) + 2 * Q), 8), STAGEW = 65 * 64;  // + a zero row
   __shared__ uint64_t tab_lds[TABW > STAGEW ? TABW : STAGEW];
   uint16_t *lg_s = reinterpret_cast<uint16_t *>(tab_lds);
   uint16_t *ex_s = lg_s + (1 << M);  // ex_s[i] = generator^(i mod q): a sum of two logs needs no reduction
-  uint64_t *stage = tab_lds;         // [pivot bit j][64 words] of the current panel
+  uint64_t *stage = tab_lds;         // [pivot bit j][64 words] of the current panel; row 64 is zero
   __shared__ uint16_t rowix[MT];     // logical row -> physical row of H
   __shared__ uint16_t lix[MT];       // physical row -> logical row
   __shared__ uint64_t pw[MT];        // the panel's word of every physical row
@@ -204,22 +204,41 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
     for (int r = tid; r < MT; r += KG_WG) H[(size_t)r * W + cw] = pw[r];
     // phase 2: the pivot of column 64 cw + j sits at logical row 64 cw + j; its words right of cw
     // are still as at the panel's start (phase 1 wrote word cw only)
+    // Rows go in groups of PB per wave: their words are loaded together before the LDS work, and
+    // a mask's bits are taken four at a time (missing ones read the zero row), so neither the HBM
+    // latency nor the LDS latency is paid once per row or per bit.
+    constexpr int PB = 4;
     for (int w0 = cw + 1; w0 < W; w0 += 64) {
       for (int i = tid; i < STAGEW; i += KG_WG) {
         const int j = i >> 6, col = 64 * cw + j, w = w0 + (i & 63);
-        stage[i] = (col >= c && col < ce && w < W) ? H[(size_t)rowix[col] * W + w] : 0ull;
+        stage[i] = (j < 64 && col >= c && col < ce && w < W) ? H[(size_t)rowix[col] * W + w] : 0ull;
       }
       __syncthreads();
-      for (int r = wave; r < MT; r += KG_WAVES) {
-        uint64_t m = mk[r];  // uniform in the wave
-        if (m == 0) continue;
-        uint64_t acc = 0;
-        while (m) {
-          const int j = __builtin_ctzll(m);
-          m &= m - 1;
-          acc ^= stage[64 * j + lane];
+      const bool wok = w0 + lane < W;
+      for (int r0 = wave; r0 < MT; r0 += KG_WAVES * PB) {
+        uint64_t mm[PB], h[PB];
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+          const int r = r0 + q * KG_WAVES;
+          const uint64_t v = r < MT ? mk[r] : 0ull;  // the same in every lane
+          mm[q] = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
+                  (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+          h[q] = (mm[q] && wok) ? H[(size_t)r * W + w0 + lane] : 0ull;
         }
-        if (w0 + lane < W) H[(size_t)r * W + w0 + lane] ^= acc;
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+          uint64_t m = mm[q], acc = 0;
+          while (m) {
+            int j[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              j[u] = m ? __builtin_ctzll(m) : 64;
+              m &= m - 1;
+            }
+            acc ^= stage[64 * j[0] + lane] ^ stage[64 * j[1] + lane] ^ stage[64 * j[2] + lane] ^ stage[64 * j[3] + lane];
+          }
+          if (mm[q] && wok) H[(size_t)(r0 + q * KG_WAVES) * W + w0 + lane] = h[q] ^ acc;
+        }
       }
       __syncthreads();  // the stage is rewritten for the next 64 words
     }
