@@ -21,7 +21,7 @@
 
 namespace tmce {
 
-constexpr int KG_WG = 512;
+constexpr int KG_WG = 1024;
 constexpr int KG_WAVES = KG_WG / 64;
 
 template <int SET, bool F>
@@ -36,11 +36,13 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
   constexpr int W = ceil_div(N, 64);  // 64-bit words per row of H
   static_assert(W <= 128, "two row words per lane");
   // field tables while H is built; afterwards the same LDS stages the panel's pivot rows
-  constexpr int TABW = ceil_div(2 * ((1 << M) + 2 * Q), 8), STAGEW = 65 * 64;  // + a zero row
+  constexpr int CW2 = 32;  // words of a row per phase-2 pass
+  constexpr int TABW = ceil_div(2 * ((1 << M) + 2 * Q), 8), STAGEW = 64 * CW2 + 256 * CW2;
   __shared__ uint64_t tab_lds[TABW > STAGEW ? TABW : STAGEW];
   uint16_t *lg_s = reinterpret_cast<uint16_t *>(tab_lds);
   uint16_t *ex_s = lg_s + (1 << M);  // ex_s[i] = generator^(i mod q): a sum of two logs needs no reduction
-  uint64_t *stage = tab_lds;         // [pivot bit j][64 words] of the current panel; row 64 is zero
+  uint64_t *srow = tab_lds;          // [pivot bit j][CW2 words] of the current panel
+  uint64_t *tbl = tab_lds + 64 * CW2;  // [nibble g][16 combinations][CW2 words]
   __shared__ uint16_t rowix[MT];     // logical row -> physical row of H
   __shared__ uint16_t lix[MT];       // physical row -> logical row
   __shared__ uint64_t pw[MT];        // the panel's word of every physical row
@@ -204,43 +206,59 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
     for (int r = tid; r < MT; r += KG_WG) H[(size_t)r * W + cw] = pw[r];
     // phase 2: the pivot of column 64 cw + j sits at logical row 64 cw + j; its words right of cw
     // are still as at the panel's start (phase 1 wrote word cw only)
-    // Rows go in groups of PB per wave: their words are loaded together before the LDS work, and
-    // a mask's bits are taken four at a time (missing ones read the zero row), so neither the HBM
-    // latency nor the LDS latency is paid once per row or per bit.
-    constexpr int PB = 4;
-    for (int w0 = cw + 1; w0 < W; w0 += 64) {
-      for (int i = tid; i < STAGEW; i += KG_WG) {
-        const int j = i >> 6, col = 64 * cw + j, w = w0 + (i & 63);
-        stage[i] = (j < 64 && col >= c && col < ce && w < W) ? H[(size_t)rowix[col] * W + w] : 0ull;
+    // Four Russians, 32 words at a time: the panel's pivot rows are staged, then for each group
+    // of four pivot bits a table of the 16 XOR combinations of those rows' words; a row's
+    // combination is then 16 table reads, one per nibble of its mask, all independent.  A wave
+    // takes two rows per pass (one per half-wave, lane = word) and loads the row words of its next
+    // PG passes while it works on the current ones.
+    constexpr int PG = 2;
+    const int hh = lane >> 5, hl = lane & 31;
+    for (int w0 = cw + 1; w0 < W; w0 += CW2) {
+      for (int i = tid; i < 64 * CW2; i += KG_WG) {
+        const int j = i / CW2, col = 64 * cw + j, w = w0 + i % CW2;
+        srow[i] = (col >= c && col < ce && w < W) ? H[(size_t)rowix[col] * W + w] : 0ull;
       }
       __syncthreads();
-      const bool wok = w0 + lane < W;
-      for (int r0 = wave; r0 < MT; r0 += KG_WAVES * PB) {
-        uint64_t mm[PB], h[PB];
+      for (int i = tid; i < 256 * CW2; i += KG_WG) {  // tbl[g][e][x]: XOR of srow[4 g + b][x], bits b of e
+        const int x = i % CW2, e = (i / CW2) & 15, g4 = 4 * (i / (16 * CW2));
+        uint64_t v = 0;
 #pragma unroll
-        for (int q = 0; q < PB; ++q) {
-          const int r = r0 + q * KG_WAVES;
-          const uint64_t v = r < MT ? mk[r] : 0ull;  // the same in every lane
-          mm[q] = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
-                  (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-          h[q] = (mm[q] && wok) ? H[(size_t)r * W + w0 + lane] : 0ull;
+        for (int bb = 0; bb < 4; ++bb)
+          if ((e >> bb) & 1) v ^= srow[(g4 + bb) * CW2 + x];
+        tbl[i] = v;
+      }
+      __syncthreads();
+      const bool wok = w0 + hl < W;
+      // pass k: rows 2 (KG_WAVES k + wave) + hh
+      auto load = [&](int k, uint64_t &m, uint64_t &h) {
+        const int r = 2 * (KG_WAVES * k + wave) + hh;
+        m = r < MT ? mk[r] : 0ull;
+        h = (m && wok) ? H[(size_t)r * W + w0 + hl] : 0ull;
+      };
+      constexpr int NPASS = ceil_div(MT, 2 * KG_WAVES);
+      uint64_t mc[PG], hc[PG];
+#pragma unroll
+      for (int q = 0; q < PG; ++q) load(q, mc[q], hc[q]);
+      for (int k0 = 0; k0 < NPASS; k0 += PG) {
+        uint64_t mn[PG], hn[PG];
+#pragma unroll
+        for (int q = 0; q < PG; ++q) load(k0 + PG + q, mn[q], hn[q]);  // rows past mt load nothing
+#pragma unroll
+        for (int q = 0; q < PG; ++q) {
+          const uint64_t m = mc[q];
+          uint64_t acc = 0;
+#pragma unroll
+          for (int g = 0; g < 16; ++g) acc ^= tbl[(16 * g + (int)((m >> (4 * g)) & 15u)) * CW2 + hl];
+          const int r = 2 * (KG_WAVES * (k0 + q) + wave) + hh;
+          if (m && wok) H[(size_t)r * W + w0 + hl] = hc[q] ^ acc;
         }
 #pragma unroll
-        for (int q = 0; q < PB; ++q) {
-          uint64_t m = mm[q], acc = 0;
-          while (m) {
-            int j[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              j[u] = m ? __builtin_ctzll(m) : 64;
-              m &= m - 1;
-            }
-            acc ^= stage[64 * j[0] + lane] ^ stage[64 * j[1] + lane] ^ stage[64 * j[2] + lane] ^ stage[64 * j[3] + lane];
-          }
-          if (mm[q] && wok) H[(size_t)(r0 + q * KG_WAVES) * W + w0 + lane] = h[q] ^ acc;
+        for (int q = 0; q < PG; ++q) {
+          mc[q] = mn[q];
+          hc[q] = hn[q];
         }
       }
-      __syncthreads();  // the stage is rewritten for the next 64 words
+      __syncthreads();  // srow and tbl are rewritten for the next words
     }
     c = ce;
   }
