@@ -335,56 +335,58 @@ bool pk_padding_bad(int set, const uint8_t *pk) {
 }
 
 // ------------------------------------------------------------------ SHAKE256 (host)
-static inline uint64_t rol(uint64_t x, int r) { return r ? (x << r) | (x >> (64 - r)) : x; }
+// Keccak-f[1600] (FIPS 202): theta, rho and pi fused into one pass over the 25 lanes (lane i of
+// the state goes to position PI_DST[i], rotated by ROT[i]), then chi and iota; the loops over
+// constant tables unroll into straight-line code with constant rotations.
+static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "state lanes are read and written as bytes");
+static inline uint64_t rol(uint64_t x, int r) { return (x << r) | (x >> ((64 - r) & 63)); }
 static void keccakf(uint64_t s[25]) {
-  static const uint64_t RC[24] = {
+  static constexpr uint64_t RC[24] = {
       0x0000000000000001ULL, 0x0000000000008082ULL, 0x800000000000808aULL, 0x8000000080008000ULL,
       0x000000000000808bULL, 0x0000000080000001ULL, 0x8000000080008081ULL, 0x8000000000008009ULL,
       0x000000000000008aULL, 0x0000000000000088ULL, 0x0000000080008009ULL, 0x000000008000000aULL,
       0x000000008000808bULL, 0x800000000000008bULL, 0x8000000000008089ULL, 0x8000000000008003ULL,
       0x8000000000008002ULL, 0x8000000000000080ULL, 0x000000000000800aULL, 0x800000008000000aULL,
       0x8000000080008081ULL, 0x8000000000008080ULL, 0x0000000080000001ULL, 0x8000000080008008ULL};
-  static const int rot[25] = {0, 1, 62, 28, 27, 36, 44, 6, 55, 20, 3, 10, 43, 25, 39, 41, 45, 15, 21, 8, 18, 2, 61, 56, 14};
+  // lane x + 5 y: rotation offset, and its position y + 5 ((2 x + 3 y) mod 5) after pi
+  static constexpr int ROT[25] = {0, 1, 62, 28, 27, 36, 44, 6, 55, 20, 3, 10, 43, 25, 39, 41, 45, 15, 21, 8, 18, 2, 61, 56, 14};
+  static constexpr int PI_DST[25] = {0, 10, 20, 5, 15, 16, 1, 11, 21, 6, 7, 17, 2, 12, 22, 23, 8, 18, 3, 13, 14, 24, 9, 19, 4};
+  uint64_t a[25], b[25];
+  std::memcpy(a, s, sizeof a);
   for (int r = 0; r < 24; ++r) {
-    uint64_t c[5], b[25];
-    for (int x = 0; x < 5; ++x) c[x] = s[x] ^ s[x + 5] ^ s[x + 10] ^ s[x + 15] ^ s[x + 20];
-    for (int x = 0; x < 5; ++x) {
-      const uint64_t dx = c[(x + 4) % 5] ^ rol(c[(x + 1) % 5], 1);
-      for (int y = 0; y < 5; ++y) s[x + 5 * y] ^= dx;
-    }
-    for (int x = 0; x < 5; ++x)
-      for (int y = 0; y < 5; ++y) b[y + 5 * ((2 * x + 3 * y) % 5)] = rol(s[x + 5 * y], rot[x + 5 * y]);
-    for (int y = 0; y < 5; ++y)
-      for (int x = 0; x < 5; ++x) s[x + 5 * y] = b[x + 5 * y] ^ (~b[(x + 1) % 5 + 5 * y] & b[(x + 2) % 5 + 5 * y]);
-    s[0] ^= RC[r];
+    uint64_t c[5], d[5];
+#pragma unroll
+    for (int x = 0; x < 5; ++x) c[x] = a[x] ^ a[x + 5] ^ a[x + 10] ^ a[x + 15] ^ a[x + 20];
+#pragma unroll
+    for (int x = 0; x < 5; ++x) d[x] = c[(x + 4) % 5] ^ rol(c[(x + 1) % 5], 1);
+#pragma unroll
+    for (int i = 0; i < 25; ++i) b[PI_DST[i]] = rol(a[i] ^ d[i % 5], ROT[i]);
+#pragma unroll
+    for (int i = 0; i < 25; ++i) a[i] = b[i] ^ (~b[(i / 5) * 5 + (i + 1) % 5] & b[(i / 5) * 5 + (i + 2) % 5]);
+    a[0] ^= RC[r];
   }
+  std::memcpy(s, a, sizeof a);
 }
 
 void shake256(uint8_t *out, long long outlen, const uint8_t *in, long long inlen) {
   uint64_t s[25] = {0};
-  uint8_t blk[136];
-  auto absorb = [&](const uint8_t *p) {
-    for (int i = 0; i < 17; ++i) {
-      uint64_t w = 0;
-      for (int b = 0; b < 8; ++b) w |= (uint64_t)p[8 * i + b] << (8 * b);
-      s[i] ^= w;
-    }
-    keccakf(s);
+  auto xor_block = [&](const uint8_t *p, size_t len) {
+    uint8_t *sb = reinterpret_cast<uint8_t *>(s);
+    for (size_t i = 0; i < len; ++i) sb[i] ^= p[i];
   };
-  while (inlen >= 136) { absorb(in); in += 136; inlen -= 136; }
-  std::memset(blk, 0, sizeof blk);
-  std::memcpy(blk, in, (size_t)inlen);
-  blk[inlen] ^= 0x1F;
-  blk[135] ^= 0x80;
-  for (int i = 0; i < 17; ++i) {
-    uint64_t w = 0;
-    for (int b = 0; b < 8; ++b) w |= (uint64_t)blk[8 * i + b] << (8 * b);
-    s[i] ^= w;
+  while (inlen >= 136) {
+    xor_block(in, 136);
+    keccakf(s);
+    in += 136;
+    inlen -= 136;
   }
+  xor_block(in, (size_t)inlen);
+  reinterpret_cast<uint8_t *>(s)[inlen] ^= 0x1F;
+  reinterpret_cast<uint8_t *>(s)[135] ^= 0x80;
   while (outlen > 0) {
     keccakf(s);
     const long long take = outlen < 136 ? outlen : 136;
-    for (long long i = 0; i < take; ++i) out[i] = (uint8_t)(s[i / 8] >> (8 * (i % 8)));
+    std::memcpy(out, s, (size_t)take);
     out += take;
     outlen -= take;
   }
@@ -533,6 +535,36 @@ static bool systematic_pk(const HostField &F, const uint16_t *g, const uint16_t 
 // seed of the chain (seed expansion by SHAKE256, Goppa polynomial, random support by sorting
 // 32-bit keys) up to the systematic-form step, and the secret key once that step has succeeded
 // (Benes control bits, rejection string).
+// The support permutation from 2^m random 32-bit keys (little-endian at pp): pi lists the indices
+// in ascending key order, as the sort of (key, index) pairs in LM:crypto_kem/<set>/vec/pk_gen.c
+// does; false when two keys are equal (the attempt fails).  LSD radix sort, four 8-bit digits.
+static bool support_perm(const uint8_t *pp, int nq, uint16_t *pi) {
+  thread_local std::vector<uint64_t> a, t;
+  a.resize(nq);
+  t.resize(nq);
+  for (int i = 0; i < nq; ++i) {
+    uint32_t k;
+    std::memcpy(&k, pp + 4 * i, 4);
+    a[i] = ((uint64_t)k << 16) | (uint64_t)i;
+  }
+  for (int sh = 16; sh < 48; sh += 8) {
+    uint32_t cnt[256] = {0};
+    for (int i = 0; i < nq; ++i) ++cnt[(a[i] >> sh) & 255];
+    uint32_t sum = 0;
+    for (int d = 0; d < 256; ++d) {
+      const uint32_t c = cnt[d];
+      cnt[d] = sum;
+      sum += c;
+    }
+    for (int i = 0; i < nq; ++i) t[cnt[(a[i] >> sh) & 255]++] = a[i];
+    a.swap(t);
+  }
+  for (int i = 1; i < nq; ++i)
+    if ((a[i - 1] >> 16) == (a[i] >> 16)) return false;
+  for (int i = 0; i < nq; ++i) pi[i] = (uint16_t)a[i];
+  return true;
+}
+
 template <int SET>
 static void candidate_t(uint8_t seed33[33], KeygenCandidate &c) {
   using P = Par<SET>;
@@ -541,7 +573,6 @@ static void candidate_t(uint8_t seed33[33], KeygenCandidate &c) {
   const size_t rlen = P::N / 8 + 4 * (size_t)NQ + 2 * P::T + 32;
   std::vector<uint8_t> r(rlen);
   std::vector<uint16_t> f(P::T);
-  std::vector<std::pair<uint32_t, uint16_t>> keys(NQ);
   c.g.assign(P::T + 1, 0);
   c.alpha.assign(P::N, 0);
   c.pi.assign(NQ, 0);
@@ -553,16 +584,7 @@ static void candidate_t(uint8_t seed33[33], KeygenCandidate &c) {
     for (int i = 0; i < P::T; ++i) f[i] = (uint16_t)((fp[2 * i] | (fp[2 * i + 1] << 8)) & ((1 << P::M) - 1));
     if (!min_poly<SET>(F, f.data(), c.g.data())) continue;
     c.g[P::T] = 1;
-    const uint8_t *pp = &r[P::N / 8];
-    for (int i = 0; i < NQ; ++i)
-      keys[i] = {(uint32_t)pp[4 * i] | ((uint32_t)pp[4 * i + 1] << 8) | ((uint32_t)pp[4 * i + 2] << 16) |
-                     ((uint32_t)pp[4 * i + 3] << 24),
-                 (uint16_t)i};
-    std::sort(keys.begin(), keys.end());
-    bool dup = false;
-    for (int i = 1; i < NQ && !dup; ++i) dup = keys[i - 1].first == keys[i].first;
-    if (dup) continue;
-    for (int i = 0; i < NQ; ++i) c.pi[i] = keys[i].second;
+    if (!support_perm(&r[P::N / 8], NQ, c.pi.data())) continue;
     for (int i = 0; i < P::N; ++i) c.alpha[i] = bitrev(c.pi[i], P::M);
     c.s.assign(r.begin(), r.begin() + P::N / 8);
     return;
@@ -574,8 +596,8 @@ static bool step_t(uint8_t seed33[33], KeygenCandidate &c) {
   using P = Par<SET>;
   constexpr int NQ = 1 << P::M;
   const size_t rlen = P::N / 8 + 4 * (size_t)NQ + 2 * P::T + 32;
-  std::vector<uint8_t> r(rlen);
-  std::vector<std::pair<uint32_t, uint16_t>> keys(NQ);
+  thread_local std::vector<uint8_t> r;
+  r.resize(rlen);
   c.f.assign(P::T, 0);
   c.g.assign(P::T + 1, 0);
   c.alpha.assign(P::N, 0);
@@ -585,15 +607,7 @@ static bool step_t(uint8_t seed33[33], KeygenCandidate &c) {
   std::memcpy(seed33 + 1, &r[rlen - 32], 32);  // the next attempt's seed, whatever happens below
   const uint8_t *fp = &r[rlen - 32 - 2 * P::T];
   for (int i = 0; i < P::T; ++i) c.f[i] = (uint16_t)((fp[2 * i] | (fp[2 * i + 1] << 8)) & ((1 << P::M) - 1));
-  const uint8_t *pp = &r[P::N / 8];
-  for (int i = 0; i < NQ; ++i)
-    keys[i] = {(uint32_t)pp[4 * i] | ((uint32_t)pp[4 * i + 1] << 8) | ((uint32_t)pp[4 * i + 2] << 16) |
-                   ((uint32_t)pp[4 * i + 3] << 24),
-               (uint16_t)i};
-  std::sort(keys.begin(), keys.end());
-  for (int i = 1; i < NQ; ++i)
-    if (keys[i - 1].first == keys[i].first) return false;
-  for (int i = 0; i < NQ; ++i) c.pi[i] = keys[i].second;
+  if (!support_perm(&r[P::N / 8], NQ, c.pi.data())) return false;
   for (int i = 0; i < P::N; ++i) c.alpha[i] = bitrev(c.pi[i], P::M);
   c.s.assign(r.begin(), r.begin() + P::N / 8);
   return true;
