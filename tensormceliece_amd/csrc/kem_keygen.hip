@@ -290,7 +290,13 @@ template <> struct ExtTaps<S6960119> { static constexpr int NT = 2; static const
 template <> struct ExtTaps<S6688128> { static constexpr int NT = 4; static constexpr int tap[4] = {7, 2, 1, 0}; static constexpr int coef[4] = {1, 1, 1, 1}; };
 template <> struct ExtTaps<S8192128> { static constexpr int NT = 4; static constexpr int tap[4] = {7, 2, 1, 0}; static constexpr int coef[4] = {1, 1, 1, 1}; };
 
-constexpr int GP_WG = 256;
+constexpr int GP_WG = 1024;
+// threads per product coefficient: the largest power of two S with S (2t - 1) <= GP_WG
+constexpr int gp_split(int t) {
+  int s = 1;
+  while (2 * s * (2 * t - 1) <= GP_WG) s *= 2;
+  return s;
+}
 
 template <int SET>
 __global__ __launch_bounds__(GP_WG) void k_genpoly(const uint16_t *__restrict__ f_all, const uint16_t *__restrict__ lg,
@@ -329,11 +335,18 @@ __global__ __launch_bounds__(GP_WG) void k_genpoly(const uint16_t *__restrict__ 
   for (int j = 2; j <= T; ++j) {
     for (int i = tid; i < T; i += GP_WG) la[i] = (uint16_t)lgz(mat[i * C + j - 1]);
     __syncthreads();
-    for (int k = tid; k < 2 * T - 1; k += GP_WG) {
+    {  // coefficient k of the product by S adjacent lanes, each over every S-th term
+      constexpr int S = gp_split(T);
+      static_assert(S <= 64 && S * (2 * T - 1) <= GP_WG, "one pass, lanes of one wave");
+      const int k = tid / S, part = tid % S;
       uint32_t acc = 0;
-      const int lo = k - T + 1 > 0 ? k - T + 1 : 0, hi = k < T - 1 ? k : T - 1;
-      for (int i = lo; i <= hi; ++i) acc ^= mul_l(la[i], lf[k - i]);
-      prod[k] = (uint16_t)acc;
+      if (k < 2 * T - 1) {
+        const int lo = k - T + 1 > 0 ? k - T + 1 : 0, hi = k < T - 1 ? k : T - 1;
+        for (int i = lo + part; i <= hi; i += S) acc ^= mul_l(la[i], lf[k - i]);
+      }
+#pragma unroll
+      for (int o = 1; o < S; o <<= 1) acc ^= (uint32_t)__shfl_xor((int)acc, o);
+      if (k < 2 * T - 1 && part == 0) prod[k] = (uint16_t)acc;
     }
     __syncthreads();
     // y^i for i >= t: the reduction from the top, in closed form.  Index q in [t, 2t-2] receives
