@@ -22,6 +22,19 @@
 namespace tmce {
 
 constexpr int KG_WG = 1024;
+
+// minimum over the wave, in every lane: DPP within each row of 16 lanes (quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror, row_mirror), then the four rows' values through scalar
+// registers -- no LDS round trips, unlike a butterfly of lane shuffles
+__device__ __forceinline__ int wave_min(int v) {
+  v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false));
+  v = min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false));
+  v = min(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false));
+  v = min(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false));
+  const int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+  const int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+  return min(min(a, b), min(c, d));
+}
 constexpr int KG_WAVES = KG_WG / 64;
 
 template <int SET, bool F>
@@ -183,8 +196,7 @@ __global__ __launch_bounds__(KG_WG) void k_pk_gen(const uint16_t *__restrict__ g
         }
       }
       if (last) break;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) best = min(best, __shfl_xor(best, o));
+      best = wave_min(best);
       if (lane == 0) red[wave] = best;
       __syncthreads();
       best = red[0];
@@ -388,8 +400,7 @@ __global__ __launch_bounds__(GP_WG) void k_genpoly(const uint16_t *__restrict__ 
         best = r;
         break;
       }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) best = min(best, __shfl_xor(best, o));
+    best = wave_min(best);
     if (lane == 0) red[wave] = best;
     __syncthreads();
     best = red[0];
