@@ -58,6 +58,22 @@ def test_keypair_batch_over_several_launches_and_usable(tm, orc):
         assert np.array_equal(ct, ct_o) and np.array_equal(k, k_o)
 
 
+def test_keypair_batch_staging_chunks(tm):
+    """8192128: more winners in one launch (about 60 of 64 keys) than the pinned staging buffer
+    holds public keys (49), so their downloads go in two chunks; keys spread over both chunks
+    against the host generator."""
+    s, n = "8192128", 64
+    rng = np.random.default_rng(64)
+    seeds = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+    print(f"keypair batch {s} n={n}", flush=True)
+    pk, sk = tm.keypair_batch(s, seeds)
+    idx = list(range(0, n, 4)) + [n - 1]
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        host = list(ex.map(lambda i: tm.keypair_seeded(s, seeds[i].tobytes()), idx))
+    for i, (hpk, hsk) in zip(idx, host):
+        assert np.array_equal(pk[i], hpk) and np.array_equal(sk[i], hsk), f"key {i}"
+
+
 def test_keypair_batch_empty(tm):
     pk, sk = tm.keypair_batch("8192128", np.zeros((0, 32), np.uint8))
     assert pk.shape == (0, tm.PARAMS["8192128"]["pk"]) and sk.shape[0] == 0
