@@ -482,6 +482,29 @@ def section_c4(tm, keys, log, total=65536):
     return out
 
 
+def verify_serving(wl, reqs, outputs, sample=32):
+    """Outputs of a serving run: every decapsulation returns the session keys of the pool
+    encapsulations its ciphertexts came from, with status 0; a sample of requests of each direction
+    equals the same request run alone through the plain batched calls."""
+    dec_ok = all(not st.any() and np.array_equal(k, wl.pool_k[key][off:off + inp.shape[0]])
+                 for (d, key, inp, off), (_, k, st) in zip(reqs, outputs) if d == "dec")
+    rng = np.random.default_rng(9)
+    alone_ok, checked = True, 0
+    for i in rng.choice(len(reqs), min(sample, len(reqs)), replace=False):
+        d, key, inp, _ = reqs[i]
+        ct, k, st = outputs[i]
+        if d == "enc":
+            ct_a, k_a, st_a = wl.pk[key].enc(inp)
+            alone_ok &= bool(np.array_equal(ct, ct_a) and np.array_equal(k, k_a) and np.array_equal(st, st_a))
+        else:
+            k_a, st_a = wl.sk[key].dec(inp)
+            alone_ok &= bool(np.array_equal(k, k_a) and np.array_equal(st, st_a))
+        checked += 1
+    n_dec = sum(r[0] == "dec" for r in reqs)
+    return {"bit_exact": bool(dec_ok and alone_ok), "dec_requests_checked_vs_pool": n_dec,
+            "requests_checked_vs_alone": checked}
+
+
 def section_c5(tm, log, nkeys=16, n_requests=6000):
     from tensormceliece_amd import serving
     s = "8192128"
@@ -493,15 +516,18 @@ def section_c5(tm, log, nkeys=16, n_requests=6000):
     try:
         reqs = serving.make_requests(wl, n_requests, seed=5)
         serving.run_poisson(server, wl, reqs[:200], 0.0)  # warm-up (first runs, graph capture)
-        sat = serving.run_poisson(server, wl, reqs, 0.0)
+        sat = serving.run_poisson(server, wl, reqs, 0.0, keep_outputs=True)
+        verified = verify_serving(wl, reqs, sat["outputs"])
+        sat["outputs"] = None
         mean_ops = float(np.mean([r[2].shape[0] for r in reqs]))
         req_rate_sat = sat["ops_per_s"] / mean_ops
         loads = {}
         for frac in (0.5, 0.8):
-            r = serving.run_poisson(server, wl, reqs, frac * req_rate_sat)
+            r = serving.run_poisson(server, wl, reqs, frac * req_rate_sat, keep_outputs=True)
+            v = verify_serving(wl, reqs, r["outputs"], sample=8)
             loads[f"{int(frac * 100)}%"] = {"offered_requests_per_s": round(frac * req_rate_sat, 1),
                                             "ops_per_s": round(r["ops_per_s"], 1), "p50_ms": round(r["p50_ms"], 3),
-                                            "p99_ms": round(r["p99_ms"], 3)}
+                                            "p99_ms": round(r["p99_ms"], 3), "bit_exact": v["bit_exact"]}
         # the same requests with inputs and outputs in pinned slabs: served in place (zero-copy)
         zreqs, inb, outb = serving.to_slab(reqs, tm.PARAMS[s])
         zsat = serving.run_poisson(server, wl, zreqs, 0.0, slab=outb)
@@ -520,6 +546,7 @@ def section_c5(tm, log, nkeys=16, n_requests=6000):
            "key_device_bytes": wl.device_bytes(),
            "saturation": {"ops_per_s": round(sat["ops_per_s"], 1), "p50_ms": round(sat["p50_ms"], 3),
                           "p99_ms": round(sat["p99_ms"], 3)},
+           "verified": verified,
            "poisson": loads, "zero_copy_slabs": zero_copy, "batches": st["batches"], "requests": st["requests"],
            "peak_in_flight_bytes": st["peak_in_flight_bytes"], "hbm_budget_bytes": 8 << 30}
     log(f"c5: {out}")

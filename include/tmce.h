@@ -141,7 +141,11 @@ extern "C" {
   const char *mceliece_dispatch_kem_##S##_compiler(long long impl);                             \
   long long mceliece_numimpl_kem_##S(void);
 
-/* LM:doc/api.md:39-116, one block per parameter set and variant */
+/* LM:doc/api.md:39-116, one block per parameter set and variant.
+   enc/dec return what libmceliece returns (0, or -1 under the 6960119 padding rules).  A device
+   failure, which libmceliece cannot have (no gfx950 device, an allocation or launch failing),
+   returns its negative TMCE_E* code instead, with the outputs cleared (ct and k for enc, k for
+   dec) and tmce_last_error() set; the process is never aborted. */
 TMCE_DECLARE_SET(348864)
 TMCE_DECLARE_SET(348864f)
 TMCE_DECLARE_SET(348864pc)
@@ -449,7 +453,10 @@ void *tmce_pipeline_stream(tmce_pipeline *p);              /* its hipStream_t */
    Requests use host buffers and key handles, which the caller keeps valid (and the keys
    unfreed) until the request is collected.
    tmce_server_wait blocks until a request completes and collects it; tmce_server_poll collects up
-   to `max` completed requests.  Latency = completion - submission (host steady clock).
+   to `max` completed requests.  A request being waited for is never returned by a concurrent
+   poll, and a second concurrent wait for it fails.  Latency = completion - submission (host
+   steady clock).  The worker learns of completions from a host function enqueued behind each
+   batch on its slot's stream (no polling of the device).
    Zero-copy requests: when every buffer of a request (in, k, status if given, ct for
    encapsulation) lies inside memory from tmce_host_alloc, aligned as the pipelines require (k 16
    bytes, status 4, in and ct 2), the GPU reads the inputs and writes the outputs there in place,
@@ -469,6 +476,7 @@ typedef struct tmce_server_stats {
   unsigned long long batches, requests, ops, pending;
   size_t in_flight_bytes, peak_in_flight_bytes, enc_bytes_per_op, dec_bytes_per_op;
   unsigned long long zero_copy_requests;  /* requests served in place from tmce_host_alloc slabs */
+  unsigned long long max_batch_keys;      /* most distinct keys in one batch (at most 16) */
 } tmce_server_stats;
 int tmce_server_create(int set, size_t slot_capacity, int slots_per_direction, size_t hbm_budget,
                        tmce_server **out);
@@ -477,6 +485,14 @@ int tmce_server_wait(tmce_server *sv, unsigned long long id, double *latency_s);
 int tmce_server_poll(tmce_server *sv, unsigned long long *ids, double *latency_s, int max);
 int tmce_server_get_stats(tmce_server *sv, tmce_server_stats *st);
 void tmce_server_free(tmce_server *sv);
+/* Open-loop load (no counterpart in the reference): submits rqs[i] at arrival_s[i] seconds after
+   the call starts (arrivals sorted; a native thread sleeps, then spins to each arrival), collects
+   every request as it completes and returns when all have: latency_s[i] = completion - submission
+   of request i (may be NULL), *elapsed_s = last completion - start.  These requests are neither
+   polled nor waited for by the other calls.  On an error the requests not yet collected go back
+   to tmce_server_poll. */
+int tmce_server_run_load(tmce_server *sv, const tmce_request *rqs, const double *arrival_s, size_t n,
+                         double *latency_s, double *elapsed_s);
 
 /* select the pipeline used by subsequent batched calls: 0 = engine form (default: fused,
    persistent sampler, row-parallel decoder), 1 = plain one-launch-per-stage form (the baseline

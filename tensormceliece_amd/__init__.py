@@ -7,6 +7,7 @@ importing works anywhere, but every KEM call needs the built library and a gfx95
 otherwise.
 """
 import ctypes
+import threading
 import os
 
 import numpy as np
@@ -95,6 +96,7 @@ def load():
         "tmce_server_poll": (I, [P, P, P, I]),
         "tmce_server_get_stats": (I, [P, P]),
         "tmce_server_free": (V, [P]),
+        "tmce_server_run_load": (I, [P, P, P, S, P, P]),
         "tmce_timing_read": (I, [I, P, P, P]),
         "tmce_stream_create": (I, [ctypes.POINTER(P)]),
         "tmce_stream_destroy": (I, [P]),
@@ -394,7 +396,8 @@ class ServerStatsC(ctypes.Structure):
     _fields_ = [("batches", ctypes.c_ulonglong), ("requests", ctypes.c_ulonglong), ("ops", ctypes.c_ulonglong),
                 ("pending", ctypes.c_ulonglong), ("in_flight_bytes", ctypes.c_size_t),
                 ("peak_in_flight_bytes", ctypes.c_size_t), ("enc_bytes_per_op", ctypes.c_size_t),
-                ("dec_bytes_per_op", ctypes.c_size_t), ("zero_copy_requests", ctypes.c_ulonglong)]
+                ("dec_bytes_per_op", ctypes.c_size_t), ("zero_copy_requests", ctypes.c_ulonglong),
+                ("max_batch_keys", ctypes.c_ulonglong)]
 
 
 class Server:
@@ -406,6 +409,9 @@ class Server:
         self.params = PARAMS[set_name]
         self._h = ctypes.c_void_p()
         self._live = {}
+        # submit() registers a request's arrays under this lock in the same critical section as the
+        # C submit, so a poll() on another thread cannot collect the id before it is registered
+        self._mu = threading.Lock()
         _check(load().tmce_server_create(SETS[set_name], int(slot_capacity), int(slots_per_direction),
                                          int(hbm_budget), ctypes.byref(self._h)), "tmce_server_create")
 
@@ -426,25 +432,50 @@ class Server:
         rq = RequestC(d, key.handle.value, n, inp.ctypes.data, ct.ctypes.data if ct is not None else None,
                       k.ctypes.data, status.ctypes.data)
         rid = ctypes.c_ulonglong()
-        _check(load().tmce_server_submit(self._h, ctypes.byref(rq), ctypes.byref(rid)), "tmce_server_submit")
-        self._live[rid.value] = (key, inp, ct, k, status)  # the key handle must outlive the request
+        with self._mu:
+            _check(load().tmce_server_submit(self._h, ctypes.byref(rq), ctypes.byref(rid)), "tmce_server_submit")
+            self._live[rid.value] = (key, inp, ct, k, status)  # the key handle must outlive the request
         return rid.value, (ct, k, status)
 
     def wait(self, rid):
         lat = ctypes.c_double()
         _check(load().tmce_server_wait(self._h, rid, ctypes.byref(lat)), "tmce_server_wait")
-        self._live.pop(rid, None)
+        with self._mu:
+            self._live.pop(rid, None)
         return lat.value
 
     def poll(self, max_items=4096):
         ids = np.zeros(max_items, np.uint64)
         lats = np.zeros(max_items, np.float64)
-        n = load().tmce_server_poll(self._h, _ptr(ids), _ptr(lats), max_items)
-        if n < 0:
-            _check(n, "tmce_server_poll")
-        for i in ids[:n]:
-            self._live.pop(int(i), None)
+        with self._mu:
+            n = load().tmce_server_poll(self._h, _ptr(ids), _ptr(lats), max_items)
+            if n < 0:
+                _check(n, "tmce_server_poll")
+            for i in ids[:n]:
+                self._live.pop(int(i), None)
         return ids[:n].tolist(), lats[:n].tolist()
+
+    def run_load(self, requests, arrival_s):
+        """Open-loop load through the native driver (tmce_server_run_load): requests[i] =
+        (direction, key, inputs, ct, k, status) with output arrays given, submitted at arrival_s[i]
+        seconds after the start (sorted).  Blocks until all complete; returns (latency_s array,
+        elapsed_s)."""
+        n = len(requests)
+        arr = (RequestC * max(n, 1))()
+        keep = []
+        for i, (direction, key, inp, ct, k, status) in enumerate(requests):
+            inp = np.ascontiguousarray(inp, np.uint8)
+            keep.append((key, inp, ct, k, status))
+            arr[i] = RequestC({"enc": 0, "dec": 1}[direction], key.handle.value, inp.shape[0], inp.ctypes.data,
+                              ct.ctypes.data if ct is not None else None, k.ctypes.data,
+                              status.ctypes.data if status is not None else None)
+        arrivals = np.ascontiguousarray(arrival_s, np.float64)
+        lat = np.zeros(n, np.float64)
+        el = ctypes.c_double()
+        _check(load().tmce_server_run_load(self._h, ctypes.cast(arr, ctypes.c_void_p), _ptr(arrivals), n, _ptr(lat),
+                                           ctypes.byref(el)), "tmce_server_run_load")
+        del keep
+        return lat, el.value
 
     def stats(self):
         st = ServerStatsC()

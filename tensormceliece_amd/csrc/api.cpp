@@ -15,7 +15,8 @@
 #include "engine.h"
 
 namespace tmce {
-KernelTimer *g_ktimer = nullptr;
+std::atomic<KernelTimer *> g_ktimer{nullptr};
+thread_local bool g_ktimer_off = false;
 }
 
 using namespace tmce;
@@ -392,9 +393,15 @@ int single_enc(int variant, unsigned char *c, unsigned char *k, const unsigned c
   host_gen_e(set, rnd_global, nullptr, pos.data());
   std::lock_guard<std::mutex> lk(g_cache_mu);
   tmce_key *key = cached_key(variant, 1, pk, (size_t)set_info(set).pk_bytes);
-  if (!key) std::abort();
   int st = 0;
-  if (enc_support(key, c, k, &st, pos.data(), 1) != 0) std::abort();
+  // a device failure (no gfx950 device, allocation, launch) returns its TMCE_E* code with the
+  // outputs cleared; tmce_last_error() says what failed
+  const int r = key ? enc_support(key, c, k, &st, pos.data(), 1) : TMCE_EDEVICE;
+  if (r != 0) {
+    std::memset(c, 0, (size_t)variant_ct_bytes(variant));
+    std::memset(k, 0, 32);
+    return r;
+  }
   return st;
 }
 
@@ -404,9 +411,12 @@ int single_dec(int variant, unsigned char *k, const unsigned char *c, const unsi
   uint8_t ctc[256];
   std::memcpy(ctc, c, (size_t)variant_ct_bytes(variant));  // k and c may alias
   tmce_key *key = cached_key(variant, 2, sk, (size_t)si.sk_bytes);
-  if (!key) std::abort();
   int st = 0;
-  if (tmce_dec_batch(key, k, &st, ctc, 1) != 0) std::abort();
+  const int r = key ? tmce_dec_batch(key, k, &st, ctc, 1) : TMCE_EDEVICE;
+  if (r != 0) {  // device failure: its TMCE_E* code, k cleared
+    std::memset(k, 0, 32);
+    return r;
+  }
   return st;
 }
 
@@ -769,7 +779,7 @@ int tmce_timing_enable(int on) {
   std::lock_guard<std::mutex> lk(e->mu);
   e->timer.drain();
   if (on) {
-    e->timer.acc.clear();
+    e->timer.reset();
     g_ktimer = &e->timer;
   } else {
     g_ktimer = nullptr;
@@ -783,7 +793,8 @@ int tmce_timing_read(int max, char *names, long *counts, double *total_ms) {
   std::lock_guard<std::mutex> lk(e->mu);
   e->timer.drain();
   int i = 0;
-  for (auto &kv : e->timer.acc) {
+  const auto acc = e->timer.snapshot();
+  for (auto &kv : acc) {
     if (i >= max) break;
     std::snprintf(names + 48 * i, 48, "%s", kv.first.c_str());
     counts[i] = kv.second.first;

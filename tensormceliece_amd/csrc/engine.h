@@ -62,6 +62,7 @@ struct DevBuf {
 
 struct EventTimer : KernelTimer {
   struct Rec { std::string name; hipEvent_t a, b; };
+  std::mutex tmu;  // begin/end/drain may come from several host threads
   std::vector<Rec> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<long, double>> acc;
@@ -71,13 +72,27 @@ struct EventTimer : KernelTimer {
     (void)hipEventCreate(&e);
     return e;
   }
-  void begin(const char *name, hipStream_t s) override {
+  long begin(const char *name, hipStream_t s) override {
+    std::lock_guard<std::mutex> lk(tmu);
     Rec r{name, get(), get()};
     (void)hipEventRecord(r.a, s);
     pending.push_back(r);
+    return (long)pending.size() - 1;
   }
-  void end(hipStream_t s) override { (void)hipEventRecord(pending.back().b, s); }
+  void end(long h, hipStream_t s) override {
+    std::lock_guard<std::mutex> lk(tmu);
+    if (h >= 0 && h < (long)pending.size()) (void)hipEventRecord(pending[(size_t)h].b, s);
+  }
+  std::map<std::string, std::pair<long, double>> snapshot() {
+    std::lock_guard<std::mutex> lk(tmu);
+    return acc;
+  }
+  void reset() {
+    std::lock_guard<std::mutex> lk(tmu);
+    acc.clear();
+  }
   void drain() {
+    std::lock_guard<std::mutex> lk(tmu);
     for (auto &r : pending) {
       (void)hipEventSynchronize(r.b);
       float ms = 0;

@@ -1,6 +1,7 @@
 // kem_kernels.h -- launch interface between the host engine (api.cpp) and the HIP kernels.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <cstdint>
 #include "params.h"
 #include "device_common.h"
@@ -238,17 +239,22 @@ hipError_t launch_fill_u32(unsigned *p, unsigned v, int n, hipStream_t s);
 hipError_t launch_fill_i32(int *p, int v, size_t n, hipStream_t s);
 
 // Optional per-kernel timing: when set, each engine launch is bracketed by events on its stream.
+// begin() returns a handle for the matching end(), so launches timed from several host threads
+// keep their own event pairs; a thread that captures a graph sets g_ktimer_off (thread-local), so
+// no event record lands inside the capture while other threads' launches are still timed.
 struct KernelTimer {
-  virtual void begin(const char *name, hipStream_t s) = 0;
-  virtual void end(hipStream_t s) = 0;
+  virtual long begin(const char *name, hipStream_t s) = 0;
+  virtual void end(long handle, hipStream_t s) = 0;
   virtual ~KernelTimer() = default;
 };
-extern KernelTimer *g_ktimer;
-#define TMCE_TIMED(name, stream, ...)                   \
-  do {                                                  \
-    if (::tmce::g_ktimer) ::tmce::g_ktimer->begin(name, stream); \
-    __VA_ARGS__;                                        \
-    if (::tmce::g_ktimer) ::tmce::g_ktimer->end(stream);  \
+extern std::atomic<KernelTimer *> g_ktimer;
+extern thread_local bool g_ktimer_off;
+#define TMCE_TIMED(name, stream, ...)                                                      \
+  do {                                                                                     \
+    ::tmce::KernelTimer *kt_ = ::tmce::g_ktimer_off ? nullptr : ::tmce::g_ktimer.load();   \
+    const long kh_ = kt_ ? kt_->begin(name, stream) : -1;                                  \
+    __VA_ARGS__;                                                                           \
+    if (kt_) kt_->end(kh_, stream);                                                        \
   } while (0)
 
 }  // namespace tmce

@@ -358,7 +358,6 @@ __global__ __launch_bounds__(256) void k_plain_reenc(const uint32_t *__restrict_
 // ------------------------------------------------------------------ launchers
 template <int SET>
 hipError_t launch_plain_enc(const PlainEncArgs &a, hipStream_t s) {
-  using P = Par<SET>;
   if (a.nops <= 0) return hipSuccess;
   TMCE_TIMED("plain_gen_e", s, k_plain_gen_e<SET><<<ceil_div(a.nops, PG_WG), PG_WG, 0, s>>>(a.seeds, a.erow, a.Ens, a.nops));
   TMCE_TIMED("plain_syndrome", s,
@@ -371,14 +370,16 @@ hipError_t launch_plain_enc(const PlainEncArgs &a, hipStream_t s) {
 
 template <int SET>
 hipError_t launch_plain_dec(const DecArgs &a, const PlainDecScratch &x, hipStream_t s) {
-  using P = Par<SET>;
   if (a.nops <= 0) return hipSuccess;
-  TMCE_TIMED("plain_front", s, launch_dec_front_only(SET, a, s));
+  hipError_t e = hipSuccess;
+  TMCE_TIMED("plain_front", s, e = launch_dec_front_only(SET, a, s));
+  if (e != hipSuccess) return e;
   TMCE_TIMED("plain_bm", s,
              k_plain_bm<SET><<<ceil_div(a.nops, PG_WG), PG_WG, 0, s>>>(a.S, a.ft.lg, a.ft.ex, x.loc, a.nops));
   TMCE_TIMED("plain_roots", s, k_plain_roots<SET><<<ceil_div(a.nops, 4), 256, 0, s>>>(x.loc, a, x.erow, x.weight));
   TMCE_TIMED("plain_reenc", s, k_plain_reenc<SET><<<ceil_div(a.nops, 4), 256, 0, s>>>(x.erow, x.weight, a));
-  TMCE_TIMED("plain_hash", s, launch_dec_hash_only(SET, a, s));
+  TMCE_TIMED("plain_hash", s, e = launch_dec_hash_only(SET, a, s));
+  if (e != hipSuccess) return e;
   return hipGetLastError();
 }
 

@@ -289,20 +289,24 @@ int tmce_pipeline_run(tmce_pipeline *p, const tmce_sub_batch *subs, size_t nsub)
     p->graph = nullptr;
   }
   if (!p->exec) {
-    KernelTimer *saved = g_ktimer;  // no per-kernel event records inside a captured graph
-    g_ktimer = nullptr;
     HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    g_ktimer_off = true;  // no per-kernel event records inside a captured graph (this thread only)
     const int r = issue(p, knobs);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(p->stream, &g);
-    g_ktimer = saved;
+    g_ktimer_off = false;
     if (r) {
       if (g) (void)hipGraphDestroy(g);
       return r;
     }
     if (ce != hipSuccess) return fail(TMCE_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    const hipError_t ie = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {  // keep no half-built graph: the next run captures afresh
+      (void)hipGraphDestroy(g);
+      p->exec = nullptr;
+      return fail(TMCE_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
     p->graph = g;
-    HIPCHK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
     std::memcpy(p->tuned, knobs, sizeof knobs);
   }
   HIPCHK(hipGraphLaunch(p->exec, p->stream));

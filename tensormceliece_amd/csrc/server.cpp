@@ -15,9 +15,13 @@
 // sub-batches point at the slabs' device addresses, so the pipeline's gather kernel reads the
 // inputs and its scatter kernel writes the outputs across the host link directly.
 //
+// Completion: a host function enqueued behind each batch on its slot's stream wakes the worker, which
+// then copies the outputs back (no polling).
+//
 // Latency = completion - submission (host steady clock), including queueing, transfers and compute.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -44,11 +48,21 @@ struct Req {
   size_t footprint = 0;
   double t_submit = 0, t_start = 0, t_done = 0;
   bool done = false;
+  bool claimed = false;  // a tmce_server_wait call owns it: tmce_server_poll leaves it alone
+  struct LoadRun *run = nullptr;  // owned by a tmce_server_run_load call (never polled or waited)
+  size_t run_index = 0;
   // zero-copy: device addresses of the request's buffers inside tmce_host_alloc slabs
   bool zero_copy = false;
   const uint8_t *d_in = nullptr;
   uint8_t *d_ct = nullptr, *d_k = nullptr;
   int *d_status = nullptr;
+};
+
+// One tmce_server_run_load call: the indices of its requests that have completed, appended by the
+// worker (under the server's mutex) and consumed by the call.
+struct LoadRun {
+  std::vector<size_t> done;
+  std::vector<Req *> req;  // by request index
 };
 
 struct Slot {
@@ -59,6 +73,8 @@ struct Slot {
   uint8_t *h_in = nullptr, *h_out = nullptr;  // pinned staging
   uint8_t *d_io = nullptr;                    // device: inputs [cap][in_row], outputs [cap][out_row]
   hipEvent_t done = nullptr;
+  std::atomic<bool> signalled{false};  // set by the stream's host callback when the batch is done
+  tmce_server *owner = nullptr;
   bool busy = false;
   size_t footprint = 0;
   std::vector<Req *> reqs;
@@ -71,24 +87,45 @@ struct tmce_server {
   bool pc = false;   // plaintext-confirmation variant
   size_t ctb = 0;    // ciphertext bytes
   size_t budget = 0, in_flight = 0, peak_in_flight = 0;
-  std::vector<Slot> slots;
+  std::deque<Slot> slots;  // stable addresses: the host callbacks hold Slot pointers
   std::deque<Req *> pending[2];
   std::unordered_map<unsigned long long, Req *> reqs;
   std::deque<unsigned long long> completed;
   unsigned long long next_id = 1;
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_done;
+  // Wake-ups of the worker: submissions, batch completions (host callbacks on the slot streams)
+  // and stop each bump wake_seq under wake_mu.  The worker never calls HIP while holding wake_mu
+  // and the callbacks take no other lock, so a callback never waits on the worker.
+  std::mutex wake_mu;
+  std::condition_variable cv_wake;
+  unsigned long long wake_seq = 0;
   std::thread worker;
   bool stop = false;
   int error = 0;  // first failure of the worker (guarded by mu)
   std::string error_msg;
   // statistics
-  unsigned long long batches = 0, ops = 0, requests = 0, zero_copy = 0;
+  unsigned long long batches = 0, ops = 0, requests = 0, zero_copy = 0, max_batch_keys = 0;
   size_t per_op_bytes[2] = {0, 0};
 };
 
 namespace tmce {
 namespace {
+
+void wake(tmce_server *sv) {
+  {
+    std::lock_guard<std::mutex> lk(sv->wake_mu);
+    ++sv->wake_seq;
+  }
+  sv->cv_wake.notify_one();
+}
+
+// host callback enqueued behind each batch on its slot's stream (runs on HIP's callback thread)
+void slot_done_cb(void *arg) {
+  Slot *sl = static_cast<Slot *>(arg);
+  sl->signalled.store(true, std::memory_order_release);
+  wake(sl->owner);
+}
 
 size_t round_up_sz(size_t a, size_t b) { return (a + b - 1) / b * b; }
 
@@ -112,6 +149,7 @@ std::vector<Req *> form_batch(tmce_server *sv, Slot &sl) {
     out.push_back(r);
   }
   sl.footprint = foot;
+  if (group.size() > sv->max_batch_keys) sv->max_batch_keys = group.size();
   return out;
 }
 
@@ -147,6 +185,8 @@ int launch_slot(tmce_server *sv, Slot &sl, std::vector<Req *> &batch) {
   if (int rc = tmce_pipeline_run(sl.pipe, subs.data(), subs.size())) return rc;
   if (pos) HIPCHK(hipMemcpyAsync(sl.h_out, d_out, pos * sl.out_row, hipMemcpyDeviceToHost, sl.stream));
   HIPCHK(hipEventRecord(sl.done, sl.stream));
+  sl.signalled.store(false, std::memory_order_relaxed);
+  HIPCHK(hipLaunchHostFunc(sl.stream, slot_done_cb, &sl));
   sl.reqs = batch;
   sl.busy = true;
   sv->batches += 1;
@@ -169,7 +209,8 @@ void finish_slot(tmce_server *sv, Slot &sl) {
     }
     r->t_done = t;
     r->done = true;
-    sv->completed.push_back(r->id);
+    if (r->run) r->run->done.push_back(r->run_index);
+    else sv->completed.push_back(r->id);
   }
   sv->in_flight -= sl.footprint;
   sl.footprint = 0;
@@ -179,40 +220,42 @@ void finish_slot(tmce_server *sv, Slot &sl) {
 
 void worker_main(tmce_server *sv) {
   (void)hipSetDevice(g_engine->device);
-  std::unique_lock<std::mutex> lk(sv->mu);
-  while (!sv->stop) {
-    bool progressed = false, any_busy = false;
-    for (Slot &sl : sv->slots) {
-      if (sl.busy && hipEventQuery(sl.done) == hipSuccess) {
-        finish_slot(sv, sl);
-        progressed = true;
-      }
+  unsigned long long seen = 0;
+  for (;;) {
+    {
+      std::lock_guard<std::mutex> wl(sv->wake_mu);
+      seen = sv->wake_seq;  // events after this read wake the wait below
     }
-    if (progressed) sv->cv_done.notify_all();
-    for (Slot &sl : sv->slots) {
-      if (!sl.busy && !sv->pending[sl.dir].empty()) {
-        std::vector<Req *> batch = form_batch(sv, sl);
-        if (batch.empty()) continue;
-        sv->in_flight += sl.footprint;
-        if (sv->in_flight > sv->peak_in_flight) sv->peak_in_flight = sv->in_flight;
-        if (int rc = launch_slot(sv, sl, batch)) {
-          sv->error = rc;
-          sv->error_msg = tmce_last_error();
-          sv->stop = true;
-          break;
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      if (sv->stop) break;
+      bool progressed = false;
+      for (Slot &sl : sv->slots) {
+        // the callback ran after the batch's last operation; the event query confirms it
+        if (sl.busy && sl.signalled.load(std::memory_order_acquire) && hipEventQuery(sl.done) == hipSuccess) {
+          finish_slot(sv, sl);
+          progressed = true;
         }
       }
-      any_busy |= sl.busy;
+      if (progressed) sv->cv_done.notify_all();
+      for (Slot &sl : sv->slots) {
+        if (!sl.busy && !sv->pending[sl.dir].empty()) {
+          std::vector<Req *> batch = form_batch(sv, sl);
+          if (batch.empty()) continue;
+          sv->in_flight += sl.footprint;
+          if (sv->in_flight > sv->peak_in_flight) sv->peak_in_flight = sv->in_flight;
+          if (int rc = launch_slot(sv, sl, batch)) {
+            sv->error = rc;
+            sv->error_msg = tmce_last_error();
+            sv->stop = true;
+            break;
+          }
+        }
+      }
+      if (sv->stop) break;
     }
-    if (sv->stop) break;
-    if (any_busy) {
-      // completions are polled: a short sleep outside the lock keeps submissions flowing
-      lk.unlock();
-      std::this_thread::sleep_for(std::chrono::microseconds(20));
-      lk.lock();
-    } else if (sv->pending[0].empty() && sv->pending[1].empty()) {
-      sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending[0].empty() || !sv->pending[1].empty(); });
-    }
+    std::unique_lock<std::mutex> wl(sv->wake_mu);
+    sv->cv_wake.wait(wl, [&] { return sv->wake_seq != seen; });
   }
   sv->cv_done.notify_all();
 }
@@ -223,7 +266,7 @@ void destroy_server(tmce_server *sv) {
     std::lock_guard<std::mutex> lk(sv->mu);
     sv->stop = true;
   }
-  sv->cv_work.notify_all();
+  wake(sv);
   if (sv->worker.joinable()) sv->worker.join();
   for (Slot &sl : sv->slots) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
@@ -236,6 +279,54 @@ void destroy_server(tmce_server *sv) {
   }
   for (auto &kv : sv->reqs) delete kv.second;
   delete sv;
+}
+
+// validates and queues one request; with `run` it belongs to a tmce_server_run_load call
+int submit_request(tmce_server *sv, const tmce_request *rq, LoadRun *run, size_t run_index, unsigned long long *id) {
+  if (!sv || !rq || !rq->key || (rq->direction != 0 && rq->direction != 1))
+    return fail(TMCE_EINVAL, "tmce_server_submit: bad argument");
+  if (rq->key->set != sv->set || rq->key->pc != sv->pc || rq->key->kind != (rq->direction == 0 ? 1 : 2))
+    return fail(TMCE_EINVAL, "tmce_server_submit: key of the wrong set, variant or kind");
+  if (rq->n == 0 || rq->n + PLAN_ALIGN - 1 > sv->slots[0].cap || !rq->in || !rq->k || (rq->direction == 0 && !rq->ct))
+    return fail(TMCE_EINVAL, "tmce_server_submit: empty request, request larger than a slot, or missing buffers");
+  auto *r = new Req();
+  r->dir = rq->direction;
+  r->key = rq->key;
+  r->n = rq->n;
+  r->in = rq->in;
+  r->ct = rq->ct;
+  r->k = rq->k;
+  r->status = rq->status;
+  r->footprint = rq->n * sv->per_op_bytes[r->dir];
+  r->run = run;
+  r->run_index = run_index;
+  r->claimed = run != nullptr;
+  {  // zero-copy when every buffer lies in a slab with the pipelines' alignment
+    const size_t n = rq->n;
+    r->d_in = slab_device_ptr(rq->in, n * (r->dir == 0 ? 32 : sv->ctb));
+    r->d_k = slab_device_ptr(rq->k, n * 32);
+    r->d_status = rq->status ? reinterpret_cast<int *>(slab_device_ptr(rq->status, n * sizeof(int))) : nullptr;
+    r->d_ct = r->dir == 0 ? slab_device_ptr(rq->ct, n * sv->ctb) : nullptr;
+    r->zero_copy = r->d_in && r->d_k && (!rq->status || r->d_status) && (r->dir == 1 || r->d_ct) &&
+                   !((uintptr_t)r->d_k & 15) && !((uintptr_t)r->d_status & 3) && !((uintptr_t)r->d_in & 1) &&
+                   !((uintptr_t)r->d_ct & 1);
+  }
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->stop) {
+      delete r;
+      return fail(sv->error ? sv->error : TMCE_EINVAL, sv->error ? sv->error_msg : "server stopped");
+    }
+    r->id = sv->next_id++;
+    r->t_submit = now_s();
+    if (r->zero_copy) sv->zero_copy += 1;
+    sv->reqs[r->id] = r;
+    if (run) run->req[run_index] = r;
+    sv->pending[r->dir].push_back(r);
+    if (id) *id = r->id;
+  }
+  wake(sv);
+  return 0;
 }
 
 }  // namespace
@@ -265,6 +356,7 @@ int tmce_server_create(int variant, size_t slot_capacity, int slots_per_directio
       sv->slots.emplace_back();
       Slot &sl = sv->slots.back();
       sl.dir = dir;
+      sl.owner = sv;
       if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(TMCE_EDEVICE, "hipStreamCreate"));
       if (int r = tmce_pipeline_create(variant, dir, slot_capacity, PLAN_MAX_GROUPS * 64 + 1024, 1, sl.stream, &sl.pipe))
@@ -289,46 +381,8 @@ int tmce_server_create(int variant, size_t slot_capacity, int slots_per_directio
 }
 
 int tmce_server_submit(tmce_server *sv, const tmce_request *rq, unsigned long long *id) {
-  if (!sv || !rq || !id || !rq->key || (rq->direction != 0 && rq->direction != 1))
-    return fail(TMCE_EINVAL, "tmce_server_submit: bad argument");
-  if (rq->key->set != sv->set || rq->key->pc != sv->pc || rq->key->kind != (rq->direction == 0 ? 1 : 2))
-    return fail(TMCE_EINVAL, "tmce_server_submit: key of the wrong set, variant or kind");
-  if (rq->n == 0 || rq->n + PLAN_ALIGN - 1 > sv->slots[0].cap || !rq->in || !rq->k || (rq->direction == 0 && !rq->ct))
-    return fail(TMCE_EINVAL, "tmce_server_submit: empty request, request larger than a slot, or missing buffers");
-  auto *r = new Req();
-  r->dir = rq->direction;
-  r->key = rq->key;
-  r->n = rq->n;
-  r->in = rq->in;
-  r->ct = rq->ct;
-  r->k = rq->k;
-  r->status = rq->status;
-  r->footprint = rq->n * sv->per_op_bytes[r->dir];
-  {  // zero-copy when every buffer lies in a slab with the pipelines' alignment
-    const size_t n = rq->n;
-    r->d_in = slab_device_ptr(rq->in, n * (r->dir == 0 ? 32 : sv->ctb));
-    r->d_k = slab_device_ptr(rq->k, n * 32);
-    r->d_status = rq->status ? reinterpret_cast<int *>(slab_device_ptr(rq->status, n * sizeof(int))) : nullptr;
-    r->d_ct = r->dir == 0 ? slab_device_ptr(rq->ct, n * sv->ctb) : nullptr;
-    r->zero_copy = r->d_in && r->d_k && (!rq->status || r->d_status) && (r->dir == 1 || r->d_ct) &&
-                   !((uintptr_t)r->d_k & 15) && !((uintptr_t)r->d_status & 3) && !((uintptr_t)r->d_in & 1) &&
-                   !((uintptr_t)r->d_ct & 1);
-  }
-  {
-    std::lock_guard<std::mutex> lk(sv->mu);
-    if (sv->stop) {
-      delete r;
-      return fail(sv->error ? sv->error : TMCE_EINVAL, sv->error ? sv->error_msg : "server stopped");
-    }
-    r->id = sv->next_id++;
-    r->t_submit = now_s();
-    if (r->zero_copy) sv->zero_copy += 1;
-    sv->reqs[r->id] = r;
-    sv->pending[r->dir].push_back(r);
-  }
-  sv->cv_work.notify_one();
-  *id = r->id;
-  return 0;
+  if (!id) return fail(TMCE_EINVAL, "tmce_server_submit: bad argument");
+  return submit_request(sv, rq, nullptr, 0, id);
 }
 
 int tmce_server_wait(tmce_server *sv, unsigned long long id, double *latency_s) {
@@ -337,10 +391,15 @@ int tmce_server_wait(tmce_server *sv, unsigned long long id, double *latency_s) 
   auto it = sv->reqs.find(id);
   if (it == sv->reqs.end()) return fail(TMCE_EINVAL, "tmce_server_wait: unknown request (or already collected)");
   Req *r = it->second;
+  if (r->claimed) return fail(TMCE_EINVAL, "tmce_server_wait: another thread is waiting for this request");
+  r->claimed = true;  // from here on only this call collects (and frees) it
   sv->cv_done.wait(lk, [&] { return r->done || sv->stop; });
-  if (!r->done) return fail(sv->error ? sv->error : TMCE_EDEVICE, sv->error ? sv->error_msg : "server stopped");
+  if (!r->done) {
+    r->claimed = false;
+    return fail(sv->error ? sv->error : TMCE_EDEVICE, sv->error ? sv->error_msg : "server stopped");
+  }
   if (latency_s) *latency_s = r->t_done - r->t_submit;
-  sv->reqs.erase(it);
+  sv->reqs.erase(id);
   delete r;
   return 0;
 }
@@ -355,6 +414,7 @@ int tmce_server_poll(tmce_server *sv, unsigned long long *ids, double *latency_s
     sv->completed.pop_front();
     auto it = sv->reqs.find(id);
     if (it == sv->reqs.end()) continue;  // already collected by tmce_server_wait
+    if (it->second->claimed) continue;   // a tmce_server_wait call collects it
     ids[n] = id;
     if (latency_s) latency_s[n] = it->second->t_done - it->second->t_submit;
     delete it->second;
@@ -362,6 +422,85 @@ int tmce_server_poll(tmce_server *sv, unsigned long long *ids, double *latency_s
     ++n;
   }
   return n;
+}
+
+int tmce_server_run_load(tmce_server *sv, const tmce_request *rqs, const double *arrival_s, size_t n,
+                         double *latency_s, double *elapsed_s) {
+  if (!sv || (n && (!rqs || !arrival_s))) return fail(TMCE_EINVAL, "tmce_server_run_load: bad argument");
+  for (size_t i = 1; i < n; ++i)
+    if (arrival_s[i] < arrival_s[i - 1]) return fail(TMCE_EINVAL, "tmce_server_run_load: arrivals not sorted");
+  LoadRun run;
+  run.req.assign(n, nullptr);
+  const auto t0 = std::chrono::steady_clock::now();
+  const double t0s = now_s();
+  int sub_rc = 0;
+  size_t submitted = 0;
+  std::thread submitter([&] {
+    for (size_t i = 0; i < n; ++i) {
+      const auto due = t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                                std::chrono::duration<double>(arrival_s[i]));
+      // sleep to about 100 us before the arrival, then spin (sleep wake-ups are coarser than the
+      // gaps between arrivals at high rates)
+      const auto early = due - std::chrono::microseconds(100);
+      if (std::chrono::steady_clock::now() < early) std::this_thread::sleep_until(early);
+      while (std::chrono::steady_clock::now() < due) {
+      }
+      const int rc = submit_request(sv, &rqs[i], &run, i, nullptr);
+      std::lock_guard<std::mutex> lk(sv->mu);
+      if (rc) {
+        sub_rc = rc;
+        sv->cv_done.notify_all();
+        return;
+      }
+      ++submitted;
+    }
+  });
+  int rc = 0;
+  size_t collected = 0;
+  double t_last = t0s;
+  {
+    std::unique_lock<std::mutex> lk(sv->mu);
+    for (;;) {
+      sv->cv_done.wait(lk, [&] { return !run.done.empty() || sv->stop || sub_rc || collected == n; });
+      for (size_t idx : run.done) {
+        Req *r = run.req[idx];
+        if (latency_s) latency_s[idx] = r->t_done - r->t_submit;
+        if (r->t_done > t_last) t_last = r->t_done;
+        sv->reqs.erase(r->id);
+        delete r;
+        run.req[idx] = nullptr;
+        ++collected;
+      }
+      run.done.clear();
+      if (collected == n) break;
+      if (sv->stop) {
+        rc = fail(sv->error ? sv->error : TMCE_EDEVICE, sv->error ? sv->error_msg : "server stopped");
+        break;
+      }
+      if (sub_rc && collected == submitted) {
+        rc = sub_rc;
+        break;
+      }
+    }
+  }
+  submitter.join();
+  if (rc) {  // requests still queued or in flight go back to the server (tmce_server_poll collects them)
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (size_t idx : run.done) {
+      sv->reqs.erase(run.req[idx]->id);
+      delete run.req[idx];
+      run.req[idx] = nullptr;
+    }
+    run.done.clear();
+    for (Req *r : run.req)
+      if (r) {
+        r->run = nullptr;
+        r->claimed = false;
+      }
+    return rc;
+  }
+  if (elapsed_s) *elapsed_s = t_last - t0s;
+  return 0;
 }
 
 int tmce_server_get_stats(tmce_server *sv, tmce_server_stats *st) {
@@ -376,6 +515,7 @@ int tmce_server_get_stats(tmce_server *sv, tmce_server_stats *st) {
   st->enc_bytes_per_op = sv->per_op_bytes[0];
   st->dec_bytes_per_op = sv->per_op_bytes[1];
   st->zero_copy_requests = sv->zero_copy;
+  st->max_batch_keys = sv->max_batch_keys;
   return 0;
 }
 

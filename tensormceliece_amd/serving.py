@@ -6,9 +6,6 @@ The workload follows the serving configuration of the reference's scope: request
 inter-arrival times at a given request rate.  Decapsulation requests take ciphertexts from a pool
 made by the engine beforehand (valid ciphertexts of the chosen key).
 """
-import threading
-import time
-
 import numpy as np
 
 import tensormceliece_amd as tm
@@ -73,49 +70,29 @@ def to_slab(reqs, params):
 
 def run_poisson(server, wl, reqs, rate, keep_outputs=False, slab=None):
     """Submits `reqs` at Poisson arrival times (`rate` requests/s; rate <= 0: all at once) and
-    collects them.  Returns dict with latencies (s), throughput (ops/s) and per-request outputs.
-    With `slab` (the output HostBuffer of to_slab) the outputs are allocated in it: with slab
-    inputs too, the requests are zero-copy."""
+    collects them, through the library's native load driver (tmce_server_run_load: a native thread
+    submits at the arrival times and completions are collected as they happen, no Python thread in
+    the loop).  Returns dict with latencies (s), throughput (ops/s) and per-request outputs
+    (ct, k, status).  With `slab` (the output HostBuffer of to_slab) the outputs are allocated in it:
+    with slab inputs too, the requests are zero-copy."""
     rng = np.random.default_rng(3)
     if slab is not None:
         slab.reset()
     gaps = rng.exponential(1.0 / rate, len(reqs)) if rate > 0 else np.zeros(len(reqs))
     arrivals = np.cumsum(gaps)
-    ids = [None] * len(reqs)
-    outputs = [None] * len(reqs)
-    err = []
-    t0 = time.perf_counter()
-
-    def submitter():
-        try:
-            for i, (d, k, inp, _) in enumerate(reqs):
-                delay = arrivals[i] - (time.perf_counter() - t0)
-                if delay > 0:
-                    time.sleep(delay)
-                key = wl.pk[k] if d == "enc" else wl.sk[k]
-                ids[i], outs = server.submit(d, key, inp, slab=slab)
-                if keep_outputs:
-                    outputs[i] = outs
-        except Exception as e:  # surfaced by the caller
-            err.append(e)
-
-    th = threading.Thread(target=submitter)
-    th.start()
-    lat = {}
-    while len(lat) < len(reqs):
-        got, lats = server.poll()
-        for i, l in zip(got, lats):
-            lat[i] = l
-        if err:
-            break
-        if not got:
-            time.sleep(0.0002)
-    th.join()
-    if err:
-        raise err[0]
-    t1 = time.perf_counter()
-    lat_s = np.array([lat[i] for i in ids])
+    ctb = server.params["ct"]
+    new = (lambda shape, dt: slab.array(shape, dt)) if slab is not None else (lambda shape, dt: np.zeros(shape, dt))
+    items, outputs = [], []
+    for d, k, inp, _ in reqs:
+        n = inp.shape[0]
+        ct = new((n, ctb), np.uint8) if d == "enc" else None
+        kk = new((n, 32), np.uint8)
+        st = new(n, np.int32)
+        key = wl.pk[k] if d == "enc" else wl.sk[k]
+        items.append((d, key, inp, ct, kk, st))
+        outputs.append((ct, kk, st))
+    lat_s, elapsed = server.run_load(items, arrivals)
     ops = int(sum(r[2].shape[0] for r in reqs))
-    return {"latency_s": lat_s, "elapsed_s": t1 - t0, "ops": ops, "ops_per_s": ops / (t1 - t0),
+    return {"latency_s": lat_s, "elapsed_s": elapsed, "ops": ops, "ops_per_s": ops / elapsed,
             "p50_ms": float(np.percentile(lat_s, 50) * 1e3), "p99_ms": float(np.percentile(lat_s, 99) * 1e3),
-            "outputs": outputs}
+            "outputs": outputs if keep_outputs else [None] * len(reqs)}

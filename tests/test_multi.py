@@ -98,6 +98,44 @@ def test_uneven_single_key_graph_replays(tm, orc, s, total):
         assert np.array_equal(k2, k_ref), f"dec run {run}"
 
 
+def test_c4_full_size_6960119(tm, orc):
+    """The uneven-batch configuration at full size: mceliece6960119, sub-batches log-uniform in
+    [1, 8192] totalling 65536 ops, through graph-captured pipelines (the second and third runs replay
+    the graph with new layouts).  Every op: the pipeline's ct and k equal one flat batched call's,
+    decapsulation returns the encapsulated key with status 0, and a sample equals the oracle."""
+    s, total = "6960119", 65536
+    pk, sk = keys(orc, s)
+    p = orc.params(s)
+    PK, SK = tm.PublicKey(s, pk), tm.SecretKey(s, sk)
+    rng = np.random.default_rng(41)
+    seeds = rng.integers(0, 256, (total, 32), dtype=np.uint8)
+    ct_ref, k_ref, st_ref = PK.enc(seeds)
+    assert not st_ref.any()
+    sample = rng.choice(total, 12, replace=False)
+    o_ct, o_k, _ = orc.enc_batch(s, pk, seeds[sample])
+    assert np.array_equal(ct_ref[sample], o_ct) and np.array_equal(k_ref[sample], o_k)
+    o_k2, o_st2 = orc.dec_batch(s, sk, ct_ref[sample[:6]])
+    assert np.array_equal(o_k2, k_ref[sample[:6]]) and not o_st2.any()
+    enc_pipe = tm.Pipeline(s, "enc", total, use_graph=True)
+    dec_pipe = tm.Pipeline(s, "dec", total, use_graph=True)
+    for run in range(3):
+        perm = rng.permutation(total)
+        sizes = log_uniform_sizes(rng, 1, 8192, total)
+        assert sum(sizes) == total and max(sizes) <= 8192
+        plan, pos = [], 0
+        for m in sizes:
+            plan.append((PK, perm[pos:pos + m]))
+            pos += m
+        print(f"pipeline {s} full size run {run}: {len(sizes)} sub-batches", flush=True)
+        ct, k, st = run_pipeline(tm, enc_pipe, plan, seeds, 32, p.synd_bytes)
+        assert not st.any()
+        assert np.array_equal(ct, ct_ref) and np.array_equal(k, k_ref), f"enc run {run}"
+        dplan = [(SK, idx) for _, idx in plan[::-1]]
+        _, k2, st2 = run_pipeline(tm, dec_pipe, dplan, ct, p.synd_bytes, 0)
+        assert not st2.any()
+        assert np.array_equal(k2, k), f"dec run {run}"
+
+
 def test_mixed_keys_enc_dec(tm, orc):
     """Three keys interleaved in one run (each key's group padded separately): results equal the
     per-key plain calls.  Decapsulation adds tampered and random ciphertexts (implicit rejection)."""

@@ -129,3 +129,119 @@ def test_zero_copy_requests(tm, orc, s):
     assert np.array_equal(outs[1], zc["outputs"][0][1])
     inb.free()
     outb.free()
+
+
+def test_wait_and_poll_on_two_threads(tm, orc):
+    """One thread waits for every other request while another polls: each request is collected
+    exactly once, by the waiter or by the poller, never both (a waited-for request is left alone
+    by poll), and every result is exact."""
+    import threading
+    s = "348864"
+    pk, _ = keypairs(orc, s, 1)[0]
+    PK = tm.PublicKey(s, pk)
+    server = tm.Server(s, slot_capacity=2048, slots_per_direction=2)
+    rng = np.random.default_rng(31)
+    try:
+        subs = []
+        for i in range(120):
+            sd = rng.integers(0, 256, (int(rng.integers(1, 200)), 32), dtype=np.uint8)
+            rid, outs = server.submit("enc", PK, sd)
+            subs.append((rid, sd, outs))
+        waited, polled, errors = set(), set(), []
+
+        def waiter():
+            try:
+                for rid, _, _ in subs[::2]:
+                    server.wait(rid)
+                    waited.add(rid)
+            except Exception as e:  # surfaced below
+                errors.append(e)
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        want_polled = {rid for rid, _, _ in subs[1::2]}
+        while not want_polled <= polled and not errors:
+            ids, _ = server.poll()
+            polled.update(ids)
+        th.join()
+    finally:
+        server.close()
+    assert not errors
+    assert waited == {rid for rid, _, _ in subs[::2]} and not (waited & polled)
+    assert polled == want_polled
+    for _, sd, (ct, k, st) in subs:
+        ct_a, k_a, st_a = PK.enc(sd)
+        assert np.array_equal(ct, ct_a) and np.array_equal(k, k_a) and not st.any()
+
+
+def _gpu_keypairs(tm, s, n, state):
+    from oracle import oracle as orc
+    seeds, _ = orc.xorshift_seeds(n, state)
+    pks, sks = tm.keypair_batch(s, seeds)
+    return list(zip(pks, sks))
+
+
+def test_serving_8192128_16_keys_poisson(tm, orc):
+    """The serving configuration at its real shape: mceliece8192128, 16 device-resident keypairs
+    (the multi-batch plan's 16 key groups can fill), Poisson arrivals, request sizes log-uniform in
+    [1, 4096], 50/50 enc/dec, keys chosen uniformly.  Every request equals the same request alone,
+    decapsulations return the pool's session keys, and enc and dec samples equal the oracle."""
+    from tensormceliece_amd import serving
+    s = "8192128"
+    kps = _gpu_keypairs(tm, s, 16, 0xC5)  # batched keypairs equal the host generator (test_keygen)
+    wl = serving.Workload(s, kps, pool_per_key=4096)
+    reqs = serving.make_requests(wl, 160, seed=51)
+    assert max(r[2].shape[0] for r in reqs) > 2048 and len({r[1] for r in reqs}) == 16
+    server = tm.Server(s, slot_capacity=32768, slots_per_direction=2, hbm_budget=8 << 30)
+    try:
+        print("serving 8192128, 16 keys, Poisson", flush=True)
+        res = serving.run_poisson(server, wl, reqs, 800.0, keep_outputs=True)
+        st = server.stats()
+    finally:
+        server.close()
+    assert st["requests"] == len(reqs) and st["batches"] < len(reqs)
+    assert st["peak_in_flight_bytes"] <= 8 << 30
+    check_alone(tm, wl, reqs, res["outputs"])
+    for (d, k, inp, off), (_, key, stt) in zip(reqs, res["outputs"]):
+        if d == "dec":
+            assert not stt.any() and np.array_equal(key, wl.pool_k[k][off:off + inp.shape[0]])
+    ie = next(i for i, r in enumerate(reqs) if r[0] == "enc")
+    ct_o, k_o, _ = orc.enc_batch(s, kps[reqs[ie][1]][0], reqs[ie][2][:4])
+    assert np.array_equal(res["outputs"][ie][0][:4], ct_o) and np.array_equal(res["outputs"][ie][1][:4], k_o)
+    idn = next(i for i, r in enumerate(reqs) if r[0] == "dec")
+    k_o, _ = orc.dec_batch(s, kps[reqs[idn][1]][1], reqs[idn][2][:4])
+    assert np.array_equal(res["outputs"][idn][1][:4], k_o)
+
+
+def test_serving_17_keys_splits_batches(tm, orc):
+    """17 resident keys with every request submitted at once: a batch holds at most 16 key groups,
+    so a request under a 17th key ends the batch being formed and goes to the next one; results
+    stay exact."""
+    from tensormceliece_amd import serving
+    s = "8192128"
+    kps = _gpu_keypairs(tm, s, 17, 0x17)
+    wl = serving.Workload(s, kps, pool_per_key=512)
+    rng = np.random.default_rng(52)
+    # one large request per direction first: while they run (one slot per direction), the small
+    # requests behind them queue up, so the next batch of each direction meets all 17 keys
+    reqs = [("enc", 0, rng.integers(0, 256, (30000, 32), dtype=np.uint8), None),
+            ("dec", 0, rng.integers(0, 256, (30000, tm.PARAMS[s]["ct"]), dtype=np.uint8), None)]
+    for i in range(68):  # keys 0..16 round-robin, so every batch of 16 groups meets a 17th key
+        k = i % 17
+        n = int(rng.integers(1, 64))
+        if i % 2 == 0:
+            reqs.append(("enc", k, rng.integers(0, 256, (n, 32), dtype=np.uint8), None))
+        else:
+            off = int(rng.integers(0, 512 - n))
+            reqs.append(("dec", k, wl.pool_ct[k][off:off + n], off))
+    server = tm.Server(s, slot_capacity=32768, slots_per_direction=1)
+    try:
+        print("serving 8192128, 17 keys, all at once", flush=True)
+        res = serving.run_poisson(server, wl, reqs, 0.0, keep_outputs=True)
+        st = server.stats()
+    finally:
+        server.close()
+    # 35 requests per direction under 17 keys: one slot per direction needs at least two batches each
+    assert st["batches"] >= 4 and st["requests"] == len(reqs)
+    assert st["max_batch_keys"] == 16  # a batch filled all 16 key groups and the 17th key waited
+    check_alone(tm, wl, reqs, res["outputs"])
