@@ -494,6 +494,20 @@ void tmce_server_free(tmce_server *sv);
 int tmce_server_run_load(tmce_server *sv, const tmce_request *rqs, const double *arrival_s, size_t n,
                          double *latency_s, double *elapsed_s);
 
+/* Decapsulation that allocates nothing (no counterpart in the reference: it is for callers that
+   cannot allocate device memory -- code under HIP graph capture, tight serving loops, harnesses
+   that forbid hipMalloc).  Same arguments, semantics and results as tmce_dec_batch_device (engine
+   form); every buffer beyond the inputs and outputs lives in a static device array of
+   TMCE_NOALLOC_SCRATCH_MB MiB that each GPU holds until the process ends, so n is limited to
+   TMCE_NOALLOC_MAX_OPS operations for every set and variant (larger n: TMCE_EINVAL).  All calls
+   on one GPU share that array: calls that overlap on one GPU (several streams or host threads)
+   race and must be ordered by the caller.  Calls no hipMalloc, hipFree or other allocating
+   routine; the key handle and the library's set-up (tmce_init) allocate beforehand. */
+#define TMCE_NOALLOC_SCRATCH_MB 40
+#define TMCE_NOALLOC_MAX_OPS 16384
+int tmce_dec_batch_device_noalloc(tmce_key *sk, unsigned char *d_k, int *d_status, const unsigned char *d_ct,
+                                  size_t n, void *stream);
+
 /* select the pipeline used by subsequent batched calls: 0 = engine form (default: fused,
    persistent sampler, row-parallel decoder), 1 = plain one-launch-per-stage form (the baseline
    mirroring the reference pipelines, mceliece-gpu-params/main.cu:213-262,326-370).  Both give

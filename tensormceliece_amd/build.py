@@ -9,7 +9,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtmce.so")
 SOURCES = ["kem_enc.hip", "kem_dec.hip", "kem_plain.hip", "kem_synd.hip", "kem_keygen.hip", "host_math.cpp", "api.cpp",
-           "keygen.cpp", "pipeline.cpp", "server.cpp"]
+           "keygen.cpp", "pipeline.cpp", "server.cpp", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",

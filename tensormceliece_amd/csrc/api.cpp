@@ -67,6 +67,7 @@ int engine_init(int device) {
       if (int r = upload(&e->st[s].pthi, hi)) return r;
     }
   }
+  if (int r = noalloc_init(device)) return r;
   g_engine = e.release();
   return 0;
 }
@@ -194,6 +195,84 @@ int enc_device(tmce_key *k, uint8_t *d_ct, uint8_t *d_k, int *d_status, const ui
   return scratch_done(e->use_enc, s);
 }
 
+}  // namespace
+
+namespace tmce {
+
+// Device scratch of one decapsulation batch of n ops of `set` (pc: the variants' syndrome copy),
+// laid out back to back at 256-byte boundaries from a base address: the decoder's work counters,
+// the syndrome planes, the message rows, the pc syndrome copy and the MFMA front end's repacked
+// ciphertext words.  Returns the total bytes; with `base` non-null fills `x`.
+size_t dec_scratch_layout(int set, bool pc, bool front_mfma, size_t n, uint8_t *base, DecScratch *x) {
+  const SetInfo &si = set_info(set);
+  const size_t TT = ((2 * (size_t)si.t + 31) / 32) * 32;
+  const size_t RW = (size_t)si.m * TT / 32;
+  auto up = [](size_t a) { return (a + 255) & ~(size_t)255; };
+  size_t off = 0;
+  const size_t q = off;
+  off = up(off + 1024);
+  const size_t sS = off;
+  off = up(off + n * RW * sizeof(uint32_t));
+  const size_t sm = off;
+  off = up(off + n * (size_t)si.msg_words * sizeof(uint32_t));
+  const size_t sc = off;
+  if (pc) off = up(off + n * (size_t)si.synd_bytes);
+  const size_t xb = front_mfma ? gf2_scratch_bytes(set, GF2_FRONT, n) : 0;
+  const size_t sx = off;
+  off = up(off + xb);
+  if (base && x) {
+    x->queue = reinterpret_cast<unsigned *>(base + q);
+    x->S = reinterpret_cast<uint32_t *>(base + sS);
+    x->msg = reinterpret_cast<uint32_t *>(base + sm);
+    x->ct_synd = pc ? base + sc : nullptr;
+    x->xs = xb ? reinterpret_cast<uint32_t *>(base + sx) : nullptr;
+  }
+  return off;
+}
+
+// Launches one decapsulation batch on stream s with the given scratch (engine form when form is
+// 0, the plain per-stage form when 1: its extra scratch in `px`).  No allocation.
+int dec_launch(Engine *e, tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, size_t n, hipStream_t s,
+               const DecScratch &x, int form, const PlainDecScratch *px) {
+  const SetInfo &si = set_info(k->set);
+  DecArgs a{};
+  a.ct = d_ct;
+  a.pc = k->pc ? 1 : 0;
+  a.ct_synd = x.ct_synd;
+  a.F = k->F;
+  a.Fsw = k->Fsw;
+  a.front_mfma = e->front_mfma[k->set];
+  a.front_cfg = e->front_cfg[k->set];
+  a.xscratch = x.xs;
+  a.Fcol = k->Fcol;
+  a.alpha = k->alpha;
+  a.la = k->la;
+  a.lu = k->lu;
+  a.ainv = k->ainv;
+  a.supp = k->supp;
+  a.rej = k->rej;
+  const int w = si.m - 12;
+  const SetTables &tb = e->st[k->set];
+  a.ft = FieldTables{e->lg[w], e->ex[w], tb.twl, tb.plo, tb.phi, tb.ptlo, tb.pthi};
+  a.key = d_k;
+  a.status = d_status;
+  a.S = x.S;
+  a.msg = x.msg;
+  a.queue = x.queue;
+  a.nops = (int)n;
+  a.decode_blocks = e->decode_blocks[k->set];
+  if (form == 1) {
+    HIPCHK(launch_plain_dec_set(k->set, a, *px, s));
+    return 0;
+  }
+  HIPCHK(launch_dec_set(k->set, a, s));
+  return 0;
+}
+
+}  // namespace tmce
+
+namespace {
+
 int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, size_t n, hipStream_t s) {
   Engine *e = engine();
   if (!e) return TMCE_ENODEV;
@@ -212,49 +291,27 @@ int dec_device(tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, si
     if (int r = e->ct_synd.ensure(n * si.synd_bytes)) return r;
   if (int r = e->msg_dec.ensure(n * MSGW * sizeof(uint32_t))) return r;
   if (int r = e->dqueue.ensure(1024)) return r;
-  DecArgs a{};
-  a.ct = d_ct;
-  a.pc = k->pc ? 1 : 0;
-  a.ct_synd = k->pc ? e->ct_synd.as<uint8_t>() : nullptr;
-  a.F = k->F;
-  a.Fsw = k->Fsw;
-  a.front_mfma = e->front_mfma[k->set];
-  a.front_cfg = e->front_cfg[k->set];
-  a.xscratch = nullptr;
-  if (a.front_mfma) {
+  DecScratch x{};
+  x.S = e->S.as<uint32_t>();
+  x.msg = e->msg_dec.as<uint32_t>();
+  x.queue = e->dqueue.as<unsigned>();
+  x.ct_synd = k->pc ? e->ct_synd.as<uint8_t>() : nullptr;
+  if (e->front_mfma[k->set]) {
     const size_t xb = gf2_scratch_bytes(k->set, GF2_FRONT, n);
     if (xb) {
       if (int r = e->xs_dec.ensure(xb)) return r;
-      a.xscratch = e->xs_dec.as<uint32_t>();
+      x.xs = e->xs_dec.as<uint32_t>();
     }
   }
-  a.Fcol = k->Fcol;
-  a.alpha = k->alpha;
-  a.la = k->la;
-  a.lu = k->lu;
-  a.ainv = k->ainv;
-  a.supp = k->supp;
-  a.rej = k->rej;
-  const int w = si.m - 12;
-  const SetTables &tb = e->st[k->set];
-  a.ft = FieldTables{e->lg[w], e->ex[w], tb.twl, tb.plo, tb.phi, tb.ptlo, tb.pthi};
-  a.key = d_k;
-  a.status = d_status;
-  a.S = e->S.as<uint32_t>();
-  a.msg = e->msg_dec.as<uint32_t>();
-  a.queue = e->dqueue.as<unsigned>();
-  a.nops = (int)n;
-  a.decode_blocks = e->decode_blocks[k->set];
+  PlainDecScratch px{};
   if (e->form == 1) {
     const int NEW2 = 2 * ((si.n + 63) / 64);
     if (int r = e->p_loc.ensure(n * (si.t + 1) * 2)) return r;
     if (int r = e->p_erow_dec.ensure(n * NEW2 * 4)) return r;
     if (int r = e->p_weight.ensure(n * 4)) return r;
-    PlainDecScratch x{e->p_loc.as<uint16_t>(), e->p_erow_dec.as<uint32_t>(), e->p_weight.as<int>()};
-    HIPCHK(launch_plain_dec_set(k->set, a, x, s));
-    return scratch_done(e->use_dec, s);
+    px = PlainDecScratch{e->p_loc.as<uint16_t>(), e->p_erow_dec.as<uint32_t>(), e->p_weight.as<int>()};
   }
-  HIPCHK(launch_dec_set(k->set, a, s));
+  if (int r = dec_launch(e, k, d_k, d_status, d_ct, n, s, x, e->form, &px)) return r;
   return scratch_done(e->use_dec, s);
 }
 

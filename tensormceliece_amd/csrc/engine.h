@@ -164,6 +164,20 @@ int engine_init(int device);
 // the engine (initialised on device 0 on first use), or null if there is no usable device
 Engine *engine();
 
+// device scratch pointers of one decapsulation batch
+struct DecScratch {
+  uint32_t *S = nullptr;         // syndrome planes [n][RW]
+  uint32_t *msg = nullptr;       // session-key messages [n][MSG_WORDS]
+  unsigned *queue = nullptr;     // decoder work counters, 1024 bytes
+  uint8_t *ct_synd = nullptr;    // pc variants: syndrome part of each ciphertext [n][SYND_BYTES]
+  uint32_t *xs = nullptr;        // MFMA front end: repacked ciphertext words (null if not needed)
+};
+size_t dec_scratch_layout(int set, bool pc, bool front_mfma, size_t n, uint8_t *base, DecScratch *x);
+int dec_launch(Engine *e, tmce_key *k, uint8_t *d_k, int *d_status, const uint8_t *d_ct, size_t n, hipStream_t s,
+               const DecScratch &x, int form, const PlainDecScratch *px);
+// records the no-allocation entry point's scratch address on the current device (engine set-up)
+int noalloc_init(int device);
+
 inline bool valid_set(int set) { return set >= 0 && set < NUM_SETS; }
 // ciphertext bytes of a key's variant
 inline size_t key_ct_bytes(const tmce_key *k) { return (size_t)set_info(k->set).synd_bytes + (k->pc ? 32 : 0); }
