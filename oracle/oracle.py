@@ -66,6 +66,11 @@ def lib():
             ("oracle_kat_shake_checksum", [I, I, ctypes.c_char_p], None),
             ("oracle_rng_seeded", [P], None),
             ("oracle_gen_e", [I, P, P], None),
+            ("oracle_fast_key", [I, P, P], P),
+            ("oracle_fast_key_free", [P], None),
+            ("oracle_fast_enc_batch", [P, P, P, P, P, LL, I], I),
+            ("oracle_fast_dec_batch", [P, P, P, P, LL, I], I),
+            ("oracle_fast_eval_all", [I, P, I, P], None),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -116,6 +121,55 @@ def dec_batch(set_name, sk, ct, nthreads=0):
     sk = np.ascontiguousarray(sk, np.uint8)
     lib().oracle_dec_batch(SET_IDS[set_name], _p(k), _p(ret), _p(ct), _p(sk), n, nthreads)
     return k, ret
+
+
+class FastKey:
+    """The word-level CPU port (oracle_fast_*): per-key tables built once, then batches of seeded
+    encapsulations / decapsulations on `nthreads` threads (0 = every core).  Same results as
+    enc_batch / dec_batch."""
+
+    def __init__(self, set_name, pk=None, sk=None):
+        self.set_name = set_name
+        self._pk = None if pk is None else np.ascontiguousarray(pk, np.uint8)
+        self._sk = None if sk is None else np.ascontiguousarray(sk, np.uint8)
+        self._h = lib().oracle_fast_key(SET_IDS[set_name], _p(self._pk) if self._pk is not None else None,
+                                        _p(self._sk) if self._sk is not None else None)
+
+    def enc(self, seeds, nthreads=0):
+        seeds = np.ascontiguousarray(seeds, np.uint8).reshape(-1, 32)
+        n = seeds.shape[0]
+        ct = np.zeros((n, ct_bytes(self.set_name)), np.uint8)
+        k = np.zeros((n, 32), np.uint8)
+        ret = np.zeros(n, np.int32)
+        lib().oracle_fast_enc_batch(self._h, _p(ct), _p(k), _p(ret), _p(seeds), n, nthreads)
+        return ct, k, ret
+
+    def dec(self, ct, nthreads=0):
+        ct = np.ascontiguousarray(ct, np.uint8).reshape(-1, ct_bytes(self.set_name))
+        n = ct.shape[0]
+        k = np.zeros((n, 32), np.uint8)
+        ret = np.zeros(n, np.int32)
+        lib().oracle_fast_dec_batch(self._h, _p(k), _p(ret), _p(ct), n, nthreads)
+        return k, ret
+
+    def close(self):
+        if self._h:
+            lib().oracle_fast_key_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fast_eval_all(m, f):
+    """f (coefficients, degree < 128) evaluated at every element of GF(2^m) by the port's FFT."""
+    f = np.ascontiguousarray(f, np.uint16)
+    vals = np.zeros(1 << m, np.uint16)
+    lib().oracle_fast_eval_all(m, _p(f), len(f) - 1, _p(vals))
+    return vals
 
 
 def syndrome_batch(set_name, pk, e, nthreads=0):

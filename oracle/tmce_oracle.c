@@ -969,6 +969,378 @@ int oracle_dec_batch(int set, uint8_t *key, int *ret, const uint8_t *ct, const u
   return run_jobs(&j, n, nthreads);
 }
 
+/* ------------------------------------------------------------------ word-level CPU port
+   The CPU baseline that is timed beside the GPU: the same results as oracle_enc_seeded and
+   oracle_dec (checked against them in tests/test_oracle_fast.py), computed the way a CPU
+   implementation would, with the per-key work done once per batch:
+   - encapsulation: seeded gen_e with a bitmap collision test; the syndrome as the XOR of the t
+     columns of [I | T] at the support of e (64-bit words; T's columns transposed once per key);
+   - decapsulation: the 2t syndromes of r as the XOR of per-position columns u_i alpha_i^j
+     (u_i = 1/g(alpha_i)^2, 16-bit field elements, built once per key; LM:crypto_kem/<set>/vec/
+     decrypt.c computes the same s by its transposed FFT); Berlekamp-Massey with division
+     bounded by the current length; the error locator evaluated at every field element by the
+     Gao-Mateer additive FFT (the method of LM:crypto_kem/<set>/vec/fft.c) and read at the
+     support; the re-encryption check by the same columns.
+   Threads: one per core by default, each on a contiguous slice. */
+typedef struct {
+  int set;
+  /* encapsulation */
+  const uint8_t *pk;
+  uint64_t *tcol; /* column j of T: cw words */
+  int cw, pk_bad;
+  /* decapsulation */
+  const uint8_t *sk;
+  dec_key K;
+  uint64_t *scol; /* position i: the 2t syndromes of the unit vector e_i, 16 bits each, sw words */
+  int sw;
+  uint16_t *pow_t; /* x^t for every field element (t = D only) */
+} fast_key;
+
+void *oracle_fast_key(int set, const uint8_t *pk, const uint8_t *sk) {
+  const orc_params *p = &PARAMS[set & 7];
+  const gf_tab *T = tab(p->m);
+  fast_key *F = (fast_key *)calloc(1, sizeof(fast_key));
+  F->set = set;
+  if (pk) {
+    F->pk = pk;
+    F->cw = (p->mt + 63) / 64;
+    F->tcol = (uint64_t *)calloc((size_t)p->k * F->cw, 8);
+    for (int r = 0; r < p->mt; r++) {
+      const uint8_t *row = pk + (size_t)r * p->pk_row_bytes;
+      for (int j = 0; j < p->k; j++)
+        if ((row[j >> 3] >> (j & 7)) & 1) F->tcol[(size_t)j * F->cw + (r >> 6)] |= 1ULL << (r & 63);
+    }
+    if ((set & 7) == ORC_6960119) { /* LM:crypto_kem/6960119/vec/kem_enc.c check_pk_padding */
+      uint8_t b = 0;
+      for (int i = 0; i < p->mt; i++) b |= pk[(size_t)i * p->pk_row_bytes + p->pk_row_bytes - 1];
+      F->pk_bad = (b >> (p->k % 8)) != 0;
+    }
+  }
+  if (sk) {
+    F->sk = sk;
+    dec_key_load(set, sk, &F->K);
+    const int tt = 2 * p->t;
+    F->sw = (tt + 3) / 4; /* 2t = 238 for 6960119: the last word holds two */
+    F->scol = (uint64_t *)calloc((size_t)p->n * F->sw, 8);
+    for (int i = 0; i < p->n; i++) {
+      uint16_t *c = (uint16_t *)(F->scol + (size_t)i * F->sw);
+      uint16_t v = F->K.u[i];
+      for (int j = 0; j < tt; j++) {
+        c[j] = v;
+        v = gmul(T, v, F->K.alpha[i]);
+      }
+    }
+    if (p->t == 64 || p->t == 128) {
+      F->pow_t = (uint16_t *)malloc(sizeof(uint16_t) << p->m);
+      for (int x = 0; x < (1 << p->m); x++) {
+        uint16_t v = (uint16_t)x;
+        for (int k = 1; k < p->t; k <<= 1) v = gmul(T, v, v);
+        F->pow_t[x] = v;
+      }
+    }
+  }
+  return F;
+}
+
+void oracle_fast_key_free(void *h) {
+  fast_key *F = (fast_key *)h;
+  if (!F) return;
+  free(F->tcol);
+  if (F->sk) dec_key_free(&F->K);
+  free(F->scol);
+  free(F->pow_t);
+  free(F);
+}
+
+/* seeded gen_e (the chain of oracle_rng_seeded) returning the sorted-free index list */
+static void fast_gen_e(int set, const uint8_t seed[32], uint16_t *ind) {
+  const orc_params *p = &PARAMS[set & 7];
+  const int m = p->m, n = p->n, t = p->t;
+  const int ndraw = ((set & 7) == ORC_8192128) ? t : 2 * t;
+  uint8_t st[33], buf[512];
+  uint64_t bm[128];
+  st[0] = 0x40;
+  memcpy(st + 1, seed, 32);
+  for (;;) {
+    oracle_shake256(buf, 2 * ndraw, st, 33);
+    memcpy(st + 1, buf, 32);
+    int count = 0;
+    for (int i = 0; i < ndraw && count < t; i++) {
+      uint16_t v = load_gf(buf + 2 * i, m);
+      if (v < n) ind[count++] = v;
+    }
+    if (count < t) continue;
+    memset(bm, 0, sizeof(uint64_t) * (size_t)((n + 63) / 64));
+    int coll = 0;
+    for (int i = 0; i < t; i++) {
+      const uint64_t b = 1ULL << (ind[i] & 63);
+      coll |= (bm[ind[i] >> 6] & b) != 0;
+      bm[ind[i] >> 6] |= b;
+    }
+    if (!coll) return;
+  }
+}
+
+static int fast_enc_one(const fast_key *F, uint8_t *ct, uint8_t *key, const uint8_t seed[32]) {
+  const int set = F->set;
+  const orc_params *p = &PARAMS[set & 7];
+  const int n = p->n, clen = oracle_ct_bytes(set);
+  if (F->pk_bad) {
+    memset(ct, 0, (size_t)clen);
+    memset(key, 0, 32);
+    return -1;
+  }
+  uint16_t ind[128];
+  fast_gen_e(set, seed, ind);
+  uint8_t pre[1 + 1024 + 240];
+  uint8_t *e = pre + 1;
+  memset(pre, 0, (size_t)(1 + n / 8));
+  uint64_t c[32];
+  memset(c, 0, sizeof c);
+  for (int i = 0; i < p->t; i++) {
+    const int v = ind[i];
+    e[v >> 3] |= (uint8_t)(1 << (v & 7));
+    if (v < p->mt) {
+      c[v >> 6] ^= 1ULL << (v & 63);
+    } else {
+      const uint64_t *col = F->tcol + (size_t)(v - p->mt) * F->cw;
+      for (int w = 0; w < F->cw; w++) c[w] ^= col[w];
+    }
+  }
+  memcpy(ct, c, (size_t)p->synd_bytes); /* little-endian words = libmceliece's bit order */
+  if (set & ORC_PC) {
+    pre[0] = 2;
+    oracle_shake256(ct + p->synd_bytes, 32, pre, 1 + n / 8);
+  }
+  pre[0] = 1;
+  memcpy(pre + 1 + n / 8, ct, (size_t)clen);
+  oracle_shake256(key, 32, pre, 1 + n / 8 + clen);
+  return 0;
+}
+
+/* Gao-Mateer additive FFT: f[0..L) (L a power of two, destroyed) evaluated at the 2^k points
+   sum_j bit_j(i) b[j], out[i] for i < 2^k */
+static void fast_afft(const gf_tab *T, int m, uint16_t *f, int L, const uint16_t *b, int k, uint16_t *out) {
+  const int NP = 1 << k;
+  if (L == 1) {
+    for (int i = 0; i < NP; i++) out[i] = f[0];
+    return;
+  }
+  /* g(x) = f(b[k-1] x) */
+  const uint16_t bk = b[k - 1];
+  uint16_t pw = 1;
+  for (int i = 1; i < L; i++) {
+    pw = gmul(T, pw, bk);
+    f[i] = gmul(T, f[i], pw);
+  }
+  /* Taylor expansion at x^2 + x: blocks of 4q coefficients [a0 a1 a2 a3] -> [a0, a1+a2+a3, a2+a3, a3] */
+  for (int q = L / 4; q >= 1; q >>= 1)
+    for (int bs = 0; bs < L; bs += 4 * q)
+      for (int i = 0; i < q; i++) {
+        f[bs + 2 * q + i] ^= f[bs + 3 * q + i];
+        f[bs + q + i] ^= f[bs + 2 * q + i];
+      }
+  uint16_t g0[128], g1[128], gam[13], del[13], pre[13];
+  for (int i = 0; i < L / 2; i++) {
+    g0[i] = f[2 * i];
+    g1[i] = f[2 * i + 1];
+  }
+  const uint16_t binv = ginv(T, m, bk);
+  for (int j = 0; j < k - 1; j++) {
+    gam[j] = gmul(T, b[j], binv);
+    del[j] = gmul(T, gam[j], gam[j]) ^ gam[j];
+    pre[j] = (j ? pre[j - 1] : 0) ^ gam[j];
+  }
+  const int half = NP / 2;
+  fast_afft(T, m, g0, L / 2, del, k - 1, out);
+  fast_afft(T, m, g1, L / 2, del, k - 1, out + half);
+  uint16_t G = 0;
+  for (int i = 0; i < half; i++) {
+    if (i) G ^= pre[__builtin_ctz((unsigned)i)];
+    const uint16_t v = out[half + i];
+    const uint16_t x = out[i] ^ gmul(T, G, v);
+    out[i] = x;
+    out[half + i] = x ^ v;
+  }
+}
+
+/* BM with division (as oracle_bm); deg C <= L (Massey), so the discrepancy sums i <= min(L, N, t),
+   and after step N deg C <= N + 1.  B is kept as logs (ZL for a zero coefficient) and the syndromes
+   as logs, so each product is one table read: xe[la + lb] with xe zero beyond 2q.
+   locator[i] = C[t-i] */
+static void fast_bm(const gf_tab *T, int t, int m, const uint16_t *s, uint16_t *locator) {
+  const int q = (1 << m) - 1, ZL = 2 * q;
+  static __thread uint16_t xe[2][4 * 8191 + 2];
+  static __thread int xe_ready[2];
+  uint16_t *ex = xe[m - 12];
+  if (!xe_ready[m - 12]) {
+    for (int i = 0; i <= 4 * q + 1; i++) ex[i] = i < 2 * q ? T->exp[i % q] : 0;
+    xe_ready[m - 12] = 1;
+  }
+  uint16_t C[130], Bl[130], Tm[130], sl[256];
+  for (int j = 0; j < 2 * t; j++) sl[j] = s[j] ? T->log[s[j]] : (uint16_t)ZL;
+  memset(C, 0, sizeof C);
+  for (int i = 0; i <= t; i++) Bl[i] = (uint16_t)ZL;
+  C[0] = 1;
+  Bl[1] = 0;
+  int lb = 0; /* log b, b = 1 */
+  int L = 0;
+  for (int N = 0; N < 2 * t; N++) {
+    uint16_t d = 0;
+    int top = L < N ? L : N;
+    if (top > t) top = t;
+    for (int i = 0; i <= top; i++)
+      if (C[i]) d ^= ex[T->log[C[i]] + sl[N - i]];
+    const int upd_len = (d != 0) && (N >= 2 * L);
+    const int hi = N + 1 < t ? N + 1 : t;
+    if (upd_len)
+      for (int i = 0; i <= hi; i++) Tm[i] = C[i] ? T->log[C[i]] : (uint16_t)ZL;
+    if (d) {
+      const int ld = T->log[d];
+      const int lf = ld >= lb ? ld - lb : ld + q - lb;
+      for (int i = 0; i <= hi; i++) C[i] ^= ex[lf + Bl[i]];
+      if (upd_len) {
+        L = N + 1 - L;
+        for (int i = 0; i <= hi; i++) Bl[i] = Tm[i];
+        for (int i = hi + 1; i <= t; i++) Bl[i] = (uint16_t)ZL;
+        lb = ld;
+      }
+    }
+    for (int i = (hi + 1 < t ? hi + 1 : t); i >= 1; i--) Bl[i] = Bl[i - 1];
+    Bl[0] = (uint16_t)ZL;
+  }
+  for (int i = 0; i <= t; i++) locator[i] = C[t - i];
+}
+
+static int fast_decrypt(const fast_key *F, const uint8_t *ct, uint8_t *e) {
+  const int set = F->set;
+  const orc_params *p = &PARAMS[set & 7];
+  const int t = p->t, n = p->n, m = p->m, sw = F->sw;
+  const gf_tab *T = tab(m);
+  uint64_t s[64], s2[64];
+  memset(s, 0, sizeof(uint64_t) * (size_t)sw);
+  const int rbits = 8 * p->synd_bytes;
+  for (int w = 0; w < (rbits + 63) / 64; w++) {
+    uint64_t x = 0;
+    const int nb = p->synd_bytes - 8 * w; /* >= 1 inside the loop */
+    memcpy(&x, ct + 8 * w, (size_t)(nb >= 8 ? 8 : (nb > 0 ? nb : 0)));
+    while (x) {
+      const int i = 64 * w + __builtin_ctzll(x);
+      x &= x - 1;
+      const uint64_t *col = F->scol + (size_t)i * sw;
+      for (int k = 0; k < sw; k++) s[k] ^= col[k];
+    }
+  }
+  uint16_t loc[130];
+  fast_bm(T, t, m, (const uint16_t *)s, loc);
+  const int D = t <= 64 ? 64 : 128;
+  uint16_t f[128], basis[13];
+  static __thread uint16_t vals[8192];
+  for (int i = 0; i < D; i++) f[i] = i <= t ? loc[i] : 0;
+  for (int j = 0; j < m; j++) basis[j] = (uint16_t)(1u << j);
+  fast_afft(T, m, f, D, basis, m, vals);
+  if (t == D)
+    for (int x = 0; x < (1 << m); x++) vals[x] ^= F->pow_t[x]; /* loc[t] = C[0] = 1 */
+  memset(e, 0, (size_t)(n / 8));
+  memset(s2, 0, sizeof(uint64_t) * (size_t)sw);
+  int wt = 0;
+  for (int i = 0; i < n; i++)
+    if (vals[F->K.alpha[i]] == 0) {
+      e[i >> 3] |= (uint8_t)(1 << (i & 7));
+      const uint64_t *col = F->scol + (size_t)i * sw;
+      for (int k = 0; k < sw; k++) s2[k] ^= col[k];
+      wt++;
+    }
+  int ok = wt == t;
+  for (int k = 0; k < sw; k++) ok &= s[k] == s2[k];
+  return ok ? 0 : 1;
+}
+
+static int fast_dec_one(const fast_key *F, uint8_t *key, const uint8_t *ct) {
+  const int set = F->set;
+  const orc_params *p = &PARAMS[set & 7];
+  const int n = p->n, clen = oracle_ct_bytes(set);
+  uint8_t e[1 + 1024], pre[1 + 1024 + 240], ctc[240];
+  memcpy(ctc, ct, (size_t)clen);
+  int pad_bad = 0;
+  if ((set & 7) == ORC_6960119) pad_bad = (ctc[p->synd_bytes - 1] >> (p->mt % 8)) != 0;
+  int fail = fast_decrypt(F, ctc, e + 1);
+  if (set & ORC_PC) {
+    uint8_t conf[32];
+    e[0] = 2;
+    oracle_shake256(conf, 32, e, 1 + n / 8);
+    if (memcmp(conf, ctc + p->synd_bytes, 32) != 0) fail = 1;
+  }
+  const uint8_t *sv = F->sk + 40 + p->irr_bytes + p->cond_bytes;
+  pre[0] = fail ? 0 : 1;
+  memcpy(pre + 1, fail ? sv : e + 1, (size_t)(n / 8));
+  memcpy(pre + 1 + n / 8, ctc, (size_t)clen);
+  oracle_shake256(key, 32, pre, 1 + n / 8 + clen);
+  if (pad_bad) {
+    memset(key, 0xFF, 32);
+    return -1;
+  }
+  return 0;
+}
+
+typedef struct {
+  const fast_key *F;
+  int is_dec;
+  uint8_t *ct, *key;
+  int *ret;
+  const uint8_t *in;
+  long lo, hi;
+} fast_job;
+
+static void *fast_job_run(void *arg) {
+  fast_job *j = (fast_job *)arg;
+  const long clen = oracle_ct_bytes(j->F->set);
+  for (long i = j->lo; i < j->hi; i++) {
+    const int r = j->is_dec ? fast_dec_one(j->F, j->key + 32 * i, j->in + (size_t)i * clen)
+                            : fast_enc_one(j->F, j->ct + (size_t)i * clen, j->key + 32 * i, j->in + 32 * i);
+    if (j->ret) j->ret[i] = r;
+  }
+  return NULL;
+}
+
+static int fast_run(fast_job *proto, long n, int nthreads) {
+  if (nthreads <= 0) nthreads = online_cores();
+  if (nthreads > n) nthreads = (int)(n > 0 ? n : 1);
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+  fast_job *jobs = (fast_job *)malloc(sizeof(fast_job) * nthreads);
+  for (int i = 0; i < nthreads; i++) {
+    jobs[i] = *proto;
+    jobs[i].lo = n * i / nthreads;
+    jobs[i].hi = n * (i + 1) / nthreads;
+    pthread_create(&th[i], NULL, fast_job_run, &jobs[i]);
+  }
+  for (int i = 0; i < nthreads; i++) pthread_join(th[i], NULL);
+  free(th);
+  free(jobs);
+  return nthreads;
+}
+
+int oracle_fast_enc_batch(void *h, uint8_t *ct, uint8_t *key, int *ret, const uint8_t *seeds, long n, int nthreads) {
+  fast_job j = {(const fast_key *)h, 0, ct, key, ret, seeds, 0, 0};
+  return fast_run(&j, n, nthreads);
+}
+
+int oracle_fast_dec_batch(void *h, uint8_t *key, int *ret, const uint8_t *ct, long n, int nthreads) {
+  fast_job j = {(const fast_key *)h, 1, NULL, key, ret, ct, 0, 0};
+  return fast_run(&j, n, nthreads);
+}
+
+/* evaluation of f[0..deg] at every field element by the FFT above (test hook) */
+void oracle_fast_eval_all(int m, const uint16_t *f, int deg, uint16_t *vals) {
+  const gf_tab *T = tab(m);
+  uint16_t g[128], basis[13];
+  int L = 1;
+  while (L < deg + 1) L <<= 1;
+  for (int i = 0; i < L; i++) g[i] = i <= deg ? f[i] : 0;
+  for (int j = 0; j < m; j++) basis[j] = (uint16_t)(1u << j);
+  fast_afft(T, m, g, L, basis, m, vals);
+}
+
 /* ------------------------------------------------------------------ KAT checksum */
 /* LM:command/mceliece-test.c: salsa20 core (:101-128), testvector/myrandom (:168-196),
    checksum (:374-388) and the KEM loop of test_kem_<set>_impl (:4634-4731). */

@@ -69,6 +69,15 @@ int oracle_syndrome_batch(int set, uint8_t *ct, const uint8_t *pk, const uint8_t
 
 /* stage-level functions (for stage parity tests) */
 void oracle_shake256(uint8_t *out, long long outlen, const uint8_t *in, long long inlen);
+
+/* word-level CPU port (the timed CPU baseline; same results as oracle_enc_seeded / oracle_dec):
+   a key handle from pk and/or sk (kept by pointer, not copied), batches on nthreads threads
+   (0 = every core) */
+void *oracle_fast_key(int set, const uint8_t *pk, const uint8_t *sk);
+void oracle_fast_key_free(void *h);
+int oracle_fast_enc_batch(void *h, uint8_t *ct, uint8_t *key, int *ret, const uint8_t *seeds, long n, int nthreads);
+int oracle_fast_dec_batch(void *h, uint8_t *key, int *ret, const uint8_t *ct, long n, int nthreads);
+void oracle_fast_eval_all(int m, const uint16_t *f, int deg, uint16_t *vals); /* deg < 128 */
 uint16_t oracle_gf_mul(int m, uint16_t a, uint16_t b);
 uint16_t oracle_gf_inv(int m, uint16_t a);
 void oracle_gen_e(int set, uint8_t *e, int *attempts); /* uses the current rng */
