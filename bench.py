@@ -40,7 +40,64 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0  # MI355X dense INT8 MFMA
+CUS, CLOCK_GHZ = 256, 2.4
+# VALU: one wave64 instruction per cycle per CU (4 SIMDs x 16 lanes, 4 cycles per wave64 instruction)
+VALU_PEAK_G = CUS * CLOCK_GHZ  # G wave-instructions/s
+# LDS: a conflict-free wave64 access of <= 32 bits per lane takes 2 cycles per CU (32 banks x 4 bytes)
+LDS_PEAK_G = CUS * CLOCK_GHZ / 2  # G wave-instructions/s
 SETS = ["348864", "460896", "6688128", "6960119", "8192128"]
+COUNTERS_JSON = os.path.join(HERE, "profiles", "counters.json")
+
+
+def load_counters(key):
+    """Per-kernel rocprofv3 counters of one workload ("<set>:<n>"), from the committed profile
+    summary (profiles/counters.json): instructions, LDS cycles and HBM bytes per launch."""
+    if not os.path.exists(COUNTERS_JSON):
+        return {}
+    with open(COUNTERS_JSON) as f:
+        return json.load(f).get(key, {})
+
+
+def hbm_bytes_per_op(kernel, p):
+    """Algorithmic HBM bytes per op: what a launch must read and write at minimum.  The decoder
+    counts the decapsulation's own inputs and outputs, SYND_BYTES + 32 per op; the intermediate
+    rows the engine passes between its launches are not algorithmic."""
+    ct = p["ct"]
+    return {"gen_e": 32, "enc_finish": ct + 32, "dec_front": ct, "dec_front_mfma": ct,
+            "dec_decode": ct + 32, "dec_hash": 32, "enc_total": 32 + ct + 32, "dec_total": ct + 32}.get(kernel)
+
+
+def kernel_roofline(kernel, avg_ms, n, p, ctr):
+    """Roofline of one kernel launch: the live average duration (HIP events) against the work one
+    launch does -- VALU and LDS wave-instructions from the committed counters (rocprofv3 SQ_INSTS_*,
+    same workload), algorithmic HBM bytes -- and the chip's peaks.  `bound` is the resource with
+    the highest fraction."""
+    sec = avg_ms * 1e-3
+    out = {"kernel": kernel, "avg_ms": round(avg_ms, 4)}
+    hb = hbm_bytes_per_op(kernel, p)
+    if hb:
+        gbs = hb * n / sec / 1e9
+        out["hbm"] = {"achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(gbs / HBM_PEAK_GBS, 5), "algorithmic_bytes_per_op": hb}
+    c = ctr.get(kernel)
+    if c:
+        v = c["valu_insts"] / sec / 1e9
+        out["valu"] = {"achieved": round(v, 1), "peak": VALU_PEAK_G, "unit": "G VALU wave-instr/s",
+                       "frac": round(v / VALU_PEAK_G, 4), "insts_per_launch": c["valu_insts"]}
+        l = c["lds_insts"] / sec / 1e9
+        out["lds"] = {"achieved": round(l, 1), "peak": LDS_PEAK_G, "unit": "G LDS wave-instr/s",
+                      "frac": round(l / LDS_PEAK_G, 4), "insts_per_launch": c["lds_insts"]}
+        if c.get("lds_idx_active"):
+            cyc = sec * CLOCK_GHZ * 1e9 * CUS
+            out["lds"]["busy_frac"] = round(c["lds_idx_active"] / cyc, 4)
+            out["lds"]["bank_conflict_frac_of_busy"] = round(c.get("lds_bank_conflict", 0) / c["lds_idx_active"], 4)
+        if c.get("traffic_bytes"):
+            out["traffic"] = c["traffic_bytes"]
+        out["counters_workload"] = c.get("workload")
+    fr = {k: out[k]["frac"] for k in ("valu", "lds", "hbm") if k in out}
+    if fr:
+        out["bound"] = max(fr, key=fr.get)
+    return out
 
 
 def xorshift_seeds(n, state=0x9E3779B97F4A7C15):
@@ -54,22 +111,6 @@ def xorshift_seeds(n, state=0x9E3779B97F4A7C15):
         x ^= (x << 17) & M
         out[i] = x & 0xFF
     return out.reshape(n, 32), x
-
-
-def algorithmic_bytes(kernel, p, n):
-    """HBM bytes one launch must move at minimum (inputs once, outputs once), per kernel."""
-    ct, e8 = p["ct"], p["n"] // 8
-    msg = ((1 + e8 + ct) // 136 + 1) * 136
-    splanes = 4 * p["m"] * ((2 * p["t"] + 31) // 32)
-    per_op = {
-        "gen_e": 32 + 2 * p["t"] + msg,           # seed in; support list + message row out
-        "enc_finish": 2 * p["t"] + msg + ct + 32,  # support + message in; ct + key out
-        "dec_front": ct + splanes,
-        "dec_front_mfma": ct + splanes,
-        "dec_decode": splanes + ct + msg,
-        "dec_hash": msg + ct + 32 + 4,
-    }[kernel]
-    return per_op * n
 
 
 def timed_kernels(tm, fn, reps):
@@ -225,30 +266,82 @@ class OverlapKem:
         self.sd.free()
 
 
-def cpu_rates(orc, s, pk, sk, seeds, cores, seconds):
-    """Oracle enc and dec on all host cores over a bounded sample (about `seconds` in total)."""
-    ne = max(cores, 32)
-    t0 = time.perf_counter()
-    ct_c, k_c, _ = orc.enc_batch(s, pk, seeds[:ne], cores)
-    te = time.perf_counter() - t0
-    ne2 = int(min(len(seeds), max(ne, ne * (seconds / 2) / max(te, 1e-3))))
-    t0 = time.perf_counter()
-    ct_c, k_c, _ = orc.enc_batch(s, pk, seeds[:ne2], cores)
-    te = time.perf_counter() - t0
-    nd = max(cores, 32)
-    t0 = time.perf_counter()
-    orc.dec_batch(s, sk, ct_c[:nd], cores)
-    td = time.perf_counter() - t0
-    nd2 = int(min(ne2, max(nd, nd * (seconds / 2) / max(td, 1e-3))))
-    t0 = time.perf_counter()
-    k_d, _ = orc.dec_batch(s, sk, ct_c[:nd2], cores)
-    td = time.perf_counter() - t0
-    assert np.array_equal(k_d, k_c[:nd2])
-    return ne2 / te, nd2 / td, ne2, nd2, ct_c, k_c
+def cpu_leg(orc, s, pk, sk, seeds, cts, cores, seconds):
+    """The CPU baseline: the word-level port (oracle FastKey) on all `cores` host threads, over a
+    bounded sample of the same inputs (about `seconds` of work in total, split between the
+    directions).  seeds -> encapsulations; cts -> decapsulations (None: decapsulate the sample's
+    own ciphertexts).  Returns rates and the sample's outputs, which the caller compares with the
+    GPU's for the same inputs."""
+    fk = orc.FastKey(s, pk=pk, sk=sk)
+    out = {}
+    if seeds is not None:
+        ne = min(len(seeds), max(cores * 8, 64))
+        t0 = time.perf_counter()
+        fk.enc(seeds[:ne], cores)
+        te = time.perf_counter() - t0
+        ne = int(min(len(seeds), max(ne, ne * (seconds / 2) / max(te, 1e-4))))
+        t0 = time.perf_counter()
+        ct_c, k_c, st_c = fk.enc(seeds[:ne], cores)
+        te = time.perf_counter() - t0
+        out.update(enc_per_s=ne / te, n_enc=ne, ct=ct_c, k_enc=k_c, st_enc=st_c)
+    if sk is not None:
+        src = cts if cts is not None else out["ct"]
+        nd = min(len(src), max(cores * 8, 64))
+        t0 = time.perf_counter()
+        fk.dec(src[:nd], cores)
+        td = time.perf_counter() - t0
+        nd = int(min(len(src), max(nd, nd * (seconds / 2) / max(td, 1e-4))))
+        t0 = time.perf_counter()
+        k_d, st_d = fk.dec(src[:nd], cores)
+        td = time.perf_counter() - t0
+        out.update(dec_per_s=nd / td, n_dec=nd, k_dec=k_d, st_dec=st_d)
+    fk.close()
+    return out
+
+
+ENC_KERNELS = ("gen_e", "enc_finish")
+DEC_KERNELS = ("dec_front_mfma", "dec_front", "dec_decode", "dec_hash")
+
+
+def direction_roofline(s, n, rates):
+    """Whole-direction VALU and LDS issue fractions at the given op rates (ops/s per direction):
+    the VALU / LDS wave-instructions one op costs in each direction (sum over the direction's
+    kernels, from the committed counters of the same set at n ops per launch) times the rate,
+    against the chip's issue peaks."""
+    ctr = load_counters(f"{s}:{n}")
+    if not ctr:
+        return None
+    out, v_tot, l_tot = {}, 0.0, 0.0
+    for d, names in (("enc", ENC_KERNELS), ("dec", DEC_KERNELS)):
+        if d not in rates:
+            continue
+        v = sum(ctr[k]["valu_insts"] for k in names if k in ctr) / n
+        l = sum(ctr[k]["lds_insts"] for k in names if k in ctr) / n
+        v_tot += v * rates[d]
+        l_tot += l * rates[d]
+        out[f"{d}_valu_insts_per_op"] = round(v, 1)
+        out[f"{d}_lds_insts_per_op"] = round(l, 1)
+    out["valu"] = {"achieved": round(v_tot / 1e9, 1), "peak": VALU_PEAK_G, "unit": "G VALU wave-instr/s",
+                   "frac": round(v_tot / 1e9 / VALU_PEAK_G, 4)}
+    out["lds"] = {"achieved": round(l_tot / 1e9, 1), "peak": LDS_PEAK_G, "unit": "G LDS wave-instr/s",
+                  "frac": round(l_tot / 1e9 / LDS_PEAK_G, 4)}
+    out["counters_workload"] = f"{s}:{n}"
+    return out
+
+
+def cpu_summary(c, cores, what):
+    r = {"cores": cores, "kind": "port",
+         "sample": f"word-level C port (oracle FastKey: column-gather syndrome, log-domain BM, additive-FFT "
+                   f"roots), {what}, {cores} threads"}
+    if "enc_per_s" in c:
+        r["encaps_per_s"] = round(c["enc_per_s"], 1)
+    if "dec_per_s" in c:
+        r["decaps_per_s"] = round(c["dec_per_s"], 1)
+    return r
 
 
 # ------------------------------------------------------------------------------- suite sections
-def section_syndrome(tm, orc, log):
+def section_syndrome(tm, orc, cores, log):
     s = "348864"
     p = tm.PARAMS[s]
     pk, _ = tm.keypair_seeded(s, bytes(range(32)))
@@ -278,8 +371,17 @@ def section_syndrome(tm, orc, log):
         res["mfma"] = best_ms
         outs["mfma"] = d_ct.download((n, p["ct"]))
         same = all(np.array_equal(outs["sparse"], outs[m]) for m in ("mfma", "valu"))
-        idx = rng.choice(n, 32, replace=False)
-        same = same and bool(np.array_equal(outs["mfma"][idx], orc.syndrome_batch(s, pk, e[idx])))
+        # linearity H(e1 + e2) = H(e1) + H(e2) on a sample, through the MFMA form
+        h = min(n, 2048)
+        e2 = e[:h] ^ e[h:2 * h] if n >= 2 * h else e[:h] ^ e[::-1][:h]
+        other = outs["mfma"][h:2 * h] if n >= 2 * h else outs["mfma"][::-1][:h]
+        d_e2, d_c2 = tm.DeviceBuffer(e2.nbytes), tm.DeviceBuffer(h * p["ct"])
+        d_e2.upload(np.ascontiguousarray(e2))
+        tm.syndrome_device(K, d_c2, d_e2, h, "mfma")
+        tm.sync()
+        same = same and bool(np.array_equal(d_c2.download((h, p["ct"])), outs["mfma"][:h] ^ other))
+        d_e2.free()
+        d_c2.free()
         tops = ops / (best_ms * 1e-3) / 1e12
         out[str(n)] = {
             "ms": {k: round(v, 4) for k, v in res.items()},
@@ -289,14 +391,20 @@ def section_syndrome(tm, orc, log):
                               "frac": round(tops / INT8_PEAK_TOPS, 4),
                               "work": "2*768*2720 int8 ops per column (0/1 bytes, int32 accumulate)"},
             "valu_lane_ops_per_column": 768 * 85 * 3 // 2,
+            "check": "three GPU forms identical; linearity H(e1+e2) = H(e1)+H(e2) on 2048 columns",
         }
         log(f"syndrome n={n}: {out[str(n)]['ms']}")
         d_e.free()
         d_ct.free()
+    # CPU context (the CPU path does not expose the syndrome alone): the port's encapsulation,
+    # dominated by the syndrome and SHAKE256, on all cores
+    c = cpu_leg(orc, s, pk, None, np.random.default_rng(5).integers(0, 256, (65536, 32), dtype=np.uint8), None,
+                cores, 4.0)
+    out["cpu_context"] = cpu_summary(c, cores, f"{c['n_enc']} encapsulations (syndrome + SHAKE256)")
     return out
 
 
-def section_variants(tm, orc, log, n=65536, reps=3):
+def section_variants(tm, orc, cores, log, n=65536, reps=3):
     """The pc variants (plaintext confirmation: one more SHAKE256 over e in each direction, 32 more
     ciphertext bytes) against their base sets at N=65536, and batched key generation of an f variant
     (semi-systematic form) against its base set.  Outputs round-trip and a sample is byte-compared
@@ -311,12 +419,14 @@ def section_variants(tm, orc, log, n=65536, reps=3):
             dk = DevKem(tm, v, n, (pk, sk), seeds)
             ee, ed = dk.rates(reps)
             ct, k1, k2, st1, st2 = dk.outputs()
-            idx = np.random.default_rng(2).choice(n, 8, replace=False)
-            ct_o, k_o, _ = orc.enc_batch(v, pk, seeds[idx])
+            c = cpu_leg(orc, v, pk, sk, seeds, None, cores, 2.0)
+            ne, nd = c["n_enc"], c["n_dec"]
             exact = bool(np.array_equal(k1, k2) and not st1.any() and not st2.any()
-                         and np.array_equal(ct[idx], ct_o) and np.array_equal(k1[idx], k_o))
+                         and np.array_equal(ct[:ne], c["ct"]) and np.array_equal(k1[:ne], c["k_enc"])
+                         and np.array_equal(k2[:nd], c["k_dec"]))
             dk.free()
-            row[v] = {"encaps_per_s": round(ee, 1), "decaps_per_s": round(ed, 1), "bit_exact": exact}
+            row[v] = {"encaps_per_s": round(ee, 1), "decaps_per_s": round(ed, 1), "bit_exact": exact,
+                      "cpu_baseline": cpu_summary(c, cores, f"{ne} encaps + {nd} decaps of the same inputs")}
         row["pc_cost"] = {"encaps": round(row[base]["encaps_per_s"] / row[base + "pc"]["encaps_per_s"], 3),
                           "decaps": round(row[base]["decaps_per_s"] / row[base + "pc"]["decaps_per_s"], 3)}
         out[base] = row
@@ -346,19 +456,19 @@ def section_per_set(tm, orc, keys, cores, cpu_seconds, log, n=65536, reps=3):
         pe, pd, _, pout = dk.plain(max(1, reps - 1))
         exact = bool(np.array_equal(k1, k2) and not st1.any() and not st2.any()
                      and all(np.array_equal(a, b) for a, b in zip((ct, k1, k2), pout[:3])))
-        idx = np.random.default_rng(1).choice(n, 16, replace=False)
-        ct_o, k_o, _ = orc.enc_batch(s, pk, seeds[idx])
-        exact = exact and bool(np.array_equal(ct[idx], ct_o) and np.array_equal(k1[idx], k_o))
-        ce, cd, ne, nd, _, _ = cpu_rates(orc, s, pk, sk, seeds, cores, cpu_seconds)
+        c = cpu_leg(orc, s, pk, sk, seeds, ct, cores, cpu_seconds)
+        ne, nd = c["n_enc"], c["n_dec"]
+        exact = exact and bool(np.array_equal(ct[:ne], c["ct"]) and np.array_equal(k1[:ne], c["k_enc"])
+                               and np.array_equal(k2[:nd], c["k_dec"]) and not c["st_dec"].any())
+        ce, cd = c["enc_per_s"], c["dec_per_s"]
         dk.free()
         out[s] = {"engine": {"encaps_per_s": round(ee, 1), "decaps_per_s": round(ed, 1)},
                   "plain_gpu_form": {"encaps_per_s": round(pe, 1), "decaps_per_s": round(pd, 1)},
-                  "cpu_oracle": {"encaps_per_s": round(ce, 1), "decaps_per_s": round(cd, 1), "cores": cores,
-                                 "sample": f"{ne} enc + {nd} dec"},
+                  "cpu_baseline": cpu_summary(c, cores, f"{ne} encaps + {nd} decaps of the same inputs"),
                   "speedup_vs_plain": {"encaps": round(ee / pe, 2), "decaps": round(ed / pd, 2)},
                   "speedup_vs_cpu": {"encaps": round(ee / ce, 1), "decaps": round(ed / cd, 1)},
                   "bit_exact": exact}
-        log(f"per_set {s}: {out[s]['engine']} plain {out[s]['plain_gpu_form']} cpu {out[s]['cpu_oracle']}")
+        log(f"per_set {s}: {out[s]['engine']} plain {out[s]['plain_gpu_form']} cpu {out[s]['cpu_baseline']}")
     return out
 
 
@@ -375,25 +485,32 @@ def section_c1(tm, orc, keys, cores, log):
         pte, ptd = wall(tm, dk.enc, 10), wall(tm, dk.dec, 10)
     finally:
         tm.set_form("engine")
-    ct_o, k_o, _ = orc.enc_batch(s, pk, seeds, cores)
-    t0 = time.perf_counter()
-    orc.enc_batch(s, pk, seeds, cores)
-    cte = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    k_d, _ = orc.dec_batch(s, sk, ct_o, cores)
-    ctd = time.perf_counter() - t0
+    # the CPU baseline on the whole batch: all cores, one batch of 256 each way (best of 3)
+    fk = orc.FastKey(s, pk=pk, sk=sk)
+    cte, ctd = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ct_o, k_o, _ = fk.enc(seeds, cores)
+        cte.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        k_d, _ = fk.dec(ct_o, cores)
+        ctd.append(time.perf_counter() - t0)
+    fk.close()
+    cte, ctd = min(cte), min(ctd)
     exact = bool(np.array_equal(ct, ct_o) and np.array_equal(k1, k_o) and np.array_equal(k2, k_d)
                  and not st1.any() and not st2.any())
     dk.free()
     out = {"batch_ms": {"engine_enc": round(te * 1e3, 4), "engine_dec": round(td * 1e3, 4),
                         "plain_enc": round(pte * 1e3, 4), "plain_dec": round(ptd * 1e3, 4),
                         "cpu_enc": round(cte * 1e3, 3), "cpu_dec": round(ctd * 1e3, 3)},
-           "cpu_cores": cores, "bit_exact_vs_oracle": exact}
+           "cpu_baseline": {"cores": cores, "kind": "port", "encaps_per_s": round(n / cte, 1),
+                            "decaps_per_s": round(n / ctd, 1), "sample": "the whole batch (256 each way)"},
+           "bit_exact_vs_cpu": exact}
     log(f"c1: {out}")
     return out
 
 
-def section_c3(tm, orc, keys, log, n=8192):
+def section_c3(tm, orc, keys, cores, log, n=8192):
     out = {}
     for s in ("460896", "6688128"):
         pk, sk = keys[s]
@@ -408,17 +525,24 @@ def section_c3(tm, orc, keys, log, n=8192):
             ptd = wall(tm, dk.dec, 3)
         finally:
             tm.set_form("engine")
-        idx = np.random.default_rng(2).choice(n, 16, replace=False)
-        k_o, _ = orc.dec_batch(s, sk, ct[idx])
-        exact = bool(np.array_equal(k1, k2) and not st2.any() and np.array_equal(k2[idx], k_o))
+        c = cpu_leg(orc, s, None, sk, None, ct, cores, 3.0)
+        nd = c["n_dec"]
+        exact = bool(np.array_equal(k1, k2) and not st2.any() and np.array_equal(k2[:nd], c["k_dec"]))
         dk.free()
+        ctr = load_counters(f"{s}:{n}")
+        dom = max(kern, key=lambda k: kern[k]["avg_ms"])
         out[s] = {"decaps_per_s": round(n / td, 1), "plain_decaps_per_s": round(n / ptd, 1),
-                  "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kern.items()}, "bit_exact": exact}
+                  "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kern.items()},
+                  "roofline": kernel_roofline(dom, kern[dom]["avg_ms"], n, tm.PARAMS[s], ctr),
+                  "hbm_decap_total": {"bytes_per_op": hbm_bytes_per_op("dec_total", tm.PARAMS[s]),
+                                      "achieved_gbs": round(hbm_bytes_per_op("dec_total", tm.PARAMS[s]) * n / td / 1e9, 2)},
+                  "cpu_baseline": cpu_summary(c, cores, f"{nd} decaps of the same ciphertexts"),
+                  "bit_exact": exact}
         log(f"c3 {s}: {out[s]}")
     return out
 
 
-def section_c4(tm, keys, log, total=65536):
+def section_c4(tm, orc, keys, cores, log, total=65536):
     from tensormceliece_amd import autotune
     s = "6960119"
     p = tm.PARAMS[s]
@@ -472,12 +596,20 @@ def section_c4(tm, keys, log, total=65536):
                  and np.array_equal(k1, k2) and not st1.any() and not st2.any())
     for b in (d_seeds, d_ct, d_k, d_st, d_k2, d_st2):
         b.free()
+    c = cpu_leg(orc, s, pk, sk, seeds, ct, cores, 4.0)
+    ne, nd = c["n_enc"], c["n_dec"]
+    exact_cpu = bool(np.array_equal(ct[:ne], c["ct"]) and np.array_equal(k1[:ne], c["k_enc"])
+                     and np.array_equal(k2[:nd], c["k_dec"]))
     out = {"sub_batches": len(sizes), "total_ops": total, "tuned": best,
            "pipeline_graph": {"encaps_per_s": round(total / te, 1), "decaps_per_s": round(total / td, 1)},
            "flat_engine_batch": {"encaps_per_s": round(fe, 1), "decaps_per_s": round(fd, 1)},
            "plain_gpu_form": {"encaps_per_s": round(pe, 1), "decaps_per_s": round(pd, 1)},
            "speedup_vs_plain": {"encaps": round(total / te / pe, 2), "decaps": round(total / td / pd, 2)},
-           "bit_exact_vs_flat_engine": exact}
+           "roofline": direction_roofline(s, total, {"enc": total / te, "dec": total / td}),
+           "cpu_baseline": cpu_summary(c, cores, f"{ne} encaps + {nd} decaps of the same inputs"),
+           "speedup_vs_cpu": {"encaps": round(total / te / c["enc_per_s"], 1),
+                              "decaps": round(total / td / c["dec_per_s"], 1)},
+           "bit_exact_vs_flat_engine": exact, "bit_exact_vs_cpu": exact_cpu}
     log(f"c4: {out}")
     return out
 
@@ -505,7 +637,36 @@ def verify_serving(wl, reqs, outputs, sample=32):
             "requests_checked_vs_alone": checked}
 
 
-def section_c5(tm, log, nkeys=16, n_requests=6000):
+def cpu_request_mix(orc, s, kps, reqs, outputs, cores, seconds):
+    """The port on the serving workload's requests (in order, until about `seconds`), whole
+    requests on `cores` threads; returns (summary, outputs identical to the served ones)."""
+    keys = [orc.FastKey(s, pk=pk, sk=sk) for pk, sk in kps]
+
+    def one(i):
+        d, k, inp, _ = reqs[i]
+        return keys[k].enc(inp, 1) if d == "enc" else keys[k].dec(inp, 1)
+
+    done, ops, t0, same = 0, 0, time.perf_counter(), True
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        while done < len(reqs) and time.perf_counter() - t0 < seconds:
+            idx = list(range(done, min(len(reqs), done + 4 * cores)))
+            for i, r in zip(idx, ex.map(one, idx)):
+                ct, k, st = outputs[i]
+                if reqs[i][0] == "enc":
+                    same &= bool(np.array_equal(r[0], ct) and np.array_equal(r[1], k) and np.array_equal(r[2], st))
+                else:
+                    same &= bool(np.array_equal(r[0], k) and np.array_equal(r[1], st))
+                ops += reqs[i][2].shape[0]
+            done = idx[-1] + 1
+    el = time.perf_counter() - t0
+    for fk in keys:
+        fk.close()
+    return {"ops_per_s": round(ops / el, 1), "cores": cores, "kind": "port",
+            "sample": f"word-level C port, the first {done} requests of the mix ({ops} ops), whole requests "
+                      f"on {cores} threads"}, same
+
+
+def section_c5(tm, orc, cores, log, nkeys=16, n_requests=6000):
     from tensormceliece_amd import serving
     s = "8192128"
     seeds, _ = xorshift_seeds(nkeys, 0x51ED)
@@ -518,7 +679,7 @@ def section_c5(tm, log, nkeys=16, n_requests=6000):
         serving.run_poisson(server, wl, reqs[:200], 0.0)  # warm-up (first runs, graph capture)
         sat = serving.run_poisson(server, wl, reqs, 0.0, keep_outputs=True)
         verified = verify_serving(wl, reqs, sat["outputs"])
-        sat["outputs"] = None
+        sat_outputs = sat["outputs"]
         mean_ops = float(np.mean([r[2].shape[0] for r in reqs]))
         req_rate_sat = sat["ops_per_s"] / mean_ops
         loads = {}
@@ -542,13 +703,23 @@ def section_c5(tm, log, nkeys=16, n_requests=6000):
         outb.free()
     finally:
         server.close()
+    # the CPU baseline on the same request mix: requests spread over all cores (one request per
+    # thread at a time), each against its own key's tables; outputs compared with the served ones
+    cpu, cpu_exact = cpu_request_mix(orc, s, kps, reqs, sat_outputs, cores, 4.0)
+    enc_ops = sum(r[2].shape[0] for r in reqs if r[0] == "enc")
+    frac_enc = enc_ops / sum(r[2].shape[0] for r in reqs)
+    rates = {"enc": sat["ops_per_s"] * frac_enc, "dec": sat["ops_per_s"] * (1 - frac_enc)}
     out = {"keys": nkeys, "requests_per_run": n_requests, "mean_request_ops": round(mean_ops, 1),
            "key_device_bytes": wl.device_bytes(),
            "saturation": {"ops_per_s": round(sat["ops_per_s"], 1), "p50_ms": round(sat["p50_ms"], 3),
                           "p99_ms": round(sat["p99_ms"], 3)},
            "verified": verified,
            "poisson": loads, "zero_copy_slabs": zero_copy, "batches": st["batches"], "requests": st["requests"],
-           "peak_in_flight_bytes": st["peak_in_flight_bytes"], "hbm_budget_bytes": 8 << 30}
+           "peak_in_flight_bytes": st["peak_in_flight_bytes"], "hbm_budget_bytes": 8 << 30,
+           "max_batch_keys": st["max_batch_keys"],
+           "roofline_at_saturation": direction_roofline(s, 65536, rates),
+           "cpu_baseline": cpu, "cpu_outputs_identical": cpu_exact,
+           "saturation_vs_cpu": round(sat["ops_per_s"] / cpu["ops_per_s"], 1)}
     log(f"c5: {out}")
     return out
 
@@ -661,35 +832,39 @@ def main():
                  "kem_ops_per_s": round(2 / (1 / pe + 1 / pd), 1), "identical_to_engine": same,
                  "kernels": pkern}
 
+    # ---- checks without the CPU: every decapsulation returns its encapsulation's key, the plain
+    #      form and the two-stream step give the same bytes
     verified = None
     if not args.no_verify:
-        ok = bool(np.array_equal(k1, k2) and not st1.any() and not st2.any())
-        idx = np.random.default_rng(0).choice(n, min(n, 256), replace=False)
-        ct_ref, k_ref, _ = orc.enc_batch(s, pk, seeds[idx])
-        verified = ok and overlap_same is not False and bool(np.array_equal(ct[idx], ct_ref) and np.array_equal(k1[idx], k_ref))
+        verified = bool(np.array_equal(k1, k2) and not st1.any() and not st2.any() and overlap_same is not False
+                        and (plain is None or plain["identical_to_engine"]))
 
-    # ---- roofline of the dominant kernel (HBM; algorithmic bytes per launch / launch time)
+    # ---- roofline of the dominant kernel: live HIP-event duration against the work one launch
+    #      does (VALU / LDS wave-instructions from the committed counters of this workload,
+    #      algorithmic HBM bytes); the binding resource is the one with the highest fraction
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
-    abytes = algorithmic_bytes(dom, p, n)
-    achieved = abytes / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(HERE, "profiles", "traffic.json")
-    if os.path.exists(tpath):  # measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (see that file)
-        with open(tpath) as f:
-            traffic = json.load(f).get(f"{s}:{n}", {}).get(dom)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
-                "algorithmic_bytes_per_launch": abytes,
-                "note": "the decoder is bound by its LDS table gathers (GF(2^m) log/exp lookups, bank "
-                        "conflicts) and VALU issue, not by HBM; rocprofv3 SQ counters in the committed profile"}
+    roofline = kernel_roofline(dom, kernels[dom]["avg_ms"], n, p, load_counters(f"{s}:{n}"))
+    rb = roofline.get(roofline.get("bound", "hbm"), {})
+    roofline.update({"achieved": rb.get("achieved"), "peak": rb.get("peak"), "unit": rb.get("unit"),
+                     "frac": rb.get("frac")})
+    roofline.setdefault("traffic", None)
+    step_roofline = direction_roofline(s, n, {"enc": n / t_step, "dec": n / t_step})
 
+    # ---- CPU baseline: the word-level port on every host core, a bounded sample of the same
+    #      inputs; its outputs are the byte-for-byte check of the GPU's
     cpu = None
     if not args.no_cpu:
-        ce, cd, ne, nd, _, _ = cpu_rates(orc, s, pk, sk, seeds, cores, args.cpu_seconds)
-        cpu = {"value": round(2.0 / (1.0 / ce + 1.0 / cd), 1), "unit": "KEM ops/s", "cores": cores,
-               "kind": "port", "encaps_per_s": round(ce, 1), "decaps_per_s": round(cd, 1),
-               "sample": f"oracle C restatement, {ne} encaps + {nd} decaps of the same seeded {s} inputs, "
-                         f"{cores} threads"}
+        c = cpu_leg(orc, s, pk, sk, seeds, ct, cores, args.cpu_seconds)
+        ne, nd = c["n_enc"], c["n_dec"]
+        cpu = {"value": round(2.0 / (1.0 / c["enc_per_s"] + 1.0 / c["dec_per_s"]), 1), "unit": "KEM ops/s",
+               "cores": cores, "kind": "port", "encaps_per_s": round(c["enc_per_s"], 1),
+               "decaps_per_s": round(c["dec_per_s"], 1),
+               "sample": f"word-level C port (oracle FastKey), {ne} encaps + {nd} decaps of the same seeded {s} "
+                         f"inputs, {cores} threads",
+               "outputs_identical_to_gpu": bool(np.array_equal(ct[:ne], c["ct"]) and np.array_equal(k1[:ne], c["k_enc"])
+                                                and np.array_equal(k2[:nd], c["k_dec"]))}
+        if verified is not None:
+            verified = verified and cpu["outputs_identical_to_gpu"]
     dk.free()
 
     suite = None
@@ -702,21 +877,21 @@ def main():
                 kp = list(ex.map(lambda x: tm.keypair_seeded(x, xorshift_seeds(1)[0][0].tobytes()), SETS))
             keys = dict(zip(SETS, kp))
         if "syndrome" in sections:
-            suite["syndrome_348864"] = section_syndrome(tm, orc, log)
+            suite["syndrome_348864"] = section_syndrome(tm, orc, cores, log)
         if "per_set" in sections:
             suite["per_set_n65536"] = section_per_set(tm, orc, keys, cores, min(4.0, args.cpu_seconds), log)
         if "c1" in sections:
             suite["c1_348864_n256"] = section_c1(tm, orc, keys, cores, log)
         if "c3" in sections:
-            suite["c3_decap_n8192"] = section_c3(tm, orc, keys, log)
+            suite["c3_decap_n8192"] = section_c3(tm, orc, keys, cores, log)
         if "c4" in sections:
-            suite["c4_6960119_uneven"] = section_c4(tm, keys, log)
+            suite["c4_6960119_uneven"] = section_c4(tm, orc, keys, cores, log)
         if "c5" in sections:
-            suite["c5_8192128_serving"] = section_c5(tm, log)
+            suite["c5_8192128_serving"] = section_c5(tm, orc, cores, log)
         if "keygen" in sections:
             suite["keygen_batch"] = section_keygen(tm, cores, log)
         if "variants" in sections:
-            suite["variants_n65536"] = section_variants(tm, orc, log)
+            suite["variants_n65536"] = section_variants(tm, orc, cores, log)
 
     kem_ops_per_s = 2 * n / t_step
     line = {
@@ -742,6 +917,7 @@ def main():
             "decaps": round((n / t_dec) / plain["decaps_per_s"], 2)},
         "kernels": kernels,
         "roofline": roofline,
+        "step_issue": step_roofline,
         "cpu_baseline": cpu,
         "suite": suite,
     }
