@@ -404,6 +404,8 @@ int tmce_syndrome_batch_device(tmce_key *pk, unsigned char *d_ct, const unsigned
    TMCE_TUNE_FRONT_MFMA     decapsulation front end S = F c: 1 = INT8 MFMA GEMM (default), 0 = VALU
    TMCE_TUNE_FRONT_CFG      tile shape of the MFMA front end (0..5)
    TMCE_TUNE_GEN_FORM       sampler: 1 = pipelined lane groups (default), 0 = one op per lane
+   TMCE_TUNE_FFT_FORM       decoder's bottom-up additive FFT: 1 = bitsliced GF(2^m) arithmetic
+                            (default), 0 = log/exp table lookups in LDS
    tmce_tune_get returns the value (>= 0) or a negative error code. */
 enum {
   TMCE_TUNE_SYND_MFMA_CFG = 0,
@@ -411,7 +413,8 @@ enum {
   TMCE_TUNE_DECODE_BLOCKS = 2,
   TMCE_TUNE_FRONT_MFMA = 3,
   TMCE_TUNE_FRONT_CFG = 4,
-  TMCE_TUNE_GEN_FORM = 5
+  TMCE_TUNE_GEN_FORM = 5,
+  TMCE_TUNE_FFT_FORM = 6
 };
 int tmce_tune_set(int set, int param, int value);
 int tmce_tune_get(int set, int param);

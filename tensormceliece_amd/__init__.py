@@ -36,7 +36,8 @@ KEY_BYTES = 32
 # syndrome methods (tmce_syndrome_batch)
 SYND_METHODS = {"sparse": 0, "mfma": 1, "valu": 2}
 # tuning parameters (tmce_tune_set / tmce_tune_get)
-TUNE_PARAMS = {"synd_mfma_cfg": 0, "gen_blocks": 1, "decode_blocks": 2, "front_mfma": 3, "front_cfg": 4, "gen_form": 5}
+TUNE_PARAMS = {"synd_mfma_cfg": 0, "gen_blocks": 1, "decode_blocks": 2, "front_mfma": 3, "front_cfg": 4, "gen_form": 5,
+               "fft_form": 6}
 
 
 class TmceError(RuntimeError):

@@ -261,6 +261,7 @@ int dec_launch(Engine *e, tmce_key *k, uint8_t *d_k, int *d_status, const uint8_
   a.queue = x.queue;
   a.nops = (int)n;
   a.decode_blocks = e->decode_blocks[k->set];
+  a.fft_form = e->fft_form[k->set];
   if (form == 1) {
     HIPCHK(launch_plain_dec_set(k->set, a, *px, s));
     return 0;
@@ -693,6 +694,10 @@ int tmce_tune_set(int variant, int param, int value) {
       if (value != 0 && value != 1) return fail(TMCE_EINVAL, "gen_form must be 0 or 1");
       e->gen_form[set] = value;
       return 0;
+    case TMCE_TUNE_FFT_FORM:
+      if (value != 0 && value != 1) return fail(TMCE_EINVAL, "fft_form must be 0 or 1");
+      e->fft_form[set] = value;
+      return 0;
   }
   return fail(TMCE_EINVAL, "tmce_tune_set: unknown parameter");
 }
@@ -710,6 +715,7 @@ int tmce_tune_get(int variant, int param) {
     case TMCE_TUNE_FRONT_MFMA: return e->front_mfma[set];
     case TMCE_TUNE_FRONT_CFG: return e->front_cfg[set];
     case TMCE_TUNE_GEN_FORM: return e->gen_form[set];
+    case TMCE_TUNE_FFT_FORM: return e->fft_form[set];
   }
   return fail(TMCE_EINVAL, "tmce_tune_get: unknown parameter");
 }

@@ -139,6 +139,7 @@ struct Engine {
   int front_mfma[NUM_SETS] = {1, 1, 1, 1, 1};  // decapsulation front end: 1 = INT8 MFMA, 0 = VALU
   int front_cfg[NUM_SETS] = {1, 1, 1, 1, 1};   // its MFMA tile shape
   int gen_form[NUM_SETS] = {1, 1, 1, 1, 1};    // sampler: 1 = pipelined lane groups, 0 = one op per lane
+  int fft_form[NUM_SETS] = {1, 1, 1, 1, 1};    // decoder's bottom-up FFT: 1 = bitsliced, 0 = log/exp tables
   EventTimer timer;
   // key generation: pinned host staging for the public-key downloads (allocated on first use and
   // kept; kg_mu serialises the calls that use it)

@@ -184,6 +184,54 @@ __device__ __forceinline__ void bm_run(int &N, const uint8_t *wl, int l16, uint3
   if constexpr (NA < NSB) bm_run<NA + 1, NSB, T, Q>(N, wl, l16, C, Bl, L, lb, lg_s, ex_s);
 }
 
+// ---- bitsliced GF(2^m) arithmetic (the decoder's bottom-up FFT, fft_form 1)
+// A bitsliced vector holds 32 field elements in m words: bit i of word b is the coefficient of x^b
+// of element i (polynomial basis, the field polynomials x^12 + x^3 + 1 and x^13 + x^4 + x^3 + x + 1
+// of LM:crypto_kem/<set>/vec/gf.c).
+__device__ __forceinline__ uint32_t and_xor(uint32_t acc, uint32_t a, uint32_t b) {  // acc ^ (a & b)
+  return __builtin_amdgcn_bitop3_b32(acc, a, b, 0x78);
+}
+__device__ __forceinline__ uint32_t andn_xor(uint32_t acc, uint32_t a, uint32_t m) {  // acc ^ (a & ~m)
+  return __builtin_amdgcn_bitop3_b32(acc, a, m, 0xB4);
+}
+__device__ __forceinline__ uint32_t select_bits(uint32_t m, uint32_t a, uint32_t b) {  // m ? a : b, per bit
+  return __builtin_amdgcn_bitop3_b32(m, a, b, 0xCA);
+}
+// r = a * b elementwise (32 products): schoolbook over the m x m coefficient pairs, one three-input
+// AND-XOR each, then reduction of the 2m - 1 product words by the field polynomial
+template <int M>
+__device__ __forceinline__ void bs_mul(const uint32_t (&a)[M], const uint32_t (&b)[M], uint32_t (&r)[M]) {
+  uint32_t p[2 * M - 1];
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      if (i == 0 || j == M - 1) p[i + j] = a[i] & b[j];  // first term of word i + j
+      else p[i + j] = and_xor(p[i + j], a[i], b[j]);
+    }
+#pragma unroll
+  for (int k = 2 * M - 2; k >= M; --k) {
+    if constexpr (M == 12) {  // x^12 = x^3 + 1
+      p[k - 12] ^= p[k];
+      p[k - 9] ^= p[k];
+    } else {  // x^13 = x^4 + x^3 + x + 1
+      p[k - 13] ^= p[k];
+      p[k - 12] ^= p[k];
+      p[k - 10] ^= p[k];
+      p[k - 9] ^= p[k];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < M; ++i) r[i] = p[i];
+}
+// the multiplier words of one bottom-up level in this lane: bit q of word b is bit b of
+// plo[d][lane] ^ phi[k(q)]; `ph` holds the phi part (bit planes, lane-uniform), plo2 is plo doubled
+template <int M>
+__device__ __forceinline__ void bs_multipliers(const uint32_t *ph, uint32_t plo2, uint32_t (&c)[M]) {
+#pragma unroll
+  for (int b = 0; b < M; ++b) c[b] = ph[b] ^ (uint32_t)(-(int)((plo2 >> (b + 1)) & 1u));
+}
+
 // One wavefront decodes four ops at a time: Berlekamp-Massey runs on the four 16-lane rows in
 // parallel (coefficient i of op `row` lives in lane 16*row + i%16, slot i/16; row reductions and
 // the x-shifts of B and of the syndrome window are DPP row operations), then the wave finishes the
@@ -191,7 +239,10 @@ __device__ __forceinline__ void bm_run(int &N, const uint8_t *wl, int l16, uint3
 // butterfly pairs two registers of one lane), root flags, weight and re-encryption checks, message.
 // m = 12: at most 128 VGPRs, so two 8-wave workgroups share a CU (LDS holds two); m = 13 keeps the
 // FFT's 2^13 values in registers (two waves per SIMD).
-template <int SET>
+// BS (fft_form 1): the bottom-up butterflies run on bitsliced values -- 32 points of a lane per
+// word and bit plane, the GF(2^m) products by AND-XOR on the VALU -- instead of log/exp table
+// gathers in LDS (fft_form 0), whose random 16-bit reads are mostly LDS bank conflicts.
+template <int SET, bool BS>
 __global__ __launch_bounds__(DEC_WG) __attribute__((amdgpu_waves_per_eu(Par<SET>::M == 12 ? 4 : 1)))
 void k_dec_decode(DecArgs a) {
   using P = Par<SET>;
@@ -210,7 +261,12 @@ void k_dec_decode(DecArgs a) {
   __shared__ uint16_t pthi_s[NR];
   __shared__ uint16_t twl_s[2 * Dn];
   __shared__ uint16_t plo_s[(KD + 1) * 64];  // butterfly multipliers' lane parts [KD][64], then (lane)^t
-  __shared__ uint16_t lpt_s[NPHI * 64];      // logs of the butterfly multipliers [phi index][lane]
+  __shared__ uint16_t lpt_s[BS ? 1 : NPHI * 64];  // logs of the butterfly multipliers [phi index][lane]
+  constexpr int NG = NR / 32;                     // bitsliced words per plane and lane (2 or 4)
+  // bitsliced form: the multipliers' register parts as bit planes, [level + KD g][b] (g: which
+  // 32-point half of a 64-point butterfly span), and the monic term's register part [group][b]
+  __shared__ uint32_t phib_s[BS ? 2 * KD * M : 1];
+  __shared__ uint32_t pthib_s[BS ? NG * M : 1];
   __shared__ DecWave<SET> Wv[DEC_WAVES];
 
   const int tid = threadIdx.x;
@@ -225,11 +281,35 @@ void k_dec_decode(DecArgs a) {
   // the bottom-up multiplier of phi index k in lane l is plo[d][l] ^ phi[k], where level d holds the
   // indices [2^(m-6) - 2^(m-6-d), 2^(m-6) - 2^(m-7-d)); its log is looked up once here, so the
   // butterflies read it lane-contiguous (no bank conflict) instead of by a table gather
-  for (int i = tid; i < NPHI * 64; i += DEC_WG) {
-    const int k = i >> 6, l = i & 63;
-    int d = 0;
-    while (d + 1 < KD && k >= (1 << (M - 6)) - (1 << (M - 7 - d))) ++d;
-    lpt_s[i] = (uint16_t)tab(lg_s, plo_s[64 * d + l] ^ phi_s[k]);
+  if constexpr (!BS) {
+    for (int i = tid; i < NPHI * 64; i += DEC_WG) {
+      const int k = i >> 6, l = i & 63;
+      int d = 0;
+      while (d + 1 < KD && k >= (1 << (M - 6)) - (1 << (M - 7 - d))) ++d;
+      lpt_s[i] = (uint16_t)tab(lg_s, plo_s[64 * d + l] ^ phi_s[k]);
+    }
+  } else {
+    // level d pairs points 64 R + l and 64 (R + h) + l, h = 2^(m-7-d), with the multiplier
+    // plo[d][l] ^ phi[off_d + (R mod h)], off_d = 2^(m-6) - 2^(m-6-d).  Bit q of plane word
+    // (d + KD g, b) is bit b of phi[off_d + (q mod h)] for h < 32 (pairs inside a 32-point word),
+    // of phi[off_d + 32 g + q] for h >= 32 (pairs of words, g = word index mod h/32)
+    for (int i = tid; i < 2 * KD * M; i += DEC_WG) {
+      const int slot = i / M, b = i % M, d = slot % KD, g = slot / KD;
+      const int h = 1 << (M - 7 - d), off = (1 << (M - 6)) - (1 << (M - 6 - d));
+      uint32_t w = 0;
+      if (h < 32 || g < h / 32)
+        for (int q = 0; q < 32; ++q) {
+          const int j = h < 32 ? (q & (h - 1)) : 32 * g + q;
+          w |= ((uint32_t)(phi_s[off + j] >> (b + 1)) & 1u) << q;  // phi_s is doubled
+        }
+      phib_s[i] = w;
+    }
+    for (int i = tid; i < NG * M; i += DEC_WG) {
+      const int G = i / M, b = i % M;
+      uint32_t w = 0;
+      for (int q = 0; q < 32; ++q) w |= ((uint32_t)(pthi_s[32 * G + q] >> (b + 1)) & 1u) << q;
+      pthib_s[i] = w;
+    }
   }
   __syncthreads();
   const int nops = a.plan ? a.plan->total : a.nops;
@@ -385,61 +465,141 @@ void k_dec_decode(DecArgs a) {
           m8[2 + P::N / 8 + 2 * h] = (uint8_t)(ctw[c] >> 8);
         }
       }
-      // bottom-up butterflies in registers: V2[r] packs point 64r+lane (low half) and point
-      // 64(r+NR/2)+lane (high half); the two butterflies of a packed pair share their multiplier
-      constexpr int NH = NR / 2;
-      const int zo = opaque_zero();
-      const uint16_t *lpq = lpt_s + zo + lane, *ptq = pthi_s + zo, *plq = plo_s + zo;
-      uint32_t V2[NH];
-#pragma unroll
-      for (int r = 0; r < NH; ++r) V2[r] = (uint32_t)fq[r] | ((uint32_t)fq[r + NH] << 16);
-#pragma unroll
-      for (int d = KD - 1; d >= 1; --d) {  // deepest level first
-        constexpr int dummy = 0;
-        (void)dummy;
-        const int step = 1 << (M - 7 - d);
-        const int phoff = (1 << (M - 6)) - (1 << (M - 6 - d));
-#pragma unroll
-        for (int j = 0; j < step; ++j) {
-          const uint32_t lp = lpq[64 * (phoff + j)];
-#pragma unroll
-          for (int r = j; r < NH; r += 2 * step) {
-            const uint32_t y2 = V2[r + step];
-            const uint32_t plo_ = tab(ex_s, tab(lg_s, y2 & 0xFFFFu) + lp);
-            const uint32_t phi_ = tab(ex_s, tab(lg_s, y2 >> 16) + lp);
-            const uint32_t x2 = V2[r] ^ (plo_ | (phi_ << 16));
-            V2[r] = x2;
-            V2[r + step] = x2 ^ y2;
+      static_assert(NG == ZPL, "one zero-flag word per 32-point group");
+      uint32_t zm[ZPL];
+      if constexpr (!BS) {
+        // bottom-up butterflies in registers: V2[r] packs point 64r+lane (low half) and point
+        // 64(r+NR/2)+lane (high half); the two butterflies of a packed pair share their multiplier
+        constexpr int NH = NR / 2;
+        const int zo = opaque_zero();
+        const uint16_t *lpq = lpt_s + zo + lane, *ptq = pthi_s + zo, *plq = plo_s + zo;
+        uint32_t V2[NH];
+  #pragma unroll
+        for (int r = 0; r < NH; ++r) V2[r] = (uint32_t)fq[r] | ((uint32_t)fq[r + NH] << 16);
+  #pragma unroll
+        for (int d = KD - 1; d >= 1; --d) {  // deepest level first
+          constexpr int dummy = 0;
+          (void)dummy;
+          const int step = 1 << (M - 7 - d);
+          const int phoff = (1 << (M - 6)) - (1 << (M - 6 - d));
+  #pragma unroll
+          for (int j = 0; j < step; ++j) {
+            const uint32_t lp = lpq[64 * (phoff + j)];
+  #pragma unroll
+            for (int r = j; r < NH; r += 2 * step) {
+              const uint32_t y2 = V2[r + step];
+              const uint32_t plo_ = tab(ex_s, tab(lg_s, y2 & 0xFFFFu) + lp);
+              const uint32_t phi_ = tab(ex_s, tab(lg_s, y2 >> 16) + lp);
+              const uint32_t x2 = V2[r] ^ (plo_ | (phi_ << 16));
+              V2[r] = x2;
+              V2[r + step] = x2 ^ y2;
+            }
           }
         }
-      }
-      {  // level 0 (half = 2^(m-1)): the pair is (low, high) of one register
-#pragma unroll
-        for (int r = 0; r < NH; ++r) {
-          const uint32_t lp = lpq[64 * r];
-          const uint32_t x = V2[r] & 0xFFFFu, y = V2[r] >> 16;
-          const uint32_t xn = x ^ tab(ex_s, tab(lg_s, y) + lp);
-          V2[r] = xn | ((xn ^ y) << 16);
+        {  // level 0 (half = 2^(m-1)): the pair is (low, high) of one register
+  #pragma unroll
+          for (int r = 0; r < NH; ++r) {
+            const uint32_t lp = lpq[64 * r];
+            const uint32_t x = V2[r] & 0xFFFFu, y = V2[r] >> 16;
+            const uint32_t xn = x ^ tab(ex_s, tab(lg_s, y) + lp);
+            V2[r] = xn | ((xn ^ y) << 16);
+          }
         }
-      }
-      // zero flags of sigma (adding the monic x^t term where it lies outside the FFT), transposed:
-      // bit r of zf[w] in lane l flags the point 64 g + l of group g = 32 w + r, where the low
-      // halves of V2 are groups 0..NH-1 and the high halves groups NH..2NH-1; masked by the
-      // support flags in the same layout, then counted (the weight of e is the sum over the wave)
-      static_assert(NH % 32 == 0 && 2 * NH == 32 * ZPL, "mask words per lane");
-      uint32_t zm[ZPL];
+        // zero flags of sigma (adding the monic x^t term where it lies outside the FFT), transposed:
+        // bit r of zf[w] in lane l flags the point 64 g + l of group g = 32 w + r, where the low
+        // halves of V2 are groups 0..NH-1 and the high halves groups NH..2NH-1; masked by the
+        // support flags in the same layout, then counted (the weight of e is the sum over the wave)
+        static_assert(NH % 32 == 0 && 2 * NH == 32 * ZPL, "mask words per lane");
+  #pragma unroll
+        for (int z = 0; z < ZPL; ++z) zm[z] = 0;
+        {
+          const uint32_t ptlo = plq[64 * KD + lane];
+          const uint32_t ptlo2 = ptlo | (ptlo << 16);
+  #pragma unroll
+          for (int r = 0; r < NH; ++r) {
+            uint32_t v = V2[r];
+            if (P::TOP_TERM) v = xor3(v, ptlo2, (uint32_t)ptq[r] | ((uint32_t)ptq[r + NH] << 16));
+            // x - 1 has bit 31 set exactly for x = 0 (field values are below 2^15)
+            zm[r / 32] |= (((v & 0xFFFFu) - 1u) >> 31) << (r % 32);
+            zm[(r + NH) / 32] |= (((v >> 16) - 1u) >> 31) << (r % 32);
+          }
+        }
+      } else {
+        // bitsliced bottom-up: Vb[G][b] bit i = bit b of the value at point 64 (32 G + i) + lane
+        const int zo = opaque_zero();
+        const uint16_t *plq = plo_s + zo;
+        const uint32_t *phq = phib_s + zo;
+        uint32_t Vb[NG][M];
+        // the D sub-polynomial constants (lane-uniform, doubled) into bit planes: lane l reads
+        // constant 64 hh + l and one ballot per bit gives two 32-point words
 #pragma unroll
-      for (int z = 0; z < ZPL; ++z) zm[z] = 0;
-      {
-        const uint32_t ptlo = plq[64 * KD + lane];
-        const uint32_t ptlo2 = ptlo | (ptlo << 16);
+        for (int hh = 0; hh < NG / 2; ++hh) {
+          const uint32_t v = fq[64 * hh + lane];
 #pragma unroll
-        for (int r = 0; r < NH; ++r) {
-          uint32_t v = V2[r];
-          if (P::TOP_TERM) v = xor3(v, ptlo2, (uint32_t)ptq[r] | ((uint32_t)ptq[r + NH] << 16));
-          // x - 1 has bit 31 set exactly for x = 0 (field values are below 2^15)
-          zm[r / 32] |= (((v & 0xFFFFu) - 1u) >> 31) << (r % 32);
-          zm[(r + NH) / 32] |= (((v >> 16) - 1u) >> 31) << (r % 32);
+          for (int b = 0; b < M; ++b) {
+            const uint64_t bal = __ballot((v >> (b + 1)) & 1u);
+            Vb[2 * hh][b] = (uint32_t)bal;
+            Vb[2 * hh + 1][b] = (uint32_t)(bal >> 32);
+          }
+        }
+#pragma unroll
+        for (int d = KD - 1; d >= 0; --d) {  // deepest level first
+          const int h = 1 << (M - 7 - d);
+          const uint32_t plo2 = plq[64 * d + lane];
+          if (h < 32) {  // pairs (i, i + h) inside each 32-point word
+            uint32_t C[M];
+            bs_multipliers<M>(phq + d * M, plo2, C);
+            uint32_t Mh = 0;  // bits i with (i & h) != 0
+#pragma unroll
+            for (int i = 0; i < 32; ++i) Mh |= (uint32_t)((i & h) != 0) << i;
+#pragma unroll
+            for (int pg = 0; pg < NG; pg += 2) {
+              // y halves of words pg (shifted down to the x positions) and pg + 1 (in place) in
+              // one word, then one 32-wide product
+              uint32_t Y[M], Z[M];
+#pragma unroll
+              for (int b = 0; b < M; ++b) Y[b] = select_bits(Mh, Vb[pg + 1][b], Vb[pg][b] >> h);
+              bs_mul<M>(Y, C, Z);
+#pragma unroll
+              for (int b = 0; b < M; ++b) {
+                uint32_t x0 = andn_xor(Vb[pg][b], Z[b], Mh);          // x' = x + c y
+                Vb[pg][b] = and_xor(x0, x0 << h, Mh);                 // y' = x' + y
+                uint32_t x1 = andn_xor(Vb[pg + 1][b], Z[b] >> h, Mh);
+                Vb[pg + 1][b] = and_xor(x1, x1 << h, Mh);
+              }
+            }
+          } else {  // pairs of words (G, G + h/32)
+            const int gap = h / 32;
+#pragma unroll
+            for (int G = 0; G < NG; ++G) {
+              if (G & gap) continue;
+              uint32_t C[M], Z[M];
+              bs_multipliers<M>(phq + (d + KD * (G % gap)) * M, plo2, C);
+              bs_mul<M>(Vb[G + gap], C, Z);
+#pragma unroll
+              for (int b = 0; b < M; ++b) {
+                Vb[G][b] ^= Z[b];
+                Vb[G + gap][b] ^= Vb[G][b];
+              }
+            }
+          }
+        }
+        if constexpr (P::TOP_TERM) {  // + x^t = (lane)^t + (64 R)^t (t a power of two)
+          const uint32_t ptlo2 = plq[64 * KD + lane];
+          const uint32_t *ptq = pthib_s + zo;
+#pragma unroll
+          for (int b = 0; b < M; ++b) {
+            const uint32_t msk = (uint32_t)(-(int)((ptlo2 >> (b + 1)) & 1u));
+#pragma unroll
+            for (int G = 0; G < NG; ++G) Vb[G][b] = xor3(Vb[G][b], ptq[G * M + b], msk);
+          }
+        }
+#pragma unroll
+        for (int G = 0; G < NG; ++G) {  // zero flags: no bit set in any plane
+          uint32_t o = Vb[G][0];
+#pragma unroll
+          for (int b = 1; b < M; ++b) o |= Vb[G][b];
+          zm[G] = ~o;
         }
       }
       uint32_t cnt = 0;
@@ -637,7 +797,10 @@ hipError_t launch_dec(const DecArgs &args, hipStream_t s) {
     const hipError_t e = launch_fill_u32(a.queue, 0u, DEC_QUEUES * DEC_QSTRIDE, s);  // a kernel, not a memset
     if (e != hipSuccess) return e;
   }
-  TMCE_TIMED("dec_decode", s, k_dec_decode<SET><<<grid, DEC_WG, 0, s>>>(a));
+  if (a.fft_form)
+    TMCE_TIMED("dec_decode", s, k_dec_decode<SET, true><<<grid, DEC_WG, 0, s>>>(a));
+  else
+    TMCE_TIMED("dec_decode", s, k_dec_decode<SET, false><<<grid, DEC_WG, 0, s>>>(a));
   if (a.pc)
     TMCE_TIMED("dec_hash", s,
                k_hash_msg<SET, true><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, args.ct, a.key, a.status, a.nops,

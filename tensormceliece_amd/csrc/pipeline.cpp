@@ -35,7 +35,7 @@ struct tmce_pipeline {
   hipGraphExec_t exec = nullptr;
   bool warm = false;                // one eager run precedes the capture (first-use set-up, e.g.
                                     // kernel attributes, stays out of the captured stream)
-  int tuned[5] = {-1, -1, -1, -1, -1};  // knobs the captured graph was built with
+  int tuned[6] = {-1, -1, -1, -1, -1, -1};  // knobs the captured graph was built with
 };
 
 namespace tmce {
@@ -45,7 +45,7 @@ namespace {
 size_t plan_image_bytes(int max_sub) { return round_up((int)sizeof(PlanDev), 16) + (size_t)max_sub * sizeof(SubDev); }
 
 // the stage launches of one run (everything after the plan upload)
-int issue(tmce_pipeline *p, const int knobs[5]) {
+int issue(tmce_pipeline *p, const int knobs[6]) {
   Engine *e = g_engine;
   const SetInfo &si = set_info(p->set);
   hipStream_t s = p->stream;
@@ -85,6 +85,7 @@ int issue(tmce_pipeline *p, const int knobs[5]) {
     a.decode_blocks = knobs[1];
     a.front_mfma = knobs[2];
     a.front_cfg = knobs[3];
+    a.fft_form = knobs[5];
     a.xscratch = p->xscratch.as<uint32_t>();
     a.plan = p->plan_dev;
     HIPCHK(launch_dec_set(p->set, a, s));
@@ -269,7 +270,7 @@ int tmce_pipeline_run(tmce_pipeline *p, const tmce_sub_batch *subs, size_t nsub)
   if (total == 0) return 0;
 
   // ---- launches: replay the captured graph, or capture / issue them
-  int knobs[5];
+  int knobs[6];
   {
     std::lock_guard<std::mutex> lk(e->mu);
     knobs[0] = e->gen_blocks[p->set];
@@ -277,6 +278,7 @@ int tmce_pipeline_run(tmce_pipeline *p, const tmce_sub_batch *subs, size_t nsub)
     knobs[2] = e->front_mfma[p->set];
     knobs[3] = e->front_cfg[p->set];
     knobs[4] = e->gen_form[p->set];
+    knobs[5] = e->fft_form[p->set];
   }
   if (!p->use_graph || !p->warm) {
     p->warm = true;

@@ -174,3 +174,37 @@ def test_sampler_forms(tm, orc, s):
     idx = rng.choice(len(seeds), 48, replace=False)
     ct_ref, k_ref, _ = orc.enc_batch(s, pk, seeds[idx])
     assert np.array_equal(out[1][0][idx], ct_ref) and np.array_equal(out[1][1][idx], k_ref)
+
+
+@pytest.mark.parametrize("s", SETS)
+def test_decoder_fft_forms_agree(tm, orc, s):
+    """The decoder's two bottom-up FFT forms -- bitsliced GF(2^m) arithmetic on the VALU and
+    log/exp table lookups in LDS -- give the same keys and statuses on valid, tampered and random
+    ciphertexts (ragged batch), and equal the oracle on a sample."""
+    pk, sk, _ = keys(orc, s)
+    p = orc.params(s)
+    seeds = seeds_for(orc, s, 1500)
+    ct, k_enc, _ = tm.PublicKey(s, pk).enc(seeds)
+    rng = np.random.default_rng(61)
+    cts = ct.copy()
+    tam = rng.choice(len(cts), 200, replace=False)
+    cts[tam, rng.integers(0, p.synd_bytes, 200)] ^= rng.integers(1, 256, 200).astype(np.uint8)
+    rnd = rng.choice(len(cts), 100, replace=False)
+    cts[rnd] = rng.integers(0, 256, (100, p.synd_bytes), dtype=np.uint8)
+    cts = cts[:1337]
+    SK = tm.SecretKey(s, sk)
+    form0 = tm.tune_get(s, "fft_form")
+    out = {}
+    try:
+        for form in (0, 1):
+            tm.tune_set(s, "fft_form", form)
+            print(f"decoder fft_form={form} {s}", flush=True)
+            out[form] = SK.dec(cts)
+    finally:
+        tm.tune_set(s, "fft_form", form0)
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    valid = np.setdiff1d(np.arange(1337), np.concatenate([tam, rnd]))
+    assert np.array_equal(out[1][0][valid], k_enc[valid])
+    idx = np.concatenate([tam[tam < 1337][:6], rnd[rnd < 1337][:6], valid[:4]])
+    k_o, st_o = orc.dec_batch(s, sk, cts[idx])
+    assert np.array_equal(out[1][0][idx], k_o) and np.array_equal(out[1][1][idx], st_o)
