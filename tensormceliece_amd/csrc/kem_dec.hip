@@ -22,8 +22,22 @@
 namespace tmce {
 
 constexpr int FRONT_WG = 256;
-constexpr int DEC_WG = 512;
-constexpr int DEC_WAVES = DEC_WG / 64;
+#ifndef DEC_WAVES_M12
+#define DEC_WAVES_M12 8
+#endif
+#ifndef DEC_WAVES_M13
+#define DEC_WAVES_M13 12
+#endif
+// decoder workgroups: the per-set field tables in LDS are shared by the workgroup's waves, so the
+// wave count per workgroup is what sets the waves per SIMD.  m = 12 (tables 40 KB): 8-wave
+// workgroups, two per CU, four waves per SIMD (at most 128 VGPRs); m = 13 (tables 80 KB): one
+// 12-wave workgroup per CU, three waves per SIMD (at most 168 VGPRs); the table form of the FFT
+// (BS false) keeps 8 waves for m = 13, its multiplier-log table does not leave LDS for 12
+template <int SET, bool BS> struct DecShape {
+  static constexpr int WAVES = Par<SET>::M == 12 ? DEC_WAVES_M12 : (BS ? DEC_WAVES_M13 : 8);
+  static constexpr int WG = 64 * WAVES;
+  static constexpr int WAVES_PER_EU = Par<SET>::M == 12 ? 4 : (BS ? 3 : 2);
+};
 // decoder work counters: DEC_QUEUES of them, DEC_QSTRIDE words apart (separate 128-byte lines)
 constexpr int DEC_QUEUES = 8, DEC_QSTRIDE = 32;
 
@@ -243,8 +257,9 @@ __device__ __forceinline__ void bs_multipliers(const uint32_t *ph, uint32_t plo2
 // word and bit plane, the GF(2^m) products by AND-XOR on the VALU -- instead of log/exp table
 // gathers in LDS (fft_form 0), whose random 16-bit reads are mostly LDS bank conflicts.
 template <int SET, bool BS>
-__global__ __launch_bounds__(DEC_WG) __attribute__((amdgpu_waves_per_eu(Par<SET>::M == 12 ? 4 : 1)))
+__global__ __launch_bounds__((DecShape<SET, BS>::WG)) __attribute__((amdgpu_waves_per_eu(DecShape<SET, BS>::WAVES_PER_EU)))
 void k_dec_decode(DecArgs a) {
+  constexpr int DEC_WG = DecShape<SET, BS>::WG, DEC_WAVES = DecShape<SET, BS>::WAVES;
   using P = Par<SET>;
   constexpr int M = P::M, T = P::T, Dn = P::D, KD = P::KD, Q = P::Q;
   constexpr int NPTS = 1 << M;
@@ -313,10 +328,10 @@ void k_dec_decode(DecArgs a) {
   }
   __syncthreads();
   const int nops = a.plan ? a.plan->total : a.nops;
-  const int wave = tid >> 6, lane = tid & 63, row = lane >> 4, l16 = lane & 15;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane0 = tid & 63;  // wave: uniform (scalar)
   constexpr int SPAD = DecWave<SET>::SPAD;
   DecWave<SET> &w = Wv[wave];
-  for (int i = lane; i < 4 * SPAD; i += 64) w.s[i / SPAD][i % SPAD] = 2 * ZL;  // window padding (zero)
+  for (int i = lane0; i < 4 * SPAD; i += 64) w.s[i / SPAD][i % SPAD] = 2 * ZL;  // window padding (zero)
 
   // the wave takes batches of four ops from work counters, one per XCD (workgroups are dispatched
   // to the 8 XCDs round-robin; counter x hands out batches x, x + 8, x + 16, ...), and from the
@@ -325,6 +340,10 @@ void k_dec_decode(DecArgs a) {
   const int nbat = (nops + 3) >> 2;
   int xq = blockIdx.x & (DEC_QUEUES - 1), tried = 0;
   for (;;) {
+    // the lane index re-derived in every batch through an opaque zero: values computed from it
+    // (per-lane addresses, bounds, masks) are then recomputed per batch instead of being hoisted
+    // out of the loop and kept live (or spilled) for the whole kernel
+    const int lane = lane0 + opaque_zero(), row = lane >> 4, l16 = lane & 15;
     int bidx = nbat;
     while (tried < DEC_QUEUES) {
       unsigned got = 0;
@@ -412,6 +431,7 @@ void k_dec_decode(DecArgs a) {
     for (int q = 0; q < 4; ++q) {
       const int op = base + q;
       if (op >= nops) break;
+      const int lane = lane0 + opaque_zero();  // per op, as above: no per-lane masks hoisted out
       // key material of the op (one key, or its key group under a plan)
       const uint16_t *k_ainv = a.ainv;
       const uint32_t *k_supp = a.supp;
@@ -791,16 +811,17 @@ hipError_t launch_dec(const DecArgs &args, hipStream_t s) {
                    a.ct, reinterpret_cast<const uint4 *>(a.F), a.S, a.nops, a.plan));
   }
   int grid = a.decode_blocks > 0 ? a.decode_blocks : (P::M == 12 ? 512 : 256);
-  const int maxgrid = ceil_div(a.nops, DEC_WAVES * 4);
+  const int dec_waves = a.fft_form ? DecShape<SET, true>::WAVES : DecShape<SET, false>::WAVES;
+  const int maxgrid = ceil_div(a.nops, dec_waves * 4);
   if (grid > maxgrid) grid = maxgrid;
   {
     const hipError_t e = launch_fill_u32(a.queue, 0u, DEC_QUEUES * DEC_QSTRIDE, s);  // a kernel, not a memset
     if (e != hipSuccess) return e;
   }
   if (a.fft_form)
-    TMCE_TIMED("dec_decode", s, k_dec_decode<SET, true><<<grid, DEC_WG, 0, s>>>(a));
+    TMCE_TIMED("dec_decode", s, k_dec_decode<SET, true><<<grid, DecShape<SET, true>::WG, 0, s>>>(a));
   else
-    TMCE_TIMED("dec_decode", s, k_dec_decode<SET, false><<<grid, DEC_WG, 0, s>>>(a));
+    TMCE_TIMED("dec_decode", s, k_dec_decode<SET, false><<<grid, DecShape<SET, false>::WG, 0, s>>>(a));
   if (a.pc)
     TMCE_TIMED("dec_hash", s,
                k_hash_msg<SET, true><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, args.ct, a.key, a.status, a.nops,
