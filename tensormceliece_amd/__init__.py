@@ -13,7 +13,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtmce.so")
+# TMCE_LIBRARY selects another build of the library (for example one compiled with different
+# launch-shape macros) in place of the in-tree libtmce.so
+LIB_PATH = os.environ.get("TMCE_LIBRARY") or os.path.join(_HERE, "libtmce.so")
 
 BASE_SETS = {"348864": 0, "460896": 1, "6688128": 2, "6960119": 3, "8192128": 4}
 # variant flags (TMCE_VARIANT_F / TMCE_VARIANT_PC): f = semi-systematic key generation,

@@ -28,6 +28,9 @@ constexpr int FRONT_WG = 256;
 #ifndef DEC_WAVES_M13
 #define DEC_WAVES_M13 12
 #endif
+#ifndef DEC_REENC_LOADS
+#define DEC_REENC_LOADS 8
+#endif
 // decoder workgroups: the per-set field tables in LDS are shared by the workgroup's waves, so the
 // wave count per workgroup is what sets the waves per SIMD.  m = 12 (tables 40 KB): 8-wave
 // workgroups, two per CU, four waves per SIMD (at most 128 VGPRs); m = 13 (tables 80 KB): one
@@ -674,12 +677,19 @@ void k_dec_decode(DecArgs a) {
         const uint2 *fc = reinterpret_cast<const uint2 *>(k_fcol);
         const int li = lane < RH ? lane : RH - 1;
         uint2 acc = make_uint2(0, 0);
-#pragma unroll 8
-        for (int k = 0; k < T; ++k) {
-          const uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)pos_r[k >> 6], k & 63);
-          const uint2 v = global_load8(fc, (size_t)p * RH + li);
-          acc.x ^= v.x;
-          acc.y ^= v.y;
+        // DEC_REENC_LOADS column loads in flight per pass; a rolled outer loop (for t = 119 the compiler would
+        // otherwise unroll all t passes and hold every load in registers at once)
+#pragma unroll 1
+        for (int k0 = 0; k0 < T; k0 += DEC_REENC_LOADS) {
+#pragma unroll
+          for (int kk = 0; kk < DEC_REENC_LOADS; ++kk) {
+            const int k = k0 + kk;
+            if (T % DEC_REENC_LOADS != 0 && k >= T) break;
+            const uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)pos_r[k >> 6], k & 63);
+            const uint2 v = global_load8(fc, (size_t)p * RH + li);
+            acc.x ^= v.x;
+            acc.y ^= v.y;
+          }
         }
         bool mism = false;
         if (lane < RH) {
