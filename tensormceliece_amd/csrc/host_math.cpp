@@ -74,121 +74,72 @@ void perm_from_controlbits(int m, const uint8_t *cb, std::vector<uint16_t> &pi) 
   }
 }
 
-// Sort of 32-bit keys: LSD radix (three 11-bit digits) for large arrays, std::sort for small ones;
-// tmp has a.size() entries
-// LSD radix sort of n 32-bit keys on 8-bit digits.  With `index_low` the low 16 bits of a[x] are
-// x itself (ascending), so sorting on the high 16 bits alone, stably, gives the full order.
-static void sort32(uint32_t *a, uint32_t *tmp, size_t n, bool index_low) {
-  if (n < 64) {
-    std::sort(a, a + n);
-    return;
-  }
-  uint32_t *src = a, *dst = tmp;
-  for (int sh = index_low ? 16 : 0; sh < 32; sh += 8) {
-    uint32_t cnt[257] = {0};
-    for (size_t i = 0; i < n; ++i) ++cnt[((src[i] >> sh) & 255u) + 1];
-    if (cnt[((src[0] >> sh) & 255u) + 1] == n) continue;  // one digit value: already ordered here
-    for (int d = 0; d < 256; ++d) cnt[d + 1] += cnt[d];
-    for (size_t i = 0; i < n; ++i) dst[cnt[(src[i] >> sh) & 255u]++] = src[i];
-    std::swap(src, dst);
-  }
-  if (src != a) std::memcpy(a, src, n * sizeof(uint32_t));
-}
-
-// Per-level work arrays of the recursion (level w holds n = 2^w entries), kept per thread.
+// Control bits of the Benes network for a permutation pi of n = 2^w indices (the bits libmceliece
+// stores in sk, LM:crypto_kem/<set>/vec/controlbits.c; the formulas of Bernstein, "Verified fast
+// formulas for control bits for permutation networks"), computed here directly from their
+// definition with index arrays:
+//   sigma(y) = pi(pi^-1(y ^ 1) ^ 1), whose cycles have length at most n/2;
+//   c(x)     = the smallest element of x's cycle under sigma;
+//   first layer: bit f_j = c(2j) mod 2, and F swaps the pair (2j, 2j + 1) when f_j = 1;
+//   last layer:  bit l_k = F(pi(2k)) mod 2, and L swaps the pair (2k, 2k + 1) when l_k = 1;
+//   middle:      M = F o pi o L, and the two half networks route q_b(j) = M(2j + b) / 2 (b = 0, 1).
+// The first layer's bits sit at pos + j step, the last layer's at pos + (2w - 2)(n/2) step + k step,
+// and the two halves continue at pos + (n/2) step and that + step, with twice the step.
+namespace {
 struct CbWork {
-  std::vector<uint32_t> A[16], B[16], T[16];
-  std::vector<uint16_t> Q[16];
-  void ensure(int w, size_t n) {
-    if (A[w].size() < n) {
-      A[w].resize(n);
-      B[w].resize(n);
-      T[w].resize(n);
-      Q[w].resize(n);
-    }
-  }
+  std::vector<uint16_t> inv, sig, cmin, q[16];
+  std::vector<uint8_t> seen;
 };
 
-// The recursion of LM:crypto_kem/<set>/vec/controlbits.c (cbrecursion).  Every sort below is on
-// distinct non-negative 32-bit keys, so any correct sort reproduces the library's order exactly.
-static void cbrec(uint8_t *out, long long pos, long long step, const uint16_t *pi, int w, size_t n, CbWork &wk) {
+void cb_level(uint8_t *out, long long pos, long long step, const uint16_t *pi, int w, CbWork &wk) {
   auto flip = [&](long long p, uint32_t b) { out[p >> 3] ^= (uint8_t)((b & 1u) << (p & 7)); };
-  if (w == 1) { flip(pos, pi[0]); return; }
-  wk.ensure(w, n);
-  uint32_t *A = wk.A[w].data(), *B = wk.B[w].data(), *tmp = wk.T[w].data();
-  for (size_t x = 0; x < n; ++x) A[x] = ((uint32_t)(pi[x] ^ 1u) << 16) | pi[x ^ 1];
-  sort32(A, tmp, n, false);
-  for (size_t x = 0; x < n; ++x) {
-    const uint32_t px = A[x] & 0xffffu;
-    B[x] = (px << 16) | std::min<uint32_t>(px, (uint32_t)x);
+  if (w == 1) {
+    flip(pos, pi[0]);
+    return;
   }
-  for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 16) | (uint32_t)x;
-  sort32(A, tmp, n, true);
-  for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 16) + (B[x] >> 16);
-  sort32(A, tmp, n, false);
-  if (w <= 10) {
-    for (size_t x = 0; x < n; ++x) B[x] = ((A[x] & 0xffffu) << 10) | (B[x] & 0x3ffu);
-    for (int i = 1; i < w - 1; ++i) {
-      for (size_t x = 0; x < n; ++x) A[x] = ((B[x] & ~0x3ffu) << 6) | (uint32_t)x;
-      sort32(A, tmp, n, true);
-      for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 20) | B[x];
-      sort32(A, tmp, n, false);
-      for (size_t x = 0; x < n; ++x)
-        B[x] = std::min<uint32_t>((A[x] & 0xffc00u) | (B[x] & 0x3ffu), A[x] & 0xfffffu);
+  const size_t n = size_t(1) << w, half = n / 2;
+  wk.inv.resize(n);
+  wk.sig.resize(n);
+  wk.cmin.resize(n);
+  wk.seen.assign(n, 0);
+  uint16_t *inv = wk.inv.data(), *sig = wk.sig.data(), *cmin = wk.cmin.data();
+  for (size_t x = 0; x < n; ++x) inv[pi[x]] = (uint16_t)x;
+  for (size_t y = 0; y < n; ++y) sig[y] = pi[inv[y ^ 1] ^ 1];
+  for (size_t x = 0; x < n; ++x) {  // cycle minima: one walk per cycle
+    if (wk.seen[x]) continue;
+    uint16_t mn = (uint16_t)x;
+    for (size_t y = sig[x]; y != x; y = sig[y]) mn = std::min<uint16_t>(mn, (uint16_t)y);
+    cmin[x] = mn;
+    wk.seen[x] = 1;
+    for (size_t y = sig[x]; y != x; y = sig[y]) {
+      cmin[y] = mn;
+      wk.seen[y] = 1;
     }
-    for (size_t x = 0; x < n; ++x) B[x] &= 0x3ffu;
-  } else {
-    for (size_t x = 0; x < n; ++x) B[x] = (A[x] << 16) | (B[x] & 0xffffu);
-    for (int i = 1; i < w - 1; ++i) {
-      for (size_t x = 0; x < n; ++x) A[x] = (B[x] & ~0xffffu) | (uint32_t)x;
-      sort32(A, tmp, n, true);
-      for (size_t x = 0; x < n; ++x) A[x] = (A[x] << 16) | (B[x] & 0xffffu);
-      if (i < w - 2) {
-        for (size_t x = 0; x < n; ++x) B[x] = (A[x] & ~0xffffu) | (B[x] >> 16);
-        sort32(B, tmp, n, false);
-        for (size_t x = 0; x < n; ++x) B[x] = (B[x] << 16) | (A[x] & 0xffffu);
-      }
-      sort32(A, tmp, n, false);
-      for (size_t x = 0; x < n; ++x) B[x] = std::min<uint32_t>(B[x], (B[x] & ~0xffffu) | (A[x] & 0xffffu));
-    }
-    for (size_t x = 0; x < n; ++x) B[x] &= 0xffffu;
   }
-  for (size_t x = 0; x < n; ++x) A[x] = ((uint32_t)pi[x] << 16) + (uint32_t)x;
-  sort32(A, tmp, n, true);
-  for (size_t j = 0; j < n / 2; ++j) {
-    const size_t z = 2 * j;
-    const uint32_t fj = B[z] & 1u, Fz = (uint32_t)z + fj;
-    flip(pos, fj);
-    pos += step;
-    B[z] = (A[z] << 16) | Fz;
-    B[z + 1] = (A[z + 1] << 16) | (Fz ^ 1u);
+  std::vector<uint16_t> &q = wk.q[w];
+  q.resize(n);
+  // f and l bits; F and L as XOR masks on the low index bit
+  auto F = [&](uint32_t x) { return x ^ (cmin[x & ~1u] & 1u); };
+  const long long last = pos + (long long)(2 * w - 2) * (long long)half * step;
+  for (size_t j = 0; j < half; ++j) flip(pos + (long long)j * step, cmin[2 * j]);
+  for (size_t k = 0; k < half; ++k) {
+    const uint32_t lk = F(pi[2 * k]) & 1u;
+    flip(last + (long long)k * step, lk);
+    // M(2k + b) = F(pi(L(2k + b))) = F(pi(2k + (b ^ lk)))
+    q[k] = (uint16_t)(F(pi[2 * k + lk]) >> 1);
+    q[half + k] = (uint16_t)(F(pi[2 * k + (lk ^ 1u)]) >> 1);
   }
-  sort32(B, tmp, n, false);
-  pos += (2 * w - 3) * step * (long long)(n / 2);
-  for (size_t k = 0; k < n / 2; ++k) {
-    const size_t y = 2 * k;
-    const uint32_t lk = B[y] & 1u, Ly = (uint32_t)y + lk;
-    flip(pos, lk);
-    pos += step;
-    A[y] = (Ly << 16) | (B[y] & 0xffffu);
-    A[y + 1] = ((Ly ^ 1u) << 16) | (B[y + 1] & 0xffffu);
-  }
-  sort32(A, tmp, n, false);
-  pos -= (2 * w - 2) * step * (long long)(n / 2);
-  uint16_t *q = wk.Q[w].data();
-  for (size_t j = 0; j < n / 2; ++j) {
-    q[j] = (uint16_t)((A[2 * j] & 0xffffu) >> 1);
-    q[j + n / 2] = (uint16_t)((A[2 * j + 1] & 0xffffu) >> 1);
-  }
-  cbrec(out, pos, step * 2, q, w - 1, n / 2, wk);
-  cbrec(out, pos + step, step * 2, q + n / 2, w - 1, n / 2, wk);
+  const long long next = pos + (long long)half * step;
+  cb_level(out, next, 2 * step, q.data(), w - 1, wk);
+  cb_level(out, next + step, 2 * step, q.data() + half, w - 1, wk);
 }
+}  // namespace
 
 void controlbits_from_perm(int m, const std::vector<uint16_t> &pi, uint8_t *cb) {
   const size_t n = size_t(1) << m;
   std::memset(cb, 0, ((2 * m - 1) * n / 2 + 7) / 8);
   static thread_local CbWork wk;
-  cbrec(cb, 0, 1, pi.data(), m, n, wk);
+  cb_level(cb, 0, 1, pi.data(), m, wk);
 }
 
 // ------------------------------------------------------------------ additive FFT tables
