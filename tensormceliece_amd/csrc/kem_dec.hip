@@ -157,8 +157,8 @@ __device__ __forceinline__ void bm_step(int N, const uint8_t *wp, int l16, uint3
   }
   const uint32_t d = row_xor16(dp);
   const uint32_t ld = tab(lg_s, d);
-  uint32_t lf = ld + Q2 - lb;  // log (d / b)
-  lf = lf >= Q2 ? lf - Q2 : lf;
+  uint32_t lf = ld + Q2 - lb;  // log (d / b): ld - lb mod Q2, the subtraction wrapping when lf < Q2
+  lf = min(lf, lf - Q2);
   lf = d != 0 ? lf : ZL2;
   const bool upd = d != 0 && 2 * L <= N;
 #pragma unroll
@@ -179,24 +179,34 @@ __device__ __forceinline__ void bm_step(int N, const uint8_t *wp, int l16, uint3
     Bl[s] = (16 * s + l16 <= T) ? v : ZL2;
   }
 }
-// 1 + (max over the wave's four rows of max(L, N + 1 - L)) / 16: the active slot count of step N
-__device__ __forceinline__ int bm_slots(int L, int N) {
+// A(N) = max over the wave's four rows of max(L, N + 1 - L): step N touches coefficients 0..A(N),
+// i.e. A(N) / 16 + 1 slots.  A grows by at most one per step (without a length change
+// max(L, N + 2 - L) <= A + 1; with one, max(N + 1 - L, L + 1) <= A + 1).
+__device__ __forceinline__ int bm_active(int L, int N) {
   const int L0 = __builtin_amdgcn_readlane(L, 0), L1 = __builtin_amdgcn_readlane(L, 16);
   const int L2 = __builtin_amdgcn_readlane(L, 32), L3 = __builtin_amdgcn_readlane(L, 48);
   const int Lmax = max(max(L0, L1), max(L2, L3)), Lmin = min(min(L0, L1), min(L2, L3));
-  return (max(Lmax, N + 1 - Lmin) >> 4) + 1;
+  return max(Lmax, N + 1 - Lmin);
 }
-// Berlekamp-Massey steps N.. while the active slot count is NA, then the next count: the count
-// never decreases (max(L, N + 1 - L) does not, in every row), so each count runs as its own loop
+// Berlekamp-Massey steps N.. while NA slots cover the active coefficients, then the next count
+// (the count never decreases).  Since A grows by at most one per step, NA slots (coefficients
+// 0..16 NA - 1) stay enough for 16 NA - A(N) more steps: those run without re-reading the lengths.
 template <int NA, int NSB, int T, int Q>
 __device__ __forceinline__ void bm_run(int &N, const uint8_t *wl, int l16, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
                                        int &L, uint32_t &lb, const uint16_t *lg_s, const uint16_t *ex_s) {
-  bool go = N < 2 * T && (NA == NSB || bm_slots(L, N) <= NA);
 #pragma unroll 1
-  while (go) {
-    bm_step<NA, NSB, T, Q>(N, wl + 2 * N, l16, C, Bl, L, lb, lg_s, ex_s);
-    ++N;
-    go = N < 2 * T && (NA == NSB || bm_slots(L, N) <= NA);
+  while (N < 2 * T) {
+    int rem = 2 * T - N;
+    if constexpr (NA < NSB) {
+      const int A = bm_active(L, N);
+      if (A > 16 * NA - 1) break;
+      rem = min(rem, 16 * NA - A);
+    }
+#pragma unroll 1
+    for (int k = 0; k < rem; ++k) {
+      bm_step<NA, NSB, T, Q>(N, wl + 2 * N, l16, C, Bl, L, lb, lg_s, ex_s);
+      ++N;
+    }
   }
   if constexpr (NA < NSB) bm_run<NA + 1, NSB, T, Q>(N, wl, l16, C, Bl, L, lb, lg_s, ex_s);
 }
