@@ -42,6 +42,7 @@ int engine_init(int device) {
   HIPCHK(hipSetDevice(device));
   auto e = std::make_unique<Engine>();
   e->device = device;
+  e->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&e->use_enc.done, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&e->use_dec.done, hipEventDisableTiming));
@@ -261,6 +262,7 @@ int dec_launch(Engine *e, tmce_key *k, uint8_t *d_k, int *d_status, const uint8_
   a.queue = x.queue;
   a.nops = (int)n;
   a.decode_blocks = e->decode_blocks[k->set];
+  a.cus = e->cus;
   a.fft_form = e->fft_form[k->set];
   if (form == 1) {
     HIPCHK(launch_plain_dec_set(k->set, a, *px, s));

@@ -121,6 +121,7 @@ struct ScratchUse {
 
 struct Engine {
   int device = -1;
+  int cus = 256;  // compute units
   hipStream_t stream = nullptr;
   std::mutex mu;
   uint16_t *lg[2] = {nullptr, nullptr}, *ex[2] = {nullptr, nullptr};

@@ -28,6 +28,7 @@ constexpr int FRONT_WG = 256;
 #ifndef DEC_WAVES_M13
 #define DEC_WAVES_M13 12
 #endif
+constexpr int DEC_MIN_WAVES = 4;  // fewest waves per decoder workgroup a launch uses
 #ifndef DEC_REENC_LOADS
 #define DEC_REENC_LOADS 8
 #endif
@@ -269,10 +270,25 @@ __device__ __forceinline__ void bs_multipliers(const uint32_t *ph, uint32_t plo2
 // BS (fft_form 1): the bottom-up butterflies run on bitsliced values -- 32 points of a lane per
 // word and bit plane, the GF(2^m) products by AND-XOR on the VALU -- instead of log/exp table
 // gathers in LDS (fft_form 0), whose random 16-bit reads are mostly LDS bank conflicts.
+// LDS table fill in a workgroup of wg threads: thread tid writes entries tid, tid + wg, ... below
+// N, dst(i, src(i)), in passes of K whose loads are issued together (src is called with indices
+// clamped to N - 1, so that it reads in bounds without a branch)
+template <int N, int K = 8, class Src, class Dst>
+__device__ __forceinline__ void wg_fill(int tid, int wg, Src &&src, Dst &&dst) {
+  for (int i0 = tid; i0 < N; i0 += K * wg) {
+    uint32_t v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = src(min(i0 + k * wg, N - 1));
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (i0 + k * wg < N) dst(i0 + k * wg, v[k]);
+  }
+}
+
 template <int SET, bool BS>
 __global__ __launch_bounds__((DecShape<SET, BS>::WG)) __attribute__((amdgpu_waves_per_eu(DecShape<SET, BS>::WAVES_PER_EU)))
 void k_dec_decode(DecArgs a) {
-  constexpr int DEC_WG = DecShape<SET, BS>::WG, DEC_WAVES = DecShape<SET, BS>::WAVES;
+  constexpr int DEC_WAVES = DecShape<SET, BS>::WAVES;  // at most; a launch may use fewer waves
   using P = Par<SET>;
   constexpr int M = P::M, T = P::T, Dn = P::D, KD = P::KD, Q = P::Q;
   constexpr int NPTS = 1 << M;
@@ -297,20 +313,32 @@ void k_dec_decode(DecArgs a) {
   __shared__ uint32_t pthib_s[BS ? NG * M : 1];
   __shared__ DecWave<SET> Wv[DEC_WAVES];
 
-  const int tid = threadIdx.x;
-  for (int i = tid; i < NPTS; i += DEC_WG) lg_s[i] = 2 * (i ? a.ft.lg[i] : ZL);
-  for (int i = tid; i < 4 * Q + 2; i += DEC_WG) ex_s[i] = i < 2 * Q ? 2 * a.ft.ex[i < Q ? i : i - Q] : 0;
-  for (int i = tid; i < NPHI; i += DEC_WG) phi_s[i] = 2 * a.ft.phi[i];
-  for (int i = tid; i < NR; i += DEC_WG) pthi_s[i] = P::TOP_TERM ? 2 * a.ft.pthi[i] : 0;
-  for (int i = tid; i < 2 * Dn; i += DEC_WG) twl_s[i] = 2 * a.ft.twl[i];
-  for (int i = tid; i < (KD + 1) * 64; i += DEC_WG)
-    plo_s[i] = i < KD * 64 ? 2 * a.ft.plo[i] : (P::TOP_TERM ? 2 * a.ft.ptlo[i - KD * 64] : 0);
+  const int tid = threadIdx.x, wg = __builtin_amdgcn_readfirstlane((int)blockDim.x);
+  // tables doubled (byte offsets of 16-bit entries); the entries that are not copies -- log 0, the
+  // zero padding past the doubled power table, the monic term's parts when t < D -- written after
+  // the copies, by the threads that copied them (same thread, program order)
+  wg_fill<NPTS>(tid, wg, [&](int i) { return 2u * a.ft.lg[i]; }, [&](int i, uint32_t x) { lg_s[i] = (uint16_t)x; });
+  wg_fill<2 * Q>(tid, wg, [&](int i) { return 2u * a.ft.ex[i < Q ? i : i - Q]; },
+                 [&](int i, uint32_t x) { ex_s[i] = (uint16_t)x; });
+  wg_fill<NPHI>(tid, wg, [&](int i) { return 2u * a.ft.phi[i]; }, [&](int i, uint32_t x) { phi_s[i] = (uint16_t)x; });
+  wg_fill<2 * Dn>(tid, wg, [&](int i) { return 2u * a.ft.twl[i]; }, [&](int i, uint32_t x) { twl_s[i] = (uint16_t)x; });
+  wg_fill<KD * 64>(tid, wg, [&](int i) { return 2u * a.ft.plo[i]; }, [&](int i, uint32_t x) { plo_s[i] = (uint16_t)x; });
+  if constexpr (P::TOP_TERM) {
+    wg_fill<NR>(tid, wg, [&](int i) { return 2u * a.ft.pthi[i]; }, [&](int i, uint32_t x) { pthi_s[i] = (uint16_t)x; });
+    wg_fill<64>(tid, wg, [&](int i) { return 2u * a.ft.ptlo[i]; },
+                [&](int i, uint32_t x) { plo_s[KD * 64 + i] = (uint16_t)x; });
+  } else {
+    for (int i = tid; i < NR; i += wg) pthi_s[i] = 0;
+    for (int i = tid; i < 64; i += wg) plo_s[KD * 64 + i] = 0;
+  }
+  for (int i = 2 * Q + tid; i < 4 * Q + 2; i += wg) ex_s[i] = 0;
+  if (tid == 0) lg_s[0] = 2 * ZL;
   __syncthreads();
   // the bottom-up multiplier of phi index k in lane l is plo[d][l] ^ phi[k], where level d holds the
   // indices [2^(m-6) - 2^(m-6-d), 2^(m-6) - 2^(m-7-d)); its log is looked up once here, so the
   // butterflies read it lane-contiguous (no bank conflict) instead of by a table gather
   if constexpr (!BS) {
-    for (int i = tid; i < NPHI * 64; i += DEC_WG) {
+    for (int i = tid; i < NPHI * 64; i += wg) {
       const int k = i >> 6, l = i & 63;
       int d = 0;
       while (d + 1 < KD && k >= (1 << (M - 6)) - (1 << (M - 7 - d))) ++d;
@@ -321,7 +349,7 @@ void k_dec_decode(DecArgs a) {
     // plo[d][l] ^ phi[off_d + (R mod h)], off_d = 2^(m-6) - 2^(m-6-d).  Bit q of plane word
     // (d + KD g, b) is bit b of phi[off_d + (q mod h)] for h < 32 (pairs inside a 32-point word),
     // of phi[off_d + 32 g + q] for h >= 32 (pairs of words, g = word index mod h/32)
-    for (int i = tid; i < 2 * KD * M; i += DEC_WG) {
+    for (int i = tid; i < 2 * KD * M; i += wg) {
       const int slot = i / M, b = i % M, d = slot % KD, g = slot / KD;
       const int h = 1 << (M - 7 - d), off = (1 << (M - 6)) - (1 << (M - 6 - d));
       uint32_t w = 0;
@@ -332,7 +360,7 @@ void k_dec_decode(DecArgs a) {
         }
       phib_s[i] = w;
     }
-    for (int i = tid; i < NG * M; i += DEC_WG) {
+    for (int i = tid; i < NG * M; i += wg) {
       const int G = i / M, b = i % M;
       uint32_t w = 0;
       for (int q = 0; q < 32; ++q) w |= ((uint32_t)(pthi_s[32 * G + q] >> (b + 1)) & 1u) << q;
@@ -830,18 +858,23 @@ hipError_t launch_dec(const DecArgs &args, hipStream_t s) {
                k_dec_front<SET><<<ceil_div(a.nops, FRONT_WG), FRONT_WG, 0, s>>>(
                    a.ct, reinterpret_cast<const uint4 *>(a.F), a.S, a.nops, a.plan));
   }
+  // waves per workgroup: the full shape when the batch gives every CU a workgroup of them, fewer
+  // (down to DEC_MIN_WAVES) when it does not, so that a small batch still reaches every CU; the
+  // workgroup is the unit that holds the tables in LDS, so only its wave count shrinks
+  const int max_waves = a.fft_form ? DecShape<SET, true>::WAVES : DecShape<SET, false>::WAVES;
+  const int nbat = ceil_div(a.nops, 4), cus = a.cus > 0 ? a.cus : 256;
+  const int waves = nbat >= cus * max_waves ? max_waves : std::max(DEC_MIN_WAVES, std::min(max_waves, ceil_div(nbat, cus)));
   int grid = a.decode_blocks > 0 ? a.decode_blocks : (P::M == 12 ? 512 : 256);
-  const int dec_waves = a.fft_form ? DecShape<SET, true>::WAVES : DecShape<SET, false>::WAVES;
-  const int maxgrid = ceil_div(a.nops, dec_waves * 4);
+  const int maxgrid = ceil_div(nbat, waves);
   if (grid > maxgrid) grid = maxgrid;
   {
     const hipError_t e = launch_fill_u32(a.queue, 0u, DEC_QUEUES * DEC_QSTRIDE, s);  // a kernel, not a memset
     if (e != hipSuccess) return e;
   }
   if (a.fft_form)
-    TMCE_TIMED("dec_decode", s, k_dec_decode<SET, true><<<grid, DecShape<SET, true>::WG, 0, s>>>(a));
+    TMCE_TIMED("dec_decode", s, k_dec_decode<SET, true><<<grid, 64 * waves, 0, s>>>(a));
   else
-    TMCE_TIMED("dec_decode", s, k_dec_decode<SET, false><<<grid, DecShape<SET, false>::WG, 0, s>>>(a));
+    TMCE_TIMED("dec_decode", s, k_dec_decode<SET, false><<<grid, 64 * waves, 0, s>>>(a));
   if (a.pc)
     TMCE_TIMED("dec_hash", s,
                k_hash_msg<SET, true><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, args.ct, a.key, a.status, a.nops,

@@ -154,6 +154,7 @@ struct DecArgs {
   unsigned *queue;       // scratch: the decoder's work counters, 1024 bytes (cleared by the launcher)
   int nops;              // op count, or with a plan the capacity (the live count is plan->total)
   int decode_blocks;     // grid of the decoder (0 = default)
+  int cus;               // compute units of the device (0: 256)
   int fft_form;          // bottom-up FFT: 1 = bitsliced GF(2^m) arithmetic, 0 = log/exp tables in LDS
   const PlanDev *plan;   // multi-batch plan (per-group keys; null: one key)
 };

@@ -83,6 +83,7 @@ int issue(tmce_pipeline *p, const int knobs[6]) {
     a.queue = p->queue.as<unsigned>();
     a.nops = p->cap;
     a.decode_blocks = knobs[1];
+    a.cus = e->cus;
     a.front_mfma = knobs[2];
     a.front_cfg = knobs[3];
     a.fft_form = knobs[5];
