@@ -303,15 +303,24 @@ ENC_KERNELS = ("gen_e", "enc_finish")
 DEC_KERNELS = ("dec_front_mfma", "dec_front", "dec_decode", "dec_hash")
 
 
-def direction_roofline(s, n, rates):
+def direction_roofline(s, n, rates, concurrent=True):
     """Whole-direction VALU and LDS issue fractions at the given op rates (ops/s per direction):
     the VALU / LDS wave-instructions one op costs in each direction (sum over the direction's
     kernels, from the committed counters of the same set at n ops per launch) times the rate,
-    against the chip's issue peaks."""
+    against the chip's issue peaks.  concurrent: the directions ran at these rates at the same time
+    (their issue rates add); otherwise they were timed one after the other, and the combined figure
+    is all instructions over the summed time, with each direction's own fraction beside it."""
     ctr = load_counters(f"{s}:{n}")
     if not ctr:
         return None
-    out, v_tot, l_tot = {}, 0.0, 0.0
+
+    def frac(v, l):
+        return ({"achieved": round(v / 1e9, 1), "peak": VALU_PEAK_G, "unit": "G VALU wave-instr/s",
+                 "frac": round(v / 1e9 / VALU_PEAK_G, 4)},
+                {"achieved": round(l / 1e9, 1), "peak": LDS_PEAK_G, "unit": "G LDS wave-instr/s",
+                 "frac": round(l / 1e9 / LDS_PEAK_G, 4)})
+
+    out, v_tot, l_tot, v_ops, l_ops, t_ops = {}, 0.0, 0.0, 0.0, 0.0, 0.0
     for d, names in (("enc", ENC_KERNELS), ("dec", DEC_KERNELS)):
         if d not in rates:
             continue
@@ -319,12 +328,17 @@ def direction_roofline(s, n, rates):
         l = sum(ctr[k]["lds_insts"] for k in names if k in ctr) / n
         v_tot += v * rates[d]
         l_tot += l * rates[d]
+        v_ops, l_ops, t_ops = v_ops + v, l_ops + l, t_ops + 1.0 / rates[d]
         out[f"{d}_valu_insts_per_op"] = round(v, 1)
         out[f"{d}_lds_insts_per_op"] = round(l, 1)
-    out["valu"] = {"achieved": round(v_tot / 1e9, 1), "peak": VALU_PEAK_G, "unit": "G VALU wave-instr/s",
-                   "frac": round(v_tot / 1e9 / VALU_PEAK_G, 4)}
-    out["lds"] = {"achieved": round(l_tot / 1e9, 1), "peak": LDS_PEAK_G, "unit": "G LDS wave-instr/s",
-                  "frac": round(l_tot / 1e9 / LDS_PEAK_G, 4)}
+        if not concurrent:
+            dv, dl = frac(v * rates[d], l * rates[d])
+            out[f"{d}_valu_frac"], out[f"{d}_lds_frac"] = dv["frac"], dl["frac"]
+    if concurrent:
+        out["valu"], out["lds"] = frac(v_tot, l_tot)
+    else:
+        out["valu"], out["lds"] = frac(v_ops / t_ops, l_ops / t_ops)
+    out["directions"] = "concurrent" if concurrent else "timed one after the other"
     out["counters_workload"] = f"{s}:{n}"
     return out
 
@@ -605,7 +619,7 @@ def section_c4(tm, orc, keys, cores, log, total=65536):
            "flat_engine_batch": {"encaps_per_s": round(fe, 1), "decaps_per_s": round(fd, 1)},
            "plain_gpu_form": {"encaps_per_s": round(pe, 1), "decaps_per_s": round(pd, 1)},
            "speedup_vs_plain": {"encaps": round(total / te / pe, 2), "decaps": round(total / td / pd, 2)},
-           "roofline": direction_roofline(s, total, {"enc": total / te, "dec": total / td}),
+           "roofline": direction_roofline(s, total, {"enc": total / te, "dec": total / td}, concurrent=False),
            "cpu_baseline": cpu_summary(c, cores, f"{ne} encaps + {nd} decaps of the same inputs"),
            "speedup_vs_cpu": {"encaps": round(total / te / c["enc_per_s"], 1),
                               "decaps": round(total / td / c["dec_per_s"], 1)},

@@ -217,7 +217,7 @@ static void build_dec_key_t(const uint8_t *sk, DecKeyHost &o) {
   // front-end matrix: row j*M + b, column i (i < KBITS): bit b of alpha_i^j u_i (syndrome-major)
   // and the same map over all N positions, column-major (the decoder's re-encryption check)
   o.F.assign((size_t)P::R * P::KWP, 0);
-  o.Fcol.assign((size_t)P::N * P::RW, 0);
+  o.Fcol.assign((size_t)P::N * P::FCW, 0);  // columns padded to 16-byte rows
   for (int i = 0; i < P::N; ++i) {
     uint16_t v = u[i];
     for (int j = 0; j < 2 * P::T; ++j) {
@@ -225,7 +225,7 @@ static void build_dec_key_t(const uint8_t *sk, DecKeyHost &o) {
         if ((v >> b) & 1) {
           const int r = j * P::M + b;
           if (i < P::KBITS) o.F[(size_t)r * P::KWP + i / 32] |= 1u << (i % 32);
-          o.Fcol[(size_t)i * P::RW + r / 32] |= 1u << (r % 32);
+          o.Fcol[(size_t)i * P::FCW + r / 32] |= 1u << (r % 32);
         }
       v = F.mul(v, o.alpha[i]);
     }

@@ -696,8 +696,13 @@ void k_dec_decode(DecArgs a) {
       // L2-resident, rows as in S) must equal the ciphertext's, word for word
       bool ok = (wt == T);
       if (ok) {
-        constexpr int RH = P::RW / 2;  // 8-byte words per column (RW is even)
-        static_assert(P::RW % 2 == 0 && RH <= 64, "one 8-byte word per lane");
+        // the columns are gathered CPI per load instruction: each span of SPN lanes (16 or 32)
+        // loads one column, lane li of the span its 16-byte word li (FQ words per column)
+        constexpr int FQ = P::FCW / 4;
+        constexpr int SPN = FQ <= 16 ? 16 : 32;
+        constexpr int CPI = 64 / SPN;
+        constexpr int NJ = ceil_div(T, CPI);  // load instructions per op
+        static_assert(FQ <= SPN, "a column fits its span");
         constexpr int KP = ceil_div(T, 64);
         // support index of each root (lane k holds root k), and its bit of e
         uint32_t pos_r[KP];
@@ -709,30 +714,54 @@ void k_dec_decode(DecArgs a) {
         }
 #pragma unroll
         for (int kk = 0; kk < KP; ++kk)
-          if (64 * kk + lane < T) atomicOr(&m32[(8u + pos_r[kk]) >> 5], 1u << ((8u + pos_r[kk]) & 31u));
-        // every lane takes part (lanes past the column re-read its last word): readlane below
-        // reads position k from lane k, so no lane may skip the position load above
-        const uint2 *fc = reinterpret_cast<const uint2 *>(k_fcol);
-        const int li = lane < RH ? lane : RH - 1;
-        uint2 acc = make_uint2(0, 0);
-        // DEC_REENC_LOADS column loads in flight per pass; a rolled outer loop (for t = 119 the compiler would
-        // otherwise unroll all t passes and hold every load in registers at once)
+          if (64 * kk + lane < T) {
+            atomicOr(&m32[(8u + pos_r[kk]) >> 5], 1u << ((8u + pos_r[kk]) & 31u));
+            w.pos[64 * kk + lane] = (uint16_t)pos_r[kk];  // support indices replace the points (read above)
+          }
+        wave_lds_sync();
+        const int g = lane / SPN, li = lane % SPN;
+        const int lq = li < FQ ? li : FQ - 1;  // lanes past the column re-read its last word
+        const uint4 *fc = reinterpret_cast<const uint4 *>(k_fcol);
+        uint4 acc = make_uint4(0, 0, 0, 0);
+        // DEC_REENC_LOADS load instructions in flight per pass; a rolled outer loop (the compiler
+        // would otherwise unroll every pass and hold all loads in registers at once)
 #pragma unroll 1
-        for (int k0 = 0; k0 < T; k0 += DEC_REENC_LOADS) {
+        for (int j0 = 0; j0 < NJ; j0 += DEC_REENC_LOADS) {
+          uint32_t pk[DEC_REENC_LOADS];
 #pragma unroll
-          for (int kk = 0; kk < DEC_REENC_LOADS; ++kk) {
-            const int k = k0 + kk;
-            if (T % DEC_REENC_LOADS != 0 && k >= T) break;
-            const uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)pos_r[k >> 6], k & 63);
-            const uint2 v = global_load8(fc, (size_t)p * RH + li);
-            acc.x ^= v.x;
-            acc.y ^= v.y;
+          for (int jj = 0; jj < DEC_REENC_LOADS; ++jj) {
+            const int k = CPI * (j0 + jj) + g;
+            pk[jj] = w.pos[k < T ? k : T - 1];
+          }
+#pragma unroll
+          for (int jj = 0; jj < DEC_REENC_LOADS; ++jj) {
+            if (NJ % DEC_REENC_LOADS != 0 && j0 + jj >= NJ) break;
+            const uint4 v = global_load16(fc, (size_t)pk[jj] * FQ + lq);
+            // a span past the last root (T not a multiple of CPI) adds nothing
+            const uint32_t keep = (T % CPI == 0 || CPI * (j0 + jj) + g < T) ? ~0u : 0u;
+            acc.x ^= v.x & keep;
+            acc.y ^= v.y & keep;
+            acc.z ^= v.z & keep;
+            acc.w ^= v.w & keep;
           }
         }
+        // XOR of the spans' partial sums
+#pragma unroll
+        for (int sh = SPN; sh < 64; sh *= 2) {
+          acc.x ^= (uint32_t)__shfl_xor((int)acc.x, sh);
+          acc.y ^= (uint32_t)__shfl_xor((int)acc.y, sh);
+          acc.z ^= (uint32_t)__shfl_xor((int)acc.z, sh);
+          acc.w ^= (uint32_t)__shfl_xor((int)acc.w, sh);
+        }
         bool mism = false;
-        if (lane < RH) {
-          const uint2 sv = global_load8(reinterpret_cast<const uint2 *>(a.S + (size_t)op * P::RW), lane);
-          mism = sv.x != acc.x || sv.y != acc.y;
+        if (lane < FQ) {  // words 4 lane .. 4 lane + 3 of the op's syndrome row (RW words, even)
+          const uint2 *srow = reinterpret_cast<const uint2 *>(a.S + (size_t)op * P::RW);
+          const uint2 s0 = global_load8(srow, 2 * lane);
+          mism = s0.x != acc.x || s0.y != acc.y;
+          if (4 * lane + 2 < P::RW) {
+            const uint2 s1 = global_load8(srow, 2 * lane + 1);
+            mism = mism || s1.x != acc.z || s1.y != acc.w;
+          }
         }
         ok = !__any(mism);
       }

@@ -69,6 +69,7 @@ template <int SET> struct Par : SetBase<SET> {
   static constexpr int TT = round_up(2 * T, 32);
   static constexpr int R = M * TT;
   static constexpr int RW = R / 32;
+  static constexpr int FCW = round_up(RW, 4);  // words per column of the re-encryption's syndrome map
   // additive FFT root search: D coefficients (the x^t term of a monic degree-64/128 locator is
   // added from a power table), KD top-down levels
   static constexpr int D = (T == 64) ? 64 : 128;
