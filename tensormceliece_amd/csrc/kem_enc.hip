@@ -144,12 +144,20 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e(const uint8_t *__restrict__ se
       const bool ok = have && ((cm >> (16 * grp)) & 0xFFFFull) == 0;
       wave_lds_sync();
       if (ok) {
+        // message row: the bitmap behind the prefix byte 1 (word k = bitmap word k << 8 | the top
+        // byte of word k - 1), every read issued before the first store
         uint32_t *mrow = msg + (size_t)opj * P::MSG_WORDS;
-        for (int k = g16; k <= NEW; k += 16) {
-          const uint32_t cur = k < NEW ? bm[k] : 0u;
-          const uint32_t prev = k ? bm[k - 1] : 0x01000000u;
-          mrow[k] = (cur << 8) | (prev >> 24);
+        constexpr int MWL = ceil_div(NEW + 1, 16);
+        uint32_t cw[MWL], pw[MWL];
+#pragma unroll
+        for (int i = 0; i < MWL; ++i) {
+          const int k = 16 * i + g16;
+          cw[i] = k < NEW ? bm[k] : 0u;
+          pw[i] = k == 0 ? 0x01000000u : (k <= NEW ? bm[k - 1] : 0u);
         }
+#pragma unroll
+        for (int i = 0; i < MWL; ++i)
+          if (16 * i + g16 <= NEW) mrow[16 * i + g16] = (cw[i] << 8) | (pw[i] >> 24);
         uint16_t *io = ind_out + (size_t)opj * P::IND_STRIDE;
         for (int idx = g16; idx < P::T; idx += 16) io[idx] = lj[idx * GEN_WG];
       }
@@ -307,57 +315,73 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
     const bool fin = evaluating && blk == B;
     wave_lds_sync();
 
-    // ---- collision check of the attempts that finished this step, four at a time
+    // ---- collision check of the attempts that finished this step, four at a time (group g of
+    //      16 lanes checks candidate g against its own bitmap).  The candidates' lanes, ops and
+    //      verdicts are kept in scalar registers; the LDS accesses of one wave execute in program
+    //      order, so the bitmap reads of the message row see the group's atomics and the clears
+    //      that follow them are seen by the next round's atomics without a wait.
     uint64_t cand = __ballot(fin && count == P::T);
     uint64_t okmask = 0;
+    constexpr int KV = ceil_div(P::T, 16);  // draws per lane of a candidate
     while (cand) {
-      uint64_t c = cand;
-      for (int k = 0; k < grp && c; ++k) c &= c - 1;
-      const bool have = c != 0;
-      const int j = have ? __ffsll((long long)c) - 1 : 0;
-      for (int k = 0; k < 4 && cand; ++k) cand &= cand - 1;
-      const int opj = __shfl(op, j);
+      int jg[4], opg[4], ng = 0;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        jg[g] = cand ? __ffsll((long long)cand) - 1 : 0;
+        opg[g] = __builtin_amdgcn_readlane(op, jg[g]);
+        ng += cand != 0;
+        cand &= cand - 1;
+      }
+      const bool have = grp < ng;
+      const int j = grp == 0 ? jg[0] : grp == 1 ? jg[1] : grp == 2 ? jg[2] : jg[3];
+      const int opj = grp == 0 ? opg[0] : grp == 1 ? opg[1] : grp == 2 ? opg[2] : opg[3];
       const uint16_t *lj = lst + (wave * 64 + j);
+      uint32_t vv[KV];
       bool coll = false;
       if (have) {
 #pragma unroll
-        for (int k = 0; k < ceil_div(P::T, 16); ++k) {
+        for (int k = 0; k < KV; ++k) {
           const int idx = k * 16 + g16;
-          if (idx < P::T) {
-            const uint32_t v = lj[idx * GEN_WG];
-            const uint32_t bit = 1u << (v & 31u);
-            coll |= (atomicOr(bm + (v >> 5), bit) & bit) != 0;
-          }
+          vv[k] = idx < P::T ? (uint32_t)lj[idx * GEN_WG] : 0u;
         }
+#pragma unroll
+        for (int k = 0; k < KV; ++k)
+          if (k * 16 + g16 < P::T) {
+            const uint32_t bit = 1u << (vv[k] & 31u);
+            coll |= (atomicOr(bm + (vv[k] >> 5), bit) & bit) != 0;
+          }
       }
       const uint64_t cm = __ballot(coll);
-      const bool ok = have && ((cm >> (16 * grp)) & 0xFFFFull) == 0;
-      wave_lds_sync();
-      if (ok) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+        if (g < ng && ((cm >> (16 * g)) & 0xFFFFull) == 0) okmask |= 1ull << jg[g];
+      if (have && ((cm >> (16 * grp)) & 0xFFFFull) == 0) {
+        // message row: the bitmap behind the prefix byte 1 (word k = bitmap word k << 8 | the top
+        // byte of word k - 1), every read issued before the first store
         uint32_t *mrow = msg + (size_t)opj * P::MSG_WORDS;
-        for (int k = g16; k <= NEW; k += 16) {
-          const uint32_t cur = k < NEW ? bm[k] : 0u;
-          const uint32_t prev = k ? bm[k - 1] : 0x01000000u;
-          mrow[k] = (cur << 8) | (prev >> 24);
+        constexpr int MWL = ceil_div(NEW + 1, 16);
+        uint32_t cw[MWL], pw[MWL];
+#pragma unroll
+        for (int i = 0; i < MWL; ++i) {
+          const int k = 16 * i + g16;
+          cw[i] = k < NEW ? bm[k] : 0u;
+          pw[i] = k == 0 ? 0x01000000u : (k <= NEW ? bm[k - 1] : 0u);
         }
+#pragma unroll
+        for (int i = 0; i < MWL; ++i)
+          if (16 * i + g16 <= NEW) mrow[16 * i + g16] = (cw[i] << 8) | (pw[i] >> 24);
         uint16_t *io = ind_out + (size_t)opj * P::IND_STRIDE;
-        for (int idx = g16; idx < P::T; idx += 16) io[idx] = lj[idx * GEN_WG];
+#pragma unroll
+        for (int k = 0; k < KV; ++k)
+          if (k * 16 + g16 < P::T) io[k * 16 + g16] = (uint16_t)vv[k];
       }
-      {
-        const uint64_t acc = __ballot(ok && g16 == 0);
-        for (int g = 0; g < 4; ++g)
-          if ((acc >> (16 * g)) & 1ull) okmask |= 1ull << __shfl(j, 16 * g);
-      }
-      wave_lds_sync();
       if (have) {
 #pragma unroll
-        for (int k = 0; k < ceil_div(P::T, 16); ++k) {
-          const int idx = k * 16 + g16;
-          if (idx < P::T) bm[lj[idx * GEN_WG] >> 5] = 0;
-        }
+        for (int k = 0; k < KV; ++k)
+          if (k * 16 + g16 < P::T) bm[vv[k] >> 5] = 0;
       }
-      wave_lds_sync();
     }
+    wave_lds_sync();
 
     // ---- accepted or abandoned ops end; the group takes a new op next step
     const bool acc_me = (okmask >> lane) & 1ull;
