@@ -196,15 +196,24 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e(const uint8_t *__restrict__ se
 }
 
 // Pipelined sampler.  An op's rejection chain is sequential only through the first SHAKE block of
-// each attempt (attempt k+1's state is the first 32 bytes of attempt k's first block), so a group of
-// G lanes works on one op: group lane 0 (the chain lane) computes first blocks S1_0, S1_1, ... one per
-// step and hands each to an evaluator lane (1 .. B-1 in rotation), which squeezes the attempt's
-// other B-1 blocks over the next B-1 steps and collects the first t draws below n.  Attempts thus
-// finish in order, one per step, B-1 steps after their first block; a finished attempt with t
-// draws goes through the collision check (as in k_gen_e), the first accepted one is the op's e
-// (libmceliece's semantics), later speculative attempts are dropped and the group takes the next op
-// from the queue.  Each step is one Keccak-f per lane.  The longest chain of a batch shrinks from
-// (attempts x B) permutations to (attempts + B - 1).
+// each attempt (attempt k+1's seed is the first 32 bytes of attempt k's first block), so a group of
+// G >= B lanes works on one op and starts one attempt per step: at the group's step s, lane s mod G
+// computes the first block of attempt s from its seed and hands the block's first 32 bytes -- the
+// seed of attempt s + 1 -- to the next lane of the group; a lane that started attempt k squeezes
+// its next B - 1 blocks in the following steps (one Keccak-f per lane and step) and collects the
+// attempt's first t draws below n in its own list.  Attempts thus finish in order, one per step,
+// B - 1 steps after their first block; a finished attempt with t draws goes through the collision
+// check (as in k_gen_e), the first accepted one is the op's e (libmceliece's semantics), later
+// speculative attempts are dropped and the group takes the next op.  The longest chain of a batch
+// shrinks from (attempts x B) permutations to (attempts + B - 1).
+#ifndef GEN_BITMAPS
+#define GEN_BITMAPS 2
+#endif
+// DPP quad_perm: lane i of each group of four reads lane (CTRL >> 2 i) & 3 of the group
+template <int CTRL>
+__device__ __forceinline__ uint32_t quad_perm(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
 template <int SET>
 __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict__ seeds,
                                                        uint16_t *__restrict__ ind_out, uint32_t *__restrict__ msg,
@@ -213,30 +222,31 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
   using P = Par<SET>;
   constexpr int B = P::DRAW_BLOCKS;
   constexpr int G = B == 2 ? 2 : 4;  // lanes per op
-  constexpr int NE = B - 1;          // evaluator lanes per op
+  static_assert(B <= G, "a lane finishes its attempt before the rotation returns to it");
   constexpr uint32_t GFMASK = (1u << P::M) - 1u;
   constexpr int NEW = ceil_div(P::N, 32);
+  constexpr int NBM = GEN_BITMAPS;                   // bitmaps (candidates per round) per 16-lane group
+  constexpr int DPP_PREV = G == 2 ? 0xB1 : 0x93;    // quad_perm: lane r reads lane r - 1 of its group
   constexpr int DPP_GROUP0 = G == 2 ? 0xA0 : 0x00;  // quad_perm: each lane reads its group's lane 0
-  __shared__ uint16_t lst[(P::T + 1) * GEN_WG];  // [index][thread]; row T takes draws past the first T
-  __shared__ uint32_t bmp[GEN_WAVES][4][NEW + 1];
+  constexpr int KV = ceil_div(P::T, 16);            // draws per lane of a candidate
+  __shared__ uint16_t lst[(P::T + 1) * GEN_WG];     // [index][thread]; row T takes draws past the first T
+  __shared__ uint32_t bmp[GEN_WAVES][4 * NBM][NEW + 1];
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wave = tid >> 6;
   const int grp = lane >> 4, g16 = lane & 15;  // collision check: four 16-lane groups
-  const int r = lane % G, gl = lane - r;        // role in the op group (0 chain, 1..NE evaluators)
+  const int r = lane % G, gl = lane - r;        // place in the op group
   const uint64_t gmask = ((1ull << G) - 1ull) << gl;
-  uint32_t *bm = bmp[wave][grp];
-  for (int i = lane; i < 4 * (NEW + 1); i += 64) bmp[wave][i / (NEW + 1)][i % (NEW + 1)] = 0;
+  for (int i = lane; i < 4 * NBM * (NEW + 1); i += 64) bmp[wave][i / (NEW + 1)][i % (NEW + 1)] = 0;
   const KeccakRC rc;
   if (plan) nops = plan->total;
-
+  // ops come from the queue as groups need them (workgroups that become resident late, behind a
+  // grid larger than the chip holds, find it drained and leave)
   int op = -1;         // the group's op (every lane of the group holds it)
-  bool done = false;   // the queue is exhausted for this group
-  uint32_t st[8];      // chain lane: seed state of the next attempt
-  int knext = 0;       // chain lane: index of the next attempt
-  int att = -1;        // evaluator: attempt being squeezed (-1: idle)
-  int blk = 0;         // evaluator: blocks of `att` collected
-  int count = 0;       // evaluator: draws below N collected
+  bool done = false;   // the queue is exhausted (wave-uniform)
+  int s = 0;           // the group's step in its op: attempt s starts now
+  uint32_t st[8];      // seed of the attempt this lane starts next
+  int count = 0;  // draws below N in this lane's list
   uint64_t A[25];
 #pragma unroll
   for (int i = 0; i < 25; ++i) A[i] = 0;
@@ -262,41 +272,41 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
   };
 
   for (;;) {
-    // ---- refill: group leaders without an op take the next ones from the queue
-    {
-      const bool need = r == 0 && op < 0 && !done;
+    // ---- refill: group leaders without an op take the next ones from the queue (one atomic per
+    //      wave and step); a group with a new op starts at step 0 with the op's seed in lane 0
+    if (!done) {
+      const bool need = r == 0 && op < 0;
       const uint64_t needmask = __ballot(need);
-      int newop = op;
-      bool newdone = done;
       if (needmask) {
         const int leader = __ffsll((long long)needmask) - 1;
+        const unsigned cnt = (unsigned)__popcll(needmask);
         unsigned base = 0;
-        if (lane == leader) base = atomicAdd(queue, (unsigned)__popcll(needmask));
-        base = (unsigned)__shfl((int)base, leader);
+        if (lane == leader) base = atomicAdd(queue, cnt);
+        base = (unsigned)__builtin_amdgcn_readlane((int)base, leader);
+        done = base + cnt >= (unsigned)nops;
+        int nop = op;
         if (need) {
           const unsigned rr = base + mbcnt64(needmask);
-          if (rr < (unsigned)nops) newop = (int)rr;
-          else newdone = true;
-        }
-      }
-      const int gop = __shfl(newop, gl);
-      done = __shfl((int)newdone, gl) != 0;
-      if (gop >= 0 && gop != op) {  // a new op for this group
-        knext = 0;
-        att = -1;
-        if (r == 0) {
-          const uint32_t *sp = reinterpret_cast<const uint32_t *>(seeds + 32 * (size_t)gop);
+          if (rr < (unsigned)nops) {
+            nop = (int)rr;
+            const uint32_t *sp = reinterpret_cast<const uint32_t *>(seeds + 32 * (size_t)rr);
 #pragma unroll
-          for (int i = 0; i < 8; ++i) st[i] = sp[i];
+            for (int i = 0; i < 8; ++i) st[i] = sp[i];
+          }
         }
+        const int gop = (int)quad_perm<DPP_GROUP0>((uint32_t)nop);
+        if (gop != op) s = 0;
+        op = gop;
       }
-      op = gop;
     }
     if (!__any(op >= 0)) break;
 
-    // ---- one Keccak-f per lane: the chain lane starts attempt `knext`, evaluators squeeze on
-    const bool chain = r == 0 && op >= 0 && knext < P::MAX_ATTEMPTS;
-    if (chain) {
+    // ---- one Keccak-f per lane: this lane's attempt k = s - b, block b (b = 0: it starts attempt s)
+    const int b = (s - r) & (G - 1);
+    const int k = s - b;
+    const bool live = op >= 0 && k >= 0 && k < P::MAX_ATTEMPTS;
+    const bool active = live && b < B;
+    if (live && b == 0) {  // absorb 0x40 || seed (33 bytes) and the SHAKE256 padding
       const uint64_t s0 = st[0] | ((uint64_t)st[1] << 32), s1 = st[2] | ((uint64_t)st[3] << 32);
       const uint64_t s2 = st[4] | ((uint64_t)st[5] << 32), s3 = st[6] | ((uint64_t)st[7] << 32);
 #pragma unroll
@@ -307,124 +317,117 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
       A[3] = (s2 >> 56) | (s3 << 8);
       A[4] = (s3 >> 56) | (0x1Full << 8);
       A[16] = 0x80ull << 56;
+      count = 0;
     }
     keccak_f1600(A, rc);
-    const bool evaluating = r >= 1 && r <= NE && att >= 0 && op >= 0;
-    extract(blk, evaluating);
-    if (evaluating) ++blk;
-    const bool fin = evaluating && blk == B;
+    extract(b, active);
+    const bool fin = active && b == B - 1;
+    // the seed of attempt s + 1 (the first 32 bytes of attempt s's first block) to the next lane
+    {
+      uint32_t x[8];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        x[2 * i] = quad_perm<DPP_PREV>((uint32_t)A[i]);
+        x[2 * i + 1] = quad_perm<DPP_PREV>((uint32_t)(A[i] >> 32));
+      }
+      if (((s + 1 - r) & (G - 1)) == 0) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) st[i] = x[i];
+      }
+    }
     wave_lds_sync();
 
-    // ---- collision check of the attempts that finished this step, four at a time (group g of
-    //      16 lanes checks candidate g against its own bitmap).  The candidates' lanes, ops and
-    //      verdicts are kept in scalar registers; the LDS accesses of one wave execute in program
-    //      order, so the bitmap reads of the message row see the group's atomics and the clears
-    //      that follow them are seen by the next round's atomics without a wait.
+    // ---- collision check of the attempts that finished this step, 4 NBM at a time (group g of 16
+    //      lanes checks candidates g, g + 4, ... against its own bitmaps).  The candidates' lanes,
+    //      ops and verdicts are kept in scalar registers; the LDS accesses of one wave execute in
+    //      program order, so the bitmap reads of a message row see the group's atomics and the
+    //      clears that follow them are seen by the next round's atomics without a wait.
     uint64_t cand = __ballot(fin && count == P::T);
     uint64_t okmask = 0;
-    constexpr int KV = ceil_div(P::T, 16);  // draws per lane of a candidate
     while (cand) {
-      int jg[4], opg[4], ng = 0;
+      int jq[4 * NBM], opq[4 * NBM], nq = 0;
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        jg[g] = cand ? __ffsll((long long)cand) - 1 : 0;
-        opg[g] = __builtin_amdgcn_readlane(op, jg[g]);
-        ng += cand != 0;
+      for (int q = 0; q < 4 * NBM; ++q) {
+        jq[q] = cand ? __ffsll((long long)cand) - 1 : 0;
+        opq[q] = __builtin_amdgcn_readlane(op, jq[q]);
+        nq += cand != 0;
         cand &= cand - 1;
       }
-      const bool have = grp < ng;
-      const int j = grp == 0 ? jg[0] : grp == 1 ? jg[1] : grp == 2 ? jg[2] : jg[3];
-      const int opj = grp == 0 ? opg[0] : grp == 1 ? opg[1] : grp == 2 ? opg[2] : opg[3];
-      const uint16_t *lj = lst + (wave * 64 + j);
-      uint32_t vv[KV];
-      bool coll = false;
-      if (have) {
+      uint32_t vv[NBM][KV];
+      bool coll[NBM];
 #pragma unroll
-        for (int k = 0; k < KV; ++k) {
-          const int idx = k * 16 + g16;
-          vv[k] = idx < P::T ? (uint32_t)lj[idx * GEN_WG] : 0u;
+      for (int c = 0; c < NBM; ++c) {
+        const int q0 = 4 * c;
+        const int j = grp == 0 ? jq[q0] : grp == 1 ? jq[q0 + 1] : grp == 2 ? jq[q0 + 2] : jq[q0 + 3];
+        const uint16_t *lj = lst + (wave * 64 + j);
+#pragma unroll
+        for (int kk = 0; kk < KV; ++kk) {
+          const int idx = kk * 16 + g16;
+          vv[c][kk] = idx < P::T ? (uint32_t)lj[idx * GEN_WG] : 0u;
         }
+      }
 #pragma unroll
-        for (int k = 0; k < KV; ++k)
-          if (k * 16 + g16 < P::T) {
-            const uint32_t bit = 1u << (vv[k] & 31u);
-            coll |= (atomicOr(bm + (vv[k] >> 5), bit) & bit) != 0;
+      for (int c = 0; c < NBM; ++c) {
+        coll[c] = false;
+        uint32_t *bm = bmp[wave][4 * c + grp];
+        if (4 * c + grp < nq) {
+#pragma unroll
+          for (int kk = 0; kk < KV; ++kk)
+            if (kk * 16 + g16 < P::T) {
+              const uint32_t bit = 1u << (vv[c][kk] & 31u);
+              coll[c] |= (atomicOr(bm + (vv[c][kk] >> 5), bit) & bit) != 0;
+            }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NBM; ++c) {
+        const uint64_t cm = __ballot(coll[c]);
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+          if (4 * c + g < nq && ((cm >> (16 * g)) & 0xFFFFull) == 0) okmask |= 1ull << jq[4 * c + g];
+        uint32_t *bm = bmp[wave][4 * c + grp];
+        const int q = 4 * c + grp;
+        if (q < nq && ((cm >> (16 * grp)) & 0xFFFFull) == 0) {
+          const int opj = grp == 0 ? opq[4 * c] : grp == 1 ? opq[4 * c + 1] : grp == 2 ? opq[4 * c + 2] : opq[4 * c + 3];
+          // message row: the bitmap behind the prefix byte 1 (word k = bitmap word k << 8 | the
+          // top byte of word k - 1), every read issued before the first store
+          uint32_t *mrow = msg + (size_t)opj * P::MSG_WORDS;
+          constexpr int MWL = ceil_div(NEW + 1, 16);
+          uint32_t cw[MWL], pw[MWL];
+#pragma unroll
+          for (int i = 0; i < MWL; ++i) {
+            const int kw = 16 * i + g16;
+            cw[i] = kw < NEW ? bm[kw] : 0u;
+            pw[i] = kw == 0 ? 0x01000000u : (kw <= NEW ? bm[kw - 1] : 0u);
           }
-      }
-      const uint64_t cm = __ballot(coll);
 #pragma unroll
-      for (int g = 0; g < 4; ++g)
-        if (g < ng && ((cm >> (16 * g)) & 0xFFFFull) == 0) okmask |= 1ull << jg[g];
-      if (have && ((cm >> (16 * grp)) & 0xFFFFull) == 0) {
-        // message row: the bitmap behind the prefix byte 1 (word k = bitmap word k << 8 | the top
-        // byte of word k - 1), every read issued before the first store
-        uint32_t *mrow = msg + (size_t)opj * P::MSG_WORDS;
-        constexpr int MWL = ceil_div(NEW + 1, 16);
-        uint32_t cw[MWL], pw[MWL];
+          for (int i = 0; i < MWL; ++i)
+            if (16 * i + g16 <= NEW) mrow[16 * i + g16] = (cw[i] << 8) | (pw[i] >> 24);
+          uint16_t *io = ind_out + (size_t)opj * P::IND_STRIDE;
 #pragma unroll
-        for (int i = 0; i < MWL; ++i) {
-          const int k = 16 * i + g16;
-          cw[i] = k < NEW ? bm[k] : 0u;
-          pw[i] = k == 0 ? 0x01000000u : (k <= NEW ? bm[k - 1] : 0u);
+          for (int kk = 0; kk < KV; ++kk)
+            if (kk * 16 + g16 < P::T) io[kk * 16 + g16] = (uint16_t)vv[c][kk];
         }
+        if (q < nq) {
 #pragma unroll
-        for (int i = 0; i < MWL; ++i)
-          if (16 * i + g16 <= NEW) mrow[16 * i + g16] = (cw[i] << 8) | (pw[i] >> 24);
-        uint16_t *io = ind_out + (size_t)opj * P::IND_STRIDE;
-#pragma unroll
-        for (int k = 0; k < KV; ++k)
-          if (k * 16 + g16 < P::T) io[k * 16 + g16] = (uint16_t)vv[k];
-      }
-      if (have) {
-#pragma unroll
-        for (int k = 0; k < KV; ++k)
-          if (k * 16 + g16 < P::T) bm[vv[k] >> 5] = 0;
+          for (int kk = 0; kk < KV; ++kk)
+            if (kk * 16 + g16 < P::T) bm[vv[c][kk] >> 5] = 0;
+        }
       }
     }
     wave_lds_sync();
 
     // ---- accepted or abandoned ops end; the group takes a new op next step
     const bool acc_me = (okmask >> lane) & 1ull;
-    if (acc_me && attempts) attempts[op] = (uint32_t)(att + 1);
-    const bool aband_me = fin && !acc_me && att == P::MAX_ATTEMPTS - 1;
+    if (acc_me && attempts) attempts[op] = (uint32_t)(k + 1);
+    const bool aband_me = fin && !acc_me && k == P::MAX_ATTEMPTS - 1;
     if (aband_me) {
       ind_out[(size_t)op * P::IND_STRIDE] = 0xFFFFu;  // never expected: status -2
       if (attempts) attempts[op] = P::MAX_ATTEMPTS;
     }
     const uint64_t endmask = okmask | __ballot(aband_me);
-    const bool gend = (endmask & gmask) != 0;
-    if (gend) {
-      op = -1;
-      att = -1;
-    }
-    if (fin) att = -1;  // the evaluator is free (it may receive the new attempt below)
-
-    // ---- hand-off: the chain lane's first block goes to evaluator 1 + knext % NE
-    const int kg = __shfl(knext, gl);
-    const bool cg = __shfl((int)chain, gl) != 0 && !gend;
-    const bool recv = cg && r == 1 + kg % NE;
-#pragma unroll
-    for (int i = 0; i < 25; ++i) {
-      const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)A[i], DPP_GROUP0, 0xF, 0xF, false);
-      const uint32_t hi =
-          (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(A[i] >> 32), DPP_GROUP0, 0xF, 0xF, false);
-      if (recv) A[i] = ((uint64_t)hi << 32) | lo;
-    }
-    if (chain && !gend) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        st[2 * i] = (uint32_t)A[i];
-        st[2 * i + 1] = (uint32_t)(A[i] >> 32);
-      }
-      ++knext;
-    }
-    if (recv) {
-      att = kg;
-      count = 0;
-    }
-    extract(0, recv);
-    if (recv) blk = 1;
-    wave_lds_sync();
+    if ((endmask & gmask) != 0) op = -1;
+    else if (op >= 0) ++s;
   }
 }
 
