@@ -453,25 +453,87 @@ __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__
     pad_bad = __builtin_amdgcn_readfirstlane(kd.pad_bad) != 0;
   }
 
-  // ---- c = sum of columns over the support of e
-  uint4 c[C4];
-#pragma unroll
-  for (int q = 0; q < C4; ++q) c[q] = make_uint4(0, 0, 0, 0);
+  // ---- c = sum of columns over the support of e.  Long gathers (t columns of C4 16-byte words,
+  //      t C4 > 960: the m = 13 sets of t >= 119) are done by the whole wave op after op: each span
+  //      of SPN lanes loads one column (lane li of the span its word li), CPI columns per load
+  //      instruction, coalesced; the spans' sums are combined across lanes and parked in LDS, from
+  //      where each lane takes its own op's c for the hash below.  Short ones stay one op per lane
+  //      (fewer instructions, and the key's columns stay in L2).
+  constexpr bool COOP = C4 * P::T > 960;
+  constexpr int SPN = C4 <= 8 ? 8 : (C4 <= 16 ? 16 : 32);
+  constexpr int CPI = 64 / SPN;
+  constexpr int NJ = ceil_div(P::T, CPI);  // load instructions per op
+  constexpr int FIN_LOADS = 8;             // load instructions in flight
+  static_assert(C4 <= SPN, "a column fits its span");
+  __shared__ uint4 cst[COOP ? 4 : 1][COOP ? 64 : 1][COOP ? C4 : 1];  // [wave][op of the wave][word]
+  __shared__ uint16_t pst[COOP ? 4 : 1][round_up(P::T, 2) + 2];     // the support of the op gathered
   const uint32_t *ip = reinterpret_cast<const uint32_t *>(ind + (size_t)op * P::IND_STRIDE);
   const bool abandoned = (ip[0] & 0xFFFFu) == 0xFFFFu;  // sampler gave up on this op (never expected)
-  if (!abandoned) {
+  uint4 c[C4];
+  if constexpr (COOP) {
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int g = lane / SPN, li = lane % SPN, lq = li < C4 ? li : C4 - 1;
+    const int wop0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+    const int jn = min(64, nops - wop0);  // the wave's ops in the batch (lanes past it store nothing)
+#pragma unroll 1
+    for (int jj = 0; jj < jn; ++jj) {
+      const int opj = wop0 + jj;
+      const uint32_t *ipj = reinterpret_cast<const uint32_t *>(ind + (size_t)opj * P::IND_STRIDE);
+      for (int i = lane; i < P::IND_STRIDE / 2; i += 64) reinterpret_cast<uint32_t *>(pst[wv])[i] = ipj[i];
+      wave_lds_sync();
+      const bool ab = pst[wv][0] == 0xFFFFu;  // uniform
+      uint4 acc = make_uint4(0, 0, 0, 0);
+      if (!ab) {
+#pragma unroll 1
+        for (int i0 = 0; i0 < NJ; i0 += FIN_LOADS) {
+          uint32_t pk[FIN_LOADS];
+#pragma unroll
+          for (int ii = 0; ii < FIN_LOADS; ++ii) {
+            const int k = CPI * (i0 + ii) + g;
+            pk[ii] = pst[wv][k < P::T ? k : P::T - 1];
+          }
+#pragma unroll
+          for (int ii = 0; ii < FIN_LOADS; ++ii) {
+            if (NJ % FIN_LOADS != 0 && i0 + ii >= NJ) break;
+            const uint4 x = global_load16(cols, (size_t)pk[ii] * C4 + lq);
+            // a span past the last position (t not a multiple of CPI) adds nothing
+            const uint32_t keep = (P::T % CPI == 0 || CPI * (i0 + ii) + g < P::T) ? ~0u : 0u;
+            acc.x ^= x.x & keep;
+            acc.y ^= x.y & keep;
+            acc.z ^= x.z & keep;
+            acc.w ^= x.w & keep;
+          }
+        }
+#pragma unroll
+        for (int sh = SPN; sh < 64; sh *= 2) {
+          acc.x ^= (uint32_t)__shfl_xor((int)acc.x, sh);
+          acc.y ^= (uint32_t)__shfl_xor((int)acc.y, sh);
+          acc.z ^= (uint32_t)__shfl_xor((int)acc.z, sh);
+          acc.w ^= (uint32_t)__shfl_xor((int)acc.w, sh);
+        }
+      }
+      if (lane < C4) cst[wv][jj][lane] = acc;
+      wave_lds_sync();
+    }
+#pragma unroll
+    for (int q = 0; q < C4; ++q) c[q] = cst[wv][lane][q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < C4; ++q) c[q] = make_uint4(0, 0, 0, 0);
+    if (!abandoned) {
 #pragma unroll 4
-    for (int i = 0; i < P::T; i += 2) {
-      const uint32_t pr = ip[i / 2];
+      for (int i = 0; i < P::T; i += 2) {
+        const uint32_t pr = ip[i / 2];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if (i + h < P::T) {
-          const uint32_t p = h ? (pr >> 16) : (pr & 0xFFFFu);
-          const uint4 *col = cols + (size_t)p * C4;
+        for (int h = 0; h < 2; ++h) {
+          if (i + h < P::T) {
+            const uint32_t p = h ? (pr >> 16) : (pr & 0xFFFFu);
+            const uint4 *col = cols + (size_t)p * C4;
 #pragma unroll
-          for (int q = 0; q < C4; ++q) {
-            const uint4 x = global_load16(col, q);
-            c[q].x ^= x.x; c[q].y ^= x.y; c[q].z ^= x.z; c[q].w ^= x.w;
+            for (int q = 0; q < C4; ++q) {
+              const uint4 x = global_load16(col, q);
+              c[q].x ^= x.x; c[q].y ^= x.y; c[q].z ^= x.z; c[q].w ^= x.w;
+            }
           }
         }
       }
