@@ -687,7 +687,7 @@ def section_c5(tm, orc, cores, log, nkeys=16, n_requests=6000):
     pks, sks = tm.keypair_batch(s, seeds)  # GPU-batched key generation
     kps = list(zip(pks, sks))
     wl = serving.Workload(s, kps, pool_per_key=4096)
-    server = tm.Server(s, slot_capacity=32768, slots_per_direction=2, hbm_budget=8 << 30)
+    server = tm.Server(s, slot_capacity=65536, slots_per_direction=2, hbm_budget=8 << 30)
     try:
         reqs = serving.make_requests(wl, n_requests, seed=5)
         serving.run_poisson(server, wl, reqs[:200], 0.0)  # warm-up (first runs, graph capture)

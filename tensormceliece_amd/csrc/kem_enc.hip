@@ -647,7 +647,7 @@ __global__ void k_plan_gather(const PlanDev *__restrict__ plan, uint8_t *__restr
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (size_t)plan->total * H) return;
   const int op = (int)(i / H), h = (int)(i - (size_t)op * H);
-  const int sb = plan_sub(plan, op);
+  const int sb = plan_sub_wave(plan, op);
   uint16_t v = 0;  // padding rows are zero
   if (sb >= 0) {
     const SubDev &d = plan->subs[sb];
@@ -663,7 +663,7 @@ __global__ void k_plan_scatter(const PlanDev *__restrict__ plan, const uint8_t *
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (size_t)plan->total * U) return;
   const int op = (int)(i / U), u = (int)(i - (size_t)op * U);
-  const int sb = plan_sub(plan, op);
+  const int sb = plan_sub_wave(plan, op);
   if (sb < 0) return;
   const SubDev &d = plan->subs[sb];
   const int j = op - d.start;

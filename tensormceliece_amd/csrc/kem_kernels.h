@@ -84,6 +84,19 @@ __device__ __forceinline__ int plan_sub(const PlanDev *p, int op) {
   }
   return (r >= 0 && op < p->subs[r].start + p->subs[r].n) ? r : -1;
 }
+// plan_sub for a lane of a wave whose ops are consecutive: the sub-batch of the wave's first active
+// lane's op is found once (uniform, scalar loads) and serves every lane whose op it holds; the
+// other lanes (a wave that crosses into the next sub-batch or into padding) search on their own
+__device__ __forceinline__ int plan_sub_wave(const PlanDev *p, int op) {
+  const int op0 = __builtin_amdgcn_readfirstlane(op);
+  const int s0 = __builtin_amdgcn_readfirstlane(plan_sub(p, op0));
+  if (s0 >= 0) {
+    const int st = __builtin_amdgcn_readfirstlane(p->subs[s0].start);
+    const int n = __builtin_amdgcn_readfirstlane(p->subs[s0].n);
+    if (op >= st && op < st + n) return s0;
+  }
+  return plan_sub(p, op);
+}
 // gather sub-batch inputs into the flat buffer (row_bytes per op; padding rows zero) and scatter
 // the flat outputs back (enc: ct_bytes > 0 also scatters ciphertexts)
 hipError_t launch_plan_gather(const PlanDev *plan, uint8_t *flat, int row_bytes, int cap, hipStream_t s);

@@ -18,9 +18,14 @@
 // Completion: a host function enqueued behind each batch on its slot's stream wakes the worker, which
 // then copies the outputs back (no polling).
 //
+// The host copies of a batch (request inputs into the slot's pinned staging, outputs back to the
+// requests) are split over a few copier threads: one thread's memcpy bandwidth into and out of
+// pinned memory, not the GPU, bounds a saturated server otherwise (a 32K-op batch moves ~7-10 MB).
+//
 // Latency = completion - submission (host steady clock), including queueing, transfers and compute.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -57,6 +62,102 @@ struct Req {
   uint8_t *d_ct = nullptr, *d_k = nullptr;
   int *d_status = nullptr;
 };
+
+// One copy of a batch's staging or unstaging: n bytes from src to dst.
+struct CopyJob {
+  void *dst;
+  const void *src;
+  size_t n;
+};
+
+// Copier threads: run() hands a list of copies to the pool, takes part itself, and returns when
+// every copy is done and no pool thread still looks at the list.  One run() at a time (the worker
+// thread is the only caller).
+class CopyPool {
+ public:
+  explicit CopyPool(int threads) {
+    for (int i = 0; i < threads; ++i) th_.emplace_back([this] { loop(); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_) t.join();
+  }
+  void run(const std::vector<CopyJob> &jobs) {
+    if (jobs.empty()) return;
+    if (th_.empty() || jobs.size() == 1) {
+      for (const CopyJob &j : jobs) std::memcpy(j.dst, j.src, j.n);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      jobs_ = &jobs;
+      next_.store(0, std::memory_order_relaxed);
+      left_ = jobs.size();
+      ++gen_;
+    }
+    cv_.notify_all();
+    work(&jobs);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_cv_.wait(lk, [&] { return left_ == 0 && active_ == 0; });
+    jobs_ = nullptr;
+  }
+
+ private:
+  // takes copies until none are left
+  void work(const std::vector<CopyJob> *jl) {
+    size_t mine = 0;
+    for (;;) {
+      const size_t i = next_.fetch_add(1, std::memory_order_relaxed);
+      if (i >= jl->size()) break;
+      const CopyJob &j = (*jl)[i];
+      std::memcpy(j.dst, j.src, j.n);
+      ++mine;
+    }
+    if (mine) {
+      std::lock_guard<std::mutex> lk(mu_);
+      left_ -= mine;
+      if (left_ == 0 && active_ == 0) done_cv_.notify_all();
+    }
+  }
+  void loop() {
+    unsigned long long seen = 0;
+    for (;;) {
+      const std::vector<CopyJob> *jl;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return stop_ || (gen_ != seen && jobs_ != nullptr); });
+        if (stop_) return;
+        seen = gen_;
+        jl = jobs_;
+        ++active_;
+      }
+      work(jl);
+      std::lock_guard<std::mutex> lk(mu_);
+      --active_;
+      if (left_ == 0 && active_ == 0) done_cv_.notify_all();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::vector<CopyJob> *jobs_ = nullptr;
+  std::atomic<size_t> next_{0};
+  size_t left_ = 0;
+  int active_ = 0;  // pool threads inside work() for the current list
+  unsigned long long gen_ = 0;
+  bool stop_ = false;
+};
+
+// A copy of n bytes appended to `jobs` in pieces of at most COPY_PIECE bytes.
+constexpr size_t COPY_PIECE = 512 << 10;
+void add_copy(std::vector<CopyJob> &jobs, void *dst, const void *src, size_t n) {
+  for (size_t o = 0; o < n; o += COPY_PIECE)
+    jobs.push_back({static_cast<uint8_t *>(dst) + o, static_cast<const uint8_t *>(src) + o, std::min(COPY_PIECE, n - o)});
+}
 
 // One tmce_server_run_load call: the indices of its requests that have completed, appended by the
 // worker (under the server's mutex) and consumed by the call.
@@ -101,6 +202,8 @@ struct tmce_server {
   std::condition_variable cv_wake;
   unsigned long long wake_seq = 0;
   std::thread worker;
+  std::unique_ptr<CopyPool> copiers;
+  std::vector<CopyJob> copy_jobs;  // the worker's scratch list
   bool stop = false;
   int error = 0;  // first failure of the worker (guarded by mu)
   std::string error_msg;
@@ -155,6 +258,8 @@ std::vector<Req *> form_batch(tmce_server *sv, Slot &sl) {
 
 int launch_slot(tmce_server *sv, Slot &sl, std::vector<Req *> &batch) {
   std::vector<tmce_sub_batch> subs;
+  std::vector<CopyJob> &jobs = sv->copy_jobs;
+  jobs.clear();
   size_t pos = 0, ops = 0;  // pos: staged ops
   uint8_t *d_in = sl.d_io, *d_out = sl.d_io + sl.cap * sl.in_row;
   const double t = now_s();
@@ -168,7 +273,7 @@ int launch_slot(tmce_server *sv, Slot &sl, std::vector<Req *> &batch) {
       b.status = r->d_status;
       b.ct = r->d_ct;
     } else {
-      std::memcpy(sl.h_in + pos * sl.in_row, r->in, r->n * sl.in_row);
+      add_copy(jobs, sl.h_in + pos * sl.in_row, r->in, r->n * sl.in_row);
       b.in = d_in + pos * sl.in_row;
       // outputs of the request: [k][status][ct (enc)] (k 16-byte aligned: out_row is a multiple of 16)
       uint8_t *o = d_out + pos * sl.out_row;
@@ -181,6 +286,7 @@ int launch_slot(tmce_server *sv, Slot &sl, std::vector<Req *> &batch) {
     r->t_start = t;
     ops += r->n;
   }
+  sv->copiers->run(jobs);
   if (pos) HIPCHK(hipMemcpyAsync(d_in, sl.h_in, pos * sl.in_row, hipMemcpyHostToDevice, sl.stream));
   if (int rc = tmce_pipeline_run(sl.pipe, subs.data(), subs.size())) return rc;
   if (pos) HIPCHK(hipMemcpyAsync(sl.h_out, d_out, pos * sl.out_row, hipMemcpyDeviceToHost, sl.stream));
@@ -198,15 +304,20 @@ int launch_slot(tmce_server *sv, Slot &sl, std::vector<Req *> &batch) {
 void finish_slot(tmce_server *sv, Slot &sl) {
   const size_t out_ct = sl.dir == 0 ? sv->ctb : 0;
   size_t pos = 0;
-  const double t = now_s();
+  std::vector<CopyJob> &jobs = sv->copy_jobs;
+  jobs.clear();
   for (Req *r : sl.reqs) {
     if (!r->zero_copy) {
       const uint8_t *o = sl.h_out + pos * sl.out_row;
-      std::memcpy(r->k, o, r->n * 32);
-      if (r->status) std::memcpy(r->status, o + r->n * 32, r->n * sizeof(int));
-      if (sl.dir == 0) std::memcpy(r->ct, o + r->n * 36, r->n * out_ct);
+      add_copy(jobs, r->k, o, r->n * 32);
+      if (r->status) add_copy(jobs, r->status, o + r->n * 32, r->n * sizeof(int));
+      if (sl.dir == 0) add_copy(jobs, r->ct, o + r->n * 36, r->n * out_ct);
       pos += r->n;
     }
+  }
+  sv->copiers->run(jobs);
+  const double t = now_s();
+  for (Req *r : sl.reqs) {
     r->t_done = t;
     r->done = true;
     if (r->run) r->run->done.push_back(r->run_index);
@@ -375,6 +486,9 @@ int tmce_server_create(int variant, size_t slot_capacity, int slots_per_directio
       const size_t per_op = (tmce_pipeline_device_bytes(sl.pipe) + sl.cap * (sl.in_row + sl.out_row)) / sl.cap;
       sv->per_op_bytes[dir] = per_op;
     }
+  // copier threads: a quarter of the host's cores, 2 to 8
+  const unsigned hc = std::thread::hardware_concurrency();
+  sv->copiers = std::make_unique<CopyPool>((int)std::min(8u, std::max(2u, hc / 4)) - 1);
   sv->worker = std::thread(worker_main, sv);
   *out = sv;
   return 0;
