@@ -237,20 +237,37 @@ __device__ __forceinline__ void bs_mul(const uint32_t (&a)[M], const uint32_t (&
       if (i == 0 || j == M - 1) p[i + j] = a[i] & b[j];  // first term of word i + j
       else p[i + j] = and_xor(p[i + j], a[i], b[j]);
     }
+  // reduction: the high words h_k (k >= M, after what folds into them from above) land on
+  // r_j = p_j + sum of h_{j + M - e} over the exponents e of the field polynomial's low terms,
+  // summed three at a time
+  if constexpr (M == 12) {  // x^12 = x^3 + 1: h_12 += h_21, h_13 += h_22; r_j = p_j + h_{j+12} + h_{j+9}
+    p[12] ^= p[21];
+    p[13] ^= p[22];
 #pragma unroll
-  for (int k = 2 * M - 2; k >= M; --k) {
-    if constexpr (M == 12) {  // x^12 = x^3 + 1
-      p[k - 12] ^= p[k];
-      p[k - 9] ^= p[k];
-    } else {  // x^13 = x^4 + x^3 + x + 1
-      p[k - 13] ^= p[k];
-      p[k - 12] ^= p[k];
-      p[k - 10] ^= p[k];
-      p[k - 9] ^= p[k];
+    for (int j = 0; j < 12; ++j) {
+      if (j < 3) r[j] = p[j] ^ p[j + 12];
+      else if (j <= 10) r[j] = xor3(p[j], p[j + 12], p[j + 9]);
+      else r[j] = p[j] ^ p[j + 9];
+    }
+  } else {  // x^13 = x^4 + x^3 + x + 1: r_j = p_j + h_{j+13} + h_{j+12} + h_{j+10} + h_{j+9}
+    p[13] = xor3(p[13], p[23], p[22]);
+    p[14] = xor3(p[14], p[24], p[23]);
+    p[15] ^= p[24];
+#pragma unroll
+    for (int j = 0; j < 13; ++j) {
+      uint32_t t[4];
+      int nt = 0;
+      if (j + 13 <= 24) t[nt++] = p[j + 13];
+      if (j + 12 >= 13 && j + 12 <= 24) t[nt++] = p[j + 12];
+      if (j + 10 >= 13 && j + 10 <= 24) t[nt++] = p[j + 10];
+      if (j + 9 >= 13 && j + 9 <= 24) t[nt++] = p[j + 9];
+      uint32_t v = p[j];
+      int u = 0;
+      for (; u + 2 <= nt; u += 2) v = xor3(v, t[u], t[u + 1]);
+      if (u < nt) v ^= t[u];
+      r[j] = v;
     }
   }
-#pragma unroll
-  for (int i = 0; i < M; ++i) r[i] = p[i];
 }
 // the multiplier words of one bottom-up level in this lane: bit q of word b is bit b of
 // plo[d][lane] ^ phi[k(q)]; `ph` holds the phi part (bit planes, lane-uniform), plo2 is plo doubled
