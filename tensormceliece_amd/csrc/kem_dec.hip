@@ -97,14 +97,24 @@ __global__ __launch_bounds__(FRONT_WG) void k_dec_front(const uint8_t *__restric
 __device__ __forceinline__ uint32_t tab(const uint16_t *t, uint32_t byte_off) {
   return *reinterpret_cast<const uint16_t *>(reinterpret_cast<const uint8_t *>(t) + byte_off);
 }
-template <int SET>
+// Ops per wave and batch (OPS): Berlekamp-Massey runs one op per row of LR = 64 / OPS lanes.
+// Four ops on 16-lane rows, or for m = 12 with the bitsliced FFT also eight ops on 8-lane rows:
+// the per-step fixed work (row reduction, length update, loop control) is then shared by twice
+// the ops and the active coefficients are padded to 8 instead of 16 (6 % fewer VALU instructions
+// per op), and the FFT and the checks take the batch as two halves of four.  Eight-op batches
+// double the work per batch, so the launcher uses them only when every resident wave gets at
+// least two (348864 at N = 65536: 0.430 -> 0.421 ms; at N = 8192 they would be 25 % slower).
+// m = 13 keeps four: eight ops' syndrome windows do not fit LDS beside 12 waves.
+template <int SET, int OPS>
 struct DecWave {
   using P = Par<SET>;
-  // syndromes of the wave's four ops (doubled logs, 2 ZL for 0) behind SPAD zero entries, so the
-  // Berlekamp-Massey window s_{N-i} reads zero for i > N
-  static constexpr int SPAD = 16 * ceil_div(P::T + 1, 16);
-  uint16_t s[4][SPAD + P::TT];
-  uint16_t f[4][P::D];            // locator coefficients (FFT input), per op
+  static constexpr int LR = 64 / OPS;
+  // syndromes of the wave's ops (doubled logs, 2 ZL for 0) behind SPAD zero entries, so the
+  // Berlekamp-Massey window s_{N-i} reads zero for i > N; after BM, rows 4..7 hold the second
+  // half's locators and FFT buffers
+  static constexpr int SPAD = LR * ceil_div(P::T + 1, LR);
+  uint16_t s[OPS][SPAD + P::TT];
+  uint16_t f[4][P::D];            // locator coefficients (FFT input), per op of the first half
   uint16_t pos[P::IND_STRIDE];    // error positions of the op being finished
   uint64_t m64[17 * P::MSG_BLOCKS];  // session-key message of the op being finished
 };
@@ -124,28 +134,43 @@ __device__ __forceinline__ uint32_t dpp(uint32_t v) {
 // row_ror:n -- lane l of a 16-lane row reads lane (l - n) mod 16 of the same row
 constexpr int DPP_ROR1 = 0x121, DPP_ROR2 = 0x122, DPP_ROR4 = 0x124, DPP_ROR8 = 0x128;
 
-__device__ __forceinline__ uint32_t row_xor16(uint32_t v) {
-  v ^= dpp<DPP_ROR8>(v);
-  v ^= dpp<DPP_ROR4>(v);
-  v ^= dpp<DPP_ROR2>(v);
-  v ^= dpp<DPP_ROR1>(v);
+// XOR over each row of LR lanes (LR = 16: four row_ror steps; LR = 8: quad_perm swaps of lanes
+// 1 and 2 apart, then row_half_mirror, which pairs the two quads of each 8-lane half-row)
+constexpr int DPP_QP_XOR1 = 0xB1, DPP_QP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_ROR9 = 0x129;
+template <int LR>
+__device__ __forceinline__ uint32_t row_xor(uint32_t v) {
+  if constexpr (LR == 16) {
+    v ^= dpp<DPP_ROR8>(v);
+    v ^= dpp<DPP_ROR4>(v);
+    v ^= dpp<DPP_ROR2>(v);
+    v ^= dpp<DPP_ROR1>(v);
+  } else {
+    static_assert(LR == 8, "rows of 8 or 16 lanes");
+    v ^= dpp<DPP_QP_XOR1>(v);
+    v ^= dpp<DPP_QP_XOR2>(v);
+    v ^= dpp<DPP_HALF_MIRROR>(v);
+  }
   return v;
 }
 
 
-// row_ror:1 without a previous value (every lane of a rotation has a source lane)
+// row_ror:n without a previous value (every lane of a rotation has a source lane)
 __device__ __forceinline__ uint32_t ror1(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, DPP_ROR1, 0xF, 0xF, true);
 }
+__device__ __forceinline__ uint32_t ror9(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, DPP_ROR9, 0xF, 0xF, true);
+}
 
-// One Berlekamp-Massey step on the wave's four rows, for NA active coefficient slots: slots >= NA
+// One Berlekamp-Massey step on the wave's rows (one op per row of LR lanes: coefficient i of the
+// row's op in lane i mod LR of the row, slot i / LR), for NA active coefficient slots: slots >= NA
 // hold zero coefficients of C and B in every row (deg C <= L, deg B <= N + 1 - L), so they are
 // skipped.  The window W_i = s_{N-i} of slot s is read from the row's padded syndrome array at
-// wp + 32 (NSB - 1 - s) bytes (wp = the lane's window base, advanced 2 bytes per step).  Straight
+// wp + 2 LR (NSB - 1 - s) bytes (wp = the lane's window base, advanced 2 bytes per step).  Straight
 // line: a zero discrepancy turns the update factor into the zero sentinel and the length change
 // into a select, so no lane diverges.  lb = log b (doubled).
-template <int NA, int NSB, int T, int Q>
-__device__ __forceinline__ void bm_step(int N, const uint8_t *wp, int l16, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
+template <int NA, int NSB, int T, int Q, int LR>
+__device__ __forceinline__ void bm_step(int N, const uint8_t *wp, int lr, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
                                         int &L, uint32_t &lb, const uint16_t *lg_s, const uint16_t *ex_s) {
   constexpr uint32_t ZL2 = 4u * Q, Q2 = 2u * Q;  // doubled sentinel and modulus
   uint32_t lc[NA], dp = 0;
@@ -153,10 +178,10 @@ __device__ __forceinline__ void bm_step(int N, const uint8_t *wp, int l16, uint3
   for (int s = 0; s < NA; ++s) lc[s] = tab(lg_s, C[s]);
 #pragma unroll
   for (int s = 0; s < NA; ++s) {
-    const uint32_t wv = *reinterpret_cast<const uint16_t *>(wp + 32 * (NSB - 1 - s));
+    const uint32_t wv = *reinterpret_cast<const uint16_t *>(wp + 2 * LR * (NSB - 1 - s));
     dp ^= tab(ex_s, lc[s] + wv);
   }
-  const uint32_t d = row_xor16(dp);
+  const uint32_t d = row_xor<LR>(dp);
   const uint32_t ld = tab(lg_s, d);
   uint32_t lf = ld + Q2 - lb;  // log (d / b): ld - lb mod Q2, the subtraction wrapping when lf < Q2
   lf = min(lf, lf - Q2);
@@ -169,47 +194,56 @@ __device__ __forceinline__ void bm_step(int N, const uint8_t *wp, int l16, uint3
   }
   L = upd ? N + 1 - L : L;
   lb = upd ? ld : lb;
-  // B <- x B on the slots that can hold a nonzero coefficient afterwards
+  // B <- x B on the slots that can hold a nonzero coefficient afterwards: within a row a rotation
+  // by one; the row's first lane takes the previous slot's last coefficient (LR = 16: that slot
+  // rotated by one; LR = 8: by nine, which brings lane 7 (15) of the 16-lane DPP row to lane 0 (8))
   constexpr int NS = NA < NSB ? NA + 1 : NSB;
-  uint32_t R[NS];
+  uint32_t R[NS], Rc[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) R[s] = ror1(Bl[s]);
 #pragma unroll
+  for (int s = 0; s + 1 < NS; ++s) Rc[s] = LR == 16 ? R[s] : ror9(Bl[s]);
+#pragma unroll
   for (int s = 0; s < NS; ++s) {
-    const uint32_t v = (l16 == 0) ? (s ? R[s > 0 ? s - 1 : 0] : ZL2) : R[s];
-    Bl[s] = (16 * s + l16 <= T) ? v : ZL2;
+    const uint32_t v = (lr == 0) ? (s ? Rc[s > 0 ? s - 1 : 0] : ZL2) : R[s];
+    Bl[s] = (s < NSB - 1 || LR * s + lr <= T) ? v : ZL2;
   }
 }
-// A(N) = max over the wave's four rows of max(L, N + 1 - L): step N touches coefficients 0..A(N),
-// i.e. A(N) / 16 + 1 slots.  A grows by at most one per step (without a length change
+// A(N) = max over the wave's rows of max(L, N + 1 - L): step N touches coefficients 0..A(N),
+// i.e. A(N) / LR + 1 slots.  A grows by at most one per step (without a length change
 // max(L, N + 2 - L) <= A + 1; with one, max(N + 1 - L, L + 1) <= A + 1).
+template <int LR>
 __device__ __forceinline__ int bm_active(int L, int N) {
-  const int L0 = __builtin_amdgcn_readlane(L, 0), L1 = __builtin_amdgcn_readlane(L, 16);
-  const int L2 = __builtin_amdgcn_readlane(L, 32), L3 = __builtin_amdgcn_readlane(L, 48);
-  const int Lmax = max(max(L0, L1), max(L2, L3)), Lmin = min(min(L0, L1), min(L2, L3));
+  int Lmax = __builtin_amdgcn_readlane(L, 0), Lmin = Lmax;
+#pragma unroll
+  for (int r = 1; r < 64 / LR; ++r) {
+    const int Lr = __builtin_amdgcn_readlane(L, LR * r);
+    Lmax = max(Lmax, Lr);
+    Lmin = min(Lmin, Lr);
+  }
   return max(Lmax, N + 1 - Lmin);
 }
 // Berlekamp-Massey steps N.. while NA slots cover the active coefficients, then the next count
 // (the count never decreases).  Since A grows by at most one per step, NA slots (coefficients
-// 0..16 NA - 1) stay enough for 16 NA - A(N) more steps: those run without re-reading the lengths.
-template <int NA, int NSB, int T, int Q>
-__device__ __forceinline__ void bm_run(int &N, const uint8_t *wl, int l16, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
+// 0..LR NA - 1) stay enough for LR NA - A(N) more steps: those run without re-reading the lengths.
+template <int NA, int NSB, int T, int Q, int LR>
+__device__ __forceinline__ void bm_run(int &N, const uint8_t *wl, int lr, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
                                        int &L, uint32_t &lb, const uint16_t *lg_s, const uint16_t *ex_s) {
 #pragma unroll 1
   while (N < 2 * T) {
     int rem = 2 * T - N;
     if constexpr (NA < NSB) {
-      const int A = bm_active(L, N);
-      if (A > 16 * NA - 1) break;
-      rem = min(rem, 16 * NA - A);
+      const int A = bm_active<LR>(L, N);
+      if (A > LR * NA - 1) break;
+      rem = min(rem, LR * NA - A);
     }
 #pragma unroll 1
     for (int k = 0; k < rem; ++k) {
-      bm_step<NA, NSB, T, Q>(N, wl + 2 * N, l16, C, Bl, L, lb, lg_s, ex_s);
+      bm_step<NA, NSB, T, Q, LR>(N, wl + 2 * N, lr, C, Bl, L, lb, lg_s, ex_s);
       ++N;
     }
   }
-  if constexpr (NA < NSB) bm_run<NA + 1, NSB, T, Q>(N, wl, l16, C, Bl, L, lb, lg_s, ex_s);
+  if constexpr (NA < NSB) bm_run<NA + 1, NSB, T, Q, LR>(N, wl, lr, C, Bl, L, lb, lg_s, ex_s);
 }
 
 // ---- bitsliced GF(2^m) arithmetic (the decoder's bottom-up FFT, fft_form 1)
@@ -302,7 +336,7 @@ __device__ __forceinline__ void wg_fill(int tid, int wg, Src &&src, Dst &&dst) {
   }
 }
 
-template <int SET, bool BS>
+template <int SET, bool BS, int OPS>
 __global__ __launch_bounds__((DecShape<SET, BS>::WG)) __attribute__((amdgpu_waves_per_eu(DecShape<SET, BS>::WAVES_PER_EU)))
 void k_dec_decode(DecArgs a) {
   constexpr int DEC_WAVES = DecShape<SET, BS>::WAVES;  // at most; a launch may use fewer waves
@@ -311,7 +345,9 @@ void k_dec_decode(DecArgs a) {
   constexpr int NPTS = 1 << M;
   constexpr int NR = NPTS / 64;  // registers per lane holding field values (== D)
   constexpr uint32_t ZL = 2u * Q;
-  constexpr int NSB = ceil_div(T + 1, 16);  // BM coefficient slots per lane
+  constexpr int LR = DecWave<SET, OPS>::LR;  // lanes per BM row
+  static_assert(OPS == 4 || (OPS == 8 && BS && P::M == 12), "batch shapes");
+  constexpr int NSB = ceil_div(T + 1, LR);  // BM coefficient slots per lane
   constexpr int LOG2D = Dn == 64 ? 6 : 7;
   static_assert((1 << LOG2D) == Dn, "D is 64 or 128");
   static_assert(M - KD == 6 && NR == Dn, "FFT layout assumes 2^(m-KD) = 64 points per constant");
@@ -328,7 +364,7 @@ void k_dec_decode(DecArgs a) {
   // 32-point half of a 64-point butterfly span), and the monic term's register part [group][b]
   __shared__ uint32_t phib_s[BS ? 2 * KD * M : 1];
   __shared__ uint32_t pthib_s[BS ? NG * M : 1];
-  __shared__ DecWave<SET> Wv[DEC_WAVES];
+  __shared__ DecWave<SET, OPS> Wv[DEC_WAVES];
 
   const int tid = threadIdx.x, wg = __builtin_amdgcn_readfirstlane((int)blockDim.x);
   // tables doubled (byte offsets of 16-bit entries); the entries that are not copies -- log 0, the
@@ -387,15 +423,15 @@ void k_dec_decode(DecArgs a) {
   __syncthreads();
   const int nops = a.plan ? a.plan->total : a.nops;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane0 = tid & 63;  // wave: uniform (scalar)
-  constexpr int SPAD = DecWave<SET>::SPAD;
-  DecWave<SET> &w = Wv[wave];
-  for (int i = lane0; i < 4 * SPAD; i += 64) w.s[i / SPAD][i % SPAD] = 2 * ZL;  // window padding (zero)
+  constexpr int SPAD = DecWave<SET, OPS>::SPAD;
+  DecWave<SET, OPS> &w = Wv[wave];
+  for (int i = lane0; i < OPS * SPAD; i += 64) w.s[i / SPAD][i % SPAD] = 2 * ZL;  // window padding (zero)
 
-  // the wave takes batches of four ops from work counters, one per XCD (workgroups are dispatched
+  // the wave takes batches of OPS ops from work counters, one per XCD (workgroups are dispatched
   // to the 8 XCDs round-robin; counter x hands out batches x, x + 8, x + 16, ...), and from the
   // other counters once its own is exhausted: workgroups that become resident late -- the GPU
   // shared with other kernels -- take fewer ops, and each counter sees an eighth of the requests
-  const int nbat = (nops + 3) >> 2;
+  const int nbat = (nops + OPS - 1) / OPS;
   int xq = blockIdx.x & (DEC_QUEUES - 1), tried = 0;
   for (;;) {
     // the lane index re-derived in every batch through an opaque zero: values computed from it
@@ -412,10 +448,10 @@ void k_dec_decode(DecArgs a) {
       ++tried;
     }
     if (bidx >= nbat) break;
-    const int base = 4 * bidx;
-    // ---- 1. syndromes of the four ops (syndrome-major bits from the front end)
+    const int base = OPS * bidx;
+    // ---- 1. syndromes of the batch's ops (syndrome-major bits from the front end)
 #pragma unroll 1
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < OPS; ++q) {
       const int op = base + q;
       const uint32_t *Srow = a.S + (size_t)(op < nops ? op : 0) * P::RW;
       for (int j = lane; j < P::TT; j += 64) w.s[q][SPAD + j] = lg_s[op < nops ? synd_value<P>(Srow, j) : 0u];
@@ -425,34 +461,43 @@ void k_dec_decode(DecArgs a) {
     // ---- 2. Berlekamp-Massey, one op per row; C in normal form, B and the syndrome window
     //         W_i = s_{N-i} in log form
     {
-      // the lane's window base: slot s reads s_{N - 16 s - l16} at wl + 2 N + 32 (NSB - 1 - s)
-      const uint8_t *wl = reinterpret_cast<const uint8_t *>(w.s[row] + (SPAD - l16 - 16 * (NSB - 1)));
+      const int brow = lane / LR, lr = lane % LR;
+      // the lane's window base: slot s reads s_{N - LR s - lr} at wl + 2 N + 2 LR (NSB - 1 - s)
+      const uint8_t *wl = reinterpret_cast<const uint8_t *>(w.s[brow] + (SPAD - lr - LR * (NSB - 1)));
       uint32_t C[NSB], Bl[NSB];
 #pragma unroll
       for (int s = 0; s < NSB; ++s) {
-        const int i = 16 * s + l16;
+        const int i = LR * s + lr;
         C[s] = (i == 0) ? 2u : 0u;  // doubled values and logs
         Bl[s] = (i == 1) ? 0u : 2 * ZL;
       }
       int L = 0, N = 0;
       uint32_t lb = 0;  // log b, b = 1
-      bm_run<1, NSB, T, Q>(N, wl, l16, C, Bl, L, lb, lg_s, ex_s);
-      // locator sigma_j = C_{t-j} (sigma_t = C_0 = 1 is added from the power table when t = D)
-      for (int j = l16; j < Dn; j += 16) w.f[row][j] = 0;
+      bm_run<1, NSB, T, Q, LR>(N, wl, lr, C, Bl, L, lb, lg_s, ex_s);
+      // locator sigma_j = C_{t-j} (sigma_t = C_0 = 1 is added from the power table when t = D):
+      // the first half's ops into f, the second half's into their (now dead) syndrome rows, past
+      // the zero padding that the next batch's windows read
+      wave_lds_sync();
+      uint16_t *lrow = brow < 4 ? w.f[brow] : w.s[brow] + SPAD;
+      for (int j = lr; j < Dn; j += LR) lrow[j] = 0;
       wave_lds_sync();
 #pragma unroll
       for (int s = 0; s < NSB; ++s) {
-        const int i = 16 * s + l16;
-        if (i <= T && T - i < Dn) w.f[row][T - i] = (uint16_t)C[s];
+        const int i = LR * s + lr;
+        if (i <= T && T - i < Dn) lrow[T - i] = (uint16_t)C[s];
       }
       wave_lds_sync();
     }
 
-    // ---- 3. additive FFT, top-down on the D coefficients of the four locators at once (row r:
-    //         op r; the row's syndrome array is free now and serves as the second buffer): twist,
-    //         Taylor expansion at x^2 + x, even/odd split
+#pragma unroll 1
+    for (int hf = 0; hf < OPS / 4; ++hf) {
+    // ---- 3. additive FFT, top-down on the D coefficients of four locators at once (row r: op r of
+    //         the half; first half: f, and the syndrome row (free now) as the second buffer;
+    //         second half: the syndrome row, and f (free again) as the second buffer; the rows'
+    //         zero padding is never written): twist, Taylor expansion at x^2 + x, even/odd split
     {
-      uint16_t *fr = w.f[row], *tr = w.s[row] + SPAD;
+      uint16_t *fr = hf ? w.s[4 + row] + SPAD : w.f[row];
+      uint16_t *tr = hf ? w.f[row] : w.s[row] + SPAD;
       static_assert(P::TT >= Dn, "syndrome array doubles as the FFT buffer");
 #pragma unroll 1
       for (int d = 0; d < KD; ++d) {
@@ -484,10 +529,10 @@ void k_dec_decode(DecArgs a) {
       }
     }
 
-    // ---- 4. finish the four ops one after the other
+    // ---- 4. finish the half's four ops one after the other
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
-      const int op = base + q;
+      const int op = base + 4 * hf + q;
       if (op >= nops) break;
       const int lane = lane0 + opaque_zero();  // per op, as above: no per-lane masks hoisted out
       // key material of the op (one key, or its key group under a plan)
@@ -532,8 +577,8 @@ void k_dec_decode(DecArgs a) {
       // [b][e] starts as zero (the bits of e are set at the roots below; b is written last)
       constexpr int EW = (1 + P::N / 8 + 3) / 4;
       for (int k = lane; k < EW; k += 64) m32[k] = 0;
-      // the top-down result: after KD buffer swaps it is in the op's row of f (KD even) or of s
-      const uint16_t *fq = (KD & 1) ? w.s[q] + SPAD : w.f[q];
+      // the top-down result: after KD buffer swaps it is in the first buffer (KD even) or the second
+      const uint16_t *fq = (KD & 1) ? (hf ? w.f[q] : w.s[q] + SPAD) : (hf ? w.s[4 + q] + SPAD : w.f[q]);
       // ciphertext bytes into the message buffer (before the butterflies need the registers)
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
@@ -802,6 +847,7 @@ void k_dec_decode(DecArgs a) {
       }
       wave_lds_sync();
     }
+    }  // halves
   }
 }
 
@@ -908,7 +954,11 @@ hipError_t launch_dec(const DecArgs &args, hipStream_t s) {
   // (down to DEC_MIN_WAVES) when it does not, so that a small batch still reaches every CU; the
   // workgroup is the unit that holds the tables in LDS, so only its wave count shrinks
   const int max_waves = a.fft_form ? DecShape<SET, true>::WAVES : DecShape<SET, false>::WAVES;
-  const int nbat = ceil_div(a.nops, 4), cus = a.cus > 0 ? a.cus : 256;
+  const int cus = a.cus > 0 ? a.cus : 256;
+  // eight-op batches (m = 12, bitsliced FFT) when every resident wave gets at least two of them
+  const int resident = (P::M == 12 ? 2 : 1) * cus * max_waves;
+  const bool ops8 = P::M == 12 && a.fft_form && ceil_div(a.nops, 8) >= 2 * resident;
+  const int nbat = ceil_div(a.nops, ops8 ? 8 : 4);
   const int waves = nbat >= cus * max_waves ? max_waves : std::max(DEC_MIN_WAVES, std::min(max_waves, ceil_div(nbat, cus)));
   int grid = a.decode_blocks > 0 ? a.decode_blocks : (P::M == 12 ? 512 : 256);
   const int maxgrid = ceil_div(nbat, waves);
@@ -917,10 +967,15 @@ hipError_t launch_dec(const DecArgs &args, hipStream_t s) {
     const hipError_t e = launch_fill_u32(a.queue, 0u, DEC_QUEUES * DEC_QSTRIDE, s);  // a kernel, not a memset
     if (e != hipSuccess) return e;
   }
-  if (a.fft_form)
-    TMCE_TIMED("dec_decode", s, k_dec_decode<SET, true><<<grid, 64 * waves, 0, s>>>(a));
-  else
-    TMCE_TIMED("dec_decode", s, k_dec_decode<SET, false><<<grid, 64 * waves, 0, s>>>(a));
+  if (!a.fft_form)
+    TMCE_TIMED("dec_decode", s, k_dec_decode<SET, false, 4><<<grid, 64 * waves, 0, s>>>(a));
+  else if constexpr (P::M == 12) {
+    if (ops8)
+      TMCE_TIMED("dec_decode", s, k_dec_decode<SET, true, 8><<<grid, 64 * waves, 0, s>>>(a));
+    else
+      TMCE_TIMED("dec_decode", s, k_dec_decode<SET, true, 4><<<grid, 64 * waves, 0, s>>>(a));
+  } else
+    TMCE_TIMED("dec_decode", s, k_dec_decode<SET, true, 4><<<grid, 64 * waves, 0, s>>>(a));
   if (a.pc)
     TMCE_TIMED("dec_hash", s,
                k_hash_msg<SET, true><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, args.ct, a.key, a.status, a.nops,

@@ -118,6 +118,31 @@ def test_roundtrip_large(tm, orc, s, n):
     assert np.array_equal(ct_s, ct_ref) and np.array_equal(k_s, k_ref)
 
 
+def test_decoder_batch_shapes_agree(tm, orc):
+    """348864 decapsulation at N = 65536 runs eight-op batches (Berlekamp-Massey on 8-lane rows); in
+    pieces of 16384 it runs four-op batches (16-lane rows).  Both give the same keys and statuses on
+    valid, tampered and random ciphertexts, and equal the oracle on a sample of each kind."""
+    s, n = "348864", 65536
+    pk, sk, _ = keys(orc, s)
+    p = orc.params(s)
+    rng = np.random.default_rng(67)
+    seeds = rng.integers(0, 256, size=(n, 32), dtype=np.uint8)
+    ct, _, _ = tm.PublicKey(s, pk).enc(seeds)
+    tam = rng.choice(n, 4000, replace=False)
+    ct[tam, rng.integers(0, p.synd_bytes, 4000)] ^= rng.integers(1, 256, 4000).astype(np.uint8)
+    rnd = rng.choice(np.setdiff1d(np.arange(n), tam), 2000, replace=False)
+    ct[rnd] = rng.integers(0, 256, (2000, p.synd_bytes), dtype=np.uint8)
+    SK = tm.SecretKey(s, sk)
+    print(f"batch shapes {s} n={n}", flush=True)
+    k_full, st_full = SK.dec(ct)
+    parts = [SK.dec(ct[i:i + 16384]) for i in range(0, n, 16384)]
+    assert np.array_equal(k_full, np.concatenate([x[0] for x in parts]))
+    assert np.array_equal(st_full, np.concatenate([x[1] for x in parts]))
+    sample = np.concatenate([tam[:24], rnd[:24], rng.choice(n, 16, replace=False)])
+    k_ref, _ = orc.dec_batch(s, sk, ct[sample])
+    assert np.array_equal(k_full[sample], k_ref)
+
+
 @pytest.mark.parametrize("s", SETS)
 def test_single_op_api(tm, orc, s):
     """mceliece_kem_<set>_enc/_dec entry points (libmceliece interface) on one op."""
