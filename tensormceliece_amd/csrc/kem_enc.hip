@@ -598,9 +598,10 @@ __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__
   uint64_t out[4];
   // The next-block prefetch of the hash costs 34 VGPRs (128 instead of 90-95: four waves per SIMD
   // instead of five).  It pays where the column gather is short (348864: t = 64 columns of 24
-  // words; 89 against 97 us at N = 65536) and loses where the gather's loads need the occupancy
-  // (8192128: 128 columns of 52 words; 468 against 351 us).
-  constexpr bool PREFETCH = P::T * P::CWP <= 64 * 24;
+  // words; 89 against 97 us at N = 65536) or done by the whole wave (the m = 13 sets with t >= 119:
+  // 8192128 214 against 222 us), and loses where each lane's own long gather needs the occupancy
+  // (460896: 96 columns of 40 words).
+  constexpr bool PREFETCH = COOP || P::T * P::CWP <= 64 * 24;
   shake256_blocks_32<PREFETCH>(reinterpret_cast<const uint64_t *>(mrow), MBLK, out, rc);
   uint64_t *kp = reinterpret_cast<uint64_t *>(key + 32 * (size_t)op);
   if (live) {
