@@ -253,9 +253,6 @@ __device__ __forceinline__ void bm_run(int &N, const uint8_t *wl, int lr, uint32
 __device__ __forceinline__ uint32_t and_xor(uint32_t acc, uint32_t a, uint32_t b) {  // acc ^ (a & b)
   return __builtin_amdgcn_bitop3_b32(acc, a, b, 0x78);
 }
-__device__ __forceinline__ uint32_t andn_xor(uint32_t acc, uint32_t a, uint32_t m) {  // acc ^ (a & ~m)
-  return __builtin_amdgcn_bitop3_b32(acc, a, m, 0xB4);
-}
 __device__ __forceinline__ uint32_t select_bits(uint32_t m, uint32_t a, uint32_t b) {  // m ? a : b, per bit
   return __builtin_amdgcn_bitop3_b32(m, a, b, 0xCA);
 }
@@ -302,6 +299,26 @@ __device__ __forceinline__ void bs_mul(const uint32_t (&a)[M], const uint32_t (&
       r[j] = v;
     }
   }
+}
+// Exchange bit j of the position inside a word with the word index: x and y are the words with
+// index bit 0 and 1; afterwards x holds the entries whose old position bit j was 0 and y those
+// whose bit was 1, each at its old position with bit j replaced by the old word index bit.
+__device__ __forceinline__ void swap_word_bit(int j, uint32_t &x, uint32_t &y) {
+  uint32_t nx, ny;
+  if (j == 3) {  // bytes: x = [x0 y0 x2 y2], y = [x1 y1 x3 y3]
+    nx = __builtin_amdgcn_perm(y, x, 0x06020400u);
+    ny = __builtin_amdgcn_perm(y, x, 0x07030501u);
+  } else if (j == 4) {  // half words: x = [x01 y01], y = [x23 y23]
+    nx = __builtin_amdgcn_perm(y, x, 0x05040100u);
+    ny = __builtin_amdgcn_perm(y, x, 0x07060302u);
+  } else {
+    const int hj = 1 << j;
+    const uint32_t mj = j == 0 ? 0xAAAAAAAAu : j == 1 ? 0xCCCCCCCCu : 0xF0F0F0F0u;  // position bit j set
+    nx = select_bits(mj, y << hj, x);
+    ny = select_bits(mj, y, x >> hj);
+  }
+  x = nx;
+  y = ny;
 }
 // the multiplier words of one bottom-up level in this lane: bit q of word b is bit b of
 // plo[d][lane] ^ phi[k(q)]; `ph` holds the phi part (bit planes, lane-uniform), plo2 is plo doubled
@@ -648,16 +665,23 @@ void k_dec_decode(DecArgs a) {
           }
         }
       } else {
-        // bitsliced bottom-up: Vb[G][b] bit i = bit b of the value at point 64 (32 G + i) + lane
+        // bitsliced bottom-up on the points 64 R + lane, R = 0 .. NR-1 (bits r0 .. r6 of R), kept
+        // in NG words per bit plane; the level with span h = 2^k pairs R with R + h, i.e. it flips
+        // r_k.  Each level's r_k is bit 0 of the word index, so every butterfly pairs two whole
+        // words: the first level (r0) starts there, and before each later level r_k moves in from
+        // bit k - 1 of the position inside the word, which takes the previous level's bit in
+        // exchange (one bit swap between the two words of a pair).  The last swap restores the
+        // standard layout: Vb[G][b] bit i = bit b of the value at R = 32 G + i.
         const int zo = opaque_zero();
         const uint16_t *plq = plo_s + zo;
         const uint32_t *phq = phib_s + zo;
         uint32_t Vb[NG][M];
-        // the D sub-polynomial constants (lane-uniform, doubled) into bit planes: lane l reads
-        // constant 64 hh + l and one ballot per bit gives two 32-point words
+        // the D sub-polynomial constants (lane-uniform, doubled) into bit planes in the starting
+        // layout -- word G = (r6 when NG = 4, r0), bit i = (R >> 1) mod 32: lane l reads constant
+        // R = 64 hh + 2 (l mod 32) + l / 32 and one ballot per bit gives the words 2 hh and 2 hh + 1
 #pragma unroll
         for (int hh = 0; hh < NG / 2; ++hh) {
-          const uint32_t v = fq[64 * hh + lane];
+          const uint32_t v = fq[64 * hh + (((lane & 31) << 1) | (lane >> 5))];
 #pragma unroll
           for (int b = 0; b < M; ++b) {
             const uint64_t bal = __ballot((v >> (b + 1)) & 1u);
@@ -666,44 +690,29 @@ void k_dec_decode(DecArgs a) {
           }
         }
 #pragma unroll
-        for (int d = KD - 1; d >= 0; --d) {  // deepest level first
-          const int h = 1 << (M - 7 - d);
+        for (int d = KD - 1; d >= 0; --d) {  // deepest level first: h = 1, 2, ..., 2^(m-7)
+          const int k = M - 7 - d, h = 1 << k;
           const uint32_t plo2 = plq[64 * d + lane];
-          if (h < 32) {  // pairs (i, i + h) inside each 32-point word
-            uint32_t C[M];
-            bs_multipliers<M>(phq + d * M, plo2, C);
-            uint32_t Mh = 0;  // bits i with (i & h) != 0
+          if (k >= 1 && k <= 5) {  // r_k (at position bit k - 1) <-> r_(k-1) (word index bit 0)
 #pragma unroll
-            for (int i = 0; i < 32; ++i) Mh |= (uint32_t)((i & h) != 0) << i;
+            for (int G = 0; G < NG; G += 2)
 #pragma unroll
-            for (int pg = 0; pg < NG; pg += 2) {
-              // y halves of words pg (shifted down to the x positions) and pg + 1 (in place) in
-              // one word, then one 32-wide product
-              uint32_t Y[M], Z[M];
+              for (int b = 0; b < M; ++b) swap_word_bit(k - 1, Vb[G][b], Vb[G + 1][b]);
+          }
+          // the multiplier of the pair (R, R + h) depends on R mod h, which is the position bits
+          // below k (h < 32), the position (h = 32), or bit 0 of the word index and the position
+          // (h = 64, pairs (G, G + 2))
+          const int gap = h < 64 ? 1 : 2;
 #pragma unroll
-              for (int b = 0; b < M; ++b) Y[b] = select_bits(Mh, Vb[pg + 1][b], Vb[pg][b] >> h);
-              bs_mul<M>(Y, C, Z);
+          for (int G = 0; G < NG; ++G) {
+            if (G & gap) continue;
+            uint32_t C[M], Z[M];
+            bs_multipliers<M>(phq + (d + KD * (gap == 2 ? G : 0)) * M, plo2, C);
+            bs_mul<M>(Vb[G + gap], C, Z);
 #pragma unroll
-              for (int b = 0; b < M; ++b) {
-                uint32_t x0 = andn_xor(Vb[pg][b], Z[b], Mh);          // x' = x + c y
-                Vb[pg][b] = and_xor(x0, x0 << h, Mh);                 // y' = x' + y
-                uint32_t x1 = andn_xor(Vb[pg + 1][b], Z[b] >> h, Mh);
-                Vb[pg + 1][b] = and_xor(x1, x1 << h, Mh);
-              }
-            }
-          } else {  // pairs of words (G, G + h/32)
-            const int gap = h / 32;
-#pragma unroll
-            for (int G = 0; G < NG; ++G) {
-              if (G & gap) continue;
-              uint32_t C[M], Z[M];
-              bs_multipliers<M>(phq + (d + KD * (G % gap)) * M, plo2, C);
-              bs_mul<M>(Vb[G + gap], C, Z);
-#pragma unroll
-              for (int b = 0; b < M; ++b) {
-                Vb[G][b] ^= Z[b];
-                Vb[G + gap][b] ^= Vb[G][b];
-              }
+            for (int b = 0; b < M; ++b) {
+              Vb[G][b] ^= Z[b];               // x' = x + c y
+              Vb[G + gap][b] ^= Vb[G][b];     // y' = x' + y
             }
           }
         }
