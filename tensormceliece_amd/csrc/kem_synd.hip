@@ -3,8 +3,8 @@
 //
 //   k_synd_mfma    the dense GF(2) GEMM on CDNA4 INT8 MFMA (v_mfma_i32_16x16x64_i8): 0/1 bytes,
 //                  int32 accumulation (exact: K < 2^31), parity = bit 0.  The public key is held
-//                  per key as 0/1 bytes (Tb, [MTP][KP], L2 resident); the workgroup's error bits
-//                  sit in LDS and are expanded to bytes on the way into the A operand.
+//                  per key as bit rows cut into 128-bit pieces (Tb, L2 resident); the key's and the
+//                  errors' bits reach the MFMA operands by one shift per VGPR (bit 0 of each byte).
 //   k_synd_valu    the bit-packed GEMM on the VALU: one op per lane, T words wave-uniform
 //                  (scalar loads), 32-bit AND and 3-input XOR per word pair, parity by popcount.
 //   k_synd_sparse  the column gather: c = XOR of the columns of [I | T] at the set bits of e
@@ -19,6 +19,19 @@
 namespace tmce {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+
+// LDS ring slots of the MFMA GEMMs.  Ring depth does not change the GEMMs' own speed (3 to 10
+// slots within 3 %); for the decapsulation front end it sets the LDS footprint, which decides how
+// many of its workgroups share a CU with the encapsulation sampler's when the two directions run on
+// concurrent streams: at 60 KB (10 slots of the default tile) a CU that holds a sampler workgroup
+// (47 KB) takes one front-end workgroup beside it, at 36 KB two, whose waves then take issue slots
+// from the sampler's chain (mceliece348864 two-stream step 0.70 -> 0.79 ms).
+#ifndef GF2_NSLOT_SYND
+#define GF2_NSLOT_SYND 4
+#endif
+#ifndef GF2_NSLOT_FRONT
+#define GF2_NSLOT_FRONT 10
+#endif
 
 // Bits [MT + 32 w, MT + 32 w + 32) of e (zero beyond N).
 template <class P>
@@ -40,15 +53,14 @@ __device__ __forceinline__ uint32_t e_ns_word(const uint8_t *__restrict__ erow, 
   }
 }
 
-// 16 bits -> 16 bytes of 0/1 (byte i = bit i): per nibble, the multiply puts bit j at bits j + 7i
-// (i = 0..3, all distinct for j < 4, so no carries) and the mask keeps bit j at byte j.
-__device__ __forceinline__ v4i expand16(uint32_t x) {
-  v4i r;
-  r[0] = (int)(__umul24(x & 0xFu, 0x00204081u) & 0x01010101u);
-  r[1] = (int)(__umul24((x >> 4) & 0xFu, 0x00204081u) & 0x01010101u);
-  r[2] = (int)(__umul24((x >> 8) & 0xFu, 0x00204081u) & 0x01010101u);
-  r[3] = (int)(__umul24((x >> 12) & 0xFu, 0x00204081u) & 0x01010101u);
-  return r;
+// The A operand of one MFMA step from a 32-bit word w of x: VGPR v = w >> (4 s + v), so byte b of
+// VGPR v holds bit 8 b + 4 s + v of w in its bit 0 (s = 0, 1: the two MFMA steps that one word
+// feeds).  The bits above bit 0 of each byte are other bits of w: a byte x (signed int8) times a
+// 0/1 matrix entry e adds (x - bit0(x)) e, an even number, beyond bit0(x) e to the int32 sum, so
+// the sum's parity is that of the 0/1 product (|sum| <= 128 K < 2^31: no overflow).  No masking or
+// spreading of the bits is needed, one shift per VGPR.
+__device__ __forceinline__ v4i shifted_bytes(uint32_t w, int s) {
+  return v4i{(int)(w >> (4 * s)), (int)(w >> (4 * s + 1)), (int)(w >> (4 * s + 2)), (int)(w >> (4 * s + 3))};
 }
 
 // u16 pair of ciphertext bytes [2h, 2h+2) = e's first bytes, masked to the MT bits of H e.
@@ -66,44 +78,54 @@ __device__ __forceinline__ uint32_t e_head16(const uint8_t *__restrict__ erow, i
 
 // ---------------------------------------------------------------------------------- INT8 MFMA
 // Generic batched GF(2) matrix product on v_mfma_i32_16x16x64_i8: for every op, output bit r =
-// parity(M[r] . x) with x a bit row of the op's input.  0/1 bytes in, int32 accumulation (exact),
-// bit 0 out.  G describes the product (rows, depth, where x sits in the input row, output row).
+// parity(M[r] . x) with x a bit row of the op's input.  Both operands are bit rows, turned into
+// MFMA bytes whose bit 0 is the bit by one shift per VGPR (shifted_bytes); int32 accumulation
+// (exact), bit 0 out.  G describes the product (rows, depth, where x sits in the input row, output
+// row).
 //
-// The matrix is stored swizzled per 16x64 MFMA fragment: Msw[((ks * RTP) + rt) * 1024 + 16 l + b] =
-// M[16 rt + l % 16][64 ks + 16 (l / 16) + b], so one fragment is 1 KB contiguous and the fragments
-// of consecutive row tiles at one k step are adjacent.
+// The k depth is taken in pairs of MFMA steps (128 bits, G::KP a multiple of 128).  Lane l of a wave
+// reads, for its 16x16 tiles, the 32-bit word l / 16 of the pair's 128 bits of its op's x row (A
+// operand, op row l % 16 of the tile) and of its matrix row (B operand, matrix row l % 16 of the
+// tile), and feeds MFMA step s of the pair with shifted_bytes(w, s): byte 4 v + b of both operands
+// carries bit 32 (l / 16) + 8 b + 4 s + v of the pair.  An MFMA pairs byte p of A lane l with byte p
+// of B lane l' for l / 16 = l' / 16, so the products are those of equal k, in an order that does not
+// change the sum.  The matrix is stored per key as Mb[pair][row][4 words] (RTP*16 rows, zero past
+// ROWS and past depth KB): the plain bit rows, cut into 16-byte pieces per pair.
 //
 // Workgroup tile: BM = 16*OPW*WO ops x BN = 16*RTW*WR matrix rows, WO x WR waves, each wave
-// OPW x RTW 16x16 int32 tiles.  The k loop runs in super-steps of KSUB MFMA steps (64 deep each)
-// over a three-slot LDS ring filled two super-steps ahead by global_load_lds (HBM/L2 -> LDS with no
-// registers):
-//   B (matrix): BN/16 fragments (1 KB each) per step, 16 bytes per lane;
-//   A (x bits): BM x 64 bits per step, one aligned 32-bit word per lane, expanded to 0/1 bytes on
-//               the way into the MFMA A operand.
+// OPW x RTW 16x16 int32 tiles.  The k loop runs in super-steps of KPAIR step pairs over an LDS ring
+// of NS slots filled NS - 1 super-steps ahead by global_load_lds (HBM/L2 -> LDS with no registers):
+// per pair BN x 16 bytes of matrix rows and BM x 16 bytes of x, one dword per lane and copy
+// instruction.  The x rows stream from HBM, so the copies need to be issued far ahead of their use.
 // The copies are issued from inline assembly with explicit vmcnt waits (the compiler, which cannot
 // tell the ring slots apart, would otherwise wait for every outstanding copy before any LDS read).
-// Before the barrier ending super-step q a wave waits until only the copies it issued for q+2 are
-// outstanding, so slot q+1 is complete for every wave after the barrier.
+// Before the barrier ending super-step q a wave waits until only the copies it issued for q+2 ..
+// q+NS-1 are outstanding (vector memory loads complete in issue order), so slot q+1 is complete
+// for every wave after the barrier.
 // Output: the D tile gives lane l column l%16 (a matrix row) of rows 4*(l/16)+v (ops); a wave ballot
 // of bit 0 per v gives, in 16-bit chunk g, 16 consecutive output bits of op 4g+v, stored as one
 // 16-bit word.
-template <class G, int OPW, int RTW, int WO, int WR, int KSUB>
+template <class G, int OPW, int RTW, int WO, int WR, int KPAIR, int NS>
 struct GemmShape {
   static constexpr int NT = 64 * WO * WR;
   static constexpr int BM = 16 * OPW * WO, BN = 16 * RTW * WR;
-  static constexpr int KS = G::KP / 64;
-  static constexpr int QS = ceil_div(KS, KSUB);     // super-steps
-  static constexpr int NF = BN / 16;                // fragments per step
-  static constexpr int FPW = NF / (WO * WR);        // fragments per wave per step
-  static constexpr int APT = 2 * BM / NT;           // x words per thread per step
-  static constexpr int BSTEP = NF * 1024, ASTEP = BM * 8;
-  static constexpr int SLOT = KSUB * (BSTEP + ASTEP);
-  static constexpr int NI = KSUB * (FPW + APT);     // copies one wave issues per super-step
-  static constexpr size_t LDS = (size_t)3 * SLOT;
-  static_assert(FPW >= 1 && FPW * WO * WR == NF, "fragments must split evenly over the waves");
-  static_assert(APT >= 1 && APT * NT == 2 * BM, "x words must split evenly over the threads");
-  static_assert(ceil_div(G::ROWS, BN) * NF <= G::RTP, "row blocks exceed the padded matrix");
-  static_assert(NI < 64, "vmcnt range");
+  static_assert(G::KP % 128 == 0, "k depth in pairs of MFMA steps");
+  static constexpr int KS2 = G::KP / 128;           // step pairs
+  static constexpr int QS = ceil_div(KS2, KPAIR);   // super-steps
+  static constexpr int MPT = 4 * BN / NT;           // matrix words per thread per pair
+  static constexpr int APT = 4 * BM / NT;           // x words per thread per pair
+  static constexpr int MPAIR = BN * 16, APAIR = BM * 16;
+  static constexpr int SLOT = KPAIR * (MPAIR + APAIR);
+  static constexpr int NI = KPAIR * (MPT + APT);    // copies one wave issues per super-step
+  // ring slots: NS, capped by the vmcnt range and the CU's LDS
+  static constexpr int NSL = NS < 2 + 63 / NI ? (NS < 160 * 1024 / SLOT ? NS : 160 * 1024 / SLOT)
+                                              : (2 + 63 / NI < 160 * 1024 / SLOT ? 2 + 63 / NI : 160 * 1024 / SLOT);
+  static constexpr size_t LDS = (size_t)NSL * SLOT;
+  static_assert(MPT >= 1 && MPT * NT == 4 * BN, "matrix words must split evenly over the threads");
+  static_assert(APT >= 1 && APT * NT == 4 * BM, "x words must split evenly over the threads");
+  static_assert(ceil_div(G::ROWS, BN) * BN <= 16 * G::RTP, "row blocks exceed the padded matrix");
+  static_assert(NSL >= 3 && (NSL - 2) * NI < 64, "vmcnt range");
+  static_assert(LDS <= 160 * 1024, "LDS ring exceeds the CU's LDS");
   static_assert(PLAN_ALIGN % BM == 0, "plan key groups must hold whole workgroup tiles");
 };
 
@@ -121,16 +143,29 @@ template <int N>
 __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
+// wait until at most k * NI copies are outstanding (k wave-uniform, 0 <= k <= K)
+template <int NI, int K>
+__device__ __forceinline__ void wait_pending(int k) {
+  if constexpr (K > 0) {
+    if (k >= K) {
+      wait_vmcnt<K * NI>();
+      return;
+    }
+    wait_pending<NI, K - 1>(k);
+  } else {
+    wait_vmcnt<0>();
+  }
+}
 
 // xw: x as aligned 32-bit words, row stride xstride words, xvalid words per row (reads past it are
 // clamped to the last word: the matrix is zero there).  head: the original input rows (G::head16).
-template <class G, int OPW, int RTW, int WO, int WR, int KSUB>
+template <class G, int OPW, int RTW, int WO, int WR, int KPAIR, int NS>
 __global__ __launch_bounds__(64 * WO * WR) void k_gf2_mfma(const uint32_t *__restrict__ xw, int xstride, int xvalid,
                                                           const uint8_t *__restrict__ head,
-                                                          const uint4 *__restrict__ Msw,
+                                                          const uint32_t *__restrict__ Mb,
                                                           uint8_t *__restrict__ out, int nops,
                                                           const PlanDev *__restrict__ plan) {
-  using S = GemmShape<G, OPW, RTW, WO, WR, KSUB>;
+  using S = GemmShape<G, OPW, RTW, WO, WR, KPAIR, NS>;
   extern __shared__ __align__(16) uint8_t lds[];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
@@ -139,44 +174,42 @@ __global__ __launch_bounds__(64 * WO * WR) void k_gf2_mfma(const uint32_t *__res
   if (plan) {  // key groups are PLAN_ALIGN-aligned (a multiple of BM): one key per workgroup
     nops = plan->total;
     if (op0 >= nops) return;
-    Msw = uniform_ptr(reinterpret_cast<const uint4 *>(G::plan_matrix(plan->keys[plan_group(plan, op0)])));
+    Mb = uniform_ptr(reinterpret_cast<const uint32_t *>(G::plan_matrix(plan->keys[plan_group(plan, op0)])));
   }
-  const int rt0 = blockIdx.y * S::NF;  // first row tile of the block
+  const int row0 = blockIdx.y * S::BN;  // first matrix row of the block
   const uint32_t lds0 = lds_off(lds);
 
-  // x word i = tid + NT*p of a step is word 2 ks + i%2 of op i/2 (rows past the batch: last op)
+  // per pair, copy i = tid + NT*p is word i%4 of x row (op) i/4, or of matrix row i/4; in LDS the
+  // pair's pieces are [row][4 words] for the matrix, then [op][4 words] for x
   const uint32_t *xrow[S::APT];
 #pragma unroll
   for (int p = 0; p < S::APT; ++p) {
-    const int op = op0 + (tid + S::NT * p) / 2;
+    const int op = op0 + (tid + S::NT * p) / 4;
     xrow[p] = xw + (size_t)(op < nops ? op : nops - 1) * xstride;
   }
+  const uint32_t *mrow = Mb + (size_t)row0 * 4 + tid;
   auto issue = [&](int q, int slot) {
 #pragma unroll
-    for (int u = 0; u < KSUB; ++u) {
-      const int ks = min(q * KSUB + u, S::KS - 1);  // a short last super-step repeats a valid step
-      const uint32_t sb = lds0 + slot * S::SLOT + u * S::BSTEP;
+    for (int u = 0; u < KPAIR; ++u) {
+      const int pr = min(q * KPAIR + u, S::KS2 - 1);  // a short last super-step repeats a valid pair
+      const uint32_t sb = lds0 + slot * S::SLOT + u * (S::MPAIR + S::APAIR);
 #pragma unroll
-      for (int f = 0; f < S::FPW; ++f) {
-        const int fr = wave * S::FPW + f;
-        dma16(Msw + ((size_t)ks * G::RTP + rt0 + fr) * 64 + lane, __builtin_amdgcn_readfirstlane(sb + fr * 1024));
-      }
-      const uint32_t sa = lds0 + slot * S::SLOT + KSUB * S::BSTEP + u * S::ASTEP;
+      for (int p = 0; p < S::MPT; ++p)
+        dma4(mrow + (size_t)pr * G::RTP * 64 + S::NT * p, __builtin_amdgcn_readfirstlane(sb + (wave * 64 + S::NT * p) * 4));
+      const uint32_t sa = sb + S::MPAIR;
 #pragma unroll
       for (int p = 0; p < S::APT; ++p) {
-        const int w = min(2 * ks + ((tid + S::NT * p) & 1), xvalid - 1);
+        const int w = min(4 * pr + ((tid + S::NT * p) & 3), xvalid - 1);
         dma4(xrow[p] + w, __builtin_amdgcn_readfirstlane(sa + (wave * 64 + S::NT * p) * 4));
       }
     }
   };
 
-  issue(0, 0);
-  if (S::QS > 1) {
-    issue(1, 1);
-    wait_vmcnt<S::NI>();
-  } else {
-    wait_vmcnt<0>();
-  }
+  // prologue: super-steps 0 .. NS-2 into slots 0 .. NS-2, slot 0 complete before the barrier
+  constexpr int NS_ = S::NSL;
+  const int pre = min(NS_ - 1, S::QS);
+  for (int q = 0; q < pre; ++q) issue(q, q);
+  wait_pending<S::NI, NS_ - 2>(pre - 1);
   __syncthreads();
 
   v4i acc[OPW][RTW];
@@ -188,32 +221,37 @@ __global__ __launch_bounds__(64 * WO * WR) void k_gf2_mfma(const uint32_t *__res
   int slot = 0;
 #pragma unroll 1
   for (int q = 0; q < S::QS; ++q) {
-    const int s1 = slot == 2 ? 0 : slot + 1, s2 = s1 == 2 ? 0 : s1 + 1;
-    const bool ahead = q + 2 < S::QS;
-    if (ahead) issue(q + 2, s2);
+    // slot (q + NS_ - 1) mod NS_ = (q - 1) mod NS_ was read in super-step q - 1, before its barrier
+    const int qa = q + NS_ - 1;
+    if (qa < S::QS) issue(qa, slot == 0 ? NS_ - 1 : slot - 1);
 #pragma unroll
-    for (int u = 0; u < KSUB; ++u) {
-      if (KSUB == 1 || q * KSUB + u < S::KS) {
-        const uint8_t *sb = lds + slot * S::SLOT + u * S::BSTEP;
-        const uint16_t *sa = reinterpret_cast<const uint16_t *>(lds + slot * S::SLOT + KSUB * S::BSTEP + u * S::ASTEP);
-        v4i b[RTW];
+    for (int u = 0; u < KPAIR; ++u) {
+      if (KPAIR == 1 || q * KPAIR + u < S::KS2) {
+        const uint32_t *sm = reinterpret_cast<const uint32_t *>(lds + slot * S::SLOT + u * (S::MPAIR + S::APAIR));
+        const uint32_t *sa = sm + S::MPAIR / 4;
+        uint32_t mw[RTW], xwd[OPW];
 #pragma unroll
-        for (int j = 0; j < RTW; ++j) {
-          const uint4 bv = reinterpret_cast<const uint4 *>(sb)[(RTW * wr + j) * 64 + lane];
-          b[j] = v4i{(int)bv.x, (int)bv.y, (int)bv.z, (int)bv.w};
-        }
+        for (int j = 0; j < RTW; ++j) mw[j] = sm[(16 * (RTW * wr + j) + li) * 4 + lg];
 #pragma unroll
-        for (int i = 0; i < OPW; ++i) {
-          const v4i a = expand16(sa[(16 * OPW * wo + 16 * i + li) * 4 + lg]);
+        for (int i = 0; i < OPW; ++i) xwd[i] = sa[(16 * OPW * wo + 16 * i + li) * 4 + lg];
 #pragma unroll
-          for (int j = 0; j < RTW; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[j], acc[i][j], 0, 0, 0);
+        for (int h = 0; h < 2; ++h) {
+          v4i b[RTW];
+#pragma unroll
+          for (int j = 0; j < RTW; ++j) b[j] = shifted_bytes(mw[j], h);
+#pragma unroll
+          for (int i = 0; i < OPW; ++i) {
+            const v4i a = shifted_bytes(xwd[i], h);
+#pragma unroll
+            for (int j = 0; j < RTW; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[j], acc[i][j], 0, 0, 0);
+          }
         }
       }
     }
-    if (ahead) wait_vmcnt<S::NI>();
-    else wait_vmcnt<0>();
+    // copies of super-steps q+2 .. min(q+NS-1, QS-1) may stay outstanding
+    wait_pending<S::NI, NS_ - 2>(min(qa, S::QS - 1) - (q + 1));
     __syncthreads();
-    slot = s1;
+    slot = slot == NS_ - 1 ? 0 : slot + 1;
   }
 
   // parity bits -> 16-bit words in LDS ([op][row tile] per wave; the ring is free after the last
@@ -237,7 +275,7 @@ __global__ __launch_bounds__(64 * WO * WR) void k_gf2_mfma(const uint32_t *__res
       uint16_t *orow = reinterpret_cast<uint16_t *>(out + (size_t)op * G::OUT_BYTES);
 #pragma unroll
       for (int j = 0; j < RTW; ++j) {
-        const int h = rt0 + RTW * wr + j;  // 16-bit word index of the output row
+        const int h = row0 / 16 + RTW * wr + j;  // 16-bit word index of the output row
         if (16 * h < G::OUT_BITS) orow[h] = (uint16_t)(ep[o * RTW + j] ^ G::head16(hrow, h));
       }
     }
@@ -249,11 +287,12 @@ __global__ __launch_bounds__(64 * WO * WR) void k_gf2_mfma(const uint32_t *__res
 template <int SET>
 struct GemmSynd {
   using P = Par<SET>;
-  static constexpr int ROWS = P::MT, KB = P::K, KP = round_up(KB, 64);
+  static constexpr int ROWS = P::MT, KB = P::K, KP = round_up(KB, 128);
   static constexpr int RTP = round_up(ceil_div(ROWS, 16), 16);
   static constexpr int IN_BYTES = P::E_BYTES, OUT_BYTES = P::SYND_BYTES, OUT_BITS = 8 * P::SYND_BYTES;
   static constexpr bool DIRECT = P::MT % 32 == 0 && P::N % 32 == 0;
   static constexpr int XOFF_WORDS = P::MT / 32, XSTRIDE = P::E_BYTES / 4, XVALID = P::K / 32;
+  static constexpr int NSLOT = GF2_NSLOT_SYND;
   __device__ static uint32_t x_word(const uint8_t *row, int w) { return e_ns_word<P>(row, w); }
   __device__ static uint32_t head16(const uint8_t *row, int h) { return e_head16<P>(row, h); }
   __device__ static const void *plan_matrix(const KeyDev &) { return nullptr; }  // no plan form
@@ -264,11 +303,12 @@ struct GemmSynd {
 template <int SET>
 struct GemmFront {
   using P = Par<SET>;
-  static constexpr int ROWS = P::R, KB = P::KBITS, KP = round_up(KB, 64);
+  static constexpr int ROWS = P::R, KB = P::KBITS, KP = round_up(KB, 128);
   static constexpr int RTP = round_up(ceil_div(ROWS, 16), 16);
   static constexpr int IN_BYTES = P::SYND_BYTES, OUT_BYTES = 4 * P::RW, OUT_BITS = P::R;
   static constexpr bool DIRECT = P::SYND_BYTES % 4 == 0;
   static constexpr int XOFF_WORDS = 0, XSTRIDE = P::SYND_BYTES / 4, XVALID = P::SYND_BYTES / 4;
+  static constexpr int NSLOT = GF2_NSLOT_FRONT;
   __device__ static uint32_t x_word(const uint8_t *row, int w) {  // ciphertext rows are 2-byte aligned
     const uint16_t *h = reinterpret_cast<const uint16_t *>(row);
     uint32_t v = 0;
@@ -293,30 +333,32 @@ __global__ void k_repack_x(const uint8_t *__restrict__ x, uint32_t *__restrict__
 }
 
 template <class G>
-struct MswDims {
-  static constexpr int KS = G::KP / 64;
-  static constexpr size_t BYTES = (size_t)KS * G::RTP * 1024;
+struct MswDims {  // Mb[pair][RTP * 16 rows][4 words]
+  static constexpr int KS2 = G::KP / 128;
+  static constexpr size_t WORDS = (size_t)KS2 * G::RTP * 16 * 4;
+  static constexpr size_t BYTES = WORDS * 4;
 };
 
-template <class G, int OPW, int RTW, int WO, int WR, int KSUB>
+template <class G, int OPW, int RTW, int WO, int WR, int KPAIR, int NS = G::NSLOT>
 hipError_t launch_gf2_cfg(const uint32_t *xw, int xstride, int xvalid, const uint8_t *x, const uint8_t *Msw,
                           uint8_t *out, int nops, const PlanDev *plan, const char *name, hipStream_t s) {
-  using S = GemmShape<G, OPW, RTW, WO, WR, KSUB>;
+  using S = GemmShape<G, OPW, RTW, WO, WR, KPAIR, NS>;
   static const hipError_t attr = hipFuncSetAttribute(
-      reinterpret_cast<const void *>(&k_gf2_mfma<G, OPW, RTW, WO, WR, KSUB>),
+      reinterpret_cast<const void *>(&k_gf2_mfma<G, OPW, RTW, WO, WR, KPAIR, NS>),
       hipFuncAttributeMaxDynamicSharedMemorySize, (int)S::LDS);
   if (attr != hipSuccess) return attr;
   dim3 grid(ceil_div(nops, S::BM), ceil_div(G::ROWS, S::BN));
   TMCE_TIMED(name, s,
-             (k_gf2_mfma<G, OPW, RTW, WO, WR, KSUB><<<grid, S::NT, S::LDS, s>>>(
-                 xw, xstride, xvalid, x, reinterpret_cast<const uint4 *>(Msw), out, nops, plan)));
+             (k_gf2_mfma<G, OPW, RTW, WO, WR, KPAIR, NS><<<grid, S::NT, S::LDS, s>>>(
+                 xw, xstride, xvalid, x, reinterpret_cast<const uint32_t *>(Msw), out, nops, plan)));
   return hipGetLastError();
 }
 
-// Tile shapes offered to the tuner: (OPW, RTW, WO, WR, KSUB) -> BM x BN workgroup tiles
-//   0: 256 x 128, 8 waves, 1 step per barrier     1: 256 x 128, 8 waves, 2 steps per barrier
-//   2: 256 x 256, 8 waves, 2 steps per barrier    3: 256 x 256, 8 waves (128 ops each), 2 steps
-//   4: 256 x 128, 4 waves (128 ops each), 2 steps 5: 256 x 256, 8 waves, 1 step per barrier
+// Tile shapes offered to the tuner: (OPW, RTW, WO, WR, KPAIR) -> BM x BN workgroup tiles
+//   0: 256 x 128, 8 waves of 32 ops x 128 rows, 1 pair per barrier
+//   1: 256 x 128, 8 waves of 64 x 64, 1 pair per barrier      2: 256 x 256, 8 waves of 64 x 128, 1 pair
+//   3: 256 x 256, 8 waves of 128 x 64, 1 pair                 4: 256 x 128, 4 waves of 128 x 64, 1 pair
+//   5: 256 x 128, 8 waves of 64 x 64, 2 pairs per barrier
 // x: input rows (IN_BYTES each); scratch: n * KP/8 bytes when the x bits are not word aligned in
 // place (G::DIRECT false), else unused.
 template <class G>
@@ -336,12 +378,12 @@ hipError_t launch_gf2(const uint8_t *x, uint32_t *scratch, const uint8_t *Msw, u
     xstride = xvalid = G::KP / 32;
   }
   switch (cfg) {
-    case 0: return launch_gf2_cfg<G, 4, 4, 4, 2, 1>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
-    case 1: return launch_gf2_cfg<G, 4, 4, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
-    case 2: return launch_gf2_cfg<G, 4, 8, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
-    case 3: return launch_gf2_cfg<G, 8, 4, 2, 4, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
-    case 4: return launch_gf2_cfg<G, 8, 4, 2, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
-    case 5: return launch_gf2_cfg<G, 4, 8, 4, 2, 1>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
+    case 0: return launch_gf2_cfg<G, 2, 8, 8, 1, 1>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
+    case 1: return launch_gf2_cfg<G, 4, 4, 4, 2, 1>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
+    case 2: return launch_gf2_cfg<G, 4, 8, 4, 2, 1>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
+    case 3: return launch_gf2_cfg<G, 8, 4, 2, 4, 1>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
+    case 4: return launch_gf2_cfg<G, 8, 4, 2, 2, 1>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
+    case 5: return launch_gf2_cfg<G, 4, 4, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
   }
   return hipErrorInvalidValue;
 }
@@ -483,17 +525,21 @@ hipError_t launch_synd(const SyndArgs &a, int method, int cfg, hipStream_t s) {
 }
 
 // Key operands of the dense forms, built on the device from a row-major bit matrix (rows of `rw`
-// 32-bit words, bits at or past `kb` ignored): Msw as described at k_gf2_mfma; Tv[(rb*KW + w)*32 + i]
-// = word w of row 32 rb + i (zero past MT).
-__global__ void k_build_msw(const uint32_t *__restrict__ rows, int nrows, int rw, int kb, int rtp, int ks_n,
-                            uint8_t *__restrict__ out) {
+// 32-bit words, bits at or past `kb` ignored): Mb as described at k_gf2_mfma (word w of row r of
+// pair pr = bits [128 pr + 32 w, +32) of row r, zero past nrows and past kb);
+// Tv[(rb*KW + w)*32 + i] = word w of row 32 rb + i (zero past MT).
+__global__ void k_build_msw(const uint32_t *__restrict__ rows, int nrows, int rw, int kb, int rtp, int ks2,
+                            uint32_t *__restrict__ out) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)ks_n * rtp * 1024) return;
-  const int b = (int)(idx & 15), l = (int)((idx >> 4) & 63);
-  const size_t frag = idx >> 10;
-  const int rt = (int)(frag % rtp), ks = (int)(frag / rtp);
-  const int r = 16 * rt + (l & 15), k = 64 * ks + 16 * (l >> 4) + b;
-  out[idx] = (r < nrows && k < kb) ? (uint8_t)((rows[(size_t)r * rw + k / 32] >> (k % 32)) & 1u) : (uint8_t)0;
+  if (idx >= (size_t)ks2 * rtp * 64) return;
+  const int w = (int)(idx & 3), r = (int)((idx >> 2) % (16 * (size_t)rtp)), pr = (int)(idx / (64 * (size_t)rtp));
+  const int k0 = 128 * pr + 32 * w;  // first bit of the word
+  uint32_t v = 0;
+  if (r < nrows && k0 < kb) {
+    v = rows[(size_t)r * rw + k0 / 32];
+    if (kb - k0 < 32) v &= (1u << (kb - k0)) - 1u;
+  }
+  out[idx] = v;
 }
 __global__ void k_build_tv(const uint32_t *__restrict__ Trows, uint32_t *__restrict__ Tv, int mt, int kw, int rbs) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -505,8 +551,9 @@ __global__ void k_build_tv(const uint32_t *__restrict__ Trows, uint32_t *__restr
 
 template <class G>
 hipError_t build_msw(const uint32_t *rows, int rw, uint8_t *out, hipStream_t s) {
-  const size_t nb = MswDims<G>::BYTES;
-  k_build_msw<<<(unsigned)((nb + 255) / 256), 256, 0, s>>>(rows, G::ROWS, rw, G::KB, G::RTP, MswDims<G>::KS, out);
+  const size_t nw = MswDims<G>::WORDS;
+  k_build_msw<<<(unsigned)((nw + 255) / 256), 256, 0, s>>>(rows, G::ROWS, rw, G::KB, G::RTP, MswDims<G>::KS2,
+                                                            reinterpret_cast<uint32_t *>(out));
   return hipGetLastError();
 }
 
