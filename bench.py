@@ -303,16 +303,23 @@ ENC_KERNELS = ("gen_e", "enc_finish")
 DEC_KERNELS = ("dec_front_mfma", "dec_front", "dec_decode", "dec_hash")
 
 
-def direction_roofline(s, n, rates, concurrent=True):
+def direction_roofline(s, n, rates, concurrent=True, sampler_key="gen_e"):
     """Whole-direction VALU and LDS issue fractions at the given op rates (ops/s per direction):
     the VALU / LDS wave-instructions one op costs in each direction (sum over the direction's
     kernels, from the committed counters of the same set at n ops per launch) times the rate,
     against the chip's issue peaks.  concurrent: the directions ran at these rates at the same time
     (their issue rates add); otherwise they were timed one after the other, and the combined figure
-    is all instructions over the summed time, with each direction's own fraction beside it."""
+    is all instructions over the summed time, with each direction's own fraction beside it.
+    sampler_key: the counters entry of the sampler launch shape that ran ("gen_e_step": the
+    two-stream step's 64-workgroup grid, whose instruction count differs from the full grid's)."""
     ctr = load_counters(f"{s}:{n}")
     if not ctr:
         return None
+    if sampler_key != "gen_e":
+        if sampler_key not in ctr:
+            return None
+        ctr = dict(ctr)
+        ctr["gen_e"] = ctr[sampler_key]
 
     def frac(v, l):
         return ({"achieved": round(v / 1e9, 1), "peak": VALU_PEAK_G, "unit": "G VALU wave-instr/s",
@@ -862,7 +869,14 @@ def main():
     roofline.update({"achieved": rb.get("achieved"), "peak": rb.get("peak"), "unit": rb.get("unit"),
                      "frac": rb.get("frac")})
     roofline.setdefault("traffic", None)
-    step_roofline = direction_roofline(s, n, {"enc": n / t_step, "dec": n / t_step})
+    # the two-stream step's sampler runs on args.step_gen_blocks workgroups; the committed counters
+    # hold that launch shape for the default grid of 64 ("gen_e_step")
+    if args.one_stream:
+        step_roofline = direction_roofline(s, n, {"enc": n / t_step, "dec": n / t_step})
+    elif args.step_gen_blocks == 64:
+        step_roofline = direction_roofline(s, n, {"enc": n / t_step, "dec": n / t_step}, sampler_key="gen_e_step")
+    else:
+        step_roofline = None
 
     # ---- CPU baseline: the word-level port on every host core, a bounded sample of the same
     #      inputs; its outputs are the byte-for-byte check of the GPU's
