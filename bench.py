@@ -760,7 +760,7 @@ def main():
     ap.add_argument("--sections", default="syndrome,per_set,c1,c3,c4,c5,keygen,variants")
     ap.add_argument("--one-stream", action="store_true",
                     help="headline steps serialised on one stream only (profiling: no concurrent kernels)")
-    ap.add_argument("--step-gen-blocks", type=int, default=64,
+    ap.add_argument("--step-gen-blocks", type=int, default=96,
                     help="sampler grid (workgroups) during the two-stream headline step: a small grid keeps "
                          "its lane groups busy from the work queue and leaves most SIMDs to the decoder")
     ap.add_argument("--tune", action="append", default=[], metavar="KNOB=VALUE",
@@ -870,10 +870,10 @@ def main():
                      "frac": rb.get("frac")})
     roofline.setdefault("traffic", None)
     # the two-stream step's sampler runs on args.step_gen_blocks workgroups; the committed counters
-    # hold that launch shape for the default grid of 64 ("gen_e_step")
+    # hold that launch shape ("gen_e_step", with its grid) for the default grid
     if args.one_stream:
         step_roofline = direction_roofline(s, n, {"enc": n / t_step, "dec": n / t_step})
-    elif args.step_gen_blocks == 64:
+    elif load_counters(f"{s}:{n}").get("gen_e_step", {}).get("grid_workgroups") == args.step_gen_blocks:
         step_roofline = direction_roofline(s, n, {"enc": n / t_step, "dec": n / t_step}, sampler_key="gen_e_step")
     else:
         step_roofline = None
