@@ -369,19 +369,31 @@ void k_dec_decode(DecArgs a) {
   static_assert((1 << LOG2D) == Dn, "D is 64 or 128");
   static_assert(M - KD == 6 && NR == Dn, "FFT layout assumes 2^(m-KD) = 64 points per constant");
   constexpr int NPHI = (1 << (M - 6)) - (1 << (M - 6 - KD));  // sum over levels of 2^(m-7-d)
-  __shared__ uint16_t lg_s[NPTS];
-  __shared__ uint16_t ex_s[4 * Q + 2];
-  __shared__ uint16_t phi_s[NPHI];
-  __shared__ uint16_t pthi_s[NR];
-  __shared__ uint16_t twl_s[2 * Dn];
-  __shared__ uint16_t plo_s[(KD + 1) * 64];  // butterfly multipliers' lane parts [KD][64], then (lane)^t
-  __shared__ uint16_t lpt_s[BS ? 1 : NPHI * 64];  // logs of the butterfly multipliers [phi index][lane]
   constexpr int NG = NR / 32;                     // bitsliced words per plane and lane (2 or 4)
-  // bitsliced form: the multipliers' register parts as bit planes, [level + KD g][b] (g: which
-  // 32-point half of a 64-point butterfly span), and the monic term's register part [group][b]
-  __shared__ uint32_t phib_s[BS ? 2 * KD * M : 1];
-  __shared__ uint32_t pthib_s[BS ? NG * M : 1];
-  __shared__ DecWave<SET, OPS> Wv[DEC_WAVES];
+  // the workgroup's LDS in one object, the field tables first: their bases then fit the 16-bit
+  // offset field of ds_read (lg at 0, ex at 2^(m+1) bytes), so a lookup at a register index costs
+  // no address add (placed by the compiler after the per-wave buffers, lg sat at 66 KB and every
+  // Berlekamp-Massey step spent one add per active slot on its base)
+  struct Lds {
+    uint16_t lg[NPTS];
+    uint16_t ex[4 * Q + 2];
+    uint16_t phi[NPHI];
+    uint16_t pthi[NR];
+    uint16_t twl[2 * Dn];
+    uint16_t plo[(KD + 1) * 64];  // butterfly multipliers' lane parts [KD][64], then (lane)^t
+    uint16_t lpt[BS ? 1 : NPHI * 64];  // logs of the butterfly multipliers [phi index][lane]
+    // bitsliced form: the multipliers' register parts as bit planes, [level + KD g][b] (g: which
+    // 32-point half of a 64-point butterfly span), and the monic term's register part [group][b]
+    alignas(4) uint32_t phib[BS ? 2 * KD * M : 1];
+    uint32_t pthib[BS ? NG * M : 1];
+    alignas(16) DecWave<SET, OPS> Wv[DEC_WAVES];
+  };
+  __shared__ Lds L;
+  static_assert(2 * NPTS < 65536, "the power table's base fits a ds_read offset");
+  uint16_t *const lg_s = L.lg, *const ex_s = L.ex, *const phi_s = L.phi, *const pthi_s = L.pthi;
+  uint16_t *const twl_s = L.twl, *const plo_s = L.plo, *const lpt_s = L.lpt;
+  uint32_t *const phib_s = L.phib, *const pthib_s = L.pthib;
+  DecWave<SET, OPS> *const Wv = L.Wv;
 
   const int tid = threadIdx.x, wg = __builtin_amdgcn_readfirstlane((int)blockDim.x);
   // tables doubled (byte offsets of 16-bit entries); the entries that are not copies -- log 0, the

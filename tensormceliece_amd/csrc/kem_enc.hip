@@ -431,6 +431,9 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
   }
 }
 
+#ifndef FIN_COOP_MIN
+#define FIN_COOP_MIN 960  // 16-byte column words per op above which the wave gathers cooperatively
+#endif
 template <int SET, bool PC>
 __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__ ind, const uint4 *__restrict__ cols,
                                                     uint32_t *__restrict__ msg, uint8_t *__restrict__ ct,
@@ -459,7 +462,7 @@ __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__
   //      instruction, coalesced; the spans' sums are combined across lanes and parked in LDS, from
   //      where each lane takes its own op's c for the hash below.  Short ones stay one op per lane
   //      (fewer instructions, and the key's columns stay in L2).
-  constexpr bool COOP = C4 * P::T > 960;
+  constexpr bool COOP = C4 * P::T > FIN_COOP_MIN;
   constexpr int SPN = C4 <= 8 ? 8 : (C4 <= 16 ? 16 : 32);
   constexpr int CPI = 64 / SPN;
   constexpr int NJ = ceil_div(P::T, CPI);  // load instructions per op
