@@ -390,7 +390,7 @@ int tmce_dec_batch_device(tmce_key *sk, unsigned char *d_k, int *d_status, const
    bytes in libmceliece's layout, any weight; ct: n * CIPHERTEXTBYTES.  Needs a public-key handle. */
 enum {
   TMCE_SYND_SPARSE = 0, /* XOR of the columns of [I|T] at the set bits of e (engine default) */
-  TMCE_SYND_MFMA = 1,   /* dense GF(2) GEMM on INT8 MFMA over 0/1 bytes, int32 accumulate, parity */
+  TMCE_SYND_MFMA = 1,   /* dense GF(2) GEMM on INT8 MFMA (bit in bit 0 of each byte), int32 accumulate, parity */
   TMCE_SYND_VALU = 2    /* dense bit-packed GEMM: 32-bit AND + XOR, popcount parity */
 };
 int tmce_syndrome_batch(tmce_key *pk, unsigned char *ct, const unsigned char *e, size_t n, int method);

@@ -23,10 +23,10 @@ struct tmce_key {
   bool pad_bad = false;
   uint32_t *cols = nullptr;
   uint32_t *Trows = nullptr;  // plain form: T as MT rows of ceil(K/32) words
-  uint8_t *Tb = nullptr;      // syndrome GEMM operands, built on first use: 0/1 bytes (MFMA) ...
+  uint8_t *Tb = nullptr;      // syndrome GEMM operands, built on first use: bit rows in MFMA step-pair pieces ...
   uint32_t *Tv = nullptr;     // ... and 32-row blocks of words (VALU)
   uint32_t *F = nullptr;
-  uint8_t *Fsw = nullptr;     // F as 0/1 bytes in MFMA fragment order (INT8 MFMA front end)
+  uint8_t *Fsw = nullptr;     // F as bit rows in MFMA step-pair pieces (INT8 MFMA front end)
   uint32_t *Fcol = nullptr;   // syndrome map over all N positions, [N][FCW] (re-encryption check)
   uint16_t *alpha = nullptr, *la = nullptr, *lu = nullptr;
   uint16_t *ainv = nullptr;   // field element -> support index (0xFFFF off the support)

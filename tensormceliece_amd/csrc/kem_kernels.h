@@ -148,7 +148,7 @@ struct DecArgs {
   int pc;                // plaintext confirmation (pc variants)
   uint8_t *ct_synd;      // pc: scratch [n][SYND_BYTES], the syndrome part of each ciphertext
   const uint32_t *F;     // key: [R][KWP] front-end matrix (plane-major rows)
-  const uint8_t *Fsw;    // key: F as 0/1 bytes in MFMA fragment order
+  const uint8_t *Fsw;    // key: F as bit rows in MFMA step-pair pieces
   int front_mfma;        // front end on INT8 MFMA (1) or on the VALU (0)
   int front_cfg;         // MFMA tile shape of the front end
   uint32_t *xscratch;    // gf2_scratch_bytes(set, GF2_FRONT, n) bytes (MFMA front end; null if 0)
@@ -194,7 +194,7 @@ struct PlainDecScratch {
 // Syndrome alone (PKE encryption of given error vectors).
 struct SyndArgs {
   const uint8_t *e;      // [n][N/8] error vectors (libmceliece byte layout)
-  const uint8_t *Tb;     // key: T as 0/1 bytes in MFMA fragment order (INT8 MFMA form), or null
+  const uint8_t *Tb;     // key: T as bit rows in MFMA step-pair pieces (INT8 MFMA form), or null
   const uint32_t *Tv;    // key: T per 32-row block as [word][row] (VALU form), or null
   const uint32_t *cols;  // key: column-major [I | T] (sparse form)
   uint8_t *ct;           // [n][SYND_BYTES]

@@ -1,5 +1,5 @@
 """Syndrome alone (PKE encryption c = H e over GF(2)): the oracle is pinned on the CPU, the three GPU
-forms (sparse column gather, INT8 MFMA GEMM over 0/1 bytes, bit-packed VALU GEMM) are checked
+forms (sparse column gather, INT8 MFMA GEMM, bit-packed VALU GEMM) are checked
 bit-exact against it and against each other.
 
 Pins of oracle_syndrome_batch (LM:crypto_kem/<set>/vec/encrypt.c syndrome, :94-132):
