@@ -37,10 +37,13 @@ constexpr int DEC_MIN_WAVES = 4;  // fewest waves per decoder workgroup a launch
 // workgroups, two per CU, four waves per SIMD (at most 128 VGPRs); m = 13 (tables 80 KB): one
 // 12-wave workgroup per CU, three waves per SIMD (at most 168 VGPRs); the table form of the FFT
 // (BS false) keeps 8 waves for m = 13, its multiplier-log table does not leave LDS for 12
-template <int SET, bool BS> struct DecShape {
-  static constexpr int WAVES = Par<SET>::M == 12 ? DEC_WAVES_M12 : (BS ? DEC_WAVES_M13 : 8);
+#ifndef DEC_OPS8_M13
+#define DEC_OPS8_M13 0  // eight-op batches for m = 13 too (8-wave workgroups: eight windows fit LDS)
+#endif
+template <int SET, bool BS, int OPS = 4> struct DecShape {
+  static constexpr int WAVES = Par<SET>::M == 12 ? DEC_WAVES_M12 : (BS ? (OPS == 8 ? 8 : DEC_WAVES_M13) : 8);
   static constexpr int WG = 64 * WAVES;
-  static constexpr int WAVES_PER_EU = Par<SET>::M == 12 ? 4 : (BS ? 3 : 2);
+  static constexpr int WAVES_PER_EU = Par<SET>::M == 12 ? 4 : (BS ? (OPS == 8 ? 2 : 3) : 2);
 };
 // decoder work counters: DEC_QUEUES of them, DEC_QSTRIDE words apart (separate 128-byte lines)
 constexpr int DEC_QUEUES = 8, DEC_QSTRIDE = 32;
@@ -354,16 +357,16 @@ __device__ __forceinline__ void wg_fill(int tid, int wg, Src &&src, Dst &&dst) {
 }
 
 template <int SET, bool BS, int OPS>
-__global__ __launch_bounds__((DecShape<SET, BS>::WG)) __attribute__((amdgpu_waves_per_eu(DecShape<SET, BS>::WAVES_PER_EU)))
+__global__ __launch_bounds__((DecShape<SET, BS, OPS>::WG)) __attribute__((amdgpu_waves_per_eu(DecShape<SET, BS, OPS>::WAVES_PER_EU)))
 void k_dec_decode(DecArgs a) {
-  constexpr int DEC_WAVES = DecShape<SET, BS>::WAVES;  // at most; a launch may use fewer waves
+  constexpr int DEC_WAVES = DecShape<SET, BS, OPS>::WAVES;  // at most; a launch may use fewer waves
   using P = Par<SET>;
   constexpr int M = P::M, T = P::T, Dn = P::D, KD = P::KD, Q = P::Q;
   constexpr int NPTS = 1 << M;
   constexpr int NR = NPTS / 64;  // registers per lane holding field values (== D)
   constexpr uint32_t ZL = 2u * Q;
   constexpr int LR = DecWave<SET, OPS>::LR;  // lanes per BM row
-  static_assert(OPS == 4 || (OPS == 8 && BS && P::M == 12), "batch shapes");
+  static_assert(OPS == 4 || (OPS == 8 && BS && (P::M == 12 || DEC_OPS8_M13)), "batch shapes");
   constexpr int NSB = ceil_div(T + 1, LR);  // BM coefficient slots per lane
   constexpr int LOG2D = Dn == 64 ? 6 : 7;
   static_assert((1 << LOG2D) == Dn, "D is 64 or 128");
@@ -974,11 +977,15 @@ hipError_t launch_dec(const DecArgs &args, hipStream_t s) {
   // waves per workgroup: the full shape when the batch gives every CU a workgroup of them, fewer
   // (down to DEC_MIN_WAVES) when it does not, so that a small batch still reaches every CU; the
   // workgroup is the unit that holds the tables in LDS, so only its wave count shrinks
-  const int max_waves = a.fft_form ? DecShape<SET, true>::WAVES : DecShape<SET, false>::WAVES;
+  const int max_waves4 = a.fft_form ? DecShape<SET, true>::WAVES : DecShape<SET, false>::WAVES;
   const int cus = a.cus > 0 ? a.cus : 256;
-  // eight-op batches (m = 12, bitsliced FFT) when every resident wave gets at least two of them
-  const int resident = (P::M == 12 ? 2 : 1) * cus * max_waves;
-  const bool ops8 = P::M == 12 && a.fft_form && ceil_div(a.nops, 8) >= 2 * resident;
+  // eight-op batches (bitsliced FFT; m = 12, and m = 13 when built with them) when every resident
+  // wave gets at least two of them
+  constexpr bool OPS8 = P::M == 12 || DEC_OPS8_M13;
+  const int max_waves8 = DecShape<SET, true, 8>::WAVES;
+  const int resident = (P::M == 12 ? 2 : 1) * cus * max_waves8;
+  const bool ops8 = OPS8 && a.fft_form && ceil_div(a.nops, 8) >= 2 * resident;
+  const int max_waves = ops8 ? max_waves8 : max_waves4;
   const int nbat = ceil_div(a.nops, ops8 ? 8 : 4);
   const int waves = nbat >= cus * max_waves ? max_waves : std::max(DEC_MIN_WAVES, std::min(max_waves, ceil_div(nbat, cus)));
   int grid = a.decode_blocks > 0 ? a.decode_blocks : (P::M == 12 ? 512 : 256);
@@ -990,7 +997,7 @@ hipError_t launch_dec(const DecArgs &args, hipStream_t s) {
   }
   if (!a.fft_form)
     TMCE_TIMED("dec_decode", s, k_dec_decode<SET, false, 4><<<grid, 64 * waves, 0, s>>>(a));
-  else if constexpr (P::M == 12) {
+  else if constexpr (OPS8) {
     if (ops8)
       TMCE_TIMED("dec_decode", s, k_dec_decode<SET, true, 8><<<grid, 64 * waves, 0, s>>>(a));
     else
