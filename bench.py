@@ -380,7 +380,7 @@ def section_syndrome(tm, orc, cores, log):
             res[meth] = sum(v["avg_ms"] for v in kt.values())
             outs[meth] = d_ct.download((n, p["ct"]))
         best_cfg, best_ms = None, None
-        for cfg in range(6):
+        for cfg in range(8):
             tm.tune_set(s, "synd_mfma_cfg", cfg)
             kt = timed_kernels(tm, lambda: tm.syndrome_device(K, d_ct, d_e, n, "mfma"), 5)
             ms = sum(v["avg_ms"] for v in kt.values())

@@ -398,11 +398,11 @@ int tmce_syndrome_batch_device(tmce_key *pk, unsigned char *d_ct, const unsigned
                                int method, void *stream);
 
 /* tuning knobs per parameter set (set by the autotuner, tensormceliece_amd.autotune):
-   TMCE_TUNE_SYND_MFMA_CFG  tile shape of the MFMA syndrome GEMM (0..5)
+   TMCE_TUNE_SYND_MFMA_CFG  tile shape of the MFMA syndrome GEMM (0..7)
    TMCE_TUNE_GEN_BLOCKS     persistent grid of the sampler (0 = built-in default)
    TMCE_TUNE_DECODE_BLOCKS  grid of the decoder (0 = built-in default)
    TMCE_TUNE_FRONT_MFMA     decapsulation front end S = F c: 1 = INT8 MFMA GEMM (default), 0 = VALU
-   TMCE_TUNE_FRONT_CFG      tile shape of the MFMA front end (0..5)
+   TMCE_TUNE_FRONT_CFG      tile shape of the MFMA front end (0..7)
    TMCE_TUNE_GEN_FORM       sampler: 1 = pipelined lane groups (default), 0 = one op per lane
    TMCE_TUNE_FFT_FORM       decoder's bottom-up additive FFT: 1 = bitsliced GF(2^m) arithmetic
                             (default), 0 = log/exp table lookups in LDS

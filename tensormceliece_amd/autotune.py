@@ -17,8 +17,8 @@ import tensormceliece_amd as tm
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 
 CANDIDATES = {
-    "synd_mfma_cfg": list(range(6)),
-    "front": [(0, 0)] + [(1, c) for c in range(6)],  # (front_mfma, front_cfg)
+    "synd_mfma_cfg": list(range(8)),
+    "front": [(0, 0)] + [(1, c) for c in range(8)],  # (front_mfma, front_cfg)
     # (gen_form, gen_blocks): the per-lane sampler keeps at most n/256 workgroups busy, the
     # pipelined one n * (lanes per op) / 256; larger grids are clamped to that
     "gen": [(0, 64), (0, 128), (0, 256), (0, 512), (1, 128), (1, 192), (1, 256), (1, 384), (1, 512), (1, 1024),

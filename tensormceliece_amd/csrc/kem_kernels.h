@@ -203,7 +203,7 @@ struct SyndArgs {
 };
 // method: 0 = sparse column gather, 1 = INT8 MFMA GEMM (cfg = tile shape index), 2 = VALU GEMM
 hipError_t launch_synd_set(int set, const SyndArgs &a, int method, int cfg, hipStream_t s);
-constexpr int SYND_MFMA_CFGS = 6;
+constexpr int SYND_MFMA_CFGS = 8;
 // Swizzled 0/1-byte operands of the INT8 MFMA GF(2) products: GF2_SYND = the public key T
 // (MT x K), GF2_FRONT = the decapsulation front-end matrix F (R x KBITS).
 enum { GF2_SYND = 0, GF2_FRONT = 1 };

@@ -359,6 +359,9 @@ hipError_t launch_gf2_cfg(const uint32_t *xw, int xstride, int xvalid, const uin
 //   1: 256 x 128, 8 waves of 64 x 64, 1 pair per barrier      2: 256 x 256, 8 waves of 64 x 128, 1 pair
 //   3: 256 x 256, 8 waves of 128 x 64, 1 pair                 4: 256 x 128, 4 waves of 128 x 64, 1 pair
 //   5: 256 x 128, 8 waves of 64 x 64, 2 pairs per barrier
+//   6: 128 x 64, 4 waves of 64 x 32, 1 pair               7: 128 x 128, 8 waves of 32 x 64, 1 pair
+// (6 and 7 give four and two times the workgroups of 1 for small batches: N = 8192 at 256 x 128
+// tiles is 192 workgroups for 256 CUs)
 // x: input rows (IN_BYTES each); scratch: n * KP/8 bytes when the x bits are not word aligned in
 // place (G::DIRECT false), else unused.
 template <class G>
@@ -384,6 +387,8 @@ hipError_t launch_gf2(const uint8_t *x, uint32_t *scratch, const uint8_t *Msw, u
     case 3: return launch_gf2_cfg<G, 8, 4, 2, 4, 1>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
     case 4: return launch_gf2_cfg<G, 8, 4, 2, 2, 1>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
     case 5: return launch_gf2_cfg<G, 4, 4, 4, 2, 2>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
+    case 6: return launch_gf2_cfg<G, 4, 2, 2, 2, 1>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
+    case 7: return launch_gf2_cfg<G, 2, 4, 4, 2, 1>(xw, xstride, xvalid, x, Msw, out, nops, plan, name, s);
   }
   return hipErrorInvalidValue;
 }

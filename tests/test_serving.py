@@ -232,10 +232,15 @@ def test_serving_17_keys_splits_batches(tm, orc):
     kps = _gpu_keypairs(tm, s, 17, 0x17)
     wl = serving.Workload(s, kps, pool_per_key=512)
     rng = np.random.default_rng(52)
-    # one large request per direction first: while they run (one slot per direction), the small
-    # requests behind them queue up, so the next batch of each direction meets all 17 keys
-    reqs = [("enc", 0, rng.integers(0, 256, (30000, 32), dtype=np.uint8), None),
-            ("dec", 0, rng.integers(0, 256, (30000, tm.PARAMS[s]["ct"]), dtype=np.uint8), None)]
+    # three slot-filling requests per direction first: while they run one after the other (one slot
+    # per direction), the small requests behind them queue up, so the batch of each direction that
+    # follows them meets all 17 keys.  (With a single large request the small ones could still be
+    # arriving when its batch completed: the worker holds the server's lock while it stages a
+    # batch, which delays the submitting thread.)
+    reqs = []
+    for _ in range(3):
+        reqs += [("enc", 0, rng.integers(0, 256, (32000, 32), dtype=np.uint8), None),
+                 ("dec", 0, rng.integers(0, 256, (32000, tm.PARAMS[s]["ct"]), dtype=np.uint8), None)]
     for i in range(68):  # keys 0..16 round-robin, so every batch of 16 groups meets a 17th key
         k = i % 17
         n = int(rng.integers(1, 64))
@@ -251,7 +256,7 @@ def test_serving_17_keys_splits_batches(tm, orc):
         st = server.stats()
     finally:
         server.close()
-    # 35 requests per direction under 17 keys: one slot per direction needs at least two batches each
+    # 37 requests per direction under 17 keys: one slot per direction needs at least two batches each
     assert st["batches"] >= 4 and st["requests"] == len(reqs)
     assert st["max_batch_keys"] == 16  # a batch filled all 16 key groups and the 17th key waited
     check_alone(tm, wl, reqs, res["outputs"])

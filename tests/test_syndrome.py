@@ -89,7 +89,7 @@ def test_syndrome_parity(tm, orc, s, method):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("cfg", range(6))
+@pytest.mark.parametrize("cfg", range(8))
 @pytest.mark.parametrize("s", ["348864", "6960119"])
 def test_syndrome_mfma_tile_shapes(tm, orc, s, cfg):
     """Every MFMA tile shape offered to the autotuner gives the oracle's bytes."""
@@ -147,7 +147,7 @@ def test_front_end_forms(tm, orc, s):
     SK = tm.SecretKey(s, sk)
     old = (tm.tune_get(s, "front_mfma"), tm.tune_get(s, "front_cfg"))
     try:
-        for mf, cfg in [(0, 0)] + [(1, c) for c in range(6)]:
+        for mf, cfg in [(0, 0)] + [(1, c) for c in range(8)]:
             tm.tune_set(s, "front_mfma", mf)
             tm.tune_set(s, "front_cfg", cfg)
             print(f"front {s} mfma={mf} cfg={cfg}", flush=True)
