@@ -81,7 +81,10 @@ def run_poisson(server, wl, reqs, rate, keep_outputs=False, slab=None):
     gaps = rng.exponential(1.0 / rate, len(reqs)) if rate > 0 else np.zeros(len(reqs))
     arrivals = np.cumsum(gaps)
     ctb = server.params["ct"]
-    new = (lambda shape, dt: slab.array(shape, dt)) if slab is not None else (lambda shape, dt: np.zeros(shape, dt))
+    # outputs written before the run (np.full touches every page; np.zeros maps pages lazily and the
+    # server's copier threads would take the first-touch page faults inside the timed run: ~0.5 GB
+    # of fresh output arrays for the serving configuration's 6000 requests)
+    new = (lambda shape, dt: slab.array(shape, dt)) if slab is not None else (lambda shape, dt: np.full(shape, 0, dt))
     items, outputs = [], []
     for d, k, inp, _ in reqs:
         n = inp.shape[0]
