@@ -209,6 +209,13 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e(const uint8_t *__restrict__ se
 #ifndef GEN_BITMAPS
 #define GEN_BITMAPS 2
 #endif
+#ifndef GEN_G3
+#define GEN_G3 1  // three-block attempts on three-lane groups (21 per wave, lane 63 idle)
+#endif
+// lanes per op of the pipelined sampler: at least the SHAKE blocks per attempt (B); two or four
+// lanes keep the group inside a DPP quad, three leave every lane of the group busy every step for
+// B = 3 (four-lane groups would leave one lane idle per step) at the cost of lane shuffles
+constexpr int gen_group(int B) { return B == 2 ? 2 : (B == 3 && GEN_G3 ? 3 : 4); }
 // DPP quad_perm: lane i of each group of four reads lane (CTRL >> 2 i) & 3 of the group
 template <int CTRL>
 __device__ __forceinline__ uint32_t quad_perm(uint32_t v) {
@@ -221,13 +228,15 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
                                                        unsigned *__restrict__ queue, const PlanDev *__restrict__ plan) {
   using P = Par<SET>;
   constexpr int B = P::DRAW_BLOCKS;
-  constexpr int G = B == 2 ? 2 : 4;  // lanes per op
+  constexpr int G = gen_group(B);  // lanes per op
   static_assert(B <= G, "a lane finishes its attempt before the rotation returns to it");
   constexpr uint32_t GFMASK = (1u << P::M) - 1u;
   constexpr int NEW = ceil_div(P::N, 32);
   constexpr int NBM = GEN_BITMAPS;                   // bitmaps (candidates per round) per 16-lane group
   constexpr int DPP_PREV = G == 2 ? 0xB1 : 0x93;    // quad_perm: lane r reads lane r - 1 of its group
   constexpr int DPP_GROUP0 = G == 2 ? 0xA0 : 0x00;  // quad_perm: each lane reads its group's lane 0
+  (void)DPP_PREV;
+  (void)DPP_GROUP0;
   constexpr int KV = ceil_div(P::T, 16);            // draws per lane of a candidate
   __shared__ uint16_t lst[(P::T + 1) * GEN_WG];     // [index][thread]; row T takes draws past the first T
   __shared__ uint32_t bmp[GEN_WAVES][4 * NBM][NEW + 1];
@@ -236,7 +245,21 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
   const int wave = tid >> 6;
   const int grp = lane >> 4, g16 = lane & 15;  // collision check: four 16-lane groups
   const int r = lane % G, gl = lane - r;        // place in the op group
+  const bool in_grp = gl + G <= 64;             // G = 3: lane 63 has no group and takes no op
   const uint64_t gmask = ((1ull << G) - 1ull) << gl;
+  // lane r of a group reads lane r - 1 (mod G) of its group, or the group's lane 0
+  const int src_prev = in_grp ? gl + (r == 0 ? G - 1 : r - 1) : lane, src_first = in_grp ? gl : lane;
+  auto from_prev = [&](uint32_t v) -> uint32_t {
+    if constexpr (G == 3) return (uint32_t)__shfl((int)v, src_prev);
+    else return quad_perm<DPP_PREV>(v);
+  };
+  auto from_first = [&](uint32_t v) -> uint32_t {
+    if constexpr (G == 3) return (uint32_t)__shfl((int)v, src_first);
+    else return quad_perm<DPP_GROUP0>(v);
+  };
+  auto mod_g = [](int x) { return G == 3 ? (x + 3) % 3 : (x & (G - 1)); };  // x >= -(G - 1)
+  (void)src_prev;
+  (void)src_first;
   for (int i = lane; i < 4 * NBM * (NEW + 1); i += 64) bmp[wave][i / (NEW + 1)][i % (NEW + 1)] = 0;
   const KeccakRC rc;
   if (plan) nops = plan->total;
@@ -275,7 +298,7 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
     // ---- refill: group leaders without an op take the next ones from the queue (one atomic per
     //      wave and step); a group with a new op starts at step 0 with the op's seed in lane 0
     if (!done) {
-      const bool need = r == 0 && op < 0;
+      const bool need = r == 0 && op < 0 && in_grp;
       const uint64_t needmask = __ballot(need);
       if (needmask) {
         const int leader = __ffsll((long long)needmask) - 1;
@@ -294,7 +317,7 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
             for (int i = 0; i < 8; ++i) st[i] = sp[i];
           }
         }
-        const int gop = (int)quad_perm<DPP_GROUP0>((uint32_t)nop);
+        const int gop = (int)from_first((uint32_t)nop);
         if (gop != op) s = 0;
         op = gop;
       }
@@ -302,7 +325,7 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
     if (!__any(op >= 0)) break;
 
     // ---- one Keccak-f per lane: this lane's attempt k = s - b, block b (b = 0: it starts attempt s)
-    const int b = (s - r) & (G - 1);
+    const int b = mod_g(s - r);
     const int k = s - b;
     const bool live = op >= 0 && k >= 0 && k < P::MAX_ATTEMPTS;
     const bool active = live && b < B;
@@ -327,10 +350,10 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
       uint32_t x[8];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        x[2 * i] = quad_perm<DPP_PREV>((uint32_t)A[i]);
-        x[2 * i + 1] = quad_perm<DPP_PREV>((uint32_t)(A[i] >> 32));
+        x[2 * i] = from_prev((uint32_t)A[i]);
+        x[2 * i + 1] = from_prev((uint32_t)(A[i] >> 32));
       }
-      if (((s + 1 - r) & (G - 1)) == 0) {
+      if (mod_g(s + 1 - r) == 0) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) st[i] = x[i];
       }
@@ -698,9 +721,9 @@ hipError_t launch_enc(const EncArgs &a, hipStream_t s) {
   // persistent grid: enough resident waves to fill the chip, lanes (or lane groups) pull ops from the
   // queue
   if (a.gen_form == 1) {
-    constexpr int G = P::DRAW_BLOCKS == 2 ? 2 : 4;
+    constexpr int G = gen_group(P::DRAW_BLOCKS);
     int grid = a.gen_blocks > 0 ? a.gen_blocks : 2048;
-    const int maxgrid = ceil_div(a.nops * G, GEN_WG);
+    const int maxgrid = ceil_div(a.nops, GEN_WAVES * (64 / G));  // 64 / G op groups per wave
     if (grid > maxgrid) grid = maxgrid;
     TMCE_TIMED("gen_e", s,
                k_gen_e_pipe<SET><<<grid, GEN_WG, 0, s>>>(a.seeds, a.ind, a.msg, a.attempts, a.nops, a.queue, a.plan));
