@@ -134,7 +134,7 @@ struct Engine {
   DevBuf S, msg_dec, dqueue, xs_dec, p_loc, p_erow_dec, p_weight, ct_synd;
   ScratchUse use_dec;
   int form = 0;                                // 0 = engine form, 1 = plain per-stage form
-  int synd_cfg[NUM_SETS] = {1, 1, 1, 1, 1};    // MFMA syndrome tile shape per set (tunable)
+  int synd_cfg[NUM_SETS] = {7, 7, 7, 7, 7};    // MFMA syndrome tile shape per set (tunable)
   int gen_blocks[NUM_SETS] = {0, 0, 0, 0, 0};  // sampler grid per set (0 = built-in default)
   int decode_blocks[NUM_SETS] = {0, 0, 0, 0, 0};
   int front_mfma[NUM_SETS] = {1, 1, 1, 1, 1};  // decapsulation front end: 1 = INT8 MFMA, 0 = VALU

@@ -20,17 +20,18 @@ namespace tmce {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-// LDS ring slots of the MFMA GEMMs.  Ring depth does not change the GEMMs' own speed (3 to 10
-// slots within 3 %); for the decapsulation front end it sets the LDS footprint, which decides how
-// many of its workgroups share a CU with the encapsulation sampler's when the two directions run on
-// concurrent streams: at 60 KB (10 slots of the default tile) a CU that holds a sampler workgroup
-// (47 KB) takes one front-end workgroup beside it, at 36 KB two, whose waves then take issue slots
-// from the sampler's chain (mceliece348864 two-stream step 0.70 -> 0.79 ms).
+// LDS ring of the MFMA GEMMs.  Ring depth does not change the GEMMs' own speed (3 to 10 slots
+// within 3 %); for the decapsulation front end it sets the LDS footprint, which decides how many of
+// its workgroups share a CU with the encapsulation sampler's when the two directions run on
+// concurrent streams: at 60 KB a CU that holds a sampler workgroup (47 KB) takes one front-end
+// workgroup beside it, at 36 KB two, whose waves then take issue slots from the sampler's chain
+// (mceliece348864 two-stream step 0.70 -> 0.79 ms).  So the syndrome GEMM uses a fixed 4 slots and
+// the front end as many slots as fill GF2_FRONT_LDS bytes, whatever its tile.
 #ifndef GF2_NSLOT_SYND
 #define GF2_NSLOT_SYND 4
 #endif
-#ifndef GF2_NSLOT_FRONT
-#define GF2_NSLOT_FRONT 10
+#ifndef GF2_FRONT_LDS
+#define GF2_FRONT_LDS (60 * 1024)
 #endif
 
 // Bits [MT + 32 w, MT + 32 w + 32) of e (zero beyond N).
@@ -106,7 +107,7 @@ __device__ __forceinline__ uint32_t e_head16(const uint8_t *__restrict__ erow, i
 // of bit 0 per v gives, in 16-bit chunk g, 16 consecutive output bits of op 4g+v, stored as one
 // 16-bit word.
 template <class G, int OPW, int RTW, int WO, int WR, int KPAIR, int NS>
-struct GemmShape {
+struct GemmShape {  // NS > 0: ring slots; NS < 0: as many slots as fill -NS bytes
   static constexpr int NT = 64 * WO * WR;
   static constexpr int BM = 16 * OPW * WO, BN = 16 * RTW * WR;
   static_assert(G::KP % 128 == 0, "k depth in pairs of MFMA steps");
@@ -117,9 +118,10 @@ struct GemmShape {
   static constexpr int MPAIR = BN * 16, APAIR = BM * 16;
   static constexpr int SLOT = KPAIR * (MPAIR + APAIR);
   static constexpr int NI = KPAIR * (MPT + APT);    // copies one wave issues per super-step
-  // ring slots: NS, capped by the vmcnt range and the CU's LDS
-  static constexpr int NSL = NS < 2 + 63 / NI ? (NS < 160 * 1024 / SLOT ? NS : 160 * 1024 / SLOT)
-                                              : (2 + 63 / NI < 160 * 1024 / SLOT ? 2 + 63 / NI : 160 * 1024 / SLOT);
+  // ring slots: NS (or -NS bytes' worth, at least 3), capped by the vmcnt range and the CU's LDS
+  static constexpr int NSQ = NS > 0 ? NS : (-NS / SLOT > 3 ? -NS / SLOT : 3);
+  static constexpr int NSL = NSQ < 2 + 63 / NI ? (NSQ < 160 * 1024 / SLOT ? NSQ : 160 * 1024 / SLOT)
+                                               : (2 + 63 / NI < 160 * 1024 / SLOT ? 2 + 63 / NI : 160 * 1024 / SLOT);
   static constexpr size_t LDS = (size_t)NSL * SLOT;
   static_assert(MPT >= 1 && MPT * NT == 4 * BN, "matrix words must split evenly over the threads");
   static_assert(APT >= 1 && APT * NT == 4 * BM, "x words must split evenly over the threads");
@@ -308,7 +310,7 @@ struct GemmFront {
   static constexpr int IN_BYTES = P::SYND_BYTES, OUT_BYTES = 4 * P::RW, OUT_BITS = P::R;
   static constexpr bool DIRECT = P::SYND_BYTES % 4 == 0;
   static constexpr int XOFF_WORDS = 0, XSTRIDE = P::SYND_BYTES / 4, XVALID = P::SYND_BYTES / 4;
-  static constexpr int NSLOT = GF2_NSLOT_FRONT;
+  static constexpr int NSLOT = -(GF2_FRONT_LDS);  // slots filling that many bytes
   __device__ static uint32_t x_word(const uint8_t *row, int w) {  // ciphertext rows are 2-byte aligned
     const uint16_t *h = reinterpret_cast<const uint16_t *>(row);
     uint32_t v = 0;
