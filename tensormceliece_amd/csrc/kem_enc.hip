@@ -457,6 +457,12 @@ __global__ __launch_bounds__(GEN_WG) void k_gen_e_pipe(const uint8_t *__restrict
 #ifndef FIN_COOP_MIN
 #define FIN_COOP_MIN 960  // 16-byte column words per op above which the wave gathers cooperatively
 #endif
+#ifndef FIN_COOP_LOADS
+// column loads in flight per round of the cooperative gather: 32 covers every set's op in one round
+// (t columns, four per load instruction), so each op costs one L2 round trip instead of four
+// (6960119 finish 0.267 -> 0.202 ms at N = 65536, 6688128 and 8192128 -7 to -8 %; 8 in flight before)
+#define FIN_COOP_LOADS 32
+#endif
 template <int SET, bool PC>
 __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__ ind, const uint4 *__restrict__ cols,
                                                     uint32_t *__restrict__ msg, uint8_t *__restrict__ ct,
@@ -489,7 +495,7 @@ __global__ __launch_bounds__(256) void k_enc_finish(const uint16_t *__restrict__
   constexpr int SPN = C4 <= 8 ? 8 : (C4 <= 16 ? 16 : 32);
   constexpr int CPI = 64 / SPN;
   constexpr int NJ = ceil_div(P::T, CPI);  // load instructions per op
-  constexpr int FIN_LOADS = 8;             // load instructions in flight
+  constexpr int FIN_LOADS = FIN_COOP_LOADS;  // load instructions in flight
   static_assert(C4 <= SPN, "a column fits its span");
   __shared__ uint4 cst[COOP ? 4 : 1][COOP ? 64 : 1][COOP ? C4 : 1];  // [wave][op of the wave][word]
   __shared__ uint16_t pst[COOP ? 4 : 1][round_up(P::T, 2) + 2];     // the support of the op gathered
