@@ -30,7 +30,7 @@ constexpr int FRONT_WG = 256;
 #endif
 constexpr int DEC_MIN_WAVES = 4;  // fewest waves per decoder workgroup a launch uses
 #ifndef DEC_REENC_LOADS
-#define DEC_REENC_LOADS 8
+#define DEC_REENC_LOADS 16  // re-encryption column loads in flight per pass (8: m = 13 decoders 1-2.5 % slower)
 #endif
 // decoder workgroups: the per-set field tables in LDS are shared by the workgroup's waves, so the
 // wave count per workgroup is what sets the waves per SIMD.  m = 12 (tables 40 KB): 8-wave
