@@ -44,6 +44,7 @@ template <int SET, bool BS, int OPS = 4> struct DecShape {
   static constexpr int WAVES = Par<SET>::M == 12 ? DEC_WAVES_M12 : (BS ? (OPS == 8 ? 8 : DEC_WAVES_M13) : 8);
   static constexpr int WG = 64 * WAVES;
   static constexpr int WAVES_PER_EU = Par<SET>::M == 12 ? 4 : (BS ? (OPS == 8 ? 2 : 3) : 2);
+  static constexpr int WGS_PER_CU = Par<SET>::M == 12 ? 16 / WAVES : 1;  // resident workgroups per CU
 };
 // decoder work counters: DEC_QUEUES of them, DEC_QSTRIDE words apart (separate 128-byte lines)
 constexpr int DEC_QUEUES = 8, DEC_QSTRIDE = 32;
@@ -100,6 +101,13 @@ __global__ __launch_bounds__(FRONT_WG) void k_dec_front(const uint8_t *__restric
 __device__ __forceinline__ uint32_t tab(const uint16_t *t, uint32_t byte_off) {
   return *reinterpret_cast<const uint16_t *>(reinterpret_cast<const uint8_t *>(t) + byte_off);
 }
+// LDS byte address of a pointer into LDS, and a 16-bit LDS read at such an address
+__device__ __forceinline__ uint32_t lds_addr(const void *p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
+}
+__device__ __forceinline__ uint32_t lds_u16(uint32_t addr) {
+  return *reinterpret_cast<const __attribute__((address_space(3))) uint16_t *>((uintptr_t)addr);
+}
 // Ops per wave and batch (OPS): Berlekamp-Massey runs one op per row of LR = 64 / OPS lanes.
 // Four ops on 16-lane rows, or for m = 12 with the bitsliced FFT also eight ops on 8-lane rows:
 // the per-step fixed work (row reduction, length update, loop control) is then shared by twice
@@ -130,123 +138,143 @@ __device__ __forceinline__ int opaque_zero() {
   return z;
 }
 
-template <int CTRL>
-__device__ __forceinline__ uint32_t dpp(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
-}
 // row_ror:n -- lane l of a 16-lane row reads lane (l - n) mod 16 of the same row
 constexpr int DPP_ROR1 = 0x121, DPP_ROR2 = 0x122, DPP_ROR4 = 0x124, DPP_ROR8 = 0x128;
 
-// XOR over each row of LR lanes (LR = 16: four row_ror steps; LR = 8: quad_perm swaps of lanes
-// 1 and 2 apart, then row_half_mirror, which pairs the two quads of each 8-lane half-row)
-constexpr int DPP_QP_XOR1 = 0xB1, DPP_QP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_ROR9 = 0x129;
+// row_ror:n without a previous value (every lane of a rotation has a source lane), so the move
+// folds into the VALU instruction that consumes it
+template <int CTRL>
+__device__ __forceinline__ uint32_t ror(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+}
+
+// Berlekamp-Massey rows (one op per row of LR lanes, coefficient i in the row's lane i mod LR of
+// register slot i / LR).  LR = 16: op r on the 16-lane DPP row r, coefficient lane l16.  LR = 8:
+// two ops per 16-lane DPP row, interleaved -- op r on the even (r even) or odd lanes of DPP row
+// r / 2, coefficient lane 2 j + (r & 1) for j = i mod 8 -- so that the x-shift of a slot is one
+// row_ror:2 (the coefficient-0 lanes take the previous slot's rotated lanes), as row_ror:1 is for
+// LR = 16, and the row sum is three row_ror steps.
+template <int LR>
+__device__ __forceinline__ int bm_row(int lane) {
+  return LR == 16 ? lane >> 4 : 2 * (lane >> 4) + (lane & 1);
+}
+template <int LR>
+__device__ __forceinline__ int bm_col(int lane) {
+  return LR == 16 ? lane & 15 : (lane & 15) >> 1;
+}
+template <int LR>
+constexpr int bm_lane(int row, int col) {
+  return LR == 16 ? 16 * row + col : 16 * (row >> 1) + 2 * col + (row & 1);
+}
+// x-shift of one slot within the rows (lane col of a row reads col - 1 of the same row)
+template <int LR>
+__device__ __forceinline__ uint32_t bm_shift(uint32_t v) {
+  static_assert(LR == 16 || LR == 8, "rows of 8 or 16 lanes");
+  return LR == 16 ? ror<DPP_ROR1>(v) : ror<DPP_ROR2>(v);
+}
+// XOR over each row of LR lanes
 template <int LR>
 __device__ __forceinline__ uint32_t row_xor(uint32_t v) {
-  if constexpr (LR == 16) {
-    v ^= dpp<DPP_ROR8>(v);
-    v ^= dpp<DPP_ROR4>(v);
-    v ^= dpp<DPP_ROR2>(v);
-    v ^= dpp<DPP_ROR1>(v);
-  } else {
-    static_assert(LR == 8, "rows of 8 or 16 lanes");
-    v ^= dpp<DPP_QP_XOR1>(v);
-    v ^= dpp<DPP_QP_XOR2>(v);
-    v ^= dpp<DPP_HALF_MIRROR>(v);
-  }
+  if constexpr (LR == 16) v ^= ror<DPP_ROR8>(v);
+  v ^= ror<LR == 16 ? DPP_ROR4 : DPP_ROR8>(v);
+  v ^= ror<LR == 16 ? DPP_ROR2 : DPP_ROR4>(v);
+  v ^= ror<LR == 16 ? DPP_ROR1 : DPP_ROR2>(v);
   return v;
-}
-
-
-// row_ror:n without a previous value (every lane of a rotation has a source lane)
-__device__ __forceinline__ uint32_t ror1(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, DPP_ROR1, 0xF, 0xF, true);
-}
-__device__ __forceinline__ uint32_t ror9(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, DPP_ROR9, 0xF, 0xF, true);
 }
 
 // One Berlekamp-Massey step on the wave's rows (one op per row of LR lanes: coefficient i of the
 // row's op in lane i mod LR of the row, slot i / LR), for NA active coefficient slots: slots >= NA
 // hold zero coefficients of C and B in every row (deg C <= L, deg B <= N + 1 - L), so they are
 // skipped.  The window W_i = s_{N-i} of slot s is read from the row's padded syndrome array at
-// wp + 2 LR (NSB - 1 - s) bytes (wp = the lane's window base, advanced 2 bytes per step).  Straight
+// wp + 2 LR (NSB - 1 - s) bytes (wp = the LDS address of the lane's window base, advanced 2 bytes per step).  Straight
 // line: a zero discrepancy turns the update factor into the zero sentinel and the length change
-// into a select, so no lane diverges.  lb = log b (doubled).
+// into a select, so no lane diverges.  lb = log b (doubled), L2 = twice the row's length L.
 template <int NA, int NSB, int T, int Q, int LR>
-__device__ __forceinline__ void bm_step(int N, const uint8_t *wp, int lr, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
-                                        int &L, uint32_t &lb, const uint16_t *lg_s, const uint16_t *ex_s) {
+__device__ __forceinline__ void bm_step(int N, uint32_t wp, int lr, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
+                                        int &L2, uint32_t &lb, const uint16_t *lg_s, const uint16_t *ex_s) {
   constexpr uint32_t ZL2 = 4u * Q, Q2 = 2u * Q;  // doubled sentinel and modulus
-  uint32_t lc[NA], dp = 0;
+  uint32_t lc[NA], pr[NA];
 #pragma unroll
   for (int s = 0; s < NA; ++s) lc[s] = tab(lg_s, C[s]);
 #pragma unroll
   for (int s = 0; s < NA; ++s) {
-    const uint32_t wv = *reinterpret_cast<const uint16_t *>(wp + 2 * LR * (NSB - 1 - s));
-    dp ^= tab(ex_s, lc[s] + wv);
+    const uint32_t wv = lds_u16(wp + 2 * LR * (NSB - 1 - s));
+    pr[s] = tab(ex_s, lc[s] + wv);
   }
+  // the products' sum by three-input XORs on 32-bit words (the entries are zero-extended: no
+  // 16-bit arithmetic and no re-masking before the row sum)
+  uint32_t dp = pr[0];
+#pragma unroll
+  for (int s = 1; s < NA; s += 2)
+    dp = s + 1 < NA ? xor3(dp, pr[s], pr[s + 1]) : __builtin_amdgcn_bitop3_b32(dp, pr[s], 0u, 0x3C);  // dp ^ pr[s]
   const uint32_t d = row_xor<LR>(dp);
   const uint32_t ld = tab(lg_s, d);
   uint32_t lf = ld + Q2 - lb;  // log (d / b): ld - lb mod Q2, the subtraction wrapping when lf < Q2
   lf = min(lf, lf - Q2);
   lf = d != 0 ? lf : ZL2;
-  const bool upd = d != 0 && 2 * L <= N;
+  const bool upd = d != 0 && L2 <= N;
 #pragma unroll
   for (int s = 0; s < NA; ++s) {
     C[s] ^= tab(ex_s, lf + Bl[s]);
     Bl[s] = upd ? lc[s] : Bl[s];
   }
-  L = upd ? N + 1 - L : L;
+  L2 = upd ? 2 * (N + 1) - L2 : L2;
   lb = upd ? ld : lb;
-  // B <- x B on the slots that can hold a nonzero coefficient afterwards: within a row a rotation
-  // by one; the row's first lane takes the previous slot's last coefficient (LR = 16: that slot
-  // rotated by one; LR = 8: by nine, which brings lane 7 (15) of the 16-lane DPP row to lane 0 (8))
+  // B <- x B on the slots that can hold a nonzero coefficient afterwards: within a row a shift by
+  // one coefficient; the row's coefficient-0 lane takes the previous slot's last coefficient,
+  // which the same rotation of that slot brings there
   constexpr int NS = NA < NSB ? NA + 1 : NSB;
-  uint32_t R[NS], Rc[NS];
+  uint32_t R[NS];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) R[s] = ror1(Bl[s]);
-#pragma unroll
-  for (int s = 0; s + 1 < NS; ++s) Rc[s] = LR == 16 ? R[s] : ror9(Bl[s]);
+  for (int s = 0; s < NS; ++s) R[s] = bm_shift<LR>(Bl[s]);
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    const uint32_t v = (lr == 0) ? (s ? Rc[s > 0 ? s - 1 : 0] : ZL2) : R[s];
+    const uint32_t v = (lr == 0) ? (s ? R[s > 0 ? s - 1 : 0] : ZL2) : R[s];
     Bl[s] = (s < NSB - 1 || LR * s + lr <= T) ? v : ZL2;
   }
 }
 // A(N) = max over the wave's rows of max(L, N + 1 - L): step N touches coefficients 0..A(N),
 // i.e. A(N) / LR + 1 slots.  A grows by at most one per step (without a length change
 // max(L, N + 2 - L) <= A + 1; with one, max(N + 1 - L, L + 1) <= A + 1).
+// The rows' lengths are kept doubled (L2 = 2 L, so a step tests L2 <= N with no shift).
 template <int LR>
-__device__ __forceinline__ int bm_active(int L, int N) {
-  int Lmax = __builtin_amdgcn_readlane(L, 0), Lmin = Lmax;
+__device__ __forceinline__ int bm_active(int L2, int N) {
+  int Lmax = __builtin_amdgcn_readlane(L2, 0), Lmin = Lmax;
 #pragma unroll
   for (int r = 1; r < 64 / LR; ++r) {
-    const int Lr = __builtin_amdgcn_readlane(L, LR * r);
+    const int Lr = __builtin_amdgcn_readlane(L2, bm_lane<LR>(r, 0));
     Lmax = max(Lmax, Lr);
     Lmin = min(Lmin, Lr);
   }
-  return max(Lmax, N + 1 - Lmin);
+  return max(Lmax >> 1, N + 1 - (Lmin >> 1));
 }
 // Berlekamp-Massey steps N.. while NA slots cover the active coefficients, then the next count
 // (the count never decreases).  Since A grows by at most one per step, NA slots (coefficients
 // 0..LR NA - 1) stay enough for LR NA - A(N) more steps: those run without re-reading the lengths.
 template <int NA, int NSB, int T, int Q, int LR>
-__device__ __forceinline__ void bm_run(int &N, const uint8_t *wl, int lr, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
-                                       int &L, uint32_t &lb, const uint16_t *lg_s, const uint16_t *ex_s) {
+__device__ __forceinline__ void bm_run(int &N, uint32_t wl, int lr, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
+                                       int &L2, uint32_t &lb, const uint16_t *lg_s, const uint16_t *ex_s) {
 #pragma unroll 1
   while (N < 2 * T) {
     int rem = 2 * T - N;
     if constexpr (NA < NSB) {
-      const int A = bm_active<LR>(L, N);
+      const int A = bm_active<LR>(L2, N);
       if (A > LR * NA - 1) break;
       rem = min(rem, LR * NA - A);
     }
+    // the window's address advances 2 bytes per step, kept whole in one register (split into a
+    // scalar step part and a lane part, every slot's read would need an address add)
+    uint32_t wp = wl + 2 * N;
+    rem = __builtin_amdgcn_readfirstlane(rem);
 #pragma unroll 1
     for (int k = 0; k < rem; ++k) {
-      bm_step<NA, NSB, T, Q, LR>(N, wl + 2 * N, lr, C, Bl, L, lb, lg_s, ex_s);
+      bm_step<NA, NSB, T, Q, LR>(N, wp, lr, C, Bl, L2, lb, lg_s, ex_s);
       ++N;
+      wp += 2;
+      asm volatile("" : "+v"(wp));
     }
   }
-  if constexpr (NA < NSB) bm_run<NA + 1, NSB, T, Q, LR>(N, wl, lr, C, Bl, L, lb, lg_s, ex_s);
+  if constexpr (NA < NSB) bm_run<NA + 1, NSB, T, Q, LR>(N, wl, lr, C, Bl, L2, lb, lg_s, ex_s);
 }
 
 // ---- bitsliced GF(2^m) arithmetic (the decoder's bottom-up FFT, fft_form 1)
@@ -493,9 +521,9 @@ void k_dec_decode(DecArgs a) {
     // ---- 2. Berlekamp-Massey, one op per row; C in normal form, B and the syndrome window
     //         W_i = s_{N-i} in log form
     {
-      const int brow = lane / LR, lr = lane % LR;
+      const int brow = bm_row<LR>(lane), lr = bm_col<LR>(lane);
       // the lane's window base: slot s reads s_{N - LR s - lr} at wl + 2 N + 2 LR (NSB - 1 - s)
-      const uint8_t *wl = reinterpret_cast<const uint8_t *>(w.s[brow] + (SPAD - lr - LR * (NSB - 1)));
+      const uint32_t wl = lds_addr(w.s[brow] + (SPAD - lr - LR * (NSB - 1)));
       uint32_t C[NSB], Bl[NSB];
 #pragma unroll
       for (int s = 0; s < NSB; ++s) {
@@ -503,9 +531,9 @@ void k_dec_decode(DecArgs a) {
         C[s] = (i == 0) ? 2u : 0u;  // doubled values and logs
         Bl[s] = (i == 1) ? 0u : 2 * ZL;
       }
-      int L = 0, N = 0;
+      int L2 = 0, N = 0;  // twice the length L
       uint32_t lb = 0;  // log b, b = 1
-      bm_run<1, NSB, T, Q, LR>(N, wl, lr, C, Bl, L, lb, lg_s, ex_s);
+      bm_run<1, NSB, T, Q, LR>(N, wl, lr, C, Bl, L2, lb, lg_s, ex_s);
       // locator sigma_j = C_{t-j} (sigma_t = C_0 = 1 is added from the power table when t = D):
       // the first half's ops into f, the second half's into their (now dead) syndrome rows, past
       // the zero padding that the next batch's windows read
@@ -983,12 +1011,12 @@ hipError_t launch_dec(const DecArgs &args, hipStream_t s) {
   // wave gets at least two of them
   constexpr bool OPS8 = P::M == 12 || DEC_OPS8_M13;
   const int max_waves8 = DecShape<SET, true, 8>::WAVES;
-  const int resident = (P::M == 12 ? 2 : 1) * cus * max_waves8;
+  const int resident = DecShape<SET, true, 8>::WGS_PER_CU * cus * max_waves8;
   const bool ops8 = OPS8 && a.fft_form && ceil_div(a.nops, 8) >= 2 * resident;
   const int max_waves = ops8 ? max_waves8 : max_waves4;
   const int nbat = ceil_div(a.nops, ops8 ? 8 : 4);
   const int waves = nbat >= cus * max_waves ? max_waves : std::max(DEC_MIN_WAVES, std::min(max_waves, ceil_div(nbat, cus)));
-  int grid = a.decode_blocks > 0 ? a.decode_blocks : (P::M == 12 ? 512 : 256);
+  int grid = a.decode_blocks > 0 ? a.decode_blocks : DecShape<SET, true>::WGS_PER_CU * cus;
   const int maxgrid = ceil_div(nbat, waves);
   if (grid > maxgrid) grid = maxgrid;
   {
