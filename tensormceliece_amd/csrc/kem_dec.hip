@@ -353,10 +353,21 @@ __device__ __forceinline__ void swap_word_bit(int j, uint32_t &x, uint32_t &y) {
 }
 // the multiplier words of one bottom-up level in this lane: bit q of word b is bit b of
 // plo[d][lane] ^ phi[k(q)]; `ph` holds the phi part (bit planes, lane-uniform), plo2 is plo doubled
+// (the planes of one multiplier set start 16-byte aligned, M rounded up to 4 words: read as 16-byte
+// LDS loads, whose 16-bit offset field holds the table's address, with no address add)
 template <int M>
 __device__ __forceinline__ void bs_multipliers(const uint32_t *ph, uint32_t plo2, uint32_t (&c)[M]) {
+  uint32_t p[(M + 3) & ~3];
 #pragma unroll
-  for (int b = 0; b < M; ++b) c[b] = ph[b] ^ (uint32_t)(-(int)((plo2 >> (b + 1)) & 1u));
+  for (int q = 0; q < (M + 3) / 4; ++q) {
+    const uint4 v = reinterpret_cast<const uint4 *>(ph)[q];
+    p[4 * q] = v.x;
+    p[4 * q + 1] = v.y;
+    p[4 * q + 2] = v.z;
+    p[4 * q + 3] = v.w;
+  }
+#pragma unroll
+  for (int b = 0; b < M; ++b) c[b] = p[b] ^ (uint32_t)(-(int)((plo2 >> (b + 1)) & 1u));
 }
 
 // One wavefront decodes four ops at a time: Berlekamp-Massey runs on the four 16-lane rows in
@@ -401,6 +412,7 @@ void k_dec_decode(DecArgs a) {
   static_assert(M - KD == 6 && NR == Dn, "FFT layout assumes 2^(m-KD) = 64 points per constant");
   constexpr int NPHI = (1 << (M - 6)) - (1 << (M - 6 - KD));  // sum over levels of 2^(m-7-d)
   constexpr int NG = NR / 32;                     // bitsliced words per plane and lane (2 or 4)
+  constexpr int MP = (M + 3) & ~3;                // plane words per multiplier set, 16-byte rows
   // the workgroup's LDS in one object, the field tables first: their bases then fit the 16-bit
   // offset field of ds_read (lg at 0, ex at 2^(m+1) bytes), so a lookup at a register index costs
   // no address add (placed by the compiler after the per-wave buffers, lg sat at 66 KB and every
@@ -415,8 +427,8 @@ void k_dec_decode(DecArgs a) {
     uint16_t lpt[BS ? 1 : NPHI * 64];  // logs of the butterfly multipliers [phi index][lane]
     // bitsliced form: the multipliers' register parts as bit planes, [level + KD g][b] (g: which
     // 32-point half of a 64-point butterfly span), and the monic term's register part [group][b]
-    alignas(4) uint32_t phib[BS ? 2 * KD * M : 1];
-    uint32_t pthib[BS ? NG * M : 1];
+    alignas(16) uint32_t phib[BS ? 2 * KD * MP : 1];
+    alignas(16) uint32_t pthib[BS ? NG * MP : 1];
     alignas(16) DecWave<SET, OPS> Wv[DEC_WAVES];
   };
   __shared__ Lds L;
@@ -462,21 +474,22 @@ void k_dec_decode(DecArgs a) {
     // plo[d][l] ^ phi[off_d + (R mod h)], off_d = 2^(m-6) - 2^(m-6-d).  Bit q of plane word
     // (d + KD g, b) is bit b of phi[off_d + (q mod h)] for h < 32 (pairs inside a 32-point word),
     // of phi[off_d + 32 g + q] for h >= 32 (pairs of words, g = word index mod h/32)
-    for (int i = tid; i < 2 * KD * M; i += wg) {
-      const int slot = i / M, b = i % M, d = slot % KD, g = slot / KD;
+    for (int i = tid; i < 2 * KD * MP; i += wg) {
+      const int slot = i / MP, b = i % MP, d = slot % KD, g = slot / KD;
       const int h = 1 << (M - 7 - d), off = (1 << (M - 6)) - (1 << (M - 6 - d));
       uint32_t w = 0;
-      if (h < 32 || g < h / 32)
+      if (b < M && (h < 32 || g < h / 32))
         for (int q = 0; q < 32; ++q) {
           const int j = h < 32 ? (q & (h - 1)) : 32 * g + q;
           w |= ((uint32_t)(phi_s[off + j] >> (b + 1)) & 1u) << q;  // phi_s is doubled
         }
       phib_s[i] = w;
     }
-    for (int i = tid; i < NG * M; i += wg) {
-      const int G = i / M, b = i % M;
+    for (int i = tid; i < NG * MP; i += wg) {
+      const int G = i / MP, b = i % MP;
       uint32_t w = 0;
-      for (int q = 0; q < 32; ++q) w |= ((uint32_t)(pthi_s[32 * G + q] >> (b + 1)) & 1u) << q;
+      if (b < M)
+        for (int q = 0; q < 32; ++q) w |= ((uint32_t)(pthi_s[32 * G + q] >> (b + 1)) & 1u) << q;
       pthib_s[i] = w;
     }
   }
@@ -750,7 +763,7 @@ void k_dec_decode(DecArgs a) {
           for (int G = 0; G < NG; ++G) {
             if (G & gap) continue;
             uint32_t C[M], Z[M];
-            bs_multipliers<M>(phq + (d + KD * (gap == 2 ? G : 0)) * M, plo2, C);
+            bs_multipliers<M>(phq + (d + KD * (gap == 2 ? G : 0)) * MP, plo2, C);
             bs_mul<M>(Vb[G + gap], C, Z);
 #pragma unroll
             for (int b = 0; b < M; ++b) {
@@ -761,12 +774,23 @@ void k_dec_decode(DecArgs a) {
         }
         if constexpr (P::TOP_TERM) {  // + x^t = (lane)^t + (64 R)^t (t a power of two)
           const uint32_t ptlo2 = plq[64 * KD + lane];
-          const uint32_t *ptq = pthib_s + zo;
+          const uint4 *ptq = reinterpret_cast<const uint4 *>(pthib_s + zo);
+          uint32_t pt[NG][MP];
+#pragma unroll
+          for (int G = 0; G < NG; ++G)
+#pragma unroll
+            for (int q = 0; q < MP / 4; ++q) {
+              const uint4 v = ptq[G * (MP / 4) + q];
+              pt[G][4 * q] = v.x;
+              pt[G][4 * q + 1] = v.y;
+              pt[G][4 * q + 2] = v.z;
+              pt[G][4 * q + 3] = v.w;
+            }
 #pragma unroll
           for (int b = 0; b < M; ++b) {
             const uint32_t msk = (uint32_t)(-(int)((ptlo2 >> (b + 1)) & 1u));
 #pragma unroll
-            for (int G = 0; G < NG; ++G) Vb[G][b] = xor3(Vb[G][b], ptq[G * M + b], msk);
+            for (int G = 0; G < NG; ++G) Vb[G][b] = xor3(Vb[G][b], pt[G][b], msk);
           }
         }
 #pragma unroll
