@@ -287,17 +287,21 @@ __device__ __forceinline__ uint32_t and_xor(uint32_t acc, uint32_t a, uint32_t b
 __device__ __forceinline__ uint32_t select_bits(uint32_t m, uint32_t a, uint32_t b) {  // m ? a : b, per bit
   return __builtin_amdgcn_bitop3_b32(m, a, b, 0xCA);
 }
-// r = a * b elementwise (32 products): schoolbook over the m x m coefficient pairs, one three-input
-// AND-XOR each, then reduction of the 2m - 1 product words by the field polynomial
+// r = x + a * b elementwise (32 products): schoolbook over the m x m coefficient pairs, one
+// three-input AND-XOR each, the low words' sums starting from x (so the addition costs nothing),
+// then reduction of the 2m - 1 product words by the field polynomial
 template <int M>
-__device__ __forceinline__ void bs_mul(const uint32_t (&a)[M], const uint32_t (&b)[M], uint32_t (&r)[M]) {
+__device__ __forceinline__ void bs_mul_add(const uint32_t (&x)[M], const uint32_t (&a)[M], const uint32_t (&b)[M],
+                                           uint32_t (&r)[M]) {
   uint32_t p[2 * M - 1];
 #pragma unroll
   for (int i = 0; i < M; ++i)
 #pragma unroll
     for (int j = 0; j < M; ++j) {
-      if (i == 0 || j == M - 1) p[i + j] = a[i] & b[j];  // first term of word i + j
-      else p[i + j] = and_xor(p[i + j], a[i], b[j]);
+      if (i == 0 || j == M - 1)  // first term of word i + j
+        p[i + j] = i + j < M ? and_xor(x[i + j], a[i], b[j]) : a[i] & b[j];
+      else
+        p[i + j] = and_xor(p[i + j], a[i], b[j]);
     }
   // reduction: the high words h_k (k >= M, after what folds into them from above) land on
   // r_j = p_j + sum of h_{j + M - e} over the exponents e of the field polynomial's low terms,
@@ -764,11 +768,11 @@ void k_dec_decode(DecArgs a) {
             if (G & gap) continue;
             uint32_t C[M], Z[M];
             bs_multipliers<M>(phq + (d + KD * (gap == 2 ? G : 0)) * MP, plo2, C);
-            bs_mul<M>(Vb[G + gap], C, Z);
+            bs_mul_add<M>(Vb[G], Vb[G + gap], C, Z);  // x' = x + c y
 #pragma unroll
             for (int b = 0; b < M; ++b) {
-              Vb[G][b] ^= Z[b];               // x' = x + c y
-              Vb[G + gap][b] ^= Vb[G][b];     // y' = x' + y
+              Vb[G][b] = Z[b];
+              Vb[G + gap][b] ^= Z[b];  // y' = x' + y
             }
           }
         }
