@@ -27,7 +27,7 @@ struct tmce_key {
   uint32_t *Tv = nullptr;     // ... and 32-row blocks of words (VALU)
   uint32_t *F = nullptr;
   uint8_t *Fsw = nullptr;     // F as bit rows in MFMA step-pair pieces (INT8 MFMA front end)
-  uint32_t *Fcol = nullptr;   // syndrome map over all N positions, [N][FCW] (re-encryption check)
+  uint32_t *Fcol = nullptr;   // syndrome map over all N positions, [N + 1][FCW], column N zero (re-encryption check)
   uint16_t *alpha = nullptr, *la = nullptr, *lu = nullptr;
   uint16_t *ainv = nullptr;   // field element -> support index (0xFFFF off the support)
   uint32_t *supp = nullptr;   // support flags over the field (decoder point order)

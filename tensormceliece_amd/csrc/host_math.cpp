@@ -217,7 +217,8 @@ static void build_dec_key_t(const uint8_t *sk, DecKeyHost &o) {
   // front-end matrix: row j*M + b, column i (i < KBITS): bit b of alpha_i^j u_i (syndrome-major)
   // and the same map over all N positions, column-major (the decoder's re-encryption check)
   o.F.assign((size_t)P::R * P::KWP, 0);
-  o.Fcol.assign((size_t)P::N * P::FCW, 0);  // columns padded to 16-byte rows
+  // columns padded to 16-byte rows, and a zero column N (the re-encryption gather's padding reads it)
+  o.Fcol.assign((size_t)(P::N + 1) * P::FCW, 0);
   for (int i = 0; i < P::N; ++i) {
     uint16_t v = u[i];
     for (int j = 0; j < 2 * P::T; ++j) {

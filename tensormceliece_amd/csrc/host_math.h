@@ -42,7 +42,7 @@ struct DecKeyHost {
   std::vector<uint16_t> ainv;  // 2^m entries: i for the field element alpha_i, 0xFFFF off the support
   std::vector<uint32_t> supp;  // 2^m bits: field element 64 g + l is in the support (bit g % 32 of word l * 2^(m-11) + g / 32)
   std::vector<uint32_t> F;     // R x KWP words
-  std::vector<uint32_t> Fcol;  // N x FCW words: column i = syndromes of unit vector i (row j*M + b)
+  std::vector<uint32_t> Fcol;  // (N + 1) x FCW words: column i = syndromes of unit vector i (row j*M + b), column N zero
   std::vector<uint8_t> rej;  // N/8 bytes
 };
 void build_dec_key(int set, const uint8_t *sk, DecKeyHost &out);

@@ -126,7 +126,7 @@ struct DecWave {
   static constexpr int SPAD = LR * ceil_div(P::T + 1, LR);
   uint16_t s[OPS][SPAD + P::TT];
   uint16_t f[4][P::D];            // locator coefficients (FFT input), per op of the first half
-  uint16_t pos[P::IND_STRIDE];    // error positions of the op being finished
+  uint16_t pos[round_up(P::T, 4)];  // error positions of the op being finished (then a zero pad)
   uint64_t m64[17 * P::MSG_BLOCKS];  // session-key message of the op being finished
 };
 
@@ -845,6 +845,7 @@ void k_dec_decode(DecArgs a) {
         constexpr int CPI = 64 / SPN;
         constexpr int NJ = ceil_div(T, CPI);  // load instructions per op
         static_assert(FQ <= SPN, "a column fits its span");
+        static_assert(CPI * NJ <= round_up(T, 4), "the spans' positions fit the padded list");
         constexpr int KP = ceil_div(T, 64);
         // support index of each root (lane k holds root k), and its bit of e
         uint32_t pos_r[KP];
@@ -855,36 +856,37 @@ void k_dec_decode(DecArgs a) {
           pos_r[kk] = k < T ? (uint32_t)inv[w.pos[k]] : 0u;
         }
 #pragma unroll
-        for (int kk = 0; kk < KP; ++kk)
-          if (64 * kk + lane < T) {
+        for (int kk = 0; kk < KP; ++kk) {
+          const int k = 64 * kk + lane;
+          if (k < T) {
             atomicOr(&m32[(8u + pos_r[kk]) >> 5], 1u << ((8u + pos_r[kk]) & 31u));
-            w.pos[64 * kk + lane] = (uint16_t)pos_r[kk];  // support indices replace the points (read above)
+            w.pos[k] = (uint16_t)pos_r[kk];  // support indices replace the points (read above)
+          } else if (k < round_up(T, CPI)) {
+            w.pos[k] = (uint16_t)P::N;  // the last load's spans past t read the key's zero column N
           }
+        }
         wave_lds_sync();
         const int g = lane / SPN, li = lane % SPN;
         const int lq = li < FQ ? li : FQ - 1;  // lanes past the column re-read its last word
         const uint4 *fc = reinterpret_cast<const uint4 *>(k_fcol);
         uint4 acc = make_uint4(0, 0, 0, 0);
+        const uint16_t *pg = w.pos + g;  // span g reads positions g, g + CPI, ... (< round_up(T, CPI))
         // DEC_REENC_LOADS load instructions in flight per pass; a rolled outer loop (the compiler
         // would otherwise unroll every pass and hold all loads in registers at once)
 #pragma unroll 1
         for (int j0 = 0; j0 < NJ; j0 += DEC_REENC_LOADS) {
           uint32_t pk[DEC_REENC_LOADS];
 #pragma unroll
-          for (int jj = 0; jj < DEC_REENC_LOADS; ++jj) {
-            const int k = CPI * (j0 + jj) + g;
-            pk[jj] = w.pos[k < T ? k : T - 1];
-          }
+          for (int jj = 0; jj < DEC_REENC_LOADS; ++jj)
+            pk[jj] = pg[CPI * (NJ % DEC_REENC_LOADS == 0 ? j0 + jj : min(j0 + jj, NJ - 1))];
 #pragma unroll
           for (int jj = 0; jj < DEC_REENC_LOADS; ++jj) {
             if (NJ % DEC_REENC_LOADS != 0 && j0 + jj >= NJ) break;
             const uint4 v = global_load16(fc, (size_t)pk[jj] * FQ + lq);
-            // a span past the last root (T not a multiple of CPI) adds nothing
-            const uint32_t keep = (T % CPI == 0 || CPI * (j0 + jj) + g < T) ? ~0u : 0u;
-            acc.x ^= v.x & keep;
-            acc.y ^= v.y & keep;
-            acc.z ^= v.z & keep;
-            acc.w ^= v.w & keep;
+            acc.x ^= v.x;
+            acc.y ^= v.y;
+            acc.z ^= v.z;
+            acc.w ^= v.w;
           }
         }
         // XOR of the spans' partial sums

@@ -21,7 +21,7 @@ struct KeyDev {
   const uint32_t *cols;   // encapsulation: column-major [I | T]
   const uint32_t *F;      // decapsulation: front-end matrix (VALU form)
   const uint8_t *Fsw;     // decapsulation: front-end matrix (MFMA form)
-  const uint32_t *Fcol;   // decapsulation: syndrome map columns [N][FCW]
+  const uint32_t *Fcol;   // decapsulation: syndrome map columns [N + 1][FCW], column N zero
   const uint16_t *alpha, *la, *lu;
   const uint16_t *ainv;   // decapsulation: field element -> support index (0xFFFF off the support)
   const uint32_t *supp;   // decapsulation: support flags over the field (decoder point order)
@@ -152,7 +152,7 @@ struct DecArgs {
   int front_mfma;        // front end on INT8 MFMA (1) or on the VALU (0)
   int front_cfg;         // MFMA tile shape of the front end
   uint32_t *xscratch;    // gf2_scratch_bytes(set, GF2_FRONT, n) bytes (MFMA front end; null if 0)
-  const uint32_t *Fcol;  // key: [N][FCW] syndromes of each unit vector (rows as in S, zero-padded)
+  const uint32_t *Fcol;  // key: [N + 1][FCW] syndromes of each unit vector (rows as in S, zero-padded), column N zero
   const uint16_t *alpha; // key: [N] support
   const uint16_t *la;    // key: [N] log alpha (0xFFFF for alpha = 0)
   const uint16_t *lu;    // key: [N] log 1/g(alpha)^2 (0xFFFF for 0)
