@@ -503,6 +503,17 @@ void k_dec_decode(DecArgs a) {
   constexpr int SPAD = DecWave<SET, OPS>::SPAD;
   DecWave<SET, OPS> &w = Wv[wave];
   for (int i = lane0; i < OPS * SPAD; i += 64) w.s[i / SPAD][i % SPAD] = 2 * ZL;  // window padding (zero)
+  // m = 12, t = D: the monic term x^t at this lane's points, (lane)^t + (64 R)^t, as bit planes in
+  // the FFT's layout -- the same for every op, so it is built once and kept in registers (2 x 12)
+  constexpr bool TT_HELD = BS && P::TOP_TERM && NG == 2;
+  uint32_t tth[TT_HELD ? NG : 1][TT_HELD ? M : 1];
+  if constexpr (TT_HELD) {
+    const uint32_t ptlo2 = plo_s[64 * KD + lane0];
+#pragma unroll
+    for (int G = 0; G < NG; ++G)
+#pragma unroll
+      for (int b = 0; b < M; ++b) tth[G][b] = pthib_s[G * MP + b] ^ (uint32_t)(-(int)((ptlo2 >> (b + 1)) & 1u));
+  }
 
   // the wave takes batches of OPS ops from work counters, one per XCD (workgroups are dispatched
   // to the 8 XCDs round-robin; counter x hands out batches x, x + 8, x + 16, ...), and from the
@@ -776,7 +787,9 @@ void k_dec_decode(DecArgs a) {
             }
           }
         }
-        if constexpr (P::TOP_TERM) {  // + x^t = (lane)^t + (64 R)^t (t a power of two)
+        if constexpr (TT_HELD) {
+          // (added inside the zero-flag OR below)
+        } else if constexpr (P::TOP_TERM) {  // + x^t = (lane)^t + (64 R)^t (t a power of two)
           const uint32_t ptlo2 = plq[64 * KD + lane];
           const uint4 *ptq = reinterpret_cast<const uint4 *>(pthib_s + zo);
           uint32_t pt[NG][MP];
@@ -799,9 +812,16 @@ void k_dec_decode(DecArgs a) {
         }
 #pragma unroll
         for (int G = 0; G < NG; ++G) {  // zero flags: no bit set in any plane
-          uint32_t o = Vb[G][0];
+          uint32_t o;
+          if constexpr (TT_HELD) {  // o | (v ^ x^t) per plane, one three-input operation each
+            o = Vb[G][0] ^ tth[G][0];
 #pragma unroll
-          for (int b = 1; b < M; ++b) o |= Vb[G][b];
+            for (int b = 1; b < M; ++b) o = __builtin_amdgcn_bitop3_b32(o, Vb[G][b], tth[G][b], 0xF6);
+          } else {
+            o = Vb[G][0];
+#pragma unroll
+            for (int b = 1; b < M; ++b) o |= Vb[G][b];
+          }
           zm[G] = ~o;
         }
       }
