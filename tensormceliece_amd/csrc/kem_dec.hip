@@ -126,7 +126,7 @@ struct DecWave {
   static constexpr int SPAD = LR * ceil_div(P::T + 1, LR);
   uint16_t s[OPS][SPAD + P::TT];
   uint16_t f[4][P::D];            // locator coefficients (FFT input), per op of the first half
-  uint16_t pos[round_up(P::T, 4)];  // error positions of the op being finished (then a zero pad)
+  uint16_t pos[round_up(P::T, 4)];  // error positions of the op being finished, then padding
   uint64_t m64[17 * P::MSG_BLOCKS];  // session-key message of the op being finished
 };
 
@@ -186,9 +186,9 @@ __device__ __forceinline__ uint32_t row_xor(uint32_t v) {
 // row's op in lane i mod LR of the row, slot i / LR), for NA active coefficient slots: slots >= NA
 // hold zero coefficients of C and B in every row (deg C <= L, deg B <= N + 1 - L), so they are
 // skipped.  The window W_i = s_{N-i} of slot s is read from the row's padded syndrome array at
-// wp + 2 LR (NSB - 1 - s) bytes (wp = the LDS address of the lane's window base, advanced 2 bytes per step).  Straight
-// line: a zero discrepancy turns the update factor into the zero sentinel and the length change
-// into a select, so no lane diverges.  lb = log b (doubled), L2 = twice the row's length L.
+// LDS address wp + 2 LR (NSB - 1 - s) (wp = the lane's window base, advanced 2 bytes per step).
+// Straight line: a zero discrepancy turns the update factor into the zero sentinel and the length
+// change into a select, so no lane diverges.  lb = log b (doubled), L2 = twice the row's length L.
 template <int NA, int NSB, int T, int Q, int LR>
 __device__ __forceinline__ void bm_step(int N, uint32_t wp, int lr, uint32_t (&C)[NSB], uint32_t (&Bl)[NSB],
                                         int &L2, uint32_t &lb, const uint16_t *lg_s, const uint16_t *ex_s) {
