@@ -120,10 +120,12 @@ template <int SET, int OPS>
 struct DecWave {
   using P = Par<SET>;
   static constexpr int LR = 64 / OPS;
-  // syndromes of the wave's ops (doubled logs, 2 ZL for 0) behind SPAD zero entries, so the
-  // Berlekamp-Massey window s_{N-i} reads zero for i > N; after BM, rows 4..7 hold the second
-  // half's locators and FFT buffers
-  static constexpr int SPAD = LR * ceil_div(P::T + 1, LR);
+  // syndromes of the wave's ops (doubled logs, 2 ZL for 0) behind SPAD = LR zero entries, so the
+  // Berlekamp-Massey window s_{N-i} reads zero for i > N: step N reads i up to LR NA - 1 for NA
+  // active slots, and NA slots are entered only once A(N) >= LR (NA - 1) (A, the highest active
+  // coefficient, at most N + 1), so i <= N + LR; after BM, rows 4..7 hold the second half's
+  // locators and FFT buffers
+  static constexpr int SPAD = LR;
   uint16_t s[OPS][SPAD + P::TT];
   uint16_t f[4][P::D];            // locator coefficients (FFT input), per op of the first half
   uint16_t pos[round_up(P::T, 4)];  // error positions of the op being finished, then padding
@@ -551,7 +553,7 @@ void k_dec_decode(DecArgs a) {
     {
       const int brow = bm_row<LR>(lane), lr = bm_col<LR>(lane);
       // the lane's window base: slot s reads s_{N - LR s - lr} at wl + 2 N + 2 LR (NSB - 1 - s)
-      const uint32_t wl = lds_addr(w.s[brow] + (SPAD - lr - LR * (NSB - 1)));
+      const uint32_t wl = (uint32_t)((int)lds_addr(w.s[brow]) + 2 * (SPAD - lr - LR * (NSB - 1)));
       uint32_t C[NSB], Bl[NSB];
 #pragma unroll
       for (int s = 0; s < NSB; ++s) {
