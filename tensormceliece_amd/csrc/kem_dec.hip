@@ -38,12 +38,12 @@ constexpr int DEC_MIN_WAVES = 4;  // fewest waves per decoder workgroup a launch
 // 12-wave workgroup per CU, three waves per SIMD (at most 168 VGPRs); the table form of the FFT
 // (BS false) keeps 8 waves for m = 13, its multiplier-log table does not leave LDS for 12
 #ifndef DEC_OPS8_M13
-#define DEC_OPS8_M13 0  // eight-op batches for m = 13 too (8-wave workgroups: eight windows fit LDS)
+#define DEC_OPS8_M13 1  // eight-op batches for m = 13 too (0: four-op batches only)
 #endif
 template <int SET, bool BS, int OPS = 4> struct DecShape {
-  static constexpr int WAVES = Par<SET>::M == 12 ? DEC_WAVES_M12 : (BS ? (OPS == 8 ? 8 : DEC_WAVES_M13) : 8);
+  static constexpr int WAVES = Par<SET>::M == 12 ? DEC_WAVES_M12 : (BS ? DEC_WAVES_M13 : 8);
   static constexpr int WG = 64 * WAVES;
-  static constexpr int WAVES_PER_EU = Par<SET>::M == 12 ? 4 : (BS ? (OPS == 8 ? 2 : 3) : 2);
+  static constexpr int WAVES_PER_EU = Par<SET>::M == 12 ? 4 : (BS ? 3 : 2);
   static constexpr int WGS_PER_CU = Par<SET>::M == 12 ? 16 / WAVES : 1;  // resident workgroups per CU
 };
 // decoder work counters: DEC_QUEUES of them, DEC_QSTRIDE words apart (separate 128-byte lines)
@@ -109,13 +109,13 @@ __device__ __forceinline__ uint32_t lds_u16(uint32_t addr) {
   return *reinterpret_cast<const __attribute__((address_space(3))) uint16_t *>((uintptr_t)addr);
 }
 // Ops per wave and batch (OPS): Berlekamp-Massey runs one op per row of LR = 64 / OPS lanes.
-// Four ops on 16-lane rows, or for m = 12 with the bitsliced FFT also eight ops on 8-lane rows:
-// the per-step fixed work (row reduction, length update, loop control) is then shared by twice
-// the ops and the active coefficients are padded to 8 instead of 16 (6 % fewer VALU instructions
-// per op), and the FFT and the checks take the batch as two halves of four.  Eight-op batches
-// double the work per batch, so the launcher uses them only when every resident wave gets at
-// least two (348864 at N = 65536: 0.430 -> 0.421 ms; at N = 8192 they would be 25 % slower).
-// m = 13 keeps four: eight ops' syndrome windows do not fit LDS beside 12 waves.
+// Four ops on 16-lane rows, or with the bitsliced FFT also eight ops on 8-lane rows: the per-step
+// fixed work (row reduction, length update, loop control) is then shared by twice the ops and the
+// active coefficients are padded to 8 instead of 16, and the FFT and the checks take the batch as
+// two halves of four.  Eight-op batches double the work per batch, so the launcher uses them only
+// when every resident wave gets at least two (348864 at N = 65536: 0.430 -> 0.421 ms, m = 13
+// 4-10 % faster; at N = 8192 they would be 25 % slower).  For m = 13 the eight windows fit LDS
+// beside 12 waves with one row width of window padding and the FFT buffers in the rows.
 template <int SET, int OPS>
 struct DecWave {
   using P = Par<SET>;
@@ -127,7 +127,15 @@ struct DecWave {
   // locators and FFT buffers
   static constexpr int SPAD = LR;
   uint16_t s[OPS][SPAD + P::TT];
-  uint16_t f[4][P::D];            // locator coefficients (FFT input), per op of the first half
+  // the top-down FFT's two buffers of op q of half h: A (the locator, then every second level's
+  // result) and B.  When a syndrome row holds 2 D entries (t = D) both lie in the op's own row, past
+  // the padding; otherwise the first half uses f and its row, the second half its row and f.
+  static constexpr bool F_IN_ROWS = P::TT >= 2 * P::D;
+  uint16_t f[F_IN_ROWS ? 1 : 4][F_IN_ROWS ? 1 : P::D];
+  __device__ uint16_t *bufA(int h, int q) { return F_IN_ROWS || h ? s[4 * h + q] + SPAD : f[q]; }
+  __device__ uint16_t *bufB(int h, int q) {
+    return F_IN_ROWS ? s[4 * h + q] + SPAD + P::D : (h ? f[q] : s[q] + SPAD);
+  }
   uint16_t pos[round_up(P::T, 4)];  // error positions of the op being finished, then padding
   uint64_t m64[17 * P::MSG_BLOCKS];  // session-key message of the op being finished
 };
@@ -564,11 +572,11 @@ void k_dec_decode(DecArgs a) {
       int L2 = 0, N = 0;  // twice the length L
       uint32_t lb = 0;  // log b, b = 1
       bm_run<1, NSB, T, Q, LR>(N, wl, lr, C, Bl, L2, lb, lg_s, ex_s);
-      // locator sigma_j = C_{t-j} (sigma_t = C_0 = 1 is added from the power table when t = D):
-      // the first half's ops into f, the second half's into their (now dead) syndrome rows, past
-      // the zero padding that the next batch's windows read
+      // locator sigma_j = C_{t-j} (sigma_t = C_0 = 1 is added from the power table when t = D)
+      // into buffer A of the op (f or its now dead syndrome row, past the zero padding that the
+      // next batch's windows read)
       wave_lds_sync();
-      uint16_t *lrow = brow < 4 ? w.f[brow] : w.s[brow] + SPAD;
+      uint16_t *lrow = w.bufA(brow >> 2, brow & 3);
       for (int j = lr; j < Dn; j += LR) lrow[j] = 0;
       wave_lds_sync();
 #pragma unroll
@@ -582,12 +590,11 @@ void k_dec_decode(DecArgs a) {
 #pragma unroll 1
     for (int hf = 0; hf < OPS / 4; ++hf) {
     // ---- 3. additive FFT, top-down on the D coefficients of four locators at once (row r: op r of
-    //         the half; first half: f, and the syndrome row (free now) as the second buffer;
-    //         second half: the syndrome row, and f (free again) as the second buffer; the rows'
-    //         zero padding is never written): twist, Taylor expansion at x^2 + x, even/odd split
+    //         the half, in its buffers A and B; the rows' zero padding is never written): twist,
+    //         Taylor expansion at x^2 + x, even/odd split
     {
-      uint16_t *fr = hf ? w.s[4 + row] + SPAD : w.f[row];
-      uint16_t *tr = hf ? w.f[row] : w.s[row] + SPAD;
+      uint16_t *fr = w.bufA(hf, row);
+      uint16_t *tr = w.bufB(hf, row);
       static_assert(P::TT >= Dn, "syndrome array doubles as the FFT buffer");
 #pragma unroll 1
       for (int d = 0; d < KD; ++d) {
@@ -668,7 +675,7 @@ void k_dec_decode(DecArgs a) {
       constexpr int EW = (1 + P::N / 8 + 3) / 4;
       for (int k = lane; k < EW; k += 64) m32[k] = 0;
       // the top-down result: after KD buffer swaps it is in the first buffer (KD even) or the second
-      const uint16_t *fq = (KD & 1) ? (hf ? w.f[q] : w.s[q] + SPAD) : (hf ? w.s[4 + q] + SPAD : w.f[q]);
+      const uint16_t *fq = (KD & 1) ? w.bufB(hf, q) : w.bufA(hf, q);
       // ciphertext bytes into the message buffer (before the butterflies need the registers)
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {

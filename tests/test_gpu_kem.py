@@ -118,11 +118,14 @@ def test_roundtrip_large(tm, orc, s, n):
     assert np.array_equal(ct_s, ct_ref) and np.array_equal(k_s, k_ref)
 
 
-def test_decoder_batch_shapes_agree(tm, orc):
-    """348864 decapsulation at N = 65536 runs eight-op batches (Berlekamp-Massey on 8-lane rows); in
-    pieces of 16384 it runs four-op batches (16-lane rows).  Both give the same keys and statuses on
-    valid, tampered and random ciphertexts, and equal the oracle on a sample of each kind."""
-    s, n = "348864", 65536
+@pytest.mark.parametrize("s", ["348864", "460896", "6960119"])
+def test_decoder_batch_shapes_agree(tm, orc, s):
+    """Decapsulation at N = 65536 runs eight-op batches (Berlekamp-Massey on interleaved 8-lane
+    rows); in pieces of 16384 it runs four-op batches (16-lane rows).  Both give the same keys and
+    statuses on valid, tampered and random ciphertexts, and equal the oracle on a sample of each
+    kind.  460896 keeps its top-down FFT's second buffer outside the syndrome rows (2t < 2D),
+    6960119 (odd t) inside them."""
+    n = 65536
     pk, sk, _ = keys(orc, s)
     p = orc.params(s)
     rng = np.random.default_rng(67)
