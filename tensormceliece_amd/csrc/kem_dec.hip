@@ -40,6 +40,9 @@ constexpr int DEC_MIN_WAVES = 4;  // fewest waves per decoder workgroup a launch
 #ifndef DEC_OPS8_M13
 #define DEC_OPS8_M13 1  // eight-op batches for m = 13 too (0: four-op batches only)
 #endif
+#ifndef DEC_OPS8_MIN
+#define DEC_OPS8_MIN 2  // eight-op batches once every resident wave gets at least this many of them
+#endif
 template <int SET, bool BS, int OPS = 4> struct DecShape {
   static constexpr int WAVES = Par<SET>::M == 12 ? DEC_WAVES_M12 : (BS ? DEC_WAVES_M13 : 8);
   static constexpr int WG = 64 * WAVES;
@@ -1067,11 +1070,11 @@ hipError_t launch_dec(const DecArgs &args, hipStream_t s) {
   const int max_waves4 = a.fft_form ? DecShape<SET, true>::WAVES : DecShape<SET, false>::WAVES;
   const int cus = a.cus > 0 ? a.cus : 256;
   // eight-op batches (bitsliced FFT; m = 12, and m = 13 when built with them) when every resident
-  // wave gets at least two of them
+  // wave gets at least DEC_OPS8_MIN of them
   constexpr bool OPS8 = P::M == 12 || DEC_OPS8_M13;
   const int max_waves8 = DecShape<SET, true, 8>::WAVES;
   const int resident = DecShape<SET, true, 8>::WGS_PER_CU * cus * max_waves8;
-  const bool ops8 = OPS8 && a.fft_form && ceil_div(a.nops, 8) >= 2 * resident;
+  const bool ops8 = OPS8 && a.fft_form && ceil_div(a.nops, 8) >= DEC_OPS8_MIN * resident;
   const int max_waves = ops8 ? max_waves8 : max_waves4;
   const int nbat = ceil_div(a.nops, ops8 ? 8 : 4);
   const int waves = nbat >= cus * max_waves ? max_waves : std::max(DEC_MIN_WAVES, std::min(max_waves, ceil_div(nbat, cus)));
