@@ -223,6 +223,100 @@ __device__ __forceinline__ void shake256_prefix_32(const uint64_t *__restrict__ 
   for (int i = 0; i < 4; ++i) out[i] = A[i];
 }
 
+// ------------------------------------------------------------------ Keccak on a lane pair
+// One state per pair of adjacent lanes: lane p = 0 holds the low 32 bits of each of the 25 state
+// words, lane p = 1 the high 32 bits.  A 64-bit rotation by R gives both lanes the same form
+// (R < 32: alignbit(own, partner, 32 - R); R > 32: alignbit(partner, own, 64 - R)), the partner's
+// half coming by a DPP swap within the pair.  ~120 instructions per round per lane against 182 for
+// a whole state in one lane: lower latency per permutation when a batch is too small to fill the
+// chip one op per lane.  Every lane of the wave must be active (round constants by v_readlane,
+// DPP swaps).
+__device__ __forceinline__ uint32_t pair_swap(uint32_t v) {  // quad_perm [1, 0, 3, 2]
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);
+}
+template <int R>
+__device__ __forceinline__ uint32_t rotl_pair(uint32_t own) {
+  if constexpr (R == 0) {
+    return own;
+  } else if constexpr (R == 32) {
+    return pair_swap(own);
+  } else if constexpr (R < 32) {
+    return __builtin_amdgcn_alignbit(own, pair_swap(own), 32 - R);
+  } else {
+    return __builtin_amdgcn_alignbit(pair_swap(own), own, 64 - R);
+  }
+}
+__device__ __forceinline__ void keccak_f1600_pair(uint32_t (&s)[25], bool hi, const KeccakRC &rc) {
+#pragma unroll 1
+  for (int round = 0; round < 24; ++round) {
+    uint32_t c[5], r1[5];
+#pragma unroll
+    for (int x = 0; x < 5; ++x) c[x] = xor3(xor3(s[x], s[x + 5], s[x + 10]), s[x + 15], s[x + 20]);
+#pragma unroll
+    for (int x = 0; x < 5; ++x) r1[x] = __builtin_amdgcn_alignbit(c[x], pair_swap(c[x]), 31);  // rotl 1
+#pragma unroll
+    for (int i = 0; i < 25; ++i) s[i] = xor3(s[i], c[(i % 5 + 4) % 5], r1[(i % 5 + 1) % 5]);
+    // rho + pi: B[y + 5*((2x+3y)%5)] = rot(A[x+5y], r[x][y])
+    uint32_t b[25];
+    b[0] = s[0];
+    b[10] = rotl_pair<1>(s[1]);
+    b[20] = rotl_pair<62>(s[2]);
+    b[5] = rotl_pair<28>(s[3]);
+    b[15] = rotl_pair<27>(s[4]);
+    b[16] = rotl_pair<36>(s[5]);
+    b[1] = rotl_pair<44>(s[6]);
+    b[11] = rotl_pair<6>(s[7]);
+    b[21] = rotl_pair<55>(s[8]);
+    b[6] = rotl_pair<20>(s[9]);
+    b[7] = rotl_pair<3>(s[10]);
+    b[17] = rotl_pair<10>(s[11]);
+    b[2] = rotl_pair<43>(s[12]);
+    b[12] = rotl_pair<25>(s[13]);
+    b[22] = rotl_pair<39>(s[14]);
+    b[23] = rotl_pair<41>(s[15]);
+    b[8] = rotl_pair<45>(s[16]);
+    b[18] = rotl_pair<15>(s[17]);
+    b[3] = rotl_pair<21>(s[18]);
+    b[13] = rotl_pair<8>(s[19]);
+    b[14] = rotl_pair<18>(s[20]);
+    b[24] = rotl_pair<2>(s[21]);
+    b[9] = rotl_pair<61>(s[22]);
+    b[19] = rotl_pair<56>(s[23]);
+    b[4] = rotl_pair<14>(s[24]);
+#pragma unroll
+    for (int y = 0; y < 5; ++y)
+#pragma unroll
+      for (int x = 0; x < 5; ++x) s[5 * y + x] = chi3(b[5 * y + x], b[5 * y + (x + 1) % 5], b[5 * y + (x + 2) % 5]);
+    const uint32_t clo = (uint32_t)__builtin_amdgcn_readlane((int)rc.lo, round);
+    const uint32_t chi = (uint32_t)__builtin_amdgcn_readlane((int)rc.hi, round);
+    s[0] ^= hi ? chi : clo;
+  }
+}
+// SHAKE256 over whole padded 136-byte blocks (as shake256_blocks_32) on a lane pair: this lane's
+// halves of the 17 rate words of each block are the 32-bit words 2 w + hi of the block; the next
+// block is loaded before the current one is permuted.  out[i] = this lane's half of output word i.
+__device__ __forceinline__ void shake256_blocks_32_pair(const uint32_t *__restrict__ msg, int nblocks, bool hi,
+                                                        uint32_t out[4], const KeccakRC &rc) {
+  uint32_t s[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) s[i] = 0;
+  const uint32_t *m = msg + (hi ? 1 : 0);
+  uint32_t nxt[17];
+#pragma unroll
+  for (int w = 0; w < 17; ++w) nxt[w] = m[2 * w];
+  for (int b = 0; b < nblocks; ++b) {
+#pragma unroll
+    for (int w = 0; w < 17; ++w) s[w] ^= nxt[w];
+    if (b + 1 < nblocks) {
+#pragma unroll
+      for (int w = 0; w < 17; ++w) nxt[w] = m[34 * (b + 1) + 2 * w];
+    }
+    keccak_f1600_pair(s, hi, rc);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) out[i] = s[i];
+}
+
 // ------------------------------------------------------------------ GF(2^m) through LDS tables
 // lg[a] = discrete log of a (lg[0] unused), ex[i] = g^i for 0 <= i < q, q = 2^m - 1.
 struct GfLds {

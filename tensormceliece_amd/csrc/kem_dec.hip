@@ -40,6 +40,9 @@ constexpr int DEC_MIN_WAVES = 4;  // fewest waves per decoder workgroup a launch
 #ifndef DEC_OPS8_M13
 #define DEC_OPS8_M13 1  // eight-op batches for m = 13 too (0: four-op batches only)
 #endif
+#ifndef DEC_HASH_PAIR_MAX
+#define DEC_HASH_PAIR_MAX 32768  // batches up to this size hash each op on a lane pair (0: never)
+#endif
 #ifndef DEC_OPS8_MIN
 #define DEC_OPS8_MIN 2  // eight-op batches once every resident wave gets at least this many of them
 #endif
@@ -1033,6 +1036,40 @@ __global__ __launch_bounds__(256) void k_hash_msg(uint32_t *__restrict__ msg, co
   if (status) status[op] = st;
 }
 
+// k_hash_msg (the variants without plaintext confirmation) with each op's SHAKE256 on a lane pair
+// (the state's low halves in the even lane, high halves in the odd one): ~1.5x lower latency per
+// permutation for twice the lanes, used for batches too small to give every SIMD a wave one op per
+// lane.  128 ops per 256-thread workgroup.
+template <int SET>
+__global__ __launch_bounds__(256) void k_hash_msg_pair(const uint32_t *__restrict__ msg, const uint8_t *__restrict__ ct,
+                                                       uint8_t *__restrict__ key, int *__restrict__ status, int nops,
+                                                       const PlanDev *__restrict__ plan) {
+  using P = Par<SET>;
+  const KeccakRC rc;  // every lane stays active through the hash
+  if (plan) nops = plan->total;
+  if ((int)(blockIdx.x * (blockDim.x / 2)) >= nops) return;  // whole workgroup past the batch (uniform)
+  const int gop = (int)(blockIdx.x * (blockDim.x / 2) + (threadIdx.x >> 1));
+  const bool hi = (threadIdx.x & 1) != 0;
+  const bool live = gop < nops;
+  const int op = live ? gop : nops - 1;
+  uint32_t out[4];
+  shake256_blocks_32_pair(msg + (size_t)op * P::MSG_WORDS, P::MSG_BLOCKS, hi, out, rc);
+  int st = 0;
+  if (SET == S6960119) {
+    const uint8_t last = ct[(size_t)op * P::SYND_BYTES + P::SYND_BYTES - 1];
+    if (last >> (P::MT % 8)) {
+      st = -1;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) out[i] = ~0u;
+    }
+  }
+  if (!live) return;
+  uint32_t *kp = reinterpret_cast<uint32_t *>(key + 32 * (size_t)op) + (hi ? 1 : 0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) kp[2 * i] = out[i];
+  if (status && !hi) status[op] = st;
+}
+
 // the syndrome part (first SYND_BYTES) of each pc ciphertext, packed densely for the front end and
 // the decoder; one thread per 16-bit half
 template <int SET>
@@ -1098,6 +1135,10 @@ hipError_t launch_dec(const DecArgs &args, hipStream_t s) {
     TMCE_TIMED("dec_hash", s,
                k_hash_msg<SET, true><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, args.ct, a.key, a.status, a.nops,
                                                                            a.plan, a.rej));
+  else if (a.nops <= DEC_HASH_PAIR_MAX)  // lane-pair hash: 128 ops per workgroup
+    TMCE_TIMED("dec_hash", s,
+               k_hash_msg_pair<SET><<<ceil_div(a.nops, 128), 256, 0, s>>>(a.msg, a.ct, a.key, a.status, a.nops,
+                                                                         a.plan));
   else
     TMCE_TIMED("dec_hash", s,
                k_hash_msg<SET, false><<<ceil_div(a.nops, 256), 256, 0, s>>>(a.msg, a.ct, a.key, a.status, a.nops,
